@@ -1,0 +1,242 @@
+#!/usr/bin/env python3
+"""Bratu3D 256^3 time-to-1e-8 residual on one MI355X.
+
+A step is one complete nonlinear solve of Bratu3D (n = 256 interior nodes per
+axis, lambda = 6.5, u0 = 0, ||F|| <= 1e-8 ||F(u0)||) with the device-resident
+Newton-Krylov solver: FD J v, GMRES(30) / FGMRES(30), 5-level GMG V(2,2) with
+the degree-3 Chebyshev smoother.  Two legs run on the same GPU:
+  * fp64:  plain FP64 GMG-GMRES (the speed-up denominator);
+  * mixed: FGMRES with FP32 Krylov basis / J v / V-cycle and FP64 F(x),
+           difference quotient, Newton update and Givens solve (the value).
+The timed region holds only solves, inputs resident in HBM.  Rank 0 prints
+one JSON line.  The CPU baseline is the FP64 CPU oracle (OpenMP) solving the
+same problem on the host, timed once; its solution is also the parity
+reference for the GPU solutions.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "chebyshev-nonlinear-lab_amd")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+for _p in (PKG, ORACLE_DIR):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("Bratu3D 256³ time-to-1e-8 residual: learned/mixed FGMRES vs plain FP64 GMG")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_params(H, forcing: str):
+    if forcing == "constant":
+        return H.krylov_params(gmres_restart=30, gmres_max_iters=600, eta_init=1e-3,
+                               forcing=H.FORCING_CONSTANT)
+    return H.krylov_params(gmres_restart=30, gmres_max_iters=600)
+
+
+def leg_configs(H, levels: int):
+    mg = H.mg_cheb(levels=levels, degree=3, pre=2, post=2)
+    fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=mg)
+    mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
+    return {"fp64": fp64, "mixed": mixed}
+
+
+def time_leg(H, ctx, n, lam, cfg, params, steps, warmup):
+    import numpy as np
+    s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+    x = ctx.empty(n ** 3)
+    zeros = np.zeros(n ** 3)
+    res = None
+    for _ in range(warmup):
+        x.upload(zeros)
+        res = s.solve(params, x)
+    times = []
+    for _ in range(steps):
+        x.upload(zeros)
+        ctx.sync()
+        t0 = time.perf_counter()
+        res = s.solve(params, x)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+        log(f"  solve {len(times)}/{steps}: {times[-1]*1e3:.1f} ms, newton={res.iters} "
+            f"gmres={res.n_lin_iters} rel={res.res/res.res0:.2e}")
+    return s, x, res, times
+
+
+def primitive_table(H, s, reps=20):
+    import ctypes as C
+    names = {"F": H.PRIM_F, "JVP": H.PRIM_JVP, "PREC": H.PRIM_PREC, "SMOOTH": H.PRIM_SMOOTH,
+             "MGS": H.PRIM_MGS}
+    out = {}
+    for name, prim in names.items():
+        ms, b = C.c_double(), C.c_double()
+        H.check(s.ctx.lib.numa_hip_primitive_cost(s.h, prim, reps, C.byref(ms), C.byref(b)))
+        out[name] = {"ms": ms.value, "bytes": b.value,
+                     "GBs": b.value / (ms.value * 1e-3) / 1e9 if ms.value > 0 else 0.0}
+    return out
+
+
+def cpu_leg(args) -> None:
+    """Runs in a child process with OMP_NUM_THREADS set: FP64 CPU oracle solve."""
+    import numpy as np
+    import numa_hip as H
+    import oracle as O
+    cfg = leg_configs(H, args.levels)["fp64"]
+    m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
+    ocfg = O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=0, flexible=0,
+                          jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
+    p = make_params(H, args.forcing)
+    op = O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
+    o = O.Oracle()
+    t0 = time.perf_counter()
+    x, r = o.bratu_solve(args.n, args.lam, ocfg, op)
+    t = time.perf_counter() - t0
+    np.save(args.cpu_out, x)
+    print(json.dumps({"seconds": t, "threads": o.threads(), "result": O.result_dict(r)}))
+
+
+def run_cpu_baseline(args):
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    import tempfile
+    out = os.path.join(tempfile.gettempdir(), f"numa_cpu_solution_{os.getpid()}.npy")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--n", str(args.n), "--lam",
+           str(args.lam), "--levels", str(args.levels), "--forcing", args.forcing,
+           "--cpu-out", out]
+    log(f"cpu baseline: FP64 oracle solve at n={args.n} with {threads} threads")
+    p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=args.cpu_timeout)
+    if p.returncode != 0:
+        raise RuntimeError("cpu leg failed: " + p.stderr[-2000:])
+    info = json.loads(p.stdout.strip().splitlines()[-1])
+    return info, out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--lam", type=float, default=6.5)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--forcing", choices=["ew", "constant"], default="ew")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--legs", default="fp64,mixed")
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-out", default="", help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    if args.cpu_leg:
+        cpu_leg(args)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build "
+              "runs the solver on a single MI355X; multi-GPU sharding is design only)",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import numa_hip as H
+
+    H.load()
+    ctx = H.Context(0)
+    params = make_params(H, args.forcing)
+    cfgs = leg_configs(H, args.levels)
+    legs = {}
+    for name in args.legs.split(","):
+        log(f"leg {name}: n={args.n} lambda={args.lam}")
+        s, x, res, times = time_leg(H, ctx, args.n, args.lam, cfgs[name], params, args.steps,
+                                    args.warmup)
+        legs[name] = {"solver": s, "x": x, "res": res, "times": times,
+                      "prims": primitive_table(H, s)}
+        log(f"  primitives: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['GBs']:.0f} GB/s"
+                                          for k, v in legs[name]["prims"].items()))
+    main_leg = "mixed" if "mixed" in legs else list(legs)[0]
+    L = legs[main_leg]
+    t_main = float(np.mean(L["times"]))
+    r = L["res"]
+
+    # roofline of the dominant kernel: the finest-level smoothing step
+    sm = L["prims"]["SMOOTH"]
+    roofline = {"bound": "hbm", "kernel": "k_stencil<EpPoly> (finest Chebyshev smoothing step)",
+                "achieved": round(sm["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(sm["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": sm["bytes"], "avg_launch_ms": sm["ms"]}
+
+    out = {"metric": METRIC, "value": round(t_main, 6), "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_main * 1e3, 3),
+           "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f32+f64 (mixed)" if main_leg == "mixed" else "f64",
+           "data": f"synthetic: u0 = 0, lambda = {args.lam}",
+           "config": {"workload": f"bratu3d_{args.n}^3_lambda{args.lam}_newton_"
+                      f"{'fgmres' if main_leg == 'mixed' else 'gmres'}30_gmg{args.levels}"
+                      "_cheb3_v22_fd_jv", "n": args.n, "lambda": args.lam,
+                      "tol": 1e-8, "forcing": args.forcing, "levels": args.levels},
+           "newton_iters": int(r.iters), "gmres_iters": int(r.n_lin_iters),
+           "n_F": int(r.n_F), "n_JVP": int(r.n_JVP), "n_prec": int(r.n_prec),
+           "final_rel_residual": r.res / r.res0, "roofline": roofline,
+           "primitives": {k: {kk: round(vv, 5) for kk, vv in v.items()}
+                          for k, v in L["prims"].items()}}
+    if "fp64" in legs and main_leg != "fp64":
+        t64 = float(np.mean(legs["fp64"]["times"]))
+        r64 = legs["fp64"]["res"]
+        out["fp64_gmg_gmres_s"] = round(t64, 6)
+        out["speedup_vs_fp64"] = round(t64 / t_main, 3)
+        out["fp64_newton_iters"] = int(r64.iters)
+        out["fp64_gmres_iters"] = int(r64.n_lin_iters)
+
+    if not args.no_cpu:
+        info, path = run_cpu_baseline(args)
+        xc = np.load(path)
+        os.remove(path)
+        parity = {"newton_iters_cpu_fp64": info["result"]["iters"]}
+        for name, leg in legs.items():
+            xg = leg["x"].numpy()
+            parity[f"{name}_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(xg - xc) /
+                                                        np.linalg.norm(xc))
+            parity[f"{name}_newton_iters"] = int(leg["res"].iters)
+        out["parity"] = parity
+        out["cpu_baseline"] = {"value": round(info["seconds"], 3), "unit": "s",
+                               "cores": info["threads"], "kind": "port",
+                               "sample": f"one full FP64 solve of the same workload "
+                                         f"(n={args.n}, same GMG, {args.forcing} forcing) by "
+                                         f"the CPU oracle with OpenMP",
+                               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                               "newton_iters": info["result"]["iters"],
+                               "gmres_iters": info["result"]["n_lin_iters"]}
+    print(json.dumps(out))
+    for leg in legs.values():
+        leg["solver"].close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,531 @@
+// mg.hpp - geometric multigrid V-cycle for -Lap_h (7-point, zero Dirichlet)
+// on gfx950.  Restates linalg/poisson_mg.hpp:24-320 (RB-GS smoother, 27-point
+// full-weighting restriction with coarse node I <-> fine node 2I+1, trilinear
+// prolongation, coarse h^-2 /4 per level) and extends it with:
+//   * even-n coarsening n -> n/2 with the same transfer operators (the
+//     reference stops at the first even n, poisson_mg.hpp:60-62);
+//   * weighted-Jacobi and Chebyshev (degree d, interval [lmax/ratio, lmax] of
+//     D^-1 A with the exact lmax = 1 + cos(pi/(n+1))) smoothers;
+//   * a Chebyshev semi-iteration over the exact spectrum
+//     [1 - cos(pi/(n+1)), 1 + cos(pi/(n+1))] as coarse solve, run by one
+//     workgroup with the level resident in LDS.
+// Level storage is FP64 or FP32 (T); arithmetic is FP64 per point.
+#pragma once
+
+#include <algorithm>
+#include <utility>
+
+#include "blas.hpp"
+#include "stencil.hpp"
+
+namespace numa_hip {
+
+// One Jacobi-type / Chebyshev step:  d <- c1 d + c2 (b - A x);  xn = x + d.
+template <class T>
+struct EpPoly {
+    double h2inv, c1, c2;
+    const T* b;
+    T* d;       // direction (read if use_d, written if write_d)
+    T* xn;      // new iterate
+    const T* x;
+    bool use_d, write_d;
+    __device__ EpPoly prep() const { return *this; }
+    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
+                                 P1 kp) const {
+#pragma clang fp contract(off)
+        const double res = dv(b, id) - h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
+        const double dn = use_d ? c1 * dv(d, id) + c2 * res : c2 * res;
+        if (write_d) d[id] = static_cast<T>(dn);
+        const T o = static_cast<T>(c.a + dn);
+        xn[id] = o;
+        return 0.0;
+    }
+};
+
+// r = b - A x   (poisson_mg.hpp:128-149)
+template <class T>
+struct EpResid {
+    double h2inv;
+    const T* b;
+    T* r;
+    __device__ EpResid prep() const { return *this; }
+    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
+                                 P1 kp) const {
+#pragma clang fp contract(off)
+        r[id] = static_cast<T>(dv(b, id) - h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a));
+        return 0.0;
+    }
+};
+
+// First step from x = 0:  d = c2 b;  xn = d.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2, const T* b, T* d,
+                                                        T* xn) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock) {
+        const double dn = c2 * dv(b, i);
+        if (d) d[i] = static_cast<T>(dn);
+        xn[i] = static_cast<T>(dn);
+    }
+}
+
+// Red-black Gauss-Seidel half sweep on one colour (poisson_mg.hpp:107-124):
+// x_id = (b_id/h2inv + sum nbrs)/6 for (i+j+k) % 2 == color, in place.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rbgs(int n, double inv, int color, const T* b, T* x) {
+#pragma clang fp contract(off)
+    const int64_t N1 = n, N2 = N1 * N1, N = N2 * N1;
+    for (int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x; id < N;
+         id += (int64_t)gridDim.x * kBlock) {
+        const int k = (int)(id % N1);
+        const int j = (int)((id / N1) % N1);
+        const int i = (int)(id / N2);
+        if (((i + j + k) & 1) != color) continue;
+        double s = dv(b, id) * inv;
+        if (i > 0) s += dv(x, id - N2);
+        if (i + 1 < n) s += dv(x, id + N2);
+        if (j > 0) s += dv(x, id - N1);
+        if (j + 1 < n) s += dv(x, id + N1);
+        if (k > 0) s += dv(x, id - 1);
+        if (k + 1 < n) s += dv(x, id + 1);
+        x[id] = static_cast<T>(s / 6.0);
+    }
+}
+
+// 27-point full weighting, coarse I <-> fine 2I+1 (poisson_mg.hpp:152-188).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(int nf, int nc, const T* r, T* bc) {
+#pragma clang fp contract(off)
+    const double w[3] = {0.25, 0.5, 0.25};
+    const int64_t C1 = nc, C2 = C1 * C1, NC = C2 * C1, F1 = nf, F2 = F1 * F1;
+    for (int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x; id < NC;
+         id += (int64_t)gridDim.x * kBlock) {
+        const int K = (int)(id % C1), J = (int)((id / C1) % C1), I = (int)(id / C2);
+        double sum = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            const int fi = 2 * I + a;
+            if (fi >= nf) continue;
+            for (int bb = 0; bb < 3; ++bb) {
+                const int fj = 2 * J + bb;
+                if (fj >= nf) continue;
+                for (int cc = 0; cc < 3; ++cc) {
+                    const int fk = 2 * K + cc;
+                    if (fk >= nf) continue;
+                    sum += w[a] * w[bb] * w[cc] * dv(r, (int64_t)fi * F2 + (int64_t)fj * F1 + fk);
+                }
+            }
+        }
+        bc[id] = static_cast<T>(sum);
+    }
+}
+
+// 1D trilinear terms of a fine index (poisson_mg.hpp:191-220).
+__device__ inline int interp_terms(int f, int nc, int ci[2], double wi[2]) {
+    if (f & 1) {
+        ci[0] = (f - 1) / 2;
+        wi[0] = 1.0;
+        return 1;
+    }
+    const int right = f / 2;
+    int cnt = 0;
+    if (right >= 1) {
+        ci[cnt] = right - 1;
+        wi[cnt] = 0.5;
+        ++cnt;
+    }
+    if (right < nc) {
+        ci[cnt] = right;
+        wi[cnt] = 0.5;
+        ++cnt;
+    }
+    return cnt;
+}
+
+// x_f += P x_c (poisson_mg.hpp:222-260).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(int nf, int nc, const T* xc, T* xf) {
+#pragma clang fp contract(off)
+    const int64_t F1 = nf, F2 = F1 * F1, NF = F2 * F1, C1 = nc, C2 = C1 * C1;
+    for (int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x; id < NF;
+         id += (int64_t)gridDim.x * kBlock) {
+        const int k = (int)(id % F1), j = (int)((id / F1) % F1), i = (int)(id / F2);
+        int ci[2], cj[2], ck[2];
+        double wi[2], wj[2], wk[2];
+        const int ni = interp_terms(i, nc, ci, wi);
+        const int nj = interp_terms(j, nc, cj, wj);
+        const int nk = interp_terms(k, nc, ck, wk);
+        double sum = 0.0;
+        for (int a = 0; a < ni; ++a)
+            for (int bb = 0; bb < nj; ++bb)
+                for (int cc = 0; cc < nk; ++cc)
+                    sum += wi[a] * wj[bb] * wk[cc] *
+                           dv(xc, (int64_t)ci[a] * C2 + (int64_t)cj[bb] * C1 + ck[cc]);
+        xf[id] = static_cast<T>(dv(xf, id) + sum);
+    }
+}
+
+// Coarse solve by one workgroup with the level in LDS (x starts at zero).
+//   mode 0: Chebyshev semi-iteration, `iters` steps over [lo, hi] of D^-1 A.
+//   mode 1: `iters` RB-GS sweeps with alternating order (poisson_mg.hpp:270-274).
+//   n == 1: direct x = b/(6 h2inv) (poisson_mg.hpp:267-269).
+constexpr int kCoarseThreads = 512;
+constexpr int kCoarseMaxPts = 8;  // points per thread: n^3 <= 4096
+constexpr int kCoarseMaxN3 = kCoarseThreads * kCoarseMaxPts;
+
+template <class T>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2inv, int mode,
+                                                                int iters, double lo, double hi,
+                                                                const T* b, T* xout) {
+#pragma clang fp contract(off)
+    __shared__ double xs[kCoarseMaxN3];
+    const int N1 = n, N2 = n * n, N = N2 * n;
+    if (n == 1) {
+        if (threadIdx.x == 0) xout[0] = static_cast<T>(dv(b, 0) / (6.0 * h2inv));
+        return;
+    }
+    double bl[kCoarseMaxPts], dl[kCoarseMaxPts], xl[kCoarseMaxPts];
+#pragma unroll
+    for (int p = 0; p < kCoarseMaxPts; ++p) {
+        const int id = threadIdx.x + p * kCoarseThreads;
+        bl[p] = id < N ? dv(b, id) : 0.0;
+        dl[p] = 0.0;
+        xl[p] = 0.0;
+    }
+    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xs[id] = 0.0;
+    __syncthreads();
+    auto nbr = [&](int id, int& i, int& j, int& k) {
+        k = id % N1;
+        j = (id / N1) % N1;
+        i = id / N2;
+    };
+    if (mode == 0) {
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+        const double sigma = theta / delta;
+        const double dinv = 1.0 / (6.0 * h2inv);
+        double rho = 1.0 / sigma;
+        for (int it = 0; it < iters; ++it) {
+            double c1 = 0.0, c2 = dinv / theta;
+            if (it > 0) {
+                const double rho_new = 1.0 / (2.0 * sigma - rho);
+                c1 = rho_new * rho;
+                c2 = 2.0 * rho_new / delta * dinv;
+                rho = rho_new;
+            }
+#pragma unroll
+            for (int p = 0; p < kCoarseMaxPts; ++p) {
+                const int id = threadIdx.x + p * kCoarseThreads;
+                if (id < N) {
+                    int i, j, k;
+                    nbr(id, i, j, k);
+                    const double c = xs[id];
+                    const double s = lap_num(c, i > 0 ? xs[id - N2] : 0.0,
+                                             i + 1 < n ? xs[id + N2] : 0.0,
+                                             j > 0 ? xs[id - N1] : 0.0,
+                                             j + 1 < n ? xs[id + N1] : 0.0,
+                                             k > 0 ? xs[id - 1] : 0.0, k + 1 < n ? xs[id + 1] : 0.0);
+                    const double res = bl[p] - h2inv * s;
+                    const double dn = it > 0 ? c1 * dl[p] + c2 * res : c2 * res;
+                    dl[p] = dn;
+                    xl[p] = c + dn;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < kCoarseMaxPts; ++p) {
+                const int id = threadIdx.x + p * kCoarseThreads;
+                if (id < N) xs[id] = xl[p];
+            }
+            __syncthreads();
+        }
+    } else {
+        const double inv = 1.0 / h2inv;
+        for (int sweep = 0; sweep < iters; ++sweep) {
+            const int order = sweep % 2;
+            for (int pass = 0; pass < 2; ++pass) {
+                const int color = order == 0 ? pass : 1 - pass;
+#pragma unroll
+                for (int p = 0; p < kCoarseMaxPts; ++p) {
+                    const int id = threadIdx.x + p * kCoarseThreads;
+                    if (id >= N) continue;
+                    int i, j, k;
+                    nbr(id, i, j, k);
+                    if (((i + j + k) & 1) != color) continue;
+                    double s = bl[p] * inv;
+                    if (i > 0) s += xs[id - N2];
+                    if (i + 1 < n) s += xs[id + N2];
+                    if (j > 0) s += xs[id - N1];
+                    if (j + 1 < n) s += xs[id + N1];
+                    if (k > 0) s += xs[id - 1];
+                    if (k + 1 < n) s += xs[id + 1];
+                    xs[id] = s / 6.0;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xout[id] = static_cast<T>(xs[id]);
+}
+
+inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
+inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1.0)); }
+
+struct MGBase {
+    virtual ~MGBase() = default;
+    virtual void apply(const void* r, void* z) = 0;
+    virtual int64_t levels() const = 0;
+    virtual int64_t level_n(int64_t l) const = 0;
+    // One finest-level smoothing step on internal buffers (primitive timing);
+    // returns its algorithmic HBM bytes.
+    virtual double smooth_step_for_timing() = 0;
+    // Algorithmic HBM bytes of one V-cycle.
+    virtual double vcycle_bytes() const = 0;
+};
+
+template <class T>
+class MG : public MGBase {
+public:
+    MG(Ctx& c, int64_t n, const numa_mg_config& cfg) : c_(c), cfg_(cfg) {
+        require(n > 0, "multigrid needs a positive grid size");
+        require(cfg.pre >= 0 && cfg.post >= 0, "multigrid needs non-negative sweep counts");
+        require(cfg.smoother >= NUMA_SMOOTH_RBGS && cfg.smoother <= NUMA_SMOOTH_CHEBYSHEV,
+                "unknown smoother");
+        require(cfg.smoother != NUMA_SMOOTH_CHEBYSHEV || (cfg.degree >= 1 && cfg.cheb_ratio > 1.0),
+                "Chebyshev smoother needs degree >= 1 and cheb_ratio > 1");
+        require(n <= 2048, "grid too large", NUMA_ERR_OVERFLOW);
+        // Hierarchy (poisson_mg.hpp:44-63, plus even coarsening and limits).
+        int64_t m = n;
+        double h2inv = (double(n) + 1.0) * (double(n) + 1.0);
+        while (true) {
+            Level L;
+            L.n = m;
+            L.h2inv = h2inv;
+            levels_.push_back(L);
+            if (m <= 1) break;
+            if (!cfg.even_coarsening && (m % 2 == 0)) break;
+            if (m <= std::max<int64_t>(1, cfg.min_coarse_n)) break;
+            if (cfg.max_levels > 0 && (int64_t)levels_.size() >= cfg.max_levels) break;
+            m = m / 2;
+            h2inv /= 4.0;
+        }
+        for (size_t l = 0; l < levels_.size(); ++l) {
+            Level& L = levels_[l];
+            const int64_t N = L.n * L.n * L.n;
+            L.x = alloc(N);
+            L.x2 = alloc(N);
+            if (l > 0) L.b = alloc(N);
+            L.r = alloc(N);
+            if (cfg.smoother == NUMA_SMOOTH_CHEBYSHEV || l + 1 == levels_.size()) L.d = alloc(N);
+            // Chebyshev smoothing coefficients (same recurrence as the coarse kernel).
+            if (cfg.smoother == NUMA_SMOOTH_CHEBYSHEV) {
+                const double hi = cheb_lmax(L.n), lo = hi / cfg.cheb_ratio;
+                cheb_coefs(L, (int)cfg.degree, lo, hi, L.c1, L.c2);
+            }
+        }
+    }
+    ~MG() override {
+        for (void* p : allocs_) (void)hipFree(p);
+    }
+
+    int64_t levels() const override { return (int64_t)levels_.size(); }
+    int64_t level_n(int64_t l) const override { return levels_[(size_t)l].n; }
+
+    double smooth_step_for_timing() override {
+        Level& L = levels_[0];
+        const int64_t N = L.n * L.n * L.n;
+        T* saved_b = L.b;
+        L.b = L.r;
+        if (cfg_.smoother == NUMA_SMOOTH_RBGS) {
+            rbgs_sweep(L, 0);
+            L.b = saved_b;
+            return (double)N * sizeof(T) * 4.0;  // each colour: read b, x (half), write x (half)
+        }
+        if (!L.d) {
+            poly_step(L, 0.0, 0.1, false, false, false);
+            L.b = saved_b;
+            return (double)N * sizeof(T) * 3.0;  // read x, b; write xn
+        }
+        poly_step(L, 0.5, 0.1, true, true, false);
+        L.b = saved_b;
+        return (double)N * sizeof(T) * 5.0;  // read x, b, d; write d, xn
+    }
+
+    double vcycle_bytes() const override {
+        // per level: smoothing steps (first step from zero reads b, writes
+        // d, x), residual (x, b -> r), restriction (r -> b_c), prolongation
+        // (x_c, x -> x), coarsest solve (b -> x); plus the output copy.
+        const double s = sizeof(T);
+        double total = 0.0;
+        for (size_t l = 0; l < levels_.size(); ++l) {
+            const double N = (double)levels_[l].n * levels_[l].n * levels_[l].n;
+            if (l + 1 == levels_.size()) {
+                total += 2.0 * N * s;
+                break;
+            }
+            const int per_sweep = cfg_.smoother == NUMA_SMOOTH_CHEBYSHEV ? (int)cfg_.degree : 1;
+            const int steps = (int)(cfg_.pre + cfg_.post) * per_sweep;
+            double step_bytes = cfg_.smoother == NUMA_SMOOTH_CHEBYSHEV ? 5.0 : 3.0;
+            if (cfg_.smoother == NUMA_SMOOTH_RBGS) step_bytes = 4.0;
+            total += N * s * (steps * step_bytes - (cfg_.pre > 0 ? step_bytes - 3.0 : 0.0));
+            total += N * s * 3.0;          // residual
+            total += N * s * (1.0 + 0.125); // restriction
+            total += N * s * (2.0 + 0.125); // prolongation
+        }
+        const double N0 = (double)levels_[0].n * levels_[0].n * levels_[0].n;
+        return total + 2.0 * N0 * s;
+    }
+
+    // z = V-cycle(r) with zero initial guess (poisson_mg.hpp:69-80).
+    void apply(const void* rv, void* zv) override {
+        const T* r = static_cast<const T*>(rv);
+        T* z = static_cast<T*>(zv);
+        Level& F = levels_[0];
+        F.b = const_cast<T*>(r);
+        cycle(0);
+        const int64_t N = F.n * F.n * F.n;
+        hipLaunchKernelGGL((k_convert<T, T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c_.stream,
+                           N, F.x, z);
+        NUMA_HIP_LAUNCH_CHECK();
+        F.b = nullptr;
+    }
+
+private:
+    struct Level {
+        int64_t n = 0;
+        double h2inv = 0.0;
+        T* x = nullptr;
+        T* x2 = nullptr;
+        T* b = nullptr;
+        T* r = nullptr;
+        T* d = nullptr;
+        std::vector<double> c1, c2;
+    };
+
+    T* alloc(int64_t N) {
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipMalloc(&p, (size_t)N * sizeof(T)));
+        allocs_.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    static void cheb_coefs(const Level& L, int deg, double lo, double hi, std::vector<double>& c1,
+                           std::vector<double>& c2) {
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+        const double sigma = theta / delta;
+        const double dinv = 1.0 / (6.0 * L.h2inv);
+        double rho = 1.0 / sigma;
+        c1.assign(deg, 0.0);
+        c2.assign(deg, 0.0);
+        c2[0] = dinv / theta;
+        for (int k = 1; k < deg; ++k) {
+            const double rho_new = 1.0 / (2.0 * sigma - rho);
+            c1[k] = rho_new * rho;
+            c2[k] = 2.0 * rho_new / delta * dinv;
+            rho = rho_new;
+        }
+    }
+
+    // One step d <- c1 d + c2 (b - A x), x <- x + d (x double-buffered).
+    void poly_step(Level& L, double c1, double c2, bool use_d, bool write_d, bool from_zero) {
+        const int64_t N = L.n * L.n * L.n;
+        if (from_zero) {
+            hipLaunchKernelGGL((k_poly_first<T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
+                               c_.stream, N, c2, (const T*)L.b, write_d ? L.d : (T*)nullptr, L.x2);
+            NUMA_HIP_LAUNCH_CHECK();
+        } else {
+            EpPoly<T> ep{L.h2inv, c1, c2, L.b, L.d, L.x2, L.x, use_d, write_d};
+            launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, ep, Epilogue{});
+        }
+        std::swap(L.x, L.x2);
+    }
+
+    void rbgs_sweep(Level& L, int order) {
+        const int64_t N = L.n * L.n * L.n;
+        for (int pass = 0; pass < 2; ++pass) {
+            const int color = order == 0 ? pass : 1 - pass;
+            hipLaunchKernelGGL((k_rbgs<T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c_.stream,
+                               (int)L.n, 1.0 / L.h2inv, color, (const T*)L.b, L.x);
+            NUMA_HIP_LAUNCH_CHECK();
+        }
+    }
+
+    // `sweeps` smoothing sweeps; if from_zero the iterate starts at 0.
+    void smooth(Level& L, int sweeps, int order, bool from_zero) {
+        const int64_t N = L.n * L.n * L.n;
+        if (cfg_.smoother == NUMA_SMOOTH_RBGS) {
+            if (from_zero) launch_fill<T>(c_, N, T(0), L.x);
+            for (int s = 0; s < sweeps; ++s) rbgs_sweep(L, order);
+            return;
+        }
+        if (sweeps == 0) {
+            if (from_zero) launch_fill<T>(c_, N, T(0), L.x);
+            return;
+        }
+        for (int s = 0; s < sweeps; ++s) {
+            if (cfg_.smoother == NUMA_SMOOTH_WJACOBI) {
+                poly_step(L, 0.0, cfg_.omega / (6.0 * L.h2inv), false, false, from_zero && s == 0);
+            } else {
+                const int deg = (int)cfg_.degree;
+                for (int k = 0; k < deg; ++k)
+                    poly_step(L, L.c1[k], L.c2[k], k > 0, k + 1 < deg, from_zero && s == 0 && k == 0);
+            }
+        }
+    }
+
+    void coarse_solve(Level& L) {
+        const int64_t N = L.n * L.n * L.n;
+        const bool rb = cfg_.smoother == NUMA_SMOOTH_RBGS;
+        if (N <= kCoarseMaxN3) {
+            const double hi = cheb_lmax(L.n), lo = cheb_lmin(L.n);
+            hipLaunchKernelGGL((k_coarse_lds<T>), dim3(1), dim3(kCoarseThreads), 0, c_.stream,
+                               (int)L.n, L.h2inv, rb ? 1 : 0, (int)cfg_.coarse_iters, lo, hi,
+                               (const T*)L.b, L.x);
+            NUMA_HIP_LAUNCH_CHECK();
+            return;
+        }
+        // Large coarsest level: the same iterations as separate launches.
+        if (rb) {
+            launch_fill<T>(c_, N, T(0), L.x);
+            for (int s = 0; s < (int)cfg_.coarse_iters; ++s) rbgs_sweep(L, s % 2);
+            return;
+        }
+        std::vector<double> c1, c2;
+        cheb_coefs(L, (int)std::max<int64_t>(1, cfg_.coarse_iters), cheb_lmin(L.n), cheb_lmax(L.n),
+                   c1, c2);
+        const int deg = (int)c1.size();
+        for (int k = 0; k < deg; ++k) poly_step(L, c1[k], c2[k], k > 0, k + 1 < deg, k == 0);
+    }
+
+    // poisson_mg.hpp:262-294
+    void cycle(size_t l) {
+        Level& L = levels_[l];
+        if (l + 1 == levels_.size()) {
+            coarse_solve(L);
+            return;
+        }
+        smooth(L, (int)cfg_.pre, 0, true);
+        launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, EpResid<T>{L.h2inv, L.b, L.r}, Epilogue{});
+        Level& C = levels_[l + 1];
+        const int64_t NC = C.n * C.n * C.n, NF = L.n * L.n * L.n;
+        hipLaunchKernelGGL((k_restrict<T>), dim3(stream_blocks(NC)), dim3(kBlock), 0, c_.stream,
+                           (int)L.n, (int)C.n, (const T*)L.r, C.b);
+        NUMA_HIP_LAUNCH_CHECK();
+        cycle(l + 1);
+        hipLaunchKernelGGL((k_prolong_add<T>), dim3(stream_blocks(NF)), dim3(kBlock), 0, c_.stream,
+                           (int)L.n, (int)C.n, (const T*)C.x, L.x);
+        NUMA_HIP_LAUNCH_CHECK();
+        smooth(L, (int)cfg_.post, 1, false);
+    }
+
+    Ctx& c_;
+    numa_mg_config cfg_;
+    std::vector<Level> levels_;
+    std::vector<void*> allocs_;
+};
+
+inline MGBase* make_mg(Ctx& c, int64_t n, const numa_mg_config& cfg, int64_t precision) {
+    if (precision == NUMA_PRECISION_MIXED) return new MG<float>(c, n, cfg);
+    return new MG<double>(c, n, cfg);
+}
+
+}  // namespace numa_hip

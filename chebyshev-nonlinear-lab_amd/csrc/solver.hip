@@ -1,0 +1,794 @@
+// solver.hip - device-resident (F)GMRES and Newton / inexact-Chebyshev outer
+// loops on Bratu3D (methods/krylov_newton.hpp:54-189, linalg/gmres.hpp:61-323),
+// plus the C entry points of the multigrid and solver handles.
+//
+// Every N-sized vector lives in HBM.  Per Arnoldi step the host launches
+//   preconditioner -> ||z|| -> J z (FD, fused first MGS dot) -> j+1 fused MGS
+// passes, then reads the j+2 new Hessenberg entries (one small copy) and
+// applies the Givens rotations in FP64, exactly in the reference's order.
+// Basis vectors are stored unnormalised with device-side scales s_i
+// (V_i = s_i * Vraw_i), which removes the normalisation pass.
+//
+// Precision: T = double (FP64) or float (mixed).  In mixed mode the Krylov
+// basis, Z vectors, J v output and the V-cycle are FP32; F(x), the difference
+// quotient's subtraction, the Newton / Chebyshev update, the linear iterate,
+// the Givens / least-squares solve and all dot products (FP64 accumulation)
+// stay FP64.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+
+#include "blas.hpp"
+#include "mg.hpp"
+#include "stencil.hpp"
+
+namespace numa_hip {
+
+struct SolverBase {
+    virtual ~SolverBase() = default;
+    virtual numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
+                                          int64_t restart, int64_t max_iters, bool zero_guess) = 0;
+    virtual void solve(const numa_krylov_params& p, double* x, numa_krylov_result& out) = 0;
+    virtual int64_t size() const = 0;
+    virtual double primitive_cost(int64_t prim, int64_t reps, double* bytes) = 0;
+};
+
+}  // namespace numa_hip
+
+struct numa_hip_solver {
+    numa_hip::Ctx* ctx;
+    std::unique_ptr<numa_hip::SolverBase> impl;
+};
+
+struct numa_hip_mg {
+    numa_hip::Ctx* ctx;
+    int64_t precision;
+    std::unique_ptr<numa_hip::MGBase> impl;
+};
+
+namespace numa_hip {
+
+// Eisenstat-Walker choice 2 with Kelley's safeguards (krylov_newton.hpp:54-65),
+// or a constant forcing term.
+inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
+                      double eta_prev, double res0) {
+    if (p.forcing == NUMA_FORCING_CONSTANT) return p.eta_init;
+    if (k == 0) return p.eta_init;
+    double eta = p.gamma * std::pow(rnorm / rprev, p.alpha);
+    const double safe = p.gamma * std::pow(eta_prev, p.alpha);
+    if (safe > 0.1) eta = std::max(eta, safe);
+    eta = std::min(eta, p.eta_max);
+    eta = std::max(eta, 0.5 * p.rtol_nl * res0 / rnorm);
+    return std::min(eta, p.eta_max);
+}
+
+template <class T>
+class BratuSolver : public SolverBase {
+public:
+    BratuSolver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg, int64_t restart)
+        : c_(c), n_(n), N_(n * n * n), lambda_(lambda), h2inv_((n + 1.0) * (n + 1.0)), cfg_(cfg),
+          m_(restart) {
+        require(n > 0, "Bratu3D needs n > 0");
+        require(n <= 2048, "grid too large", NUMA_ERR_OVERFLOW);
+        require(restart > 0 && restart <= 256, "gmres restart must be in [1, 256]");
+        require(cfg.precond >= NUMA_PRECOND_NONE && cfg.precond <= NUMA_PRECOND_GMG,
+                "unknown preconditioner");
+        if (cfg.precond == NUMA_PRECOND_GMG) mg_.reset(make_mg(c, n, cfg.mg, cfg.precision));
+        x2_ = allocd();
+        Fx_ = allocd();
+        Fn_ = allocd();
+        z_ = allocd();
+        u_ = allocd();
+        w_ = allocd();
+        r_ = allocd();
+        if (cfg.precond == NUMA_PRECOND_JACOBI) diag_ = allocd();
+        V_.resize((size_t)m_ + 1);
+        for (auto& v : V_) v = alloct();
+        if (cfg.flexible && cfg.precond != NUMA_PRECOND_NONE) {
+            Z_.resize((size_t)m_);
+            for (auto& v : Z_) v = alloct();
+        }
+        zb_ = alloct();
+        comb_ = alloct();
+        // device scalars: H ((m+1) x m, column major), s (m+1), rhs (m+1), misc
+        nsc_ = (m_ + 1) * m_ + 2 * (m_ + 1) + kMisc;
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipMalloc(&p, (size_t)nsc_ * sizeof(double)));
+        allocs_.push_back(p);
+        sd_ = static_cast<double*>(p);
+        NUMA_HIP_CHECK(hipHostMalloc(&p, (size_t)nsc_ * sizeof(double), hipHostMallocDefault));
+        hs_ = static_cast<double*>(p);
+    }
+    ~BratuSolver() override {
+        for (void* p : allocs_) (void)hipFree(p);
+        if (hs_) (void)hipHostFree(hs_);
+    }
+    int64_t size() const override { return N_; }
+
+    numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
+                                  int64_t restart, int64_t max_iters, bool zero_guess) override {
+        point_setup(x0);
+        eval_F(x0, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
+        auto r = gmres(x0, Fx_, b, z, rtol, restart, max_iters, zero_guess, nullptr, false);
+        c_.sync();
+        return r;
+    }
+
+    void solve(const numa_krylov_params& p, double* xdev, numa_krylov_result& out) override;
+
+    // Average device time of one primitive call (HIP events on the solver's
+    // stream) and its algorithmic HBM bytes; the counterpart of the
+    // reference's frozen per-call costs c_F, c_JVP, c_D2, c_prec.
+    double primitive_cost(int64_t prim, int64_t reps, double* bytes) override {
+        require(reps > 0, "reps must be positive");
+        const double N = (double)N_, st = sizeof(T);
+        // well-defined inputs: x = 0.1, F = F(x), Krylov vectors = 1
+        launch_fill<double>(c_, N_, 0.1, x2_);
+        eval_F(x2_, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
+        launch_fill<double>(c_, N_, 0.01, z_);
+        for (int i = 0; i < 3 && i <= m_; ++i) launch_fill<T>(c_, N_, T(1), V_[(size_t)i]);
+        launch_fill<T>(c_, N_, T(1), zb_);
+        point_setup(x2_);
+        norm(zb_, MISC(S_VNORM));
+        launch_fill<double>(c_, 1, 1.0, S(0));
+        launch_fill<double>(c_, 1, 1.0, H(0, 0));
+        auto run = [&]() -> double {
+            switch (prim) {
+                case NUMA_PRIM_F:
+                    eval_F(x2_, nullptr, 0.0, nullptr, 0.0, Fn_, nullptr);
+                    return 16.0 * N;
+                case NUMA_PRIM_JVP: {  // stencil kernel only (||v|| precomputed)
+                    if (cfg_.jv_mode == NUMA_DERIV_FD) {
+                        const FDStep stp{std::sqrt(std::numeric_limits<double>::epsilon()),
+                                         MISC(S_XNORM), MISC(S_VNORM)};
+                        launch_stencil<true>(c_, n_, LdXpHV<T>{x2_, zb_, stp, 0.0},
+                                             EpJvFD<T>{h2inv_, lambda_, stp, S(0), Fx_, V_[1],
+                                                       V_[0], nullptr, 0.0, 1.0},
+                                             Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
+                        return N * (8.0 + 8.0 + 3.0 * st);  // x, F0, v, out, V0
+                    }
+                    launch_stencil<true>(c_, n_, LdVec<T>{zb_},
+                                         EpJvAn<T>{h2inv_, lambda_, S(0), x2_, V_[1], V_[0],
+                                                   nullptr, 1.0},
+                                         Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
+                    return N * (8.0 + 3.0 * st);
+                }
+                case NUMA_PRIM_D2:
+                    d2(x2_, Fx_, z_, w_);
+                    return N * 8.0 * (cfg_.d2_mode == NUMA_DERIV_FD ? 4.0 : 3.0) +
+                           (cfg_.d2_mode == NUMA_DERIV_FD ? 8.0 * N : 0.0);
+                case NUMA_PRIM_PREC:
+                    precond(V_[0], zb_);
+                    return mg_ ? mg_->vcycle_bytes() : 3.0 * N * st;
+                case NUMA_PRIM_SMOOTH:
+                    require(mg_ != nullptr, "no multigrid in this solver");
+                    return mg_->smooth_step_for_timing();
+                case NUMA_PRIM_MGS:
+                    hipLaunchKernelGGL((k_mgs<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0,
+                                       c_.stream, N_, V_[2], (const T*)V_[0], (const double*)H(0, 0),
+                                       (const double*)S(0), (const T*)V_[1], c_.red,
+                                       Epilogue{EPI_STORE, 1.0, nullptr, MISC(S_TMP), nullptr});
+                    NUMA_HIP_LAUNCH_CHECK();
+                    return 4.0 * N * st;
+                default:
+                    throw Error(NUMA_ERR_ARG, "unknown primitive");
+            }
+        };
+        require(prim != NUMA_PRIM_MGS || m_ >= 2, "MGS timing needs restart >= 2");
+        double b = run();  // warm-up
+        hipEvent_t e0, e1;
+        NUMA_HIP_CHECK(hipEventCreate(&e0));
+        NUMA_HIP_CHECK(hipEventCreate(&e1));
+        NUMA_HIP_CHECK(hipEventRecord(e0, c_.stream));
+        for (int64_t r = 0; r < reps; ++r) b = run();
+        NUMA_HIP_CHECK(hipEventRecord(e1, c_.stream));
+        NUMA_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        NUMA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (bytes) *bytes = b;
+        return (double)ms / (double)reps;
+    }
+
+private:
+    // misc scalar slots
+    enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_TMP, kMisc = 8 };
+    double* H(int64_t i, int64_t j) { return sd_ + j * (m_ + 1) + i; }
+    double* S(int64_t i) { return sd_ + (m_ + 1) * m_ + i; }
+    double* RHS(int64_t i) { return sd_ + (m_ + 1) * m_ + (m_ + 1) + i; }
+    double* MISC(int i) { return sd_ + (m_ + 1) * m_ + 2 * (m_ + 1) + i; }
+
+    double read1(const double* dev) {
+        NUMA_HIP_CHECK(hipMemcpyAsync(hs_, dev, sizeof(double), hipMemcpyDeviceToHost, c_.stream));
+        c_.sync();
+        return hs_[0];
+    }
+    void readn(const double* dev, int64_t cnt, double* host) {
+        NUMA_HIP_CHECK(hipMemcpyAsync(hs_, dev, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                      c_.stream));
+        c_.sync();
+        std::copy(hs_, hs_ + cnt, host);
+    }
+
+    double* allocd() {
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipMalloc(&p, (size_t)N_ * sizeof(double)));
+        allocs_.push_back(p);
+        return static_cast<double*>(p);
+    }
+    T* alloct() {
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipMalloc(&p, (size_t)N_ * sizeof(T)));
+        allocs_.push_back(p);
+        return static_cast<T*>(p);
+    }
+
+    // ||v|| -> slot (EPI_SQRT) or ||v||, 1/||v|| -> slot, slot2
+    template <class TV>
+    void norm(const TV* v, double* slot, double* slot2 = nullptr) {
+        launch_dot<TV, TV>(c_, N_, v, v,
+                           Epilogue{slot2 ? EPI_NORM_SCALE : EPI_SQRT, 1.0, nullptr, slot, slot2});
+    }
+
+    // F at the trial point w = (x - a z) - b u (z,u may be null); writes F and,
+    // if wout, w; ||F|| -> S_FNORM.
+    void eval_F(const double* x, const double* z, double a, const double* u, double b, double* F,
+                double* wout) {
+        launch_stencil<true>(c_, n_, LdStep<double>{x, z, a, u, b},
+                             EpF{h2inv_, lambda_, F, wout},
+                             Epilogue{EPI_SQRT, 1.0, nullptr, MISC(S_FNORM), nullptr});
+    }
+
+    // Everything that depends only on the Newton point x: ||x|| for the FD
+    // step, the Jacobi diagonal (BratuJacobi::make, bratu3d.hpp:251-255).
+    void point_setup(const double* x) {
+        if (cfg_.jv_mode == NUMA_DERIV_FD || cfg_.d2_mode == NUMA_DERIV_FD) norm(x, MISC(S_XNORM));
+        if (cfg_.precond == NUMA_PRECOND_JACOBI) {
+            hipLaunchKernelGGL(k_jacobi_diag, dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
+                               N_, h2inv_, lambda_, x, diag_);
+            NUMA_HIP_LAUNCH_CHECK();
+        }
+    }
+
+    // out = scale * J(x) v  [or b - J(x) v]; reduction epilogue ep if reducing.
+    template <class TV>
+    void jv(const double* x, const double* F0, const TV* v, T* out, const double* scale,
+            const T* dotv, const double* b, const Epilogue& ep, bool reduce) {
+        if (cfg_.jv_mode == NUMA_DERIV_FD) {
+            norm(v, MISC(S_VNORM));
+            const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), MISC(S_XNORM),
+                            MISC(S_VNORM)};
+            LdXpHV<TV> L{x, v, st, 0.0};
+            EpJvFD<T> E{h2inv_, lambda_, st, scale, F0, out, dotv, b, 0.0, 1.0};
+            if (reduce)
+                launch_stencil<true>(c_, n_, L, E, ep);
+            else
+                launch_stencil<false>(c_, n_, L, E, ep);
+        } else {
+            EpJvAn<T> E{h2inv_, lambda_, scale, x, out, dotv, b, 1.0};
+            if (reduce)
+                launch_stencil<true>(c_, n_, LdVec<TV>{v}, E, ep);
+            else
+                launch_stencil<false>(c_, n_, LdVec<TV>{v}, E, ep);
+        }
+    }
+
+    // Preconditioner M^-1 in -> out; returns the result pointer (in itself
+    // for no preconditioner, IdentityPrecond of gmres.hpp:37-39).
+    const T* precond(const T* in, T* out) {
+        switch (cfg_.precond) {
+            case NUMA_PRECOND_GMG:
+                mg_->apply(in, out);
+                return out;
+            case NUMA_PRECOND_JACOBI:
+                hipLaunchKernelGGL((k_diag_solve<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0,
+                                   c_.stream, N_, in, (const double*)diag_, out);
+                NUMA_HIP_LAUNCH_CHECK();
+                return out;
+            default:
+                return in;
+        }
+    }
+
+    void d2(const double* x, const double* F0, const double* v, double* out) {
+        if (cfg_.d2_mode == NUMA_DERIV_FD) {
+            norm(v, MISC(S_VNORM));
+            const FDStep st{std::pow(std::numeric_limits<double>::epsilon(), 0.25), MISC(S_XNORM),
+                            MISC(S_VNORM)};
+            launch_stencil<false>(c_, n_, LdPM<double>{x, v, st, 0.0},
+                                  EpD2FD{h2inv_, lambda_, st, F0, out, 0.0}, Epilogue{});
+        } else {
+            hipLaunchKernelGGL(k_d2_analytic, dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
+                               N_, lambda_, x, v, out);
+            NUMA_HIP_LAUNCH_CHECK();
+        }
+    }
+
+    numa_gmres_result gmres(const double* x0, const double* F0, const double* b, double* xl,
+                            double rtol, int64_t restart, int64_t max_iters, bool zero_guess,
+                            const double* r_init, bool save);
+    numa_gmres_result gmres_recycled(const double* x0, const double* F0, const double* b,
+                                     double* xl, double rtol, int64_t restart, int64_t max_iters);
+
+    Ctx& c_;
+    int64_t n_, N_;
+    double lambda_, h2inv_;
+    numa_solver_config cfg_;
+    int64_t m_;
+    std::unique_ptr<MGBase> mg_;
+    double *x2_ = nullptr, *Fx_ = nullptr, *Fn_ = nullptr, *z_ = nullptr, *u_ = nullptr,
+           *w_ = nullptr, *r_ = nullptr, *diag_ = nullptr;
+    std::vector<T*> V_, Z_;
+    T* zb_ = nullptr;
+    T* comb_ = nullptr;
+    int64_t nsc_ = 0;
+    double* sd_ = nullptr;
+    double* hs_ = nullptr;
+    std::vector<void*> allocs_;
+    // saved last Arnoldi cycle for recycling (gmres.hpp:210-220)
+    int64_t saved_k_ = 0;
+    std::vector<std::vector<double>> saved_H_;
+    std::vector<double> saved_s_;
+};
+
+// linalg/gmres.hpp:61-223 (and FGMRES when cfg.flexible).
+template <class T>
+numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, const double* b,
+                                        double* xl, double rtol, int64_t restart,
+                                        int64_t max_iters, bool zero_guess, const double* r_init,
+                                        bool save) {
+    numa_gmres_result res{};
+    if (save) saved_k_ = 0;
+    norm(b, MISC(S_BNORM));
+    const double bnorm = read1(MISC(S_BNORM));
+    if (bnorm == 0.0) {
+        launch_fill<double>(c_, N_, 0.0, xl);
+        res.converged = 1;
+        return res;
+    }
+    if (restart == 0) {
+        res.rel_res = 1.0;
+        res.breakdown = 1;
+        return res;
+    }
+    require(restart <= m_, "gmres restart exceeds the solver's basis capacity");
+    if (max_iters == 0) {
+        res.rel_res = 1.0;
+        return res;
+    }
+    if (zero_guess) launch_fill<double>(c_, N_, 0.0, xl);
+
+    const bool flex = cfg_.flexible != 0;
+    const size_t m = (size_t)restart;
+    std::vector<std::vector<double>> Hh(m + 1, std::vector<double>(m, 0.0)),
+        Hraw(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m), sn(m), g(m + 1), y(m), s(m + 1), col(m + 2);
+    bool use_given = r_init != nullptr;
+    bool first = zero_guess;
+    size_t k_last = 0;
+    const T* zlast[1] = {nullptr};
+    (void)zlast;
+
+    while (res.iters < max_iters) {
+        double beta;
+        if (use_given) {
+            launch_convert<double, T>(c_, N_, r_init, V_[0]);
+            norm(r_init, MISC(S_BETA), S(0));
+            use_given = false;
+            first = false;
+        } else if (first) {
+            launch_convert<double, T>(c_, N_, b, V_[0]);
+            norm(b, MISC(S_BETA), S(0));
+            first = false;
+        } else {
+            jv<double>(x0, F0, xl, V_[0], nullptr, nullptr, b,
+                       Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)}, true);
+            ++res.matvecs;
+        }
+        beta = read1(MISC(S_BETA));
+        res.rel_res = beta / bnorm;
+        if (res.rel_res <= rtol) {
+            res.converged = 1;
+            k_last = 0;
+            break;
+        }
+        for (auto& row : Hraw) std::fill(row.begin(), row.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        s[0] = beta > 0.0 ? 1.0 / beta : 0.0;
+
+        size_t k = 0;
+        k_last = 0;
+        while (k < m && res.iters < max_iters) {
+            const size_t j = k;
+            T* zdst = flex && !Z_.empty() ? Z_[j] : zb_;
+            const T* zj = precond(V_[j], zdst);
+            ++res.precs;
+            // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
+            jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
+                  Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr}, true);
+            ++res.matvecs;
+            for (size_t i = 0; i <= j; ++i) {
+                const bool last = i == j;
+                const Epilogue ep = last ? Epilogue{EPI_NORM_SCALE, 1.0, nullptr, H(j + 1, j), S(j + 1)}
+                                         : Epilogue{EPI_STORE, 1.0, S(i + 1), H(i + 1, j), nullptr};
+                hipLaunchKernelGGL((k_mgs<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
+                                   N_, V_[j + 1], (const T*)V_[i], (const double*)H(i, j),
+                                   (const double*)S(i), last ? (const T*)nullptr : (const T*)V_[i + 1],
+                                   c_.red, ep);
+                NUMA_HIP_LAUNCH_CHECK();
+            }
+            readn(H(0, j), (int64_t)j + 2, col.data());
+            for (size_t i = 0; i <= j + 1; ++i) Hh[i][j] = col[i];
+            s[j + 1] = Hh[j + 1][j] > 0.0 ? 1.0 / Hh[j + 1][j] : 0.0;
+            for (size_t i = 0; i <= j + 1; ++i) Hraw[i][j] = Hh[i][j];
+            for (size_t i = 0; i < j; ++i) {
+                const double t = cs[i] * Hh[i][j] + sn[i] * Hh[i + 1][j];
+                Hh[i + 1][j] = -sn[i] * Hh[i][j] + cs[i] * Hh[i + 1][j];
+                Hh[i][j] = t;
+            }
+            const double denom = std::hypot(Hh[j][j], Hh[j + 1][j]);
+            if (!(denom > 0.0) || !std::isfinite(denom)) {
+                res.breakdown = 1;
+                return res;
+            }
+            cs[j] = Hh[j][j] / denom;
+            sn[j] = Hh[j + 1][j] / denom;
+            Hh[j][j] = denom;
+            Hh[j + 1][j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++k;
+            ++res.iters;
+            k_last = k;
+            res.rel_res = std::abs(g[j + 1]) / bnorm;
+            if (res.rel_res <= rtol) break;
+        }
+        for (size_t ii = k; ii-- > 0;) {
+            const double pivot = Hh[ii][ii];
+            if (!(std::abs(pivot) > 0.0) || !std::isfinite(pivot)) {
+                res.breakdown = 1;
+                return res;
+            }
+            double acc = g[ii];
+            for (size_t jj = ii + 1; jj < k; ++jj) acc -= Hh[ii][jj] * y[jj];
+            y[ii] = acc / pivot;
+        }
+        std::vector<const T*> vecs(k);
+        std::vector<double> coef(k);
+        for (size_t i = 0; i < k; ++i) coef[i] = y[i] * s[i];
+        if (flex) {
+            for (size_t i = 0; i < k; ++i) vecs[i] = Z_.empty() ? V_[i] : Z_[i];
+            launch_combine<T, double>(c_, N_, vecs, coef, xl, true);
+        } else {
+            for (size_t i = 0; i < k; ++i) vecs[i] = V_[i];
+            launch_combine<T, T>(c_, N_, vecs, coef, comb_, false);
+            const T* zq = precond(comb_, zb_);
+            ++res.precs;
+            launch_axpy<T, double>(c_, N_, 1.0, zq, xl);
+        }
+        if (res.rel_res <= rtol) {
+            res.converged = 1;
+            break;
+        }
+    }
+    if (save && k_last > 0) {
+        saved_k_ = (int64_t)k_last;
+        saved_H_.assign(k_last + 1, std::vector<double>(k_last, 0.0));
+        for (size_t i = 0; i <= k_last; ++i)
+            for (size_t j = 0; j < k_last; ++j) saved_H_[i][j] = Hraw[i][j];
+        saved_s_.assign(s.begin(), s.begin() + k_last + 1);
+    }
+    return res;
+}
+
+// Second solve on the saved Arnoldi relation (gmres.hpp:229-323).  The saved
+// basis is still resident in V; it is consumed before GMRES overwrites it.
+template <class T>
+numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double* F0,
+                                                 const double* b, double* xl, double rtol,
+                                                 int64_t restart, int64_t max_iters) {
+    launch_fill<double>(c_, N_, 0.0, xl);
+    const int64_t k = saved_k_;
+    if (k == 0) return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+    std::vector<double> rhs((size_t)k + 1);
+    for (int64_t i = 0; i <= k; ++i)
+        launch_dot<T, double>(c_, N_, V_[(size_t)i], b,
+                              Epilogue{EPI_STORE, 1.0, S(i), RHS(i), nullptr});
+    readn(RHS(0), k + 1, rhs.data());
+    auto R = saved_H_;
+    std::vector<double> cs((size_t)k, 0.0), sn((size_t)k, 0.0);
+    for (int64_t j = 0; j < k; ++j) {
+        for (int64_t i = 0; i < j; ++i) {
+            const double t = cs[i] * R[i][j] + sn[i] * R[i + 1][j];
+            R[i + 1][j] = -sn[i] * R[i][j] + cs[i] * R[i + 1][j];
+            R[i][j] = t;
+        }
+        const double denom = std::hypot(R[j][j], R[j + 1][j]);
+        if (!(denom > 0.0) || !std::isfinite(denom))
+            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+        cs[j] = R[j][j] / denom;
+        sn[j] = R[j + 1][j] / denom;
+        R[j][j] = denom;
+        R[j + 1][j] = 0.0;
+        const double t = cs[j] * rhs[j] + sn[j] * rhs[j + 1];
+        rhs[j + 1] = -sn[j] * rhs[j] + cs[j] * rhs[j + 1];
+        rhs[j] = t;
+    }
+    std::vector<double> y((size_t)k, 0.0);
+    for (int64_t ii = k; ii-- > 0;) {
+        const double pivot = R[ii][ii];
+        if (!(std::abs(pivot) > 0.0) || !std::isfinite(pivot)) {
+            launch_fill<double>(c_, N_, 0.0, xl);
+            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+        }
+        double acc = rhs[ii];
+        for (int64_t jj = ii + 1; jj < k; ++jj) acc -= R[ii][jj] * y[jj];
+        y[ii] = acc / pivot;
+    }
+    // residual = b - V_{k+1} Hbar y  (no A v products)
+    std::vector<const T*> vecs((size_t)k + 1);
+    std::vector<double> coef((size_t)k + 1);
+    for (int64_t i = 0; i <= k; ++i) {
+        double cf = 0.0;
+        for (int64_t j = 0; j < k; ++j) cf += saved_H_[i][j] * y[j];
+        vecs[i] = V_[(size_t)i];
+        coef[i] = -cf * saved_s_[i];
+    }
+    launch_convert<double, double>(c_, N_, b, r_);
+    launch_combine<T, double>(c_, N_, vecs, coef, r_, true);
+    // x0 = M^-1 (V_k y)
+    std::vector<const T*> vk((size_t)k);
+    std::vector<double> ck((size_t)k);
+    for (int64_t j = 0; j < k; ++j) {
+        vk[j] = V_[(size_t)j];
+        ck[j] = y[j] * saved_s_[j];
+    }
+    launch_combine<T, T>(c_, N_, vk, ck, comb_, false);
+    const T* px = precond(comb_, zb_);
+    launch_convert<T, double>(c_, N_, px, xl);
+    auto result = gmres(x0, F0, b, xl, rtol, restart, max_iters, false, r_, false);
+    ++result.precs;
+    return result;
+}
+
+// methods/krylov_newton.hpp:69-189 (Newton when method = NEWTON, inexact
+// Chebyshev with optional recycling when method = CHEBYSHEV).
+template <class T>
+void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylov_result& out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    out = numa_krylov_result{};
+    const bool cheb = cfg_.method == NUMA_METHOD_CHEBYSHEV;
+    const bool rec = cheb && p.recycle;
+    require(p.gmres_restart >= 0 && p.gmres_restart <= m_,
+            "gmres_restart exceeds the solver's basis capacity");
+    double* xc = xdev;  // current iterate
+    double* xt = x2_;   // trial point
+    double* Fc = Fx_;
+    double* Ft = Fn_;
+    eval_F(xc, nullptr, 0.0, nullptr, 0.0, Fc, nullptr);
+    ++out.n_F;
+    double rnorm = read1(MISC(S_FNORM));
+    out.res0 = rnorm;
+    double rprev = rnorm, eta_prev = p.eta_init;
+    for (int64_t k = 0; k < p.max_nl; ++k) {
+        if (rnorm <= p.rtol_nl * out.res0) {
+            out.converged = 1;
+            break;
+        }
+        const double eta1 = forcing(p, k, rnorm, rprev, eta_prev, out.res0);
+        const double eta2 = std::min(p.eta2_max, p.eta2_factor * eta1);
+        point_setup(xc);
+        const auto g1 = gmres(xc, Fc, Fc, z_, eta1, p.gmres_restart, p.gmres_max_iters, true,
+                              nullptr, rec);
+        out.n_JVP += g1.matvecs;
+        out.n_lin_iters += g1.iters;
+        out.n_prec += g1.precs;
+        if (!g1.converged) ++out.n_lin_failures;
+        if (g1.breakdown) break;
+        bool ok = false;
+        double fn = 0.0;
+        if (cheb) {
+            d2(xc, Fc, z_, w_);
+            ++out.n_D2;
+            const auto g2 = rec ? gmres_recycled(xc, Fc, w_, u_, eta2, p.gmres_restart,
+                                                 p.gmres_max_iters)
+                                : gmres(xc, Fc, w_, u_, eta2, p.gmres_restart, p.gmres_max_iters,
+                                        true, nullptr, false);
+            out.n_JVP += g2.matvecs;
+            out.n_lin_iters += g2.iters;
+            out.n_prec += g2.precs;
+            if (!g2.converged) ++out.n_lin_failures;
+            if (!g2.breakdown) {
+                eval_F(xc, z_, 1.0, u_, 0.5, Ft, xt);
+                ++out.n_F;
+                fn = read1(MISC(S_FNORM));
+                if (fn < rnorm) ok = true;
+            }
+        }
+        if (!ok) {
+            double lam = 1.0;
+            while (lam >= p.min_lambda) {
+                eval_F(xc, z_, lam, nullptr, 0.0, Ft, xt);
+                ++out.n_F;
+                fn = read1(MISC(S_FNORM));
+                if (fn <= (1.0 - p.armijo * lam) * rnorm) {
+                    ok = true;
+                    break;
+                }
+                lam *= 0.5;
+            }
+        }
+        if (!ok) break;
+        std::swap(xc, xt);
+        std::swap(Fc, Ft);
+        rprev = rnorm;
+        rnorm = fn;
+        eta_prev = eta1;
+        ++out.iters;
+    }
+    if (rnorm <= p.rtol_nl * out.res0) out.converged = 1;
+    out.res = rnorm;
+    if (xc != xdev) {
+        launch_convert<double, double>(c_, N_, xc, xdev);
+        x2_ = xc;  // the internal buffer now holds the previous trial storage
+    }
+    // keep Fx_/Fn_ ownership consistent for the next call
+    Fx_ = Fc;
+    Fn_ = Ft;
+    c_.sync();
+    out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace numa_hip
+
+// ===========================================================================
+// C entry points: multigrid and solver handles
+// ===========================================================================
+using numa_hip::Error;
+
+#define NUMA_API_BEGIN try {
+#define NUMA_API_END                                    \
+    }                                                   \
+    catch (const Error& e) {                            \
+        numa_hip::set_last_error(e.what());             \
+        return e.code;                                  \
+    }                                                   \
+    catch (const std::bad_alloc&) {                     \
+        numa_hip::set_last_error("host allocation failed"); \
+        return NUMA_ERR_NOMEM;                          \
+    }                                                   \
+    catch (const std::exception& e) {                   \
+        numa_hip::set_last_error(e.what());             \
+        return NUMA_ERR_HIP;                            \
+    }                                                   \
+    return NUMA_OK;
+
+extern "C" {
+
+int numa_hip_mg_create(numa_hip_ctx* ctx, int64_t n, const numa_mg_config* cfg,
+                       int64_t precision, numa_hip_mg** out) {
+    NUMA_API_BEGIN
+    numa_hip::require(ctx && cfg && out, "null argument");
+    numa_hip::require(precision == NUMA_PRECISION_FP64 || precision == NUMA_PRECISION_MIXED,
+                      "unknown precision");
+    NUMA_HIP_CHECK(hipSetDevice(ctx->impl->device));
+    auto h = std::make_unique<numa_hip_mg>();
+    h->ctx = ctx->impl;
+    h->precision = precision;
+    h->impl.reset(numa_hip::make_mg(*ctx->impl, n, *cfg, precision));
+    ++ctx->impl->live_handles;
+    *out = h.release();
+    NUMA_API_END
+}
+
+int numa_hip_mg_levels(const numa_hip_mg* mg, int64_t* levels, int64_t* sizes_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(mg && levels, "null argument");
+    *levels = mg->impl->levels();
+    if (sizes_host)
+        for (int64_t l = 0; l < *levels; ++l) sizes_host[l] = mg->impl->level_n(l);
+    NUMA_API_END
+}
+
+int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev) {
+    NUMA_API_BEGIN
+    numa_hip::require(mg && r_dev && z_dev, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(mg->ctx->device));
+    mg->impl->apply(r_dev, z_dev);
+    NUMA_API_END
+}
+
+int numa_hip_mg_destroy(numa_hip_mg* mg) {
+    NUMA_API_BEGIN
+    if (mg) {
+        NUMA_HIP_CHECK(hipStreamSynchronize(mg->ctx->stream));
+        --mg->ctx->live_handles;
+        delete mg;
+    }
+    NUMA_API_END
+}
+
+int numa_hip_bratu_solver_create(numa_hip_ctx* ctx, int64_t n, double lambda,
+                                 const numa_solver_config* cfg, int64_t gmres_restart,
+                                 numa_hip_solver** out) {
+    NUMA_API_BEGIN
+    numa_hip::require(ctx && cfg && out, "null argument");
+    numa_hip::require(std::isfinite(lambda), "lambda must be finite");
+    NUMA_HIP_CHECK(hipSetDevice(ctx->impl->device));
+    auto h = std::make_unique<numa_hip_solver>();
+    h->ctx = ctx->impl;
+    if (cfg->precision == NUMA_PRECISION_MIXED)
+        h->impl.reset(new numa_hip::BratuSolver<float>(*ctx->impl, n, lambda, *cfg, gmres_restart));
+    else if (cfg->precision == NUMA_PRECISION_FP64)
+        h->impl.reset(new numa_hip::BratuSolver<double>(*ctx->impl, n, lambda, *cfg, gmres_restart));
+    else
+        throw Error(NUMA_ERR_ARG, "unknown precision");
+    ++ctx->impl->live_handles;
+    *out = h.release();
+    NUMA_API_END
+}
+
+int numa_hip_solver_destroy(numa_hip_solver* s) {
+    NUMA_API_BEGIN
+    if (s) {
+        NUMA_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        --s->ctx->live_handles;
+        delete s;
+    }
+    NUMA_API_END
+}
+
+int numa_hip_gmres(numa_hip_solver* s, const double* x0_dev, const double* b_dev, double* z_dev,
+                   double rtol, int64_t restart, int64_t max_iters, int64_t zero_guess,
+                   numa_gmres_result* out) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && x0_dev && b_dev && z_dev && out, "null argument");
+    numa_hip::require(restart >= 0 && max_iters >= 0, "negative restart / max_iters");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *out = s->impl->gmres_entry(x0_dev, b_dev, z_dev, rtol, restart, max_iters, zero_guess != 0);
+    NUMA_API_END
+}
+
+int numa_hip_solve(numa_hip_solver* s, const numa_krylov_params* p, double* x_dev,
+                   numa_krylov_result* out) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && p && x_dev && out, "null argument");
+    numa_hip::require(p->max_nl >= 0 && p->gmres_max_iters >= 0 && p->gmres_restart >= 0,
+                      "negative iteration limits");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    s->impl->solve(*p, x_dev, *out);
+    NUMA_API_END
+}
+
+int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
+                            double* ms_per_call, double* bytes_per_call) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && ms_per_call, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *ms_per_call = s->impl->primitive_cost(prim, reps, bytes_per_call);
+    NUMA_API_END
+}
+
+int numa_hip_solve_host(numa_hip_solver* s, const numa_krylov_params* p, double* x_host,
+                        numa_krylov_result* out) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && p && x_host && out, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    const size_t bytes = (size_t)s->impl->size() * sizeof(double);
+    void* d = nullptr;
+    NUMA_HIP_CHECK(hipMalloc(&d, bytes));
+    struct Guard {
+        void* p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{d};
+    NUMA_HIP_CHECK(hipMemcpyAsync(d, x_host, bytes, hipMemcpyHostToDevice, s->ctx->stream));
+    numa_hip::require(p->max_nl >= 0, "negative iteration limits");
+    s->impl->solve(*p, static_cast<double*>(d), *out);
+    NUMA_HIP_CHECK(hipMemcpyAsync(x_host, d, bytes, hipMemcpyDeviceToHost, s->ctx->stream));
+    NUMA_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    NUMA_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+"""ctypes bindings of lib/libnuma_hip.so (include/numa_hip.h).
+
+This is the Python face of the MI355X backend used by the tests and bench.py.
+It mirrors the reference's interface for the solver path
+(include/numa/methods/krylov_newton.hpp, linalg/gmres.hpp,
+linalg/poisson_mg.hpp, problems/bratu3d.hpp): KrylovParams / KrylovResult /
+GmresResult with the same fields, Bratu3D with eval / jvp / d2, PoissonMG
+with levels() and operator(), newton_krylov / chebyshev_inexact / gmres.
+Device memory is owned by DeviceArray (allocated through the C ABI, no torch).
+
+There is no CPU fallback: loading fails loudly if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnuma_hip.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include", "numa_hip.h")
+
+_i64 = C.c_int64
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+NUMA_OK = 0
+ERR_NAMES = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "overflow",
+             5: "bad state"}
+
+METHOD_NEWTON, METHOD_CHEBYSHEV = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+DERIV_FD, DERIV_ANALYTIC = 0, 1
+FORCING_EW, FORCING_CONSTANT = 0, 1
+SMOOTH_RBGS, SMOOTH_WJACOBI, SMOOTH_CHEBYSHEV = 0, 1, 2
+PRIM_F, PRIM_JVP, PRIM_D2, PRIM_PREC, PRIM_SMOOTH, PRIM_MGS = 0, 1, 2, 3, 4, 5
+
+
+class KrylovParams(C.Structure):
+    _fields_ = [("rtol_nl", C.c_double), ("max_nl", _i64), ("eta_init", C.c_double),
+                ("eta_max", C.c_double), ("gamma", C.c_double), ("alpha", C.c_double),
+                ("eta2_factor", C.c_double), ("eta2_max", C.c_double),
+                ("gmres_restart", _i64), ("gmres_max_iters", _i64), ("armijo", C.c_double),
+                ("min_lambda", C.c_double), ("recycle", _i64), ("forcing", _i64)]
+
+
+class KrylovResult(C.Structure):
+    _fields_ = [("converged", _i64), ("iters", _i64), ("n_F", _i64), ("n_JVP", _i64),
+                ("n_D2", _i64), ("n_lin_iters", _i64), ("n_lin_failures", _i64),
+                ("n_prec", _i64), ("res0", C.c_double), ("res", C.c_double),
+                ("t_total", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class GmresResult(C.Structure):
+    _fields_ = [("iters", _i64), ("matvecs", _i64), ("precs", _i64), ("rel_res", C.c_double),
+                ("converged", _i64), ("breakdown", _i64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MGConfig(C.Structure):
+    _fields_ = [("smoother", _i64), ("pre", _i64), ("post", _i64), ("degree", _i64),
+                ("max_levels", _i64), ("min_coarse_n", _i64), ("coarse_iters", _i64),
+                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double)]
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [("method", _i64), ("precond", _i64), ("precision", _i64), ("flexible", _i64),
+                ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
+
+
+def krylov_params(**kw) -> KrylovParams:
+    """Defaults of numa::op::KrylovParams (methods/krylov_newton.hpp:24-38)."""
+    p = KrylovParams(rtol_nl=1e-8, max_nl=50, eta_init=0.5, eta_max=0.9, gamma=0.9, alpha=2.0,
+                     eta2_factor=10.0, eta2_max=0.5, gmres_restart=50, gmres_max_iters=500,
+                     armijo=1e-4, min_lambda=1e-4, recycle=0, forcing=FORCING_EW)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def mg_reference() -> MGConfig:
+    """PoissonMG(n) (linalg/poisson_mg.hpp:26-64): RB-GS V(2,2), odd-n coarsening."""
+    return MGConfig(smoother=SMOOTH_RBGS, pre=2, post=2, degree=3, max_levels=0, min_coarse_n=1,
+                    coarse_iters=30, even_coarsening=0, omega=6.0 / 7.0, cheb_ratio=4.0)
+
+
+def mg_config(**kw) -> MGConfig:
+    m = mg_reference()
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def mg_cheb(levels: int = 5, degree: int = 3, pre: int = 2, post: int = 2,
+            cheb_ratio: float = 4.0, coarse_iters: int = 24) -> MGConfig:
+    """5-level even-n V(2,2) with the degree-3 Chebyshev smoother (256^3 config)."""
+    return mg_config(smoother=SMOOTH_CHEBYSHEV, pre=pre, post=post, degree=degree,
+                     max_levels=levels, min_coarse_n=2, coarse_iters=coarse_iters,
+                     even_coarsening=1, cheb_ratio=cheb_ratio)
+
+
+def mg_wjacobi(levels: int = 0, pre: int = 2, post: int = 2, omega: float = 6.0 / 7.0,
+               coarse_iters: int = 24) -> MGConfig:
+    """Even-n V(2,2) with the weighted-Jacobi smoother (32^3 CPU config)."""
+    return mg_config(smoother=SMOOTH_WJACOBI, pre=pre, post=post, max_levels=levels,
+                     min_coarse_n=2, coarse_iters=coarse_iters, even_coarsening=1, omega=omega)
+
+
+def solver_config(method=METHOD_NEWTON, precond=PRECOND_GMG, precision=PRECISION_FP64,
+                  flexible=0, jv_mode=DERIV_FD, d2_mode=DERIV_FD, mg=None) -> SolverConfig:
+    return SolverConfig(method=method, precond=precond, precision=precision, flexible=flexible,
+                        jv_mode=jv_mode, d2_mode=d2_mode,
+                        mg=mg if mg is not None else mg_reference())
+
+
+class NumaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NumaError(f"{path} not found: build it with __graft_entry__.build() "
+                        "(there is no CPU fallback)")
+    L = C.CDLL(path)
+    L.numa_hip_last_error.restype = C.c_char_p
+    sig = {
+        "numa_hip_ctx_create": [C.c_int, C.POINTER(_vp)],
+        "numa_hip_ctx_destroy": [_vp],
+        "numa_hip_ctx_sync": [_vp],
+        "numa_hip_malloc": [_vp, C.c_size_t, C.POINTER(_vp)],
+        "numa_hip_free": [_vp, _vp],
+        "numa_hip_memcpy_h2d": [_vp, _vp, _vp, C.c_size_t],
+        "numa_hip_memcpy_d2h": [_vp, _vp, _vp, C.c_size_t],
+        "numa_hip_bratu_eval": [_vp, _i64, C.c_double, _vp, _vp, _dp],
+        "numa_hip_bratu_jvp": [_vp, _i64, C.c_double, _vp, _vp, _vp],
+        "numa_hip_bratu_d2": [_vp, _i64, C.c_double, _vp, _vp, _vp],
+        "numa_hip_bratu_jacobi_diag": [_vp, _i64, C.c_double, _vp, _vp],
+        "numa_hip_bratu_jvp_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
+        "numa_hip_bratu_jvp_fd_mixed": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
+        "numa_hip_bratu_d2_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
+        "numa_hip_dot": [_vp, _i64, _vp, _vp, _dp],
+        "numa_hip_dot_f32": [_vp, _i64, _vp, _vp, _dp],
+        "numa_hip_axpy": [_vp, _i64, C.c_double, _vp, _vp],
+        "numa_hip_mg_create": [_vp, _i64, C.POINTER(MGConfig), _i64, C.POINTER(_vp)],
+        "numa_hip_mg_levels": [_vp, C.POINTER(_i64), C.POINTER(_i64)],
+        "numa_hip_mg_apply": [_vp, _vp, _vp],
+        "numa_hip_mg_destroy": [_vp],
+        "numa_hip_bratu_solver_create": [_vp, _i64, C.c_double, C.POINTER(SolverConfig), _i64,
+                                         C.POINTER(_vp)],
+        "numa_hip_solver_destroy": [_vp],
+        "numa_hip_gmres": [_vp, _vp, _vp, _vp, C.c_double, _i64, _i64, _i64,
+                           C.POINTER(GmresResult)],
+        "numa_hip_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
+        "numa_hip_solve_host": [_vp, C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)],
+        "numa_hip_primitive_cost": [_vp, _i64, _i64, _dp, _dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != NUMA_OK:
+        msg = _lib.numa_hip_last_error().decode() if _lib is not None else ""
+        raise NumaError(f"numa_hip error {rc} ({ERR_NAMES.get(rc, '?')}): {msg}")
+
+
+def declared_symbols(header: str = INCLUDE) -> list[str]:
+    """Entry points declared in include/numa_hip.h."""
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"\b(numa_hip_[a-z0-9_]+)\s*\(", txt)))
+
+
+class Context:
+    """A device + stream (numa_hip_ctx)."""
+
+    def __init__(self, device: int = 0):
+        import weakref
+        self.lib = load()
+        h = _vp()
+        check(self.lib.numa_hip_ctx_create(device, C.byref(h)))
+        self.h = h
+        # handles created on this context; closed before the context itself
+        # (a context must outlive its solvers, multigrids and buffers)
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+        return obj
+
+    def sync(self):
+        check(self.lib.numa_hip_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            for child in list(self._children):
+                child.close()
+            check(self.lib.numa_hip_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def array(self, a) -> "DeviceArray":
+        return DeviceArray.from_numpy(self, np.asarray(a))
+
+    def empty(self, n: int, dtype=np.float64) -> "DeviceArray":
+        return DeviceArray(self, n, dtype)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class DeviceArray:
+    """A 1-D device buffer of float64 or float32 owned through the C ABI."""
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        p = _vp()
+        check(ctx.lib.numa_hip_malloc(ctx.h, max(1, self.nbytes), C.byref(p)))
+        self.ptr = p
+        ctx._adopt(self)
+
+    @property
+    def nbytes(self) -> int:
+        return self.n * self.dtype.itemsize
+
+    @classmethod
+    def from_numpy(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        assert a.dtype in (np.float64, np.float32)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        check(self.ctx.lib.numa_hip_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        check(self.ctx.lib.numa_hip_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            check(self.ctx.lib.numa_hip_free(self.ctx.h, self.ptr))
+        self.ptr = None
+
+    close = free
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Bratu3D:
+    """problems/bratu3d.hpp:12-100 on the device: F(u) = -Lap_h u - lambda e^u."""
+
+    def __init__(self, ctx: Context, n: int, lam: float = 1.0):
+        self.ctx, self.n, self.lam = ctx, int(n), float(lam)
+
+    def size(self) -> int:
+        return self.n ** 3
+
+    def eval(self, u: DeviceArray, out: DeviceArray) -> float:
+        nrm = C.c_double()
+        check(self.ctx.lib.numa_hip_bratu_eval(self.ctx.h, self.n, self.lam, u.ptr, out.ptr,
+                                               C.byref(nrm)))
+        return nrm.value
+
+    def jvp(self, u, v, out):
+        check(self.ctx.lib.numa_hip_bratu_jvp(self.ctx.h, self.n, self.lam, u.ptr, v.ptr, out.ptr))
+
+    def d2(self, u, v, out):
+        check(self.ctx.lib.numa_hip_bratu_d2(self.ctx.h, self.n, self.lam, u.ptr, v.ptr, out.ptr))
+
+    def jacobi_diag(self, u, d):
+        check(self.ctx.lib.numa_hip_bratu_jacobi_diag(self.ctx.h, self.n, self.lam, u.ptr, d.ptr))
+
+    def jvp_fd(self, x, F0, v, out):
+        f = (self.ctx.lib.numa_hip_bratu_jvp_fd_mixed if v.dtype == np.float32
+             else self.ctx.lib.numa_hip_bratu_jvp_fd)
+        check(f(self.ctx.h, self.n, self.lam, x.ptr, F0.ptr, v.ptr, out.ptr))
+
+    def d2_fd(self, x, F0, v, out):
+        check(self.ctx.lib.numa_hip_bratu_d2_fd(self.ctx.h, self.n, self.lam, x.ptr, F0.ptr,
+                                                v.ptr, out.ptr))
+
+
+class PoissonMG:
+    """linalg/poisson_mg.hpp:24-299 on the device (plus even-n / WJ / Chebyshev)."""
+
+    def __init__(self, ctx: Context, n: int, cfg: MGConfig | None = None,
+                 precision: int = PRECISION_FP64):
+        self.ctx, self.n, self.precision = ctx, int(n), precision
+        self.cfg = cfg if cfg is not None else mg_reference()
+        h = _vp()
+        check(ctx.lib.numa_hip_mg_create(ctx.h, self.n, C.byref(self.cfg), precision, C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def levels(self) -> int:
+        L = _i64()
+        check(self.ctx.lib.numa_hip_mg_levels(self.h, C.byref(L), None))
+        return L.value
+
+    def sizes(self) -> list[int]:
+        L = _i64()
+        s = (_i64 * 64)()
+        check(self.ctx.lib.numa_hip_mg_levels(self.h, C.byref(L), s))
+        return [int(s[i]) for i in range(L.value)]
+
+    def __call__(self, r: DeviceArray, z: DeviceArray) -> None:
+        check(self.ctx.lib.numa_hip_mg_apply(self.h, r.ptr, z.ptr))
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_mg_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BratuSolver:
+    """Device-resident Newton-Krylov / inexact Chebyshev on Bratu3D."""
+
+    def __init__(self, ctx: Context, n: int, lam: float, cfg: SolverConfig, restart: int = 50):
+        self.ctx, self.n, self.lam, self.cfg = ctx, int(n), float(lam), cfg
+        h = _vp()
+        check(ctx.lib.numa_hip_bratu_solver_create(ctx.h, self.n, self.lam, C.byref(cfg),
+                                                   int(restart), C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def gmres(self, x0: DeviceArray, b: DeviceArray, z: DeviceArray, rtol: float, restart: int,
+              max_iters: int, zero_guess: bool = True) -> GmresResult:
+        r = GmresResult()
+        check(self.ctx.lib.numa_hip_gmres(self.h, x0.ptr, b.ptr, z.ptr, rtol, restart, max_iters,
+                                          1 if zero_guess else 0, C.byref(r)))
+        return r
+
+    def solve(self, params: KrylovParams, x: DeviceArray) -> KrylovResult:
+        r = KrylovResult()
+        check(self.ctx.lib.numa_hip_solve(self.h, C.byref(params), x.ptr, C.byref(r)))
+        return r
+
+    def solve_host(self, params: KrylovParams, x: np.ndarray) -> KrylovResult:
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        r = KrylovResult()
+        check(self.ctx.lib.numa_hip_solve_host(self.h, C.byref(params),
+                                               x.ctypes.data_as(_dp), C.byref(r)))
+        return r
+
+    def primitive_cost(self, prim: int, reps: int = 20) -> tuple[float, float]:
+        """(ms per call, algorithmic bytes per call) of one primitive."""
+        ms, b = C.c_double(), C.c_double()
+        check(self.ctx.lib.numa_hip_primitive_cost(self.h, prim, reps, C.byref(ms), C.byref(b)))
+        return ms.value, b.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_solver_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def newton_krylov(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restart=None):
+    """numa::op::newton_krylov (krylov_newton.hpp:69-114) on the device."""
+    cfg = cfg if cfg is not None else solver_config()
+    cfg.method = METHOD_NEWTON
+    params = params if params is not None else krylov_params()
+    s = BratuSolver(ctx, n, lam, cfg, restart or max(1, params.gmres_restart))
+    x = np.array(x0, np.float64)
+    r = s.solve_host(params, x)
+    s.close()
+    return x, r
+
+
+def chebyshev_inexact(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restart=None):
+    """numa::op::chebyshev_inexact (krylov_newton.hpp:116-189) on the device."""
+    cfg = cfg if cfg is not None else solver_config()
+    cfg.method = METHOD_CHEBYSHEV
+    params = params if params is not None else krylov_params()
+    s = BratuSolver(ctx, n, lam, cfg, restart or max(1, params.gmres_restart))
+    x = np.array(x0, np.float64)
+    r = s.solve_host(params, x)
+    s.close()
+    return x, r

@@ -1,0 +1,251 @@
+/*
+ * numa_hip.h - C ABI of the MI355X (gfx950) backend of the matrix-free
+ * nonlinear-solver path: Bratu3D residual and directional derivatives,
+ * BLAS-1 / Arnoldi reductions, geometric multigrid, (F)GMRES and the
+ * Newton / inexact-Chebyshev outer loops, plus the dense Chandrasekhar
+ * H-equation operator.
+ *
+ * Every entry point returns an int status (NUMA_OK = 0).  On failure the
+ * message is available from numa_hip_last_error() (thread local).  Numerical
+ * outcomes (non-convergence, breakdown) are reported in result structs and
+ * never as an error status, as in the reference.
+ *
+ * Pointer arguments named *_dev are device (HBM) addresses; *_host are host
+ * addresses.  All device work of a context is issued on the context's stream.
+ * Handles are not thread safe: one handle per host thread.
+ *
+ * References below are to kapumota/chebyshev-nonlinear-lab, include/numa/.
+ */
+#ifndef NUMA_HIP_H
+#define NUMA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    NUMA_OK = 0,
+    NUMA_ERR_ARG = 1,      /* invalid argument (reference: std::invalid_argument) */
+    NUMA_ERR_HIP = 2,      /* HIP runtime error                                   */
+    NUMA_ERR_NOMEM = 3,    /* device allocation failed                            */
+    NUMA_ERR_OVERFLOW = 4, /* size not representable (reference: std::overflow_error) */
+    NUMA_ERR_STATE = 5     /* handle used in a state that does not allow the call */
+};
+
+/* ---- enumerations used in the POD configs ------------------------------ */
+enum { NUMA_METHOD_NEWTON = 0, NUMA_METHOD_CHEBYSHEV = 1 };
+enum {
+    NUMA_PRECOND_NONE = 0,
+    NUMA_PRECOND_JACOBI = 1, /* BratuJacobi, bratu3d.hpp:104-120            */
+    NUMA_PRECOND_GMG = 2     /* geometric multigrid V-cycle, see numa_mg_config */
+};
+enum { NUMA_PRECISION_FP64 = 0, NUMA_PRECISION_MIXED = 1 };
+enum { NUMA_DERIV_FD = 0, NUMA_DERIV_ANALYTIC = 1 };
+enum { NUMA_FORCING_EW = 0, NUMA_FORCING_CONSTANT = 1 };
+enum {
+    NUMA_SMOOTH_RBGS = 0,     /* red-black Gauss-Seidel, poisson_mg.hpp:99-126 */
+    NUMA_SMOOTH_WJACOBI = 1,  /* weighted Jacobi                               */
+    NUMA_SMOOTH_CHEBYSHEV = 2 /* Chebyshev polynomial of D^-1 A                */
+};
+
+/* ---- POD parameter / result structs ------------------------------------ */
+/* All fields are 8 bytes wide so the layout is identical for C, C++,
+ * ctypes and any FFI. */
+
+/* Mirrors numa::op::KrylovParams (methods/krylov_newton.hpp:24-38) field for
+ * field, with the additions after `recycle`. */
+typedef struct numa_krylov_params {
+    double rtol_nl;          /* ||F|| <= rtol_nl * ||F(x0)||           */
+    int64_t max_nl;
+    double eta_init;
+    double eta_max;
+    double gamma;
+    double alpha;
+    double eta2_factor;
+    double eta2_max;
+    int64_t gmres_restart;
+    int64_t gmres_max_iters;
+    double armijo;
+    double min_lambda;
+    int64_t recycle;
+    /* additions (no reference counterpart) */
+    int64_t forcing;         /* NUMA_FORCING_EW (reference) or _CONSTANT (eta = eta_init) */
+} numa_krylov_params;
+
+/* Mirrors numa::op::KrylovResult (krylov_newton.hpp:40-49) without the
+ * solution vector (returned in place), plus the Timers fields of
+ * operators/timing.hpp:16-24 measured with HIP events on the stream. */
+typedef struct numa_krylov_result {
+    int64_t converged;
+    int64_t iters;
+    int64_t n_F;
+    int64_t n_JVP;
+    int64_t n_D2;
+    int64_t n_lin_iters;
+    int64_t n_lin_failures;
+    int64_t n_prec;
+    double res0;
+    double res;
+    double t_total;          /* seconds, wall clock of the whole solve        */
+} numa_krylov_result;
+
+/* Mirrors numa::op::GmresResult (linalg/gmres.hpp:14-21). */
+typedef struct numa_gmres_result {
+    int64_t iters;
+    int64_t matvecs;
+    int64_t precs;
+    double rel_res;
+    int64_t converged;
+    int64_t breakdown;
+} numa_gmres_result;
+
+/* Multigrid V-cycle configuration.  The reference PoissonMG
+ * (linalg/poisson_mg.hpp:26-64) is smoother = RBGS, pre = post = 2,
+ * even_coarsening = 0, max_levels = 0, min_coarse_n = 1, coarse_iters = 30. */
+typedef struct numa_mg_config {
+    int64_t smoother;        /* NUMA_SMOOTH_*                                      */
+    int64_t pre;             /* pre-smoothing sweeps                               */
+    int64_t post;            /* post-smoothing sweeps                              */
+    int64_t degree;          /* Chebyshev smoother degree (matvecs per sweep)      */
+    int64_t max_levels;      /* 0 = no limit                                       */
+    int64_t min_coarse_n;    /* stop coarsening once n <= min_coarse_n             */
+    int64_t coarse_iters;    /* coarsest level: RB-GS sweeps or Chebyshev degree   */
+    int64_t even_coarsening; /* 1: coarsen any n to n/2; 0: only odd n (reference) */
+    double omega;            /* weighted-Jacobi damping                            */
+    double cheb_ratio;       /* Chebyshev smoothing interval [lmax/ratio, lmax]    */
+} numa_mg_config;
+
+/* Solver configuration: what the reference selects by template arguments and
+ * provider types, plus the precision / flexibility additions. */
+typedef struct numa_solver_config {
+    int64_t method;          /* NUMA_METHOD_*                                   */
+    int64_t precond;         /* NUMA_PRECOND_*                                  */
+    int64_t precision;       /* NUMA_PRECISION_*                                */
+    int64_t flexible;        /* 0: GMRES (gmres.hpp:61-223), 1: FGMRES          */
+    int64_t jv_mode;         /* NUMA_DERIV_FD (jvp_fd) or _ANALYTIC (Bratu3D::jvp) */
+    int64_t d2_mode;         /* NUMA_DERIV_FD (d2_fd) or _ANALYTIC (Bratu3D::d2)   */
+    numa_mg_config mg;
+} numa_solver_config;
+
+/* ---- opaque handles ---------------------------------------------------- */
+typedef struct numa_hip_ctx numa_hip_ctx;
+typedef struct numa_hip_mg numa_hip_mg;
+typedef struct numa_hip_solver numa_hip_solver;
+
+/* ---- context and memory ------------------------------------------------ */
+const char* numa_hip_last_error(void);
+int numa_hip_ctx_create(int device, numa_hip_ctx** out);
+/* Fails with NUMA_ERR_STATE while multigrid or solver handles created on the
+ * context are alive: destroy those first. */
+int numa_hip_ctx_destroy(numa_hip_ctx* ctx);
+int numa_hip_ctx_sync(numa_hip_ctx* ctx);
+int numa_hip_malloc(numa_hip_ctx* ctx, size_t bytes, void** out_dev);
+int numa_hip_free(numa_hip_ctx* ctx, void* dev);
+int numa_hip_memcpy_h2d(numa_hip_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int numa_hip_memcpy_d2h(numa_hip_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+
+/* ---- Bratu3D operator (problems/bratu3d.hpp) ---------------------------- */
+/* F(u) = (n+1)^2 (6u - sum of 6 neighbours) - lambda e^u, zero Dirichlet,
+ * index (i*n+j)*n+k.  Replaces Bratu3D::eval (bratu3d.hpp:21-31, 73-95).
+ * If norm_host is not NULL it receives ||F||_2 (the call then synchronises). */
+int numa_hip_bratu_eval(numa_hip_ctx* ctx, int64_t n, double lambda,
+                        const double* u_dev, double* F_dev, double* norm_host);
+/* Analytic J v.  Replaces Bratu3D::jvp (bratu3d.hpp:33-43). */
+int numa_hip_bratu_jvp(numa_hip_ctx* ctx, int64_t n, double lambda,
+                       const double* u_dev, const double* v_dev, double* out_dev);
+/* Analytic F''[v,v] = -lambda e^u v^2.  Replaces Bratu3D::d2 (bratu3d.hpp:46-56). */
+int numa_hip_bratu_d2(numa_hip_ctx* ctx, int64_t n, double lambda,
+                      const double* u_dev, const double* v_dev, double* out_dev);
+/* Jacobi diagonal 6(n+1)^2 - lambda e^u.  Replaces Bratu3D::jacobi_diag
+ * (bratu3d.hpp:59-69). */
+int numa_hip_bratu_jacobi_diag(numa_hip_ctx* ctx, int64_t n, double lambda,
+                               const double* u_dev, double* d_dev);
+/* Forward-difference J v = (F(x+hv) - F0)/h, h = sqrt(eps)(1+||x||)/||v||,
+ * out = 0 if ||v|| = 0, with F0 = F(x) supplied by the caller (the reference
+ * recomputes it: operators/operator.hpp:101-114).  FP64 throughout. */
+int numa_hip_bratu_jvp_fd(numa_hip_ctx* ctx, int64_t n, double lambda,
+                          const double* x_dev, const double* F0_dev,
+                          const double* v_dev, double* out_dev);
+/* Mixed form of the above: x, F0, F(x+hv) and the subtraction in FP64,
+ * v and out in FP32. */
+int numa_hip_bratu_jvp_fd_mixed(numa_hip_ctx* ctx, int64_t n, double lambda,
+                                const double* x_dev, const double* F0_dev,
+                                const float* v_dev, float* out_dev);
+/* Central-difference F''(x)[v,v] = (F(x+hv) - 2F0 + F(x-hv))/h^2,
+ * h = eps^(1/4)(1+||x||)/||v||.  Replaces d2_fd (operator.hpp:118-134). */
+int numa_hip_bratu_d2_fd(numa_hip_ctx* ctx, int64_t n, double lambda,
+                         const double* x_dev, const double* F0_dev,
+                         const double* v_dev, double* out_dev);
+
+/* ---- BLAS-1 (operators/operator.hpp:19-41) ------------------------------ */
+/* Deterministic fixed-order two-stage reductions, FP64 accumulation. */
+int numa_hip_dot(numa_hip_ctx* ctx, int64_t n, const double* a_dev,
+                 const double* b_dev, double* result_host);
+int numa_hip_dot_f32(numa_hip_ctx* ctx, int64_t n, const float* a_dev,
+                     const float* b_dev, double* result_host);
+int numa_hip_axpy(numa_hip_ctx* ctx, int64_t n, double a, const double* x_dev,
+                  double* y_dev);
+
+/* ---- geometric multigrid (linalg/poisson_mg.hpp) ------------------------ */
+/* Builds the level hierarchy of -Lap_h on n^3 interior nodes.  Replaces
+ * PoissonMG::PoissonMG (poisson_mg.hpp:26-64); precision selects FP64 or FP32
+ * level storage. */
+int numa_hip_mg_create(numa_hip_ctx* ctx, int64_t n, const numa_mg_config* cfg,
+                       int64_t precision, numa_hip_mg** out);
+int numa_hip_mg_levels(const numa_hip_mg* mg, int64_t* levels, int64_t* sizes_host);
+/* z = one V-cycle applied to r (zero initial guess).  Replaces
+ * PoissonMG::operator() (poisson_mg.hpp:69-80).  r and z are double* for
+ * NUMA_PRECISION_FP64 and float* for NUMA_PRECISION_MIXED. */
+int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev);
+int numa_hip_mg_destroy(numa_hip_mg* mg);
+
+/* ---- device-resident solvers on Bratu3D ---------------------------------- */
+int numa_hip_bratu_solver_create(numa_hip_ctx* ctx, int64_t n, double lambda,
+                                 const numa_solver_config* cfg,
+                                 int64_t gmres_restart,
+                                 numa_hip_solver** out);
+int numa_hip_solver_destroy(numa_hip_solver* s);
+/* Solves J(x0) z = b with restarted right-preconditioned GMRES, or FGMRES if
+ * the solver was created flexible.  Replaces numa::op::gmres
+ * (linalg/gmres.hpp:61-65) with apply = J(x0)v, precond = the configured
+ * preconditioner.  x0, b and z are FP64 device vectors. */
+int numa_hip_gmres(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
+                   double* z_dev, double rtol, int64_t restart, int64_t max_iters,
+                   int64_t zero_guess, numa_gmres_result* out);
+/* Newton-Krylov or inexact Chebyshev from x (device, FP64, overwritten by
+ * the final iterate).  Replaces numa::op::newton_krylov /
+ * chebyshev_inexact (methods/krylov_newton.hpp:69-189). */
+int numa_hip_solve(numa_hip_solver* s, const numa_krylov_params* p,
+                   double* x_dev, numa_krylov_result* out);
+/* Same with the iterate in host memory (staged through the device). */
+int numa_hip_solve_host(numa_hip_solver* s, const numa_krylov_params* p,
+                        double* x_host, numa_krylov_result* out);
+
+/* ---- per-call primitive costs (instrumentation) -------------------------- */
+/* The reference freezes per-call costs c_F, c_JVP, c_D2, c_prec of its
+ * primitives (experiments/c1/frozen/primitive_costs.csv, measured by
+ * scripts/freeze_primitive_costs.py, semantics docs/research/C1_WEQ_PROTOCOL.md)
+ * and times them with Timed / TimedPrecond (operators/timing.hpp:37-108).
+ * This entry point measures one primitive of a solver on its own stream with
+ * HIP events: average milliseconds per call over `reps` calls, and the call's
+ * algorithmic HBM bytes. */
+enum {
+    NUMA_PRIM_F = 0,      /* F(x), FP64, fused ||F||^2                      */
+    NUMA_PRIM_JVP = 1,    /* J v stencil kernel with the fused first MGS dot */
+    NUMA_PRIM_D2 = 2,     /* F''[v,v]                                        */
+    NUMA_PRIM_PREC = 3,   /* one preconditioner application (V-cycle)        */
+    NUMA_PRIM_SMOOTH = 4, /* one finest-level smoothing step                 */
+    NUMA_PRIM_MGS = 5     /* one fused MGS pass (w -= h V_i; dot(w, V_i+1))  */
+};
+int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
+                            double* ms_per_call, double* bytes_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUMA_HIP_H */

@@ -1,0 +1,76 @@
+/*
+ * numa_oracle.h - C ABI of the CPU FP64 oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a CPU restatement of the
+ * reference's matrix-free solver path (kapumota/chebyshev-nonlinear-lab,
+ * include/numa/...) plus the new components the GPU path adds (even-n
+ * multigrid, weighted-Jacobi and Chebyshev smoothers, FGMRES, constant
+ * forcing, midpoint / Kelley-form Chandrasekhar).  Only tests/, the smoke()
+ * check and bench.py's cpu_baseline leg may load it, and only as the checker
+ * or the timed CPU baseline, never as a product path.
+ *
+ * Parity pins for every function are listed in the header comment of
+ * numa_oracle.cpp.
+ */
+#ifndef NUMA_ORACLE_H
+#define NUMA_ORACLE_H
+
+#include <stdint.h>
+#include "../include/numa_hip.h" /* shared POD config / result structs only */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_num_threads(void);
+
+/* Bratu3D (problems/bratu3d.hpp) */
+void orc_bratu_eval(int64_t n, double lambda, const double* u, double* out);
+void orc_bratu_jvp(int64_t n, double lambda, const double* u, const double* v, double* out);
+void orc_bratu_d2(int64_t n, double lambda, const double* u, const double* v, double* out);
+void orc_bratu_jacobi_diag(int64_t n, double lambda, const double* u, double* d);
+void orc_bratu_jvp_fd(int64_t n, double lambda, const double* x, const double* v, double* out);
+void orc_bratu_d2_fd(int64_t n, double lambda, const double* x, const double* v, double* out);
+void orc_poisson_apply(int64_t n, const double* v, double* out);
+
+/* BLAS-1 (operators/operator.hpp:19-41) */
+double orc_dot(int64_t n, const double* a, const double* b);
+
+/* multigrid (linalg/poisson_mg.hpp) */
+int64_t orc_mg_levels(int64_t n, const numa_mg_config* cfg, int64_t* sizes);
+int orc_mg_apply(int64_t n, const numa_mg_config* cfg, const double* r, double* z);
+
+/* GMRES on the Bratu Jacobian at x0 (linalg/gmres.hpp:61-223); flexible = 1
+ * selects FGMRES.  precond / jv_mode / mg come from cfg. */
+int orc_bratu_gmres(int64_t n, double lambda, const numa_solver_config* cfg,
+                    const double* x0, const double* b, double* x, double rtol,
+                    int64_t restart, int64_t max_iters, int64_t zero_guess,
+                    numa_gmres_result* out);
+/* GMRES on the plain Poisson operator -Lap_h (test_operators.cpp:207-233). */
+int orc_poisson_gmres(int64_t n, const numa_mg_config* mg, int64_t use_mg,
+                      const double* b, double* x, double rtol, int64_t restart,
+                      int64_t max_iters, numa_gmres_result* out);
+
+/* Newton-Krylov / inexact Chebyshev on Bratu3D (methods/krylov_newton.hpp) */
+int orc_bratu_solve(int64_t n, double lambda, const numa_solver_config* cfg,
+                    const numa_krylov_params* p, double* x, numa_krylov_result* out);
+
+/* Chandrasekhar H-equation (problems/chandrasekhar_op.hpp and the midpoint /
+ * Kelley form).  rule: 0 = Gauss-Legendre on [0,1], 1 = midpoint.
+ * form: 0 = F = 1 + x(Ax) - x (reference), 1 = F = x - 1/(1 - Ax). */
+int orc_gauss_legendre_unit(int64_t n, double* nodes, double* weights);
+int orc_chandra_eval(int64_t n, double c, int64_t rule, int64_t form,
+                     const double* x, double* out);
+int orc_chandra_jvp(int64_t n, double c, int64_t rule, int64_t form,
+                    const double* x, const double* v, double* out);
+int orc_chandra_d2(int64_t n, double c, int64_t rule, int64_t form,
+                   const double* x, const double* v, double* out);
+/* Newton-GMRES (no preconditioner) or Chebyshev; method from cfg->method. */
+int orc_chandra_solve(int64_t n, double c, int64_t rule, int64_t form,
+                      const numa_solver_config* cfg, const numa_krylov_params* p,
+                      double* x, numa_krylov_result* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

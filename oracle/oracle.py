@@ -1,0 +1,357 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the reference's own
+header library compiled into oracle/_ref/libnuma_ref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker / timed CPU baseline.  It is never
+a product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libnuma_ref.so")
+
+_dp = C.POINTER(C.c_double)
+_i64 = C.c_int64
+
+
+# ---- POD structs (include/numa_hip.h) -------------------------------------
+class KrylovParams(C.Structure):
+    _fields_ = [("rtol_nl", C.c_double), ("max_nl", _i64), ("eta_init", C.c_double),
+                ("eta_max", C.c_double), ("gamma", C.c_double), ("alpha", C.c_double),
+                ("eta2_factor", C.c_double), ("eta2_max", C.c_double),
+                ("gmres_restart", _i64), ("gmres_max_iters", _i64), ("armijo", C.c_double),
+                ("min_lambda", C.c_double), ("recycle", _i64), ("forcing", _i64)]
+
+
+class KrylovResult(C.Structure):
+    _fields_ = [("converged", _i64), ("iters", _i64), ("n_F", _i64), ("n_JVP", _i64),
+                ("n_D2", _i64), ("n_lin_iters", _i64), ("n_lin_failures", _i64),
+                ("n_prec", _i64), ("res0", C.c_double), ("res", C.c_double),
+                ("t_total", C.c_double)]
+
+
+class GmresResult(C.Structure):
+    _fields_ = [("iters", _i64), ("matvecs", _i64), ("precs", _i64), ("rel_res", C.c_double),
+                ("converged", _i64), ("breakdown", _i64)]
+
+
+class MGConfig(C.Structure):
+    _fields_ = [("smoother", _i64), ("pre", _i64), ("post", _i64), ("degree", _i64),
+                ("max_levels", _i64), ("min_coarse_n", _i64), ("coarse_iters", _i64),
+                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double)]
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [("method", _i64), ("precond", _i64), ("precision", _i64), ("flexible", _i64),
+                ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
+
+
+METHOD_NEWTON, METHOD_CHEBYSHEV = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+DERIV_FD, DERIV_ANALYTIC = 0, 1
+FORCING_EW, FORCING_CONSTANT = 0, 1
+SMOOTH_RBGS, SMOOTH_WJACOBI, SMOOTH_CHEBYSHEV = 0, 1, 2
+
+
+def krylov_params(**kw) -> KrylovParams:
+    """Defaults of numa::op::KrylovParams (methods/krylov_newton.hpp:24-38)."""
+    p = KrylovParams(rtol_nl=1e-8, max_nl=50, eta_init=0.5, eta_max=0.9, gamma=0.9, alpha=2.0,
+                     eta2_factor=10.0, eta2_max=0.5, gmres_restart=50, gmres_max_iters=500,
+                     armijo=1e-4, min_lambda=1e-4, recycle=0, forcing=FORCING_EW)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def mg_reference() -> MGConfig:
+    """PoissonMG(n) of linalg/poisson_mg.hpp:26-64: RB-GS V(2,2), odd-n coarsening."""
+    return MGConfig(smoother=SMOOTH_RBGS, pre=2, post=2, degree=3, max_levels=0, min_coarse_n=1,
+                    coarse_iters=30, even_coarsening=0, omega=6.0 / 7.0, cheb_ratio=4.0)
+
+
+def mg_config(**kw) -> MGConfig:
+    m = mg_reference()
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def solver_config(method=METHOD_NEWTON, precond=PRECOND_GMG, precision=PRECISION_FP64,
+                  flexible=0, jv_mode=DERIV_FD, d2_mode=DERIV_FD, mg=None) -> SolverConfig:
+    return SolverConfig(method=method, precond=precond, precision=precision, flexible=flexible,
+                        jv_mode=jv_mode, d2_mode=d2_mode, mg=mg if mg is not None else mg_reference())
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liboracle.so (and _ref when the reference sources exist)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+class Oracle:
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        self.lib = L
+        for name in ("orc_bratu_eval", "orc_bratu_jacobi_diag"):
+            getattr(L, name).argtypes = [_i64, C.c_double, _dp, _dp]
+        for name in ("orc_bratu_jvp", "orc_bratu_d2", "orc_bratu_jvp_fd", "orc_bratu_d2_fd"):
+            getattr(L, name).argtypes = [_i64, C.c_double, _dp, _dp, _dp]
+        L.orc_poisson_apply.argtypes = [_i64, _dp, _dp]
+        L.orc_dot.argtypes = [_i64, _dp, _dp]
+        L.orc_dot.restype = C.c_double
+        L.orc_mg_levels.argtypes = [_i64, C.POINTER(MGConfig), C.POINTER(_i64)]
+        L.orc_mg_levels.restype = _i64
+        L.orc_mg_apply.argtypes = [_i64, C.POINTER(MGConfig), _dp, _dp]
+        L.orc_bratu_gmres.argtypes = [_i64, C.c_double, C.POINTER(SolverConfig), _dp, _dp, _dp,
+                                      C.c_double, _i64, _i64, _i64, C.POINTER(GmresResult)]
+        L.orc_poisson_gmres.argtypes = [_i64, C.POINTER(MGConfig), _i64, _dp, _dp, C.c_double,
+                                        _i64, _i64, C.POINTER(GmresResult)]
+        L.orc_bratu_solve.argtypes = [_i64, C.c_double, C.POINTER(SolverConfig),
+                                      C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)]
+        L.orc_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
+        L.orc_chandra_eval.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp]
+        L.orc_chandra_jvp.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp, _dp]
+        L.orc_chandra_d2.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp, _dp]
+        L.orc_chandra_solve.argtypes = [_i64, C.c_double, _i64, _i64, C.POINTER(SolverConfig),
+                                        C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)]
+        L.orc_num_threads.restype = C.c_int
+
+    def threads(self) -> int:
+        return int(self.lib.orc_num_threads())
+
+    # Bratu3D ---------------------------------------------------------------
+    def bratu_eval(self, n, lam, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.empty(n ** 3)
+        self.lib.orc_bratu_eval(n, lam, _ptr(u), _ptr(out))
+        return out
+
+    def _uv(self, name, n, lam, u, v):
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.empty(n ** 3)
+        getattr(self.lib, name)(n, lam, _ptr(u), _ptr(v), _ptr(out))
+        return out
+
+    def bratu_jvp(self, n, lam, u, v):
+        return self._uv("orc_bratu_jvp", n, lam, u, v)
+
+    def bratu_d2(self, n, lam, u, v):
+        return self._uv("orc_bratu_d2", n, lam, u, v)
+
+    def bratu_jvp_fd(self, n, lam, x, v):
+        return self._uv("orc_bratu_jvp_fd", n, lam, x, v)
+
+    def bratu_d2_fd(self, n, lam, x, v):
+        return self._uv("orc_bratu_d2_fd", n, lam, x, v)
+
+    def bratu_jacobi_diag(self, n, lam, u):
+        u = np.ascontiguousarray(u, np.float64)
+        out = np.empty(n ** 3)
+        self.lib.orc_bratu_jacobi_diag(n, lam, _ptr(u), _ptr(out))
+        return out
+
+    def poisson_apply(self, n, v):
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.empty(n ** 3)
+        self.lib.orc_poisson_apply(n, _ptr(v), _ptr(out))
+        return out
+
+    def dot(self, a, b):
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        return float(self.lib.orc_dot(a.size, _ptr(a), _ptr(b)))
+
+    # multigrid ---------------------------------------------------------------
+    def mg_levels(self, n, cfg=None):
+        sizes = (_i64 * 64)()
+        L = self.lib.orc_mg_levels(n, C.byref(cfg) if cfg else None, sizes)
+        return [int(sizes[i]) for i in range(L)]
+
+    def mg_apply(self, n, r, cfg=None):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(n ** 3)
+        rc = self.lib.orc_mg_apply(n, C.byref(cfg) if cfg else None, _ptr(r), _ptr(z))
+        assert rc == 0
+        return z
+
+    # Krylov ------------------------------------------------------------------
+    def bratu_gmres(self, n, lam, cfg, x0, b, rtol, restart, max_iters, x=None, zero_guess=1):
+        x0 = np.ascontiguousarray(x0, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n ** 3) if x is None else np.array(x, np.float64)
+        res = GmresResult()
+        rc = self.lib.orc_bratu_gmres(n, lam, C.byref(cfg), _ptr(x0), _ptr(b), _ptr(x), rtol,
+                                      restart, max_iters, zero_guess, C.byref(res))
+        assert rc == 0
+        return x, res
+
+    def poisson_gmres(self, n, b, rtol, restart, max_iters, mg=None, use_mg=1):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n ** 3)
+        res = GmresResult()
+        self.lib.orc_poisson_gmres(n, C.byref(mg) if mg else None, use_mg, _ptr(b), _ptr(x),
+                                   rtol, restart, max_iters, C.byref(res))
+        return x, res
+
+    def bratu_solve(self, n, lam, cfg, params, x0=None):
+        x = np.zeros(n ** 3) if x0 is None else np.array(x0, np.float64)
+        res = KrylovResult()
+        rc = self.lib.orc_bratu_solve(n, lam, C.byref(cfg), C.byref(params), _ptr(x), C.byref(res))
+        assert rc == 0
+        return x, res
+
+    # Chandrasekhar -------------------------------------------------------------
+    def gauss_legendre_unit(self, n):
+        a, b = np.empty(n), np.empty(n)
+        self.lib.orc_gauss_legendre_unit(n, _ptr(a), _ptr(b))
+        return a, b
+
+    def chandra_eval(self, n, c, x, rule=0, form=0):
+        x = np.ascontiguousarray(x, np.float64)
+        o = np.empty(n)
+        self.lib.orc_chandra_eval(n, c, rule, form, _ptr(x), _ptr(o))
+        return o
+
+    def chandra_jvp(self, n, c, x, v, rule=0, form=0):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        o = np.empty(n)
+        self.lib.orc_chandra_jvp(n, c, rule, form, _ptr(x), _ptr(v), _ptr(o))
+        return o
+
+    def chandra_d2(self, n, c, x, v, rule=0, form=0):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        o = np.empty(n)
+        self.lib.orc_chandra_d2(n, c, rule, form, _ptr(x), _ptr(v), _ptr(o))
+        return o
+
+    def chandra_solve(self, n, c, cfg, params, x0=None, rule=0, form=0):
+        x = np.ones(n) if x0 is None else np.array(x0, np.float64)
+        res = KrylovResult()
+        self.lib.orc_chandra_solve(n, c, rule, form, C.byref(cfg), C.byref(params), _ptr(x),
+                                   C.byref(res))
+        return x, res
+
+
+def params_vector(p: KrylovParams) -> np.ndarray:
+    """KrylovParams in the 13-double layout ref_driver.cpp reads."""
+    return np.array([p.rtol_nl, p.max_nl, p.eta_init, p.eta_max, p.gamma, p.alpha,
+                     p.eta2_factor, p.eta2_max, p.gmres_restart, p.gmres_max_iters, p.armijo,
+                     p.min_lambda, p.recycle], np.float64)
+
+
+class Reference:
+    """The reference's own headers compiled into oracle/_ref (this container only)."""
+
+    def __init__(self, path: str = REF_SO):
+        L = C.CDLL(path)
+        self.lib = L
+        for name in ("ref_bratu_eval", "ref_bratu_jacobi_diag"):
+            getattr(L, name).argtypes = [_i64, C.c_double, _dp, _dp]
+        for name in ("ref_bratu_jvp", "ref_bratu_d2", "ref_bratu_jvp_fd", "ref_bratu_d2_fd"):
+            getattr(L, name).argtypes = [_i64, C.c_double, _dp, _dp, _dp]
+        L.ref_mg_levels.argtypes = [_i64]
+        L.ref_mg_levels.restype = _i64
+        L.ref_mg_apply.argtypes = [_i64, _dp, _dp]
+        L.ref_bratu_gmres.argtypes = [_i64, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                      C.c_double, _i64, _i64, _dp]
+        L.ref_bratu_solve.argtypes = [_i64, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ref_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
+        L.ref_chandra_eval.argtypes = [_i64, C.c_double, _dp, _dp]
+        L.ref_chandra_jvp.argtypes = [_i64, C.c_double, _dp, _dp, _dp]
+        L.ref_chandra_d2.argtypes = [_i64, C.c_double, _dp, _dp, _dp]
+        L.ref_chandra_solve.argtypes = [_i64, C.c_double, C.c_int, _dp, _dp, _dp]
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_SO)
+
+    def bratu(self, name, n, lam, *vecs):
+        vecs = [np.ascontiguousarray(v, np.float64) for v in vecs]
+        out = np.empty(n ** 3)
+        getattr(self.lib, "ref_bratu_" + name)(n, lam, *[_ptr(v) for v in vecs], _ptr(out))
+        return out
+
+    def mg_levels(self, n):
+        return int(self.lib.ref_mg_levels(n))
+
+    def mg_apply(self, n, r):
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty(n ** 3)
+        self.lib.ref_mg_apply(n, _ptr(r), _ptr(z))
+        return z
+
+    def bratu_gmres(self, n, lam, fd, precond, x0, b, rtol, restart, max_iters):
+        x0 = np.ascontiguousarray(x0, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(n ** 3)
+        res = np.zeros(6)
+        self.lib.ref_bratu_gmres(n, lam, fd, precond, _ptr(x0), _ptr(b), _ptr(x), rtol, restart,
+                                 max_iters, _ptr(res))
+        return x, dict(iters=int(res[0]), matvecs=int(res[1]), precs=int(res[2]),
+                       rel_res=float(res[3]), converged=bool(res[4]), breakdown=bool(res[5]))
+
+    def bratu_solve(self, n, lam, method, precond, fd, params: KrylovParams, x0=None):
+        x = np.zeros(n ** 3) if x0 is None else np.array(x0, np.float64)
+        pv = params_vector(params)
+        res = np.zeros(10)
+        self.lib.ref_bratu_solve(n, lam, method, precond, fd, _ptr(pv), _ptr(x), _ptr(res))
+        return x, _result_dict(res)
+
+    def gauss_legendre_unit(self, n):
+        a, b = np.empty(n), np.empty(n)
+        self.lib.ref_gauss_legendre_unit(n, _ptr(a), _ptr(b))
+        return a, b
+
+    def chandra(self, name, n, c, *vecs):
+        vecs = [np.ascontiguousarray(v, np.float64) for v in vecs]
+        out = np.empty(n)
+        getattr(self.lib, "ref_chandra_" + name)(n, c, *[_ptr(v) for v in vecs], _ptr(out))
+        return out
+
+    def chandra_solve(self, n, c, method, params: KrylovParams, x0=None):
+        x = np.ones(n) if x0 is None else np.array(x0, np.float64)
+        pv = params_vector(params)
+        res = np.zeros(10)
+        self.lib.ref_chandra_solve(n, c, method, _ptr(pv), _ptr(x), _ptr(res))
+        return x, _result_dict(res)
+
+
+def _result_dict(r):
+    return dict(converged=bool(r[0]), iters=int(r[1]), n_F=int(r[2]), n_JVP=int(r[3]),
+                n_D2=int(r[4]), n_lin_iters=int(r[5]), n_lin_failures=int(r[6]),
+                n_prec=int(r[7]), res0=float(r[8]), res=float(r[9]))
+
+
+def result_dict(r: KrylovResult) -> dict:
+    return dict(converged=bool(r.converged), iters=int(r.iters), n_F=int(r.n_F),
+                n_JVP=int(r.n_JVP), n_D2=int(r.n_D2), n_lin_iters=int(r.n_lin_iters),
+                n_lin_failures=int(r.n_lin_failures), n_prec=int(r.n_prec), res0=float(r.res0),
+                res=float(r.res))
+
+
+def sin_point(n: int, scale: float) -> np.ndarray:
+    """x_i = scale sin(0.37 i + 0.5), the reference tests' deterministic input
+    (tests/test_operators.cpp:19-23)."""
+    i = np.arange(n, dtype=np.float64)
+    return scale * np.sin(0.37 * i + 0.5)

@@ -1,0 +1,124 @@
+"""GPU (F)GMRES and Newton / inexact-Chebyshev solves against the oracle.
+
+FP64 runs follow the reference's algorithm step by step, so the iteration
+counters must be identical and the iterates agree to 1e-10 relative.  Mixed
+runs (FP32 Krylov / V-cycle) must converge with the Newton count within +-1 of
+the FP64 oracle and a final solution within 1e-7 relative of the oracle's
+(both stop at ||F|| <= 1e-8 ||F0||)."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("converged", "iters", "n_F", "n_JVP", "n_D2", "n_lin_iters", "n_lin_failures", "n_prec")
+
+
+def _ocfg(cfg):
+    import oracle as O
+    m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
+    return O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=cfg.precision,
+                          flexible=cfg.flexible, jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
+
+
+def _oparams(p):
+    import oracle as O
+    return O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2])
+@pytest.mark.parametrize("fd", [0, 1])
+@pytest.mark.parametrize("flexible", [0, 1])
+def test_gmres_fp64(ctx, oracle, precond, fd, flexible):
+    import numa_hip as H
+    import oracle as O
+    n = 8
+    N = n ** 3
+    x0 = O.sin_point(N, 0.1)
+    b = O.sin_point(N, 1.0)
+    cfg = H.solver_config(precond=precond, flexible=flexible,
+                          jv_mode=H.DERIV_FD if fd else H.DERIV_ANALYTIC)
+    s = H.BratuSolver(ctx, n, 1.0, cfg, restart=40)
+    dz = ctx.empty(N)
+    g = s.gmres(ctx.array(x0), ctx.array(b), dz, 1e-10, 40, 400)
+    xo, go = oracle.bratu_gmres(n, 1.0, _ocfg(cfg), x0, b, 1e-10, 40, 400)
+    assert (g.iters, g.matvecs, g.precs, g.converged, g.breakdown) == \
+        (go.iters, go.matvecs, go.precs, go.converged, go.breakdown)
+    assert rel(dz.numpy(), xo) < 1e-9
+    # residual truth (test_operators.cpp:78-89)
+    if not fd:
+        Az = oracle.bratu_jvp(n, 1.0, x0, dz.numpy())
+        assert rel(Az, b) < 1e-9
+
+
+@pytest.mark.parametrize("jv", [1, 0])
+def test_gmres_restarts(ctx, oracle, jv):
+    """GMRES(10) over many restarts.  With the analytic J v the GPU follows the
+    oracle exactly; with FD J v at rtol 1e-9 the difference quotient's noise
+    (~1e-8 relative) is at the tolerance, so ulp-level differences of exp()
+    may move convergence by an iteration or two."""
+    import numa_hip as H
+    import oracle as O
+    n = 12
+    N = n ** 3
+    x0 = O.sin_point(N, 0.2)
+    b = O.sin_point(N, 1.0)
+    cfg = H.solver_config(precond=H.PRECOND_JACOBI, jv_mode=jv)
+    s = H.BratuSolver(ctx, n, 2.0, cfg, restart=10)
+    dz = ctx.empty(N)
+    g = s.gmres(ctx.array(x0), ctx.array(b), dz, 1e-9, 10, 300)
+    xo, go = oracle.bratu_gmres(n, 2.0, _ocfg(cfg), x0, b, 1e-9, 10, 300)
+    assert g.converged and go.converged and g.iters > 10
+    if jv == H.DERIV_ANALYTIC:
+        assert (g.iters, g.matvecs, g.precs) == (go.iters, go.matvecs, go.precs)
+        assert rel(dz.numpy(), xo) < 1e-9
+    else:
+        assert abs(g.iters - go.iters) <= 2
+        assert rel(dz.numpy(), xo) < 1e-7
+
+
+SOLVES = [
+    # n, lam, method, precond, jv_mode, recycle, flexible
+    (8, 1.0, 0, 1, 1, 0, 0),
+    (8, 1.0, 1, 1, 1, 0, 0),
+    (16, 1.0, 0, 2, 1, 0, 0),
+    (16, 1.0, 1, 2, 1, 0, 0),
+    (16, 1.0, 1, 2, 1, 1, 0),
+    (15, 1.0, 1, 2, 0, 1, 0),
+    (15, 3.0, 0, 2, 0, 0, 1),
+    (17, 6.0, 1, 1, 0, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", SOLVES)
+def test_solve_fp64_matches_oracle(ctx, oracle, case):
+    import numa_hip as H
+    n, lam, method, precond, jv, rec, flex = case
+    cfg = H.solver_config(method=method, precond=precond, flexible=flex, jv_mode=jv, d2_mode=jv)
+    p = H.krylov_params(recycle=rec)
+    x = np.zeros(n ** 3)
+    s = H.BratuSolver(ctx, n, lam, cfg, restart=p.gmres_restart)
+    r = s.solve_host(p, x)
+    xo, ro = oracle.bratu_solve(n, lam, _ocfg(cfg), _oparams(p))
+    assert {k: getattr(r, k) for k in KEYS} == {k: getattr(ro, k) for k in KEYS}
+    assert rel(x, xo) < 1e-10
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_solve_mixed_even_gmg(ctx, oracle, method):
+    import numa_hip as H
+    n, lam = 32, 6.0
+    mg = H.mg_cheb(levels=4)
+    cfg64 = H.solver_config(method=method, precond=H.PRECOND_GMG, mg=mg)
+    p = H.krylov_params(gmres_restart=30, eta_init=1e-3, forcing=H.FORCING_CONSTANT)
+    xo, ro = oracle.bratu_solve(n, lam, _ocfg(cfg64), _oparams(p))
+    for flex in (0, 1):
+        cfg = H.solver_config(method=method, precond=H.PRECOND_GMG, mg=mg,
+                              precision=H.PRECISION_MIXED, flexible=flex)
+        x = np.zeros(n ** 3)
+        s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+        r = s.solve_host(p, x)
+        assert r.converged and r.res <= 1e-8 * r.res0
+        assert abs(r.iters - ro.iters) <= 1
+        assert rel(x, xo) < 1e-7
