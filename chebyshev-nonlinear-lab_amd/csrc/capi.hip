@@ -108,7 +108,8 @@ void jvp_fd_impl(Ctx& c, int64_t n, double lambda, const double* x, const double
     const double h2inv = (n + 1.0) * (n + 1.0);
     numa_hip::launch_stencil<false>(
         c, n, numa_hip::LdXpHV<TV>{x, v, st, 0.0},
-        numa_hip::EpJvFD<TO>{h2inv, lambda, st, nullptr, F0, out, nullptr, nullptr, 0.0, 1.0},
+        numa_hip::EpJvFD<TO, false, false>{h2inv, lambda, st, nullptr, F0, out, nullptr, nullptr,
+                                           0.0, 1.0},
         Epilogue{});
 }
 }  // namespace
@@ -183,8 +184,7 @@ int numa_hip_bratu_eval(numa_hip_ctx* ctx, int64_t n, double lambda, const doubl
     require(u_dev && F_dev, "null device pointer");
     const double h2inv = (n + 1.0) * (n + 1.0);
     numa_hip::launch_stencil<true>(
-        c, n, numa_hip::LdStep<double>{u_dev, nullptr, 0.0, nullptr, 0.0},
-        numa_hip::EpF{h2inv, lambda, F_dev, nullptr},
+        c, n, numa_hip::LdVec<double>{u_dev}, numa_hip::EpF<false>{h2inv, lambda, F_dev, nullptr},
         Epilogue{numa_hip::EPI_SQRT, 1.0, nullptr, c.dscal + K_OUT, nullptr});
     if (norm_host) c.read_scalars(c.dscal + K_OUT, norm_host, 1);
     NUMA_API_END
@@ -199,7 +199,8 @@ int numa_hip_bratu_jvp(numa_hip_ctx* ctx, int64_t n, double lambda, const double
     const double h2inv = (n + 1.0) * (n + 1.0);
     numa_hip::launch_stencil<false>(
         c, n, numa_hip::LdVec<double>{v_dev},
-        numa_hip::EpJvAn<double>{h2inv, lambda, nullptr, u_dev, out_dev, nullptr, nullptr, 1.0},
+        numa_hip::EpJvAn<double, false, false>{h2inv, lambda, nullptr, u_dev, out_dev, nullptr,
+                                               nullptr, 1.0},
         Epilogue{});
     NUMA_API_END
 }

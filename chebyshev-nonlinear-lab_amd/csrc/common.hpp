@@ -147,28 +147,15 @@ struct Ctx {
     void sync() { NUMA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
 
-// Launch shape of grid-stride streaming kernels over N points: enough blocks
-// to fill 256 CUs several times over, capped for the reduction partials.
-inline unsigned stream_blocks(int64_t N, int items_per_thread = 4) {
-    int64_t b = (N + (int64_t)kBlock * items_per_thread - 1) / ((int64_t)kBlock * items_per_thread);
+// Launch shape of grid-stride streaming kernels over N points: each thread
+// handles 4 consecutive elements per iteration; up to 4096 blocks (16 per CU
+// of the 256-CU part), capped for the reduction partials.
+constexpr int kVec = 4;
+inline unsigned stream_blocks(int64_t N) {
+    int64_t b = (N + (int64_t)kBlock * kVec - 1) / ((int64_t)kBlock * kVec);
     if (b < 1) b = 1;
-    if (b > 2048) b = 2048;
+    if (b > 4096) b = 4096;
     return (unsigned)b;
-}
-
-// Launch shape of the 7-point stencil kernels: 64 (k) x 4 (j) threads per
-// block, each thread marching over kIChunk planes in i.
-constexpr int kTK = 64;
-constexpr int kTJ = 4;
-constexpr int kIChunk = 16;
-
-inline dim3 stencil_grid(int64_t n) {
-    return dim3((unsigned)((n + kTK - 1) / kTK), (unsigned)((n + kTJ - 1) / kTJ),
-                (unsigned)((n + kIChunk - 1) / kIChunk));
-}
-inline unsigned stencil_nblocks(int64_t n) {
-    dim3 g = stencil_grid(n);
-    return g.x * g.y * g.z;
 }
 
 }  // namespace numa_hip

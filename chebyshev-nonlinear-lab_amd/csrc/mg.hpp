@@ -21,23 +21,30 @@
 namespace numa_hip {
 
 // One Jacobi-type / Chebyshev step:  d <- c1 d + c2 (b - A x);  xn = x + d.
-template <class T>
+// USE_D reads the previous direction, WRITE_D stores the new one.
+template <class T, bool USE_D, bool WRITE_D>
 struct EpPoly {
     double h2inv, c1, c2;
     const T* b;
-    T* d;       // direction (read if use_d, written if write_d)
-    T* xn;      // new iterate
-    const T* x;
-    bool use_d, write_d;
+    T* d;
+    T* xn;
+    struct Extra {
+        T b, d;
+    };
     __device__ EpPoly prep() const { return *this; }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
-                                 P1 kp) const {
+    __device__ Extra preload(int64_t id) const {
+        Extra e{b[id], T(0)};
+        if constexpr (USE_D) e.d = d[id];
+        return e;
+    }
+    __device__ double operator()(int64_t id, const Extra& e, P1 c, P1 im, P1 ip, P1 jm, P1 jp,
+                                 P1 km, P1 kp) const {
 #pragma clang fp contract(off)
-        const double res = dv(b, id) - h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
-        const double dn = use_d ? c1 * dv(d, id) + c2 * res : c2 * res;
-        if (write_d) d[id] = static_cast<T>(dn);
-        const T o = static_cast<T>(c.a + dn);
-        xn[id] = o;
+        const double res = static_cast<double>(e.b) -
+                           h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
+        const double dn = USE_D ? c1 * static_cast<double>(e.d) + c2 * res : c2 * res;
+        if constexpr (WRITE_D) d[id] = static_cast<T>(dn);
+        xn[id] = static_cast<T>(c.a + dn);
         return 0.0;
     }
 };
@@ -49,25 +56,36 @@ struct EpResid {
     const T* b;
     T* r;
     __device__ EpResid prep() const { return *this; }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
+    __device__ T preload(int64_t id) const { return b[id]; }
+    __device__ double operator()(int64_t id, T bv, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
                                  P1 kp) const {
 #pragma clang fp contract(off)
-        r[id] = static_cast<T>(dv(b, id) - h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a));
+        r[id] = static_cast<T>(static_cast<double>(bv) -
+                               h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a));
         return 0.0;
     }
 };
 
 // First step from x = 0:  d = c2 b;  xn = d.
-template <class T>
+template <bool VEC, class T>
 __global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2, const T* b, T* d,
                                                         T* xn) {
 #pragma clang fp contract(off)
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * kBlock) {
-        const double dn = c2 * dv(b, i);
-        if (d) d[i] = static_cast<T>(dn);
-        xn[i] = static_cast<T>(dn);
-    }
+    for_each4<VEC>(
+        N,
+        [&](int64_t i) {
+            const V4<T> bv = ld4<VEC>(b, i);
+            V4<T> o;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o.v[e] = static_cast<T>(c2 * (double)bv.v[e]);
+            if (d) st4<VEC>(d, i, o);
+            st4<VEC>(xn, i, o);
+        },
+        [&](int64_t i) {
+            const double dn = c2 * dv(b, i);
+            if (d) d[i] = static_cast<T>(dn);
+            xn[i] = static_cast<T>(dn);
+        });
 }
 
 // Red-black Gauss-Seidel half sweep on one colour (poisson_mg.hpp:107-124):
@@ -94,30 +112,32 @@ __global__ __launch_bounds__(kBlock) void k_rbgs(int n, double inv, int color, c
 }
 
 // 27-point full weighting, coarse I <-> fine 2I+1 (poisson_mg.hpp:152-188).
+// One thread per coarse node, 64 (K) x 4 (J) threads per block, grid z = I.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_restrict(int nf, int nc, const T* r, T* bc) {
+__global__ __launch_bounds__(256) void k_restrict(int nf, int nc, const T* r, T* bc) {
 #pragma clang fp contract(off)
     const double w[3] = {0.25, 0.5, 0.25};
-    const int64_t C1 = nc, C2 = C1 * C1, NC = C2 * C1, F1 = nf, F2 = F1 * F1;
-    for (int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x; id < NC;
-         id += (int64_t)gridDim.x * kBlock) {
-        const int K = (int)(id % C1), J = (int)((id / C1) % C1), I = (int)(id / C2);
-        double sum = 0.0;
-        for (int a = 0; a < 3; ++a) {
-            const int fi = 2 * I + a;
-            if (fi >= nf) continue;
-            for (int bb = 0; bb < 3; ++bb) {
-                const int fj = 2 * J + bb;
-                if (fj >= nf) continue;
-                for (int cc = 0; cc < 3; ++cc) {
-                    const int fk = 2 * K + cc;
-                    if (fk >= nf) continue;
-                    sum += w[a] * w[bb] * w[cc] * dv(r, (int64_t)fi * F2 + (int64_t)fj * F1 + fk);
-                }
+    const int K = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int I = blockIdx.z;
+    if (K >= nc || J >= nc) return;
+    const int64_t F1 = nf, F2 = F1 * F1;
+    double sum = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const int fi = 2 * I + a;
+        if (fi >= nf) continue;
+        for (int bb = 0; bb < 3; ++bb) {
+            const int fj = 2 * J + bb;
+            if (fj >= nf) continue;
+            const int64_t row = (int64_t)fi * F2 + (int64_t)fj * F1;
+            for (int cc = 0; cc < 3; ++cc) {
+                const int fk = 2 * K + cc;
+                if (fk >= nf) continue;
+                sum += w[a] * w[bb] * w[cc] * dv(r, row + fk);
             }
         }
-        bc[id] = static_cast<T>(sum);
     }
+    bc[((int64_t)I * nc + J) * nc + K] = static_cast<T>(sum);
 }
 
 // 1D trilinear terms of a fine index (poisson_mg.hpp:191-220).
@@ -142,26 +162,71 @@ __device__ inline int interp_terms(int f, int nc, int ci[2], double wi[2]) {
     return cnt;
 }
 
-// x_f += P x_c (poisson_mg.hpp:222-260).
+// x_f += P x_c (poisson_mg.hpp:222-260).  One thread per cell (I,J,K) that
+// covers fine nodes {2I,2I+1} x {2J,2J+1} x {2K,2K+1}: those depend only on the
+// coarse nodes {I-1,I}^3.  Per axis an odd fine index takes coarse I with
+// weight 1, an even one takes I-1 (if I >= 1) and I (if I < nc) with weight
+// 1/2; terms are summed in the reference's order (zero-weight terms add 0).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_prolong_add(int nf, int nc, const T* xc, T* xf) {
+__global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc, T* xf) {
 #pragma clang fp contract(off)
-    const int64_t F1 = nf, F2 = F1 * F1, NF = F2 * F1, C1 = nc, C2 = C1 * C1;
-    for (int64_t id = (int64_t)blockIdx.x * kBlock + threadIdx.x; id < NF;
-         id += (int64_t)gridDim.x * kBlock) {
-        const int k = (int)(id % F1), j = (int)((id / F1) % F1), i = (int)(id / F2);
-        int ci[2], cj[2], ck[2];
-        double wi[2], wj[2], wk[2];
-        const int ni = interp_terms(i, nc, ci, wi);
-        const int nj = interp_terms(j, nc, cj, wj);
-        const int nk = interp_terms(k, nc, ck, wk);
-        double sum = 0.0;
-        for (int a = 0; a < ni; ++a)
-            for (int bb = 0; bb < nj; ++bb)
-                for (int cc = 0; cc < nk; ++cc)
-                    sum += wi[a] * wj[bb] * wk[cc] *
-                           dv(xc, (int64_t)ci[a] * C2 + (int64_t)cj[bb] * C1 + ck[cc]);
-        xf[id] = static_cast<T>(dv(xf, id) + sum);
+    const int K = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int I = blockIdx.z;
+    const int ncell = (nf + 1) / 2;
+    if (K >= ncell || J >= ncell) return;
+    const int64_t C1 = nc, C2 = C1 * C1, F1 = nf, F2 = F1 * F1;
+    // coarse block c[a][b][e] at (I-1+a, J-1+b, K-1+e), zero where invalid
+    double c[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int ci = I - 1 + a, cj = J - 1 + b, ck = K - 1 + e;
+                const bool ok = ci >= 0 && ci < nc && cj >= 0 && cj < nc && ck >= 0 && ck < nc;
+                c[a][b][e] = ok ? dv(xc, ci * C2 + cj * C1 + ck) : 0.0;
+            }
+    // weights: [parity][slot] for slot 0 = coarse I-1, 1 = coarse I
+    auto wts = [nc](int idx, double w[2]) {
+        // even fine index 2*idx
+        w[0] = idx >= 1 ? 0.5 : 0.0;
+        w[1] = idx < nc ? 0.5 : 0.0;
+    };
+    double we_i[2], we_j[2], we_k[2];
+    wts(I, we_i);
+    wts(J, we_j);
+    wts(K, we_k);
+    const double wo[2] = {0.0, 1.0};
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
+        const int fi = 2 * I + pi;
+        if (fi >= nf) continue;
+        const double* wi = pi ? wo : we_i;
+#pragma unroll
+        for (int pj = 0; pj < 2; ++pj) {
+            const int fj = 2 * J + pj;
+            if (fj >= nf) continue;
+            const double* wj = pj ? wo : we_j;
+#pragma unroll
+            for (int pk = 0; pk < 2; ++pk) {
+                const int fk = 2 * K + pk;
+                if (fk >= nf) continue;
+                const double* wk = pk ? wo : we_k;
+                double sum = 0.0;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int e = 0; e < 2; ++e)
+                            if (wi[a] != 0.0 && wj[b] != 0.0 && wk[e] != 0.0)
+                                sum += wi[a] * wj[b] * wk[e] * c[a][b][e];
+                const int64_t id = (int64_t)fi * F2 + (int64_t)fj * F1 + fk;
+                xf[id] = static_cast<T>(dv(xf, id) + sum);
+            }
+        }
     }
 }
 
@@ -383,9 +448,7 @@ public:
         F.b = const_cast<T*>(r);
         cycle(0);
         const int64_t N = F.n * F.n * F.n;
-        hipLaunchKernelGGL((k_convert<T, T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c_.stream,
-                           N, F.x, z);
-        NUMA_HIP_LAUNCH_CHECK();
+        launch_convert<T, T>(c_, N, F.x, z);
         F.b = nullptr;
     }
 
@@ -429,12 +492,28 @@ private:
     void poly_step(Level& L, double c1, double c2, bool use_d, bool write_d, bool from_zero) {
         const int64_t N = L.n * L.n * L.n;
         if (from_zero) {
-            hipLaunchKernelGGL((k_poly_first<T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
-                               c_.stream, N, c2, (const T*)L.b, write_d ? L.d : (T*)nullptr, L.x2);
+            T* d = write_d ? L.d : (T*)nullptr;
+            if (aligned4(L.b, sizeof(T)) && aligned4(d, sizeof(T)) && aligned4(L.x2, sizeof(T)))
+                hipLaunchKernelGGL((k_poly_first<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
+                                   c_.stream, N, c2, (const T*)L.b, d, L.x2);
+            else
+                hipLaunchKernelGGL((k_poly_first<false, T>), dim3(stream_blocks(N)), dim3(kBlock),
+                                   0, c_.stream, N, c2, (const T*)L.b, d, L.x2);
             NUMA_HIP_LAUNCH_CHECK();
         } else {
-            EpPoly<T> ep{L.h2inv, c1, c2, L.b, L.d, L.x2, L.x, use_d, write_d};
-            launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, ep, Epilogue{});
+            const LdVec<T> ld{L.x};
+            if (use_d && write_d)
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, true>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                                      Epilogue{});
+            else if (use_d)
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, false>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                                      Epilogue{});
+            else if (write_d)
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, true>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                                      Epilogue{});
+            else
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, false>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                                      Epilogue{});
         }
         std::swap(L.x, L.x2);
     }
@@ -507,13 +586,21 @@ private:
         launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, EpResid<T>{L.h2inv, L.b, L.r}, Epilogue{});
         Level& C = levels_[l + 1];
         const int64_t NC = C.n * C.n * C.n, NF = L.n * L.n * L.n;
-        hipLaunchKernelGGL((k_restrict<T>), dim3(stream_blocks(NC)), dim3(kBlock), 0, c_.stream,
-                           (int)L.n, (int)C.n, (const T*)L.r, C.b);
+        hipLaunchKernelGGL((k_restrict<T>),
+                           dim3((unsigned)((C.n + 63) / 64), (unsigned)((C.n + 3) / 4), (unsigned)C.n),
+                           dim3(256), 0, c_.stream, (int)L.n, (int)C.n, (const T*)L.r, C.b);
         NUMA_HIP_LAUNCH_CHECK();
+        (void)NC;
         cycle(l + 1);
-        hipLaunchKernelGGL((k_prolong_add<T>), dim3(stream_blocks(NF)), dim3(kBlock), 0, c_.stream,
-                           (int)L.n, (int)C.n, (const T*)C.x, L.x);
-        NUMA_HIP_LAUNCH_CHECK();
+        {
+            const int64_t ncell = (L.n + 1) / 2;
+            hipLaunchKernelGGL((k_prolong_add<T>),
+                               dim3((unsigned)((ncell + 63) / 64), (unsigned)((ncell + 3) / 4),
+                                    (unsigned)ncell),
+                               dim3(256), 0, c_.stream, (int)L.n, (int)C.n, (const T*)C.x, L.x);
+            NUMA_HIP_LAUNCH_CHECK();
+            (void)NF;
+        }
         smooth(L, (int)cfg_.post, 1, false);
     }
 

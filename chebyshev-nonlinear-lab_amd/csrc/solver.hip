@@ -142,14 +142,14 @@ public:
                         const FDStep stp{std::sqrt(std::numeric_limits<double>::epsilon()),
                                          MISC(S_XNORM), MISC(S_VNORM)};
                         launch_stencil<true>(c_, n_, LdXpHV<T>{x2_, zb_, stp, 0.0},
-                                             EpJvFD<T>{h2inv_, lambda_, stp, S(0), Fx_, V_[1],
-                                                       V_[0], nullptr, 0.0, 1.0},
+                                             EpJvFD<T, true, false>{h2inv_, lambda_, stp, S(0), Fx_,
+                                                                    V_[1], V_[0], nullptr, 0.0, 1.0},
                                              Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
                         return N * (8.0 + 8.0 + 3.0 * st);  // x, F0, v, out, V0
                     }
                     launch_stencil<true>(c_, n_, LdVec<T>{zb_},
-                                         EpJvAn<T>{h2inv_, lambda_, S(0), x2_, V_[1], V_[0],
-                                                   nullptr, 1.0},
+                                         EpJvAn<T, true, false>{h2inv_, lambda_, S(0), x2_, V_[1],
+                                                                V_[0], nullptr, 1.0},
                                          Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
                     return N * (8.0 + 3.0 * st);
                 }
@@ -164,11 +164,8 @@ public:
                     require(mg_ != nullptr, "no multigrid in this solver");
                     return mg_->smooth_step_for_timing();
                 case NUMA_PRIM_MGS:
-                    hipLaunchKernelGGL((k_mgs<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0,
-                                       c_.stream, N_, V_[2], (const T*)V_[0], (const double*)H(0, 0),
-                                       (const double*)S(0), (const T*)V_[1], c_.red,
-                                       Epilogue{EPI_STORE, 1.0, nullptr, MISC(S_TMP), nullptr});
-                    NUMA_HIP_LAUNCH_CHECK();
+                    launch_mgs<T>(c_, N_, V_[2], V_[0], H(0, 0), S(0), V_[1],
+                                  Epilogue{EPI_STORE, 1.0, nullptr, MISC(S_TMP), nullptr});
                     return 4.0 * N * st;
                 default:
                     throw Error(NUMA_ERR_ARG, "unknown primitive");
@@ -235,9 +232,16 @@ private:
     // if wout, w; ||F|| -> S_FNORM.
     void eval_F(const double* x, const double* z, double a, const double* u, double b, double* F,
                 double* wout) {
-        launch_stencil<true>(c_, n_, LdStep<double>{x, z, a, u, b},
-                             EpF{h2inv_, lambda_, F, wout},
-                             Epilogue{EPI_SQRT, 1.0, nullptr, MISC(S_FNORM), nullptr});
+        const Epilogue ep{EPI_SQRT, 1.0, nullptr, MISC(S_FNORM), nullptr};
+        if (!z && !u)
+            launch_stencil<true>(c_, n_, LdVec<double>{x}, EpF<false>{h2inv_, lambda_, F, nullptr},
+                                 ep);
+        else if (!u)
+            launch_stencil<true>(c_, n_, LdStep<double, true, false>{x, z, a, nullptr, 0.0},
+                                 EpF<true>{h2inv_, lambda_, F, wout}, ep);
+        else
+            launch_stencil<true>(c_, n_, LdStep<double, true, true>{x, z, a, u, b},
+                                 EpF<true>{h2inv_, lambda_, F, wout}, ep);
     }
 
     // Everything that depends only on the Newton point x: ||x|| for the FD
@@ -252,25 +256,46 @@ private:
     }
 
     // out = scale * J(x) v  [or b - J(x) v]; reduction epilogue ep if reducing.
+    // Exactly one of dotv (reduce out*dotv) / b (out = b - J v, reduce out^2)
+    // is set when the launch reduces.
     template <class TV>
     void jv(const double* x, const double* F0, const TV* v, T* out, const double* scale,
-            const T* dotv, const double* b, const Epilogue& ep, bool reduce) {
+            const T* dotv, const double* b, const Epilogue& ep) {
         if (cfg_.jv_mode == NUMA_DERIV_FD) {
             norm(v, MISC(S_VNORM));
             const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), MISC(S_XNORM),
                             MISC(S_VNORM)};
-            LdXpHV<TV> L{x, v, st, 0.0};
-            EpJvFD<T> E{h2inv_, lambda_, st, scale, F0, out, dotv, b, 0.0, 1.0};
-            if (reduce)
-                launch_stencil<true>(c_, n_, L, E, ep);
+            const LdXpHV<TV> L{x, v, st, 0.0};
+            if (dotv)
+                launch_stencil<true>(c_, n_, L,
+                                     EpJvFD<T, true, false>{h2inv_, lambda_, st, scale, F0, out,
+                                                            dotv, nullptr, 0.0, 1.0},
+                                     ep);
+            else if (b)
+                launch_stencil<true>(c_, n_, L,
+                                     EpJvFD<T, false, true>{h2inv_, lambda_, st, scale, F0, out,
+                                                            nullptr, b, 0.0, 1.0},
+                                     ep);
             else
-                launch_stencil<false>(c_, n_, L, E, ep);
+                launch_stencil<false>(c_, n_, L,
+                                      EpJvFD<T, false, false>{h2inv_, lambda_, st, scale, F0, out,
+                                                              nullptr, nullptr, 0.0, 1.0},
+                                      ep);
         } else {
-            EpJvAn<T> E{h2inv_, lambda_, scale, x, out, dotv, b, 1.0};
-            if (reduce)
-                launch_stencil<true>(c_, n_, LdVec<TV>{v}, E, ep);
+            const LdVec<TV> L{v};
+            if (dotv)
+                launch_stencil<true>(
+                    c_, n_, L,
+                    EpJvAn<T, true, false>{h2inv_, lambda_, scale, x, out, dotv, nullptr, 1.0}, ep);
+            else if (b)
+                launch_stencil<true>(
+                    c_, n_, L,
+                    EpJvAn<T, false, true>{h2inv_, lambda_, scale, x, out, nullptr, b, 1.0}, ep);
             else
-                launch_stencil<false>(c_, n_, LdVec<TV>{v}, E, ep);
+                launch_stencil<false>(
+                    c_, n_, L,
+                    EpJvAn<T, false, false>{h2inv_, lambda_, scale, x, out, nullptr, nullptr, 1.0},
+                    ep);
         }
     }
 
@@ -282,9 +307,7 @@ private:
                 mg_->apply(in, out);
                 return out;
             case NUMA_PRECOND_JACOBI:
-                hipLaunchKernelGGL((k_diag_solve<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0,
-                                   c_.stream, N_, in, (const double*)diag_, out);
-                NUMA_HIP_LAUNCH_CHECK();
+                launch_diag_solve<T>(c_, N_, in, diag_, out);
                 return out;
             default:
                 return in;
@@ -383,7 +406,7 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             first = false;
         } else {
             jv<double>(x0, F0, xl, V_[0], nullptr, nullptr, b,
-                       Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)}, true);
+                       Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)});
             ++res.matvecs;
         }
         beta = read1(MISC(S_BETA));
@@ -407,17 +430,14 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             ++res.precs;
             // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
             jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
-                  Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr}, true);
+                  Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
             ++res.matvecs;
             for (size_t i = 0; i <= j; ++i) {
                 const bool last = i == j;
                 const Epilogue ep = last ? Epilogue{EPI_NORM_SCALE, 1.0, nullptr, H(j + 1, j), S(j + 1)}
                                          : Epilogue{EPI_STORE, 1.0, S(i + 1), H(i + 1, j), nullptr};
-                hipLaunchKernelGGL((k_mgs<T>), dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
-                                   N_, V_[j + 1], (const T*)V_[i], (const double*)H(i, j),
-                                   (const double*)S(i), last ? (const T*)nullptr : (const T*)V_[i + 1],
-                                   c_.red, ep);
-                NUMA_HIP_LAUNCH_CHECK();
+                launch_mgs<T>(c_, N_, V_[j + 1], V_[i], H(i, j), S(i),
+                              last ? (const T*)nullptr : (const T*)V_[i + 1], ep);
             }
             readn(H(0, j), (int64_t)j + 2, col.data());
             for (size_t i = 0; i <= j + 1; ++i) Hh[i][j] = col[i];

@@ -3,9 +3,10 @@
 // Layout: interior nodes only, id = (i*n + j)*n + k, k fastest (the reference's
 // layout, problems/bratu3d.hpp:85), zero Dirichlet values outside.
 //
-// One generic driver k_stencil: a block is 64 (k) x 4 (j) threads; each thread
-// owns one (j,k) column of a slab of `ichunk` planes and marches along i,
-// keeping the i-1 / i / i+1 values of the field in registers.  The j+-1 and
+// One generic driver k_stencil: a block is 64 (k) x 4 (j) threads (32 x 8 for
+// n <= 32); each thread owns one (j,k) column of a slab of `ichunk` planes and
+// marches along i, keeping the i-1 / i / i+1 values of the field in registers
+// and prefetching the next plane's loads before computing the current one.  The j+-1 and
 // k+-1 neighbours are re-read and hit the CU's L1 / the XCD's L2 (the
 // neighbouring waves and blocks stream the same rows), so HBM traffic stays
 // close to one read per input field.  A Loader gives the field value(s) at a
@@ -49,55 +50,100 @@ __device__ inline double lap_num(double c, double im, double ip, double jm, doub
     return s;
 }
 
-template <int BLOCK, bool REDUCE, class Load, class Epi>
-__global__ __launch_bounds__(BLOCK) void k_stencil(int n, int ichunk, Load ldr, Epi epi,
-                                                    Reduce red, Epilogue ep) {
+// Loader:   Raw load(id) issues the loads of one node (no branches),
+//           Val make(raw) combines them into the field value(s).
+// Epilogue: Extra preload(id) issues the node's other loads (F0, b, d, ...),
+//           operator()(id, extra, centre, 6 neighbours) computes, stores and
+//           returns the node's reduction contribution.
+// Out-of-domain neighbours are loaded from a valid address (the centre) and
+// replaced by zero with a select, so every load is unconditional and the
+// whole next plane is in flight while the current one is computed.
+template <int TK, bool REDUCE, class Load, class Epi>
+__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Epi epi, Reduce red,
+                                                  Epilogue ep) {
+    constexpr int TJ = 256 / TK;
     const Load L = ldr.prep();
     const Epi E = epi.prep();
-    using P = decltype(L(int64_t(0)));
-    const int tk = threadIdx.x % kTK, tj = threadIdx.x / kTK;
-    const int k = blockIdx.x * kTK + tk, j = blockIdx.y * kTJ + tj;
+    using P = decltype(L.make(L.load(int64_t(0))));
+    using X = decltype(E.preload(int64_t(0)));
+    const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
+    const int k = blockIdx.x * TK + tk, j = blockIdx.y * TJ + tj;
     const int i0 = blockIdx.z * ichunk;
     const int i1 = min(n, i0 + ichunk);
     double acc = 0.0;
     if (k < n && j < n && i0 < n) {
         const int64_t N1 = n, N2 = N1 * N1;
         int64_t id = ((int64_t)i0 * N1 + j) * N1 + k;
-        P pm = i0 > 0 ? L(id - N2) : P::zero();
-        P pc = L(id);
+        const bool hj0 = j > 0, hj1 = j + 1 < n, hk0 = k > 0, hk1 = k + 1 < n;
+        auto fetch = [&](int64_t idx, int64_t safe, bool ok) -> P {
+            const P v = L.make(L.load(ok ? idx : safe));
+            return ok ? v : P::zero();
+        };
+        P pm = fetch(id - N2, id, i0 > 0);
+        P pc = L.make(L.load(id));
+        P pp = fetch(id + N2, id, i0 + 1 < n);
+        P jm = fetch(id - N1, id, hj0);
+        P jp = fetch(id + N1, id, hj1);
+        P km = fetch(id - 1, id, hk0);
+        P kp = fetch(id + 1, id, hk1);
+        X ex = E.preload(id);
         for (int i = i0; i < i1; ++i, id += N2) {
-            const P pp = (i + 1 < n) ? L(id + N2) : P::zero();
-            const P jm = j > 0 ? L(id - N1) : P::zero();
-            const P jp = (j + 1 < n) ? L(id + N1) : P::zero();
-            const P km = k > 0 ? L(id - 1) : P::zero();
-            const P kp = (k + 1 < n) ? L(id + 1) : P::zero();
-            acc += E(id, pc, pm, pp, jm, jp, km, kp);
+            // prefetch plane i+1 (its centre pp is already resident)
+            const bool more = i + 1 < i1;
+            const int64_t nx = more ? id + N2 : id;
+            const P qp = fetch(nx + N2, nx, more && i + 2 < n);
+            const P qjm = fetch(nx - N1, nx, hj0);
+            const P qjp = fetch(nx + N1, nx, hj1);
+            const P qkm = fetch(nx - 1, nx, hk0);
+            const P qkp = fetch(nx + 1, nx, hk1);
+            const X qex = E.preload(nx);
+            acc += E(id, ex, pc, pm, pp, jm, jp, km, kp);
             pm = pc;
             pc = pp;
+            pp = qp;
+            jm = qjm;
+            jp = qjp;
+            km = qkm;
+            kp = qkp;
+            ex = qex;
         }
     }
     if constexpr (REDUCE) {
-        const double t = block_sum<BLOCK>(acc);
+        const double t = block_sum<256>(acc);
         const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        reduce_finish<BLOCK>(t, red, ep, gridDim.x * gridDim.y * gridDim.z, bid);
+        reduce_finish<256>(t, red, ep, gridDim.x * gridDim.y * gridDim.z, bid);
     }
 }
 
-// Planes per thread such that the grid has at most kMaxPartials blocks.
-inline int stencil_ichunk(int64_t n) {
-    int ich = kIChunk;
-    const int64_t gxy = ((n + kTK - 1) / kTK) * ((n + kTJ - 1) / kTJ);
-    while (gxy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
-    return ich;
+// Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below; planes per thread
+// chosen so large grids keep a long register queue and small grids still get
+// enough blocks to fill the chip; at most kMaxPartials blocks.
+struct StencilShape {
+    int tk, ichunk;
+    dim3 grid;
+};
+
+inline StencilShape stencil_shape(int64_t n) {
+    StencilShape s;
+    s.tk = n > 32 ? 64 : 32;
+    const int tj = 256 / s.tk;
+    const int64_t gx = (n + s.tk - 1) / s.tk, gy = (n + tj - 1) / tj;
+    int ich = n >= 192 ? 8 : (n >= 96 ? 2 : 1);
+    while (gx * gy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
+    s.ichunk = ich;
+    s.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)((n + ich - 1) / ich));
+    return s;
 }
 
 template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
-    const int ich = stencil_ichunk(n);
-    const dim3 grid((unsigned)((n + kTK - 1) / kTK), (unsigned)((n + kTJ - 1) / kTJ),
-                    (unsigned)((n + ich - 1) / ich));
-    hipLaunchKernelGGL((k_stencil<kTK * kTJ, REDUCE, Load, Epi>), grid, dim3(kTK * kTJ), 0,
-                       c.stream, (int)n, ich, ldr, epi, c.red, ep);
+    const StencilShape s = stencil_shape(n);
+    if (s.tk == 64)
+        hipLaunchKernelGGL((k_stencil<64, REDUCE, Load, Epi>), s.grid, dim3(256), 0, c.stream,
+                           (int)n, s.ichunk, ldr, epi, c.red, ep);
+    else
+        hipLaunchKernelGGL((k_stencil<32, REDUCE, Load, Epi>), s.grid, dim3(256), 0, c.stream,
+                           (int)n, s.ichunk, ldr, epi, c.red, ep);
     NUMA_HIP_LAUNCH_CHECK();
 }
 
@@ -120,28 +166,42 @@ struct FDStep {
 // ---------------------------------------------------------------------------
 // Loaders
 // ---------------------------------------------------------------------------
+struct NoExtra {};
+
 template <class T>
 struct LdVec {  // w = v
     const T* v;
     __device__ LdVec prep() const { return *this; }
-    __device__ P1 operator()(int64_t id) const { return {ld(v, id)}; }
+    __device__ T load(int64_t id) const { return v[id]; }
+    __device__ P1 make(T r) const { return {static_cast<double>(r)}; }
 };
 
 // w = (x - a z) - b u   (Newton trial x - lambda z, krylov_newton.hpp:99;
-// Chebyshev step x - z - u/2, :166).  z and u may be null.
-template <class TZ>
+// Chebyshev step x - z - u/2, :166); HZ / HU select the terms at compile time.
+template <class TZ, bool HZ, bool HU>
 struct LdStep {
     const double* x;
     const TZ* z;
     double a;
     const double* u;
     double b;
+    struct Raw {
+        double x;
+        TZ z;
+        double u;
+    };
     __device__ LdStep prep() const { return *this; }
-    __device__ P1 operator()(int64_t id) const {
+    __device__ Raw load(int64_t id) const {
+        Raw r{x[id], TZ(0), 0.0};
+        if constexpr (HZ) r.z = z[id];
+        if constexpr (HU) r.u = u[id];
+        return r;
+    }
+    __device__ P1 make(const Raw& r) const {
 #pragma clang fp contract(off)
-        double w = x[id];
-        if (z) w = w - a * ld(z, id);
-        if (u) w = w - b * u[id];
+        double w = r.x;
+        if constexpr (HZ) w = w - a * static_cast<double>(r.z);
+        if constexpr (HU) w = w - b * r.u;
         return {w};
     }
 };
@@ -153,14 +213,19 @@ struct LdXpHV {
     const TV* v;
     FDStep step;
     double h;
+    struct Raw {
+        double x;
+        TV v;
+    };
     __device__ LdXpHV prep() const {
         LdXpHV r = *this;
         r.h = step.h();
         return r;
     }
-    __device__ P1 operator()(int64_t id) const {
+    __device__ Raw load(int64_t id) const { return {x[id], v[id]}; }
+    __device__ P1 make(const Raw& r) const {
 #pragma clang fp contract(off)
-        return {x[id] + h * ld(v, id)};
+        return {r.x + h * static_cast<double>(r.v)};
     }
 };
 
@@ -171,44 +236,58 @@ struct LdPM {
     const TV* v;
     FDStep step;
     double h;
+    struct Raw {
+        double x;
+        TV v;
+    };
     __device__ LdPM prep() const {
         LdPM r = *this;
         r.h = step.h();
         return r;
     }
-    __device__ P2 operator()(int64_t id) const {
+    __device__ Raw load(int64_t id) const { return {x[id], v[id]}; }
+    __device__ P2 make(const Raw& r) const {
 #pragma clang fp contract(off)
-        const double xv = x[id], vv = ld(v, id);
-        return {xv + h * vv, xv + (-h) * vv};
+        const double vv = static_cast<double>(r.v);
+        return {r.x + h * vv, r.x + (-h) * vv};
     }
 };
 
 // ---------------------------------------------------------------------------
 // Epilogues
 // ---------------------------------------------------------------------------
-// F = h2inv*(6w - sum nbrs) - lambda e^w (bratu3d.hpp:21-31); optional write of
-// the trial point w itself; returns F^2.
+// F = h2inv*(6w - sum nbrs) - lambda e^w (bratu3d.hpp:21-31); with WOUT also
+// writes the trial point w itself; returns F^2.
+template <bool WOUT>
 struct EpF {
     double h2inv, lambda;
     double* F;
     double* wout;
     __device__ EpF prep() const { return *this; }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
+    __device__ NoExtra preload(int64_t) const { return {}; }
+    __device__ double operator()(int64_t id, NoExtra, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
                                  P1 kp) const {
 #pragma clang fp contract(off)
         double f = h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
         f -= lambda * exp(c.a);
         F[id] = f;
-        if (wout) wout[id] = c.a;
+        if constexpr (WOUT) wout[id] = c.a;
         return f * f;
     }
 };
 
-// Forward-difference J v = (F(x+hv) - F0)/h (operator.hpp:113), multiplied by
-// the Arnoldi scale *scale_ptr (1 if null), optionally as a residual b - J v.
-// Output type TO.  Returns out*dotv[id] (first MGS dot) or out^2 (residual
-// norm) or 0.
+// J v epilogue options: DOT reduces out*dotv[id] (first MGS dot), RESID writes
+// b - J v and reduces its square.
 template <class TO>
+struct JvExtra {
+    double f0;   // F0 (FD) or x (analytic)
+    TO dv;
+    double b;
+};
+
+// Forward-difference J v = (F(x+hv) - F0)/h (operator.hpp:113), multiplied by
+// the Arnoldi scale *scale_ptr (1 if null).
+template <class TO, bool DOT, bool RESID>
 struct EpJvFD {
     double h2inv, lambda;
     FDStep step;
@@ -224,28 +303,33 @@ struct EpJvFD {
         r.scale = scale_ptr ? *scale_ptr : 1.0;
         return r;
     }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
-                                 P1 kp) const {
+    __device__ JvExtra<TO> preload(int64_t id) const {
+        JvExtra<TO> e{F0[id], TO(0), 0.0};
+        if constexpr (DOT) e.dv = dotv[id];
+        if constexpr (RESID) e.b = b[id];
+        return e;
+    }
+    __device__ double operator()(int64_t id, const JvExtra<TO>& e, P1 c, P1 im, P1 ip, P1 jm,
+                                 P1 jp, P1 km, P1 kp) const {
 #pragma clang fp contract(off)
         double val = 0.0;
         if (h != 0.0) {  // ||v|| == 0 gives J v = 0 (operator.hpp:105-106)
             double f1 = h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
             f1 -= lambda * exp(c.a);
-            val = (f1 - F0[id]) / h;
+            val = (f1 - e.f0) / h;
             val = val * scale;
         }
-        if (b) val = b[id] - val;
+        if constexpr (RESID) val = e.b - val;
         const TO o = static_cast<TO>(val);
         out[id] = o;
-        if (dotv) return static_cast<double>(o) * ld(dotv, id);
-        if (b) return static_cast<double>(o) * static_cast<double>(o);
+        if constexpr (DOT) return static_cast<double>(o) * static_cast<double>(e.dv);
+        if constexpr (RESID) return static_cast<double>(o) * static_cast<double>(o);
         return 0.0;
     }
 };
 
-// Analytic J v = h2inv*(6v - sum nbrs) - lambda e^x v (bratu3d.hpp:33-43),
-// same output options as EpJvFD.
-template <class TO>
+// Analytic J v = h2inv*(6v - sum nbrs) - lambda e^x v (bratu3d.hpp:33-43).
+template <class TO, bool DOT, bool RESID>
 struct EpJvAn {
     double h2inv, lambda;
     const double* scale_ptr;
@@ -259,17 +343,23 @@ struct EpJvAn {
         r.scale = scale_ptr ? *scale_ptr : 1.0;
         return r;
     }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
-                                 P1 kp) const {
+    __device__ JvExtra<TO> preload(int64_t id) const {
+        JvExtra<TO> e{x[id], TO(0), 0.0};
+        if constexpr (DOT) e.dv = dotv[id];
+        if constexpr (RESID) e.b = b[id];
+        return e;
+    }
+    __device__ double operator()(int64_t id, const JvExtra<TO>& e, P1 c, P1 im, P1 ip, P1 jm,
+                                 P1 jp, P1 km, P1 kp) const {
 #pragma clang fp contract(off)
         double val = h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
-        val -= lambda * exp(x[id]) * c.a;
+        val -= lambda * exp(e.f0) * c.a;
         val = val * scale;
-        if (b) val = b[id] - val;
+        if constexpr (RESID) val = e.b - val;
         const TO o = static_cast<TO>(val);
         out[id] = o;
-        if (dotv) return static_cast<double>(o) * ld(dotv, id);
-        if (b) return static_cast<double>(o) * static_cast<double>(o);
+        if constexpr (DOT) return static_cast<double>(o) * static_cast<double>(e.dv);
+        if constexpr (RESID) return static_cast<double>(o) * static_cast<double>(o);
         return 0.0;
     }
 };
@@ -286,7 +376,8 @@ struct EpD2FD {
         r.h = step.h();
         return r;
     }
-    __device__ double operator()(int64_t id, P2 c, P2 im, P2 ip, P2 jm, P2 jp, P2 km,
+    __device__ double preload(int64_t id) const { return F0[id]; }
+    __device__ double operator()(int64_t id, double f0, P2 c, P2 im, P2 ip, P2 jm, P2 jp, P2 km,
                                  P2 kp) const {
 #pragma clang fp contract(off)
         if (h == 0.0) {
@@ -297,7 +388,7 @@ struct EpD2FD {
         fp -= lambda * exp(c.a);
         double fm = h2inv * lap_num(c.b, im.b, ip.b, jm.b, jp.b, km.b, kp.b);
         fm -= lambda * exp(c.b);
-        out[id] = (fp - 2.0 * F0[id] + fm) / (h * h);
+        out[id] = (fp - 2.0 * f0 + fm) / (h * h);
         return 0.0;
     }
 };
@@ -308,7 +399,8 @@ struct EpLap {
     double h2inv;
     TO* out;
     __device__ EpLap prep() const { return *this; }
-    __device__ double operator()(int64_t id, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
+    __device__ NoExtra preload(int64_t) const { return {}; }
+    __device__ double operator()(int64_t id, NoExtra, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
                                  P1 kp) const {
 #pragma clang fp contract(off)
         out[id] = static_cast<TO>(h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a));

@@ -1,0 +1,148 @@
+"""Writes the golden fixtures under tests/golden/.
+
+Run in the build container, where the reference sources exist:
+    python tests/golden/make_golden.py
+
+Two kinds of fixtures:
+  * reference-held data: numbers copied from the reference's own data files
+    (experiments/c1/internal_calibration/internal_calibration_raw.csv,
+    experiments/c1/starts/frozen/B1_start_hashes.csv,
+    experiments/c1/fold_calibration/bratu_n*_coarse.csv,
+    docs/research/R1_OPERATOR_INTERFACE.md:113-121,
+    tests/test_chandrasekhar_op.cpp:106-115);
+  * vectors computed by the reference's own header library compiled into
+    oracle/_ref (oracle/Makefile) on deterministic inputs.
+Only data is written; no reference source is copied.
+"""
+import csv
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+REF = "/root/reference"
+
+
+def calibration_rows():
+    """NK / Chebyshev / Chebyshev+recycling rows of the C1 internal
+    calibration at n = 31 from the natural start (u0 = 0)."""
+    path = os.path.join(REF, "experiments/c1/internal_calibration/internal_calibration_raw.csv")
+    out, seen = [], set()
+    for r in csv.DictReader(open(path)):
+        if r["family"] != "B1" or r["start_kind"] != "natural" or r["n"] != "31":
+            continue
+        if r["method"] not in ("nk", "cheb", "rec") or r["precond"] not in ("jacobi", "mg"):
+            continue
+        key = (r["param"], r["method"], r["precond"], r["eta_init"], r["restart"], r["maxlin"])
+        if key in seen:
+            continue
+        seen.add(key)
+        out.append({
+            "source": "experiments/c1/internal_calibration/internal_calibration_raw.csv",
+            "n": int(r["n"]), "lambda": float(r["param"]), "method": r["method"],
+            "precond": r["precond"],
+            "params": {"rtol_nl": float(r["rtol"]), "max_nl": int(r["max_nl"]),
+                       "eta_init": float(r["eta_init"]), "eta_max": float(r["eta_max"]),
+                       "gamma": float(r["gamma"]), "alpha": float(r["alpha"]),
+                       "eta2_factor": float(r["eta2_factor"]), "eta2_max": float(r["eta2_max"]),
+                       "gmres_restart": int(r["restart"]), "gmres_max_iters": int(r["maxlin"])},
+            "expect": {"iters": int(r["nonlinear_iters"]), "n_F": int(r["n_F"]),
+                       "n_JVP": int(r["n_JVP"]), "n_D2": int(r["n_D2"]),
+                       "n_lin_iters": int(r["n_lin"]), "n_lin_failures": int(r["n_lin_failures"]),
+                       "n_prec": int(r["n_prec"]), "residual_rel": float(r["residual_rel"])}})
+    return out
+
+
+def start_norms():
+    """Lower-branch solution norms ||u*(lambda)|| (natural-mode continuation)."""
+    path = os.path.join(REF, "experiments/c1/starts/frozen/B1_start_hashes.csv")
+    out = []
+    for r in csv.DictReader(open(path)):
+        if r["mode"] == "natural":
+            out.append({"n": int(r["n"]), "lambda": float(r["reference_lambda"]),
+                        "norm": float(r["reference_norm"])})
+    return out
+
+
+def spectral_gaps():
+    out = []
+    for n in (31, 63, 127):
+        path = os.path.join(REF, f"experiments/c1/fold_calibration/bratu_n{n}_coarse.csv")
+        for r in csv.DictReader(open(path)):
+            lam = float(r["lambda"])
+            if lam in (0.0, 6.5):
+                out.append({"n": n, "lambda": lam, "gap": float(r["spectral_gap"])})
+    return out
+
+
+def main():
+    ref = O.Reference()
+    data = {
+        "calibration_n31": calibration_rows(),
+        "start_norms": start_norms(),
+        "spectral_gaps": spectral_gaps(),
+        # docs/research/R1_OPERATOR_INTERFACE.md:113-121 (n = 16, lambda = 1, u0 = 0)
+        "r1_smoke_n16": {
+            "mg": {"nk": [4, 5, 11, 0, 15], "cheb": [4, 5, 20, 4, 28], "rec": [4, 5, 18, 4, 30]},
+            "jacobi": {"nk": [5, None, 74, None, 79], "cheb": [5, None, 123, None, 133],
+                       "rec": [5, None, 99, None, 112]},
+            "fields": ["iters", "n_F", "n_JVP", "n_D2", "n_prec"]},
+        # tests/test_chandrasekhar_op.cpp:106-115 (N = 8, c = 0.5, Gauss-Legendre)
+        "chandra_n8_c05": [1.021719731462, 1.073186381734, 1.125724893657, 1.169753312169,
+                           1.203071751305, 1.226490874633, 1.241524600594, 1.249448516693],
+    }
+    with open(os.path.join(HERE, "reference_data.json"), "w") as f:
+        json.dump(data, f, indent=1)
+
+    # vectors from the reference's compiled headers on deterministic inputs
+    arrays = {}
+    for n in (6, 8, 15, 16):
+        N = n ** 3
+        x = O.sin_point(N, 0.3)
+        v = O.sin_point(N, 1.0)[::-1].copy()
+        for lam in (1.0, 6.5):
+            tag = f"n{n}_l{lam}"
+            arrays[f"eval_{tag}"] = ref.bratu("eval", n, lam, x)
+            arrays[f"jvp_{tag}"] = ref.bratu("jvp", n, lam, x, v)
+            arrays[f"d2_{tag}"] = ref.bratu("d2", n, lam, x, v)
+            arrays[f"diag_{tag}"] = ref.bratu("jacobi_diag", n, lam, x)
+            arrays[f"jvpfd_{tag}"] = ref.bratu("jvp_fd", n, lam, x, v)
+            arrays[f"d2fd_{tag}"] = ref.bratu("d2_fd", n, lam, x, v)
+    for n in (7, 15, 16):
+        arrays[f"mg_n{n}"] = ref.mg_apply(n, O.sin_point(n ** 3, 1.0))
+    for n, lam, fd, pc in ((8, 1.0, 0, 2), (8, 1.0, 1, 1), (12, 2.0, 0, 1)):
+        N = n ** 3
+        xg, g = ref.bratu_gmres(n, lam, fd, pc, O.sin_point(N, 0.1), O.sin_point(N, 1.0), 1e-10,
+                                 40 if n == 8 else 10, 400)
+        tag = f"gmres_n{n}_fd{fd}_pc{pc}"
+        arrays[tag] = xg
+        arrays[tag + "_counts"] = np.array([g["iters"], g["matvecs"], g["precs"], g["converged"],
+                                            g["breakdown"]], np.float64)
+    for n, lam, method, pc, rec in ((8, 1.0, 0, 1, 0), (15, 1.0, 1, 2, 1), (16, 6.0, 1, 1, 0)):
+        p = O.krylov_params(recycle=rec)
+        xs, r = ref.bratu_solve(n, lam, method, pc, 0, p)
+        tag = f"solve_n{n}_m{method}_pc{pc}_r{rec}"
+        arrays[tag] = xs
+        arrays[tag + "_counts"] = np.array([r["iters"], r["n_F"], r["n_JVP"], r["n_D2"],
+                                            r["n_lin_iters"], r["n_prec"]], np.float64)
+    for N in (8, 24):
+        nodes, weights = ref.gauss_legendre_unit(N)
+        arrays[f"gl_nodes_{N}"] = nodes
+        arrays[f"gl_weights_{N}"] = weights
+        x = 1.0 + 0.2 * np.sin(0.37 * np.arange(N) + 0.5)
+        d = 0.5 * np.cos(0.23 * np.arange(N) + 0.1)
+        arrays[f"chandra_eval_{N}"] = ref.chandra("eval", N, 0.8, x)
+        arrays[f"chandra_jvp_{N}"] = ref.chandra("jvp", N, 0.8, x, d)
+        arrays[f"chandra_d2_{N}"] = ref.chandra("d2", N, 0.8, x, d)
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **arrays)
+    print(f"wrote {len(data['calibration_n31'])} calibration rows, {len(arrays)} arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -122,3 +122,32 @@ def test_solve_mixed_even_gmg(ctx, oracle, method):
         assert r.converged and r.res <= 1e-8 * r.res0
         assert abs(r.iters - ro.iters) <= 1
         assert rel(x, xo) < 1e-7
+
+
+def _cal_rows():
+    import json
+    import os
+    from conftest import GOLDEN
+    return json.load(open(os.path.join(GOLDEN, "reference_data.json")))["calibration_n31"]
+
+
+@pytest.mark.parametrize("row", _cal_rows(),
+                         ids=lambda r: f"{r['method']}-{r['precond']}-l{r['lambda']:.3f}-e{r['params']['eta_init']}")
+def test_gpu_reproduces_reference_calibration_rows(ctx, row):
+    """The device FP64 solver against the reference's own recorded C1 runs
+    (n = 31, u0 = 0, analytic J v; experiments/c1/internal_calibration/
+    internal_calibration_raw.csv): every counter identical, final relative
+    residual within 1e-3 (roundoff floor, amplified near the fold)."""
+    import numa_hip as H
+    p = H.krylov_params(recycle=1 if row["method"] == "rec" else 0, **row["params"])
+    cfg = H.solver_config(method=0 if row["method"] == "nk" else 1,
+                          precond=H.PRECOND_JACOBI if row["precond"] == "jacobi" else H.PRECOND_GMG,
+                          jv_mode=H.DERIV_ANALYTIC, d2_mode=H.DERIV_ANALYTIC)
+    s = H.BratuSolver(ctx, row["n"], row["lambda"], cfg, restart=p.gmres_restart)
+    x = np.zeros(row["n"] ** 3)
+    r = s.solve_host(p, x)
+    s.close()
+    e = row["expect"]
+    got = {k: getattr(r, k) for k in KEYS if k != "converged"}
+    assert got == {k: e[k] for k in got}
+    assert abs(r.res / r.res0 - e["residual_rel"]) <= 1e-3 * e["residual_rel"]

@@ -1,0 +1,335 @@
+"""CPU pins of the oracle (oracle/numa_oracle.cpp) — no GPU.
+
+The oracle is what every GPU parity test compares against, so it is pinned
+here against:
+  * the reference's own header library compiled into oracle/_ref, through the
+    fixtures tests/golden/reference_vectors.npz (bit-exact where the operation
+    order is the same);
+  * data the reference itself recorded: the C1 internal-calibration rows
+    (experiments/c1/internal_calibration/internal_calibration_raw.csv), the R1
+    smoke table (docs/research/R1_OPERATOR_INTERFACE.md:113-121), the
+    lower-branch solution norms (experiments/c1/starts/frozen/B1_start_hashes.csv),
+    the lambda = 0 spectral gaps (experiments/c1/fold_calibration), the N = 8
+    Chandrasekhar vector (tests/test_chandrasekhar_op.cpp:106-115);
+  * mathematics: Laplacian sine eigenpairs, F(0) = -lambda, the Chebyshev
+    residual polynomial, symmetry / contraction of the V-cycle by brute force,
+    dense LU against GMRES, the Chandrasekhar moment identity.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, rel
+
+REFDATA = json.load(open(os.path.join(GOLDEN, "reference_data.json")))
+VEC = np.load(os.path.join(GOLDEN, "reference_vectors.npz"))
+
+
+def inputs(n):
+    N = n ** 3
+    return O.sin_point(N, 0.3), O.sin_point(N, 1.0)[::-1].copy()
+
+
+# ---- Bratu3D operators -------------------------------------------------------
+@pytest.mark.parametrize("n", [6, 8, 16])
+@pytest.mark.parametrize("lam", [1.0, 6.5])
+def test_bratu_ops_match_reference_bitwise(oracle, n, lam):
+    x, v = inputs(n)
+    tag = f"n{n}_l{lam}"
+    assert np.array_equal(oracle.bratu_eval(n, lam, x), VEC[f"eval_{tag}"])
+    assert np.array_equal(oracle.bratu_jvp(n, lam, x, v), VEC[f"jvp_{tag}"])
+    assert np.array_equal(oracle.bratu_d2(n, lam, x, v), VEC[f"d2_{tag}"])
+    assert np.array_equal(oracle.bratu_jacobi_diag(n, lam, x), VEC[f"diag_{tag}"])
+    # FD quotients: same formulas; the reference's OpenMP norm2 may order its sum
+    # differently, which moves h by an ulp
+    assert rel(oracle.bratu_jvp_fd(n, lam, x, v), VEC[f"jvpfd_{tag}"]) < 1e-9
+    assert rel(oracle.bratu_d2_fd(n, lam, x, v), VEC[f"d2fd_{tag}"]) < 1e-7
+
+
+@pytest.mark.parametrize("n", [4, 7, 8])
+def test_laplacian_sine_eigenpairs(oracle, n):
+    """phi_pqr = sin(p pi (i+1)/(n+1)) ... is an eigenvector of -Lap_h with
+    eigenvalue 4(n+1)^2 sum sin^2(p pi / (2(n+1)))."""
+    idx = np.arange(1, n + 1)
+    h2inv = (n + 1.0) ** 2
+    for p, q, r in ((1, 1, 1), (2, 1, 3), (n, n, n), (1, n, 2)):
+        s = [np.sin(m * np.pi * idx / (n + 1)) for m in (p, q, r)]
+        phi = np.einsum("i,j,k->ijk", *s).ravel()
+        mu = 4 * h2inv * sum(np.sin(m * np.pi / (2 * (n + 1))) ** 2 for m in (p, q, r))
+        assert rel(oracle.poisson_apply(n, phi), mu * phi) < 1e-13
+        # F(u) with lambda = 0 is the same operator
+        assert rel(oracle.bratu_eval(n, 0.0, phi), mu * phi) < 1e-13
+
+
+def test_F_at_zero_and_lambda0_gap(oracle):
+    for n in (5, 16):
+        assert np.array_equal(oracle.bratu_eval(n, 6.5, np.zeros(n ** 3)), np.full(n ** 3, -6.5))
+    # the fold CSVs' lambda = 0 spectral gap is the smallest Laplacian eigenvalue
+    for row in REFDATA["spectral_gaps"]:
+        if row["lambda"] == 0.0:
+            n = row["n"]
+            mu = 12 * (n + 1.0) ** 2 * np.sin(np.pi / (2 * (n + 1))) ** 2
+            assert abs(mu - row["gap"]) < 5e-4
+
+
+def test_jvp_fd_and_d2_fd_against_analytic(oracle):
+    """test_operators.cpp:39-61: analytic vs FD at n = 6, lambda = 1."""
+    n = 6
+    x = O.sin_point(n ** 3, 0.3)
+    assert rel(oracle.bratu_jvp_fd(n, 1.0, x, O.sin_point(n ** 3, 1.0)),
+               oracle.bratu_jvp(n, 1.0, x, O.sin_point(n ** 3, 1.0))) < 1e-6
+    v = O.sin_point(n ** 3, 0.5)
+    assert rel(oracle.bratu_d2_fd(n, 1.0, x, v), oracle.bratu_d2(n, 1.0, x, v)) < 1e-4
+
+
+def test_dot_exact_on_integers(oracle):
+    a = np.arange(1, 100001, dtype=np.float64)
+    assert oracle.dot(a, a) == float(sum(int(i) * int(i) for i in range(1, 100001)))
+
+
+# ---- multigrid -----------------------------------------------------------------
+def test_reference_mg_levels_and_vcycle(oracle):
+    assert oracle.mg_levels(15) == [15, 7, 3, 1]  # test_operators.cpp:193-194
+    assert oracle.mg_levels(16) == [16]           # poisson_mg.hpp:60-62
+    for n in (7, 15, 16):
+        z = oracle.mg_apply(n, O.sin_point(n ** 3, 1.0))
+        assert rel(z, VEC[f"mg_n{n}"]) < 1e-14
+    # one V-cycle halves the residual (test_operators.cpp:189-205)
+    b = O.sin_point(15 ** 3, 1.0)
+    z = oracle.mg_apply(15, b)
+    assert np.linalg.norm(oracle.poisson_apply(15, z) - b) < 0.5 * np.linalg.norm(b)
+
+
+def cheb_cfg(levels=0, **kw):
+    return O.mg_config(smoother=O.SMOOTH_CHEBYSHEV, pre=2, post=2, degree=3,
+                       max_levels=levels, min_coarse_n=2, coarse_iters=24, even_coarsening=1,
+                       cheb_ratio=4.0, **kw)
+
+
+def wj_cfg(levels=0):
+    return O.mg_config(smoother=O.SMOOTH_WJACOBI, pre=2, post=2, max_levels=levels,
+                       min_coarse_n=2, coarse_iters=24, even_coarsening=1, omega=6 / 7)
+
+
+def test_even_hierarchy_sizes(oracle):
+    assert oracle.mg_levels(256, cheb_cfg(5)) == [256, 128, 64, 32, 16]
+    assert oracle.mg_levels(32, cheb_cfg(0)) == [32, 16, 8, 4, 2]
+    assert oracle.mg_levels(33, cheb_cfg(0)) == [33, 16, 8, 4, 2]
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 24])
+def test_chebyshev_residual_polynomial(oracle, k):
+    """Single-level cycle = Chebyshev semi-iteration over the exact spectrum of
+    D^-1 A from x = 0; on a sine eigenvector with D^-1 A eigenvalue t the
+    residual is T_k((theta - t)/delta) / T_k(theta/delta) times the vector."""
+    n = 6
+    idx = np.arange(1, n + 1)
+    h2inv = (n + 1.0) ** 2
+    lo, hi = 1 - np.cos(np.pi / (n + 1)), 1 + np.cos(np.pi / (n + 1))
+    theta, delta = (hi + lo) / 2, (hi - lo) / 2
+    cfg = cheb_cfg(1, )
+    cfg.coarse_iters = k
+    for p, q, r in ((1, 1, 1), (3, 2, 5), (6, 6, 6)):
+        phi = np.einsum("i,j,k->ijk", *[np.sin(m * np.pi * idx / (n + 1)) for m in (p, q, r)]).ravel()
+        t = (4 * h2inv * sum(np.sin(m * np.pi / (2 * (n + 1))) ** 2 for m in (p, q, r))) / (6 * h2inv)
+        z = oracle.mg_apply(n, phi, cfg)
+        res = phi - oracle.poisson_apply(n, z)
+        Tk = np.polynomial.chebyshev.Chebyshev.basis(k)
+        expect = Tk((theta - t) / delta) / Tk(theta / delta)
+        assert rel(res, expect * phi) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["reference", "wjacobi", "cheb3"])
+def test_vcycle_symmetric_and_contracting(oracle, name):
+    """The V-cycle M is a fixed symmetric operator (restriction = P^T/8,
+    symmetric smoothing); by brute force at n = 8 (even) or 7 (reference),
+    ||I - M A|| in the A-norm is well below 1."""
+    n = 7 if name == "reference" else 8
+    cfg = {"reference": O.mg_reference(), "wjacobi": wj_cfg(), "cheb3": cheb_cfg()}[name]
+    N = n ** 3
+    M = np.stack([oracle.mg_apply(n, e, cfg) for e in np.eye(N)], axis=1)
+    A = np.stack([oracle.poisson_apply(n, e) for e in np.eye(N)], axis=1)
+    assert np.allclose(M, M.T, atol=1e-12 * np.abs(M).max())
+    # error propagation E = I - M A in the A-inner product: spectral radius
+    L = np.linalg.cholesky(A)
+    E = np.eye(N) - M @ A
+    S = L.T @ E @ np.linalg.inv(L.T)
+    rho = np.max(np.abs(np.linalg.eigvals(S)))
+    assert rho < {"reference": 0.2, "wjacobi": 0.35, "cheb3": 0.1}[name]
+
+
+def test_mg_gmres_iterations_bounded(oracle):
+    """test_operators.cpp:207-233 for the reference, and the even-n Chebyshev
+    hierarchy at 16, 32, 64."""
+    its = []
+    for n in (15, 31):
+        _, r = oracle.poisson_gmres(n, O.sin_point(n ** 3, 1.0), 1e-8, 50, 200, O.mg_reference())
+        assert r.converged
+        its.append(r.iters)
+    assert its[1] <= its[0] + 3
+    its = []
+    for n in (16, 32, 64):
+        _, r = oracle.poisson_gmres(n, O.sin_point(n ** 3, 1.0), 1e-8, 30, 200, cheb_cfg())
+        assert r.converged
+        its.append(r.iters)
+    assert max(its) <= min(its) + 2
+
+
+# ---- GMRES ---------------------------------------------------------------------
+@pytest.mark.parametrize("n,lam,fd,pc,restart", [(8, 1.0, 0, 2, 40), (8, 1.0, 1, 1, 40),
+                                                 (12, 2.0, 0, 1, 10)])
+def test_gmres_matches_reference(oracle, n, lam, fd, pc, restart):
+    N = n ** 3
+    cfg = O.solver_config(precond=pc, jv_mode=O.DERIV_FD if fd else O.DERIV_ANALYTIC)
+    x, g = oracle.bratu_gmres(n, lam, cfg, O.sin_point(N, 0.1), O.sin_point(N, 1.0), 1e-10,
+                              restart, 400)
+    tag = f"gmres_n{n}_fd{fd}_pc{pc}"
+    c = VEC[tag + "_counts"]
+    assert (g.iters, g.matvecs, g.precs, g.converged, g.breakdown) == tuple(int(v) for v in c)
+    assert rel(x, VEC[tag]) < 1e-11
+
+
+def test_gmres_against_dense_lu(oracle):
+    n, lam = 4, 2.0
+    N = n ** 3
+    x0 = O.sin_point(N, 0.2)
+    J = np.stack([oracle.bratu_jvp(n, lam, x0, e) for e in np.eye(N)], axis=1)
+    b = O.sin_point(N, 1.0)
+    cfg = O.solver_config(precond=O.PRECOND_JACOBI, jv_mode=O.DERIV_ANALYTIC)
+    x, g = oracle.bratu_gmres(n, lam, cfg, x0, b, 1e-13, 64, 200)
+    assert g.converged
+    assert rel(x, np.linalg.solve(J, b)) < 1e-11
+
+
+def test_gmres_breakdown_and_restart_zero(oracle):
+    """test_operators.cpp:141-157 (null operator via lambda = 0 and v in the
+    kernel is not available; use restart = 0 and a zero RHS)."""
+    n = 4
+    N = n ** 3
+    cfg = O.solver_config(precond=O.PRECOND_NONE, jv_mode=O.DERIV_ANALYTIC)
+    x, g = oracle.bratu_gmres(n, 1.0, cfg, np.zeros(N), O.sin_point(N, 1.0), 1e-10, 0, 20)
+    assert g.breakdown and not g.converged
+    x, g = oracle.bratu_gmres(n, 1.0, cfg, np.zeros(N), np.zeros(N), 1e-10, 10, 20)
+    assert g.converged and np.all(x == 0)
+
+
+def test_fgmres_equals_gmres_for_linear_preconditioner(oracle):
+    n = 16
+    N = n ** 3
+    x0, b = O.sin_point(N, 0.1), O.sin_point(N, 1.0)
+    res = []
+    for flex in (0, 1):
+        cfg = O.solver_config(precond=O.PRECOND_GMG, flexible=flex, mg=cheb_cfg())
+        res.append(oracle.bratu_gmres(n, 3.0, cfg, x0, b, 1e-10, 30, 200))
+    (x0_, g0), (x1_, g1) = res
+    assert g0.iters == g1.iters and g1.precs == g0.precs - 1
+    assert rel(x1_, x0_) < 1e-12
+
+
+# ---- Newton / Chebyshev --------------------------------------------------------
+@pytest.mark.parametrize("n,lam,method,pc,rec", [(8, 1.0, 0, 1, 0), (15, 1.0, 1, 2, 1),
+                                                 (16, 6.0, 1, 1, 0)])
+def test_solves_match_reference(oracle, n, lam, method, pc, rec):
+    cfg = O.solver_config(method=method, precond=pc, jv_mode=O.DERIV_ANALYTIC,
+                          d2_mode=O.DERIV_ANALYTIC)
+    x, r = oracle.bratu_solve(n, lam, cfg, O.krylov_params(recycle=rec))
+    tag = f"solve_n{n}_m{method}_pc{pc}_r{rec}"
+    c = VEC[tag + "_counts"]
+    assert (r.iters, r.n_F, r.n_JVP, r.n_D2, r.n_lin_iters, r.n_prec) == tuple(int(v) for v in c)
+    assert rel(x, VEC[tag]) < 1e-12
+
+
+CAL = REFDATA["calibration_n31"]
+
+
+@pytest.mark.parametrize("row", CAL, ids=[f"{r['method']}-{r['precond']}-l{r['lambda']:.3f}-e{r['params']['eta_init']}"
+                                          for r in CAL])
+def test_reference_calibration_rows(oracle, row):
+    """Every counter and the final relative residual of the reference's own
+    recorded C1 calibration runs (analytic J v, u0 = 0)."""
+    p = O.krylov_params(recycle=1 if row["method"] == "rec" else 0, **row["params"])
+    cfg = O.solver_config(method=0 if row["method"] == "nk" else 1,
+                          precond=O.PRECOND_JACOBI if row["precond"] == "jacobi" else O.PRECOND_GMG,
+                          jv_mode=O.DERIV_ANALYTIC, d2_mode=O.DERIV_ANALYTIC)
+    _, r = oracle.bratu_solve(row["n"], row["lambda"], cfg, p)
+    e = row["expect"]
+    got = {k: getattr(r, k) for k in ("iters", "n_F", "n_JVP", "n_D2", "n_lin_iters",
+                                      "n_lin_failures", "n_prec")}
+    assert got == {k: e[k] for k in got}
+    # counters are exact; the final ||F|| (~1e-9 ||F0||) sits at the roundoff
+    # floor of F (the reference summed norms in one thread, the oracle in fixed
+    # chunks), amplified near the fold (lambda -> 9.9) where J is nearly singular
+    assert abs(r.res / r.res0 - e["residual_rel"]) <= 1e-3 * e["residual_rel"]
+
+
+@pytest.mark.parametrize("pc", ["mg", "jacobi"])
+def test_r1_smoke_table(oracle, pc):
+    t = REFDATA["r1_smoke_n16"]
+    for m, want in t[pc].items():
+        cfg = O.solver_config(method=0 if m == "nk" else 1,
+                              precond=O.PRECOND_GMG if pc == "mg" else O.PRECOND_JACOBI,
+                              jv_mode=O.DERIV_ANALYTIC, d2_mode=O.DERIV_ANALYTIC)
+        _, r = oracle.bratu_solve(16, 1.0, cfg, O.krylov_params(recycle=1 if m == "rec" else 0))
+        got = [r.iters, r.n_F, r.n_JVP, r.n_D2, r.n_prec]
+        for g, w in zip(got, want):
+            if w is not None:
+                assert g == w, (m, got, want)
+
+
+def test_reference_solution_norms(oracle):
+    """||u*(lambda)|| of the reference's lower-branch continuation solutions."""
+    rows = [r for r in REFDATA["start_norms"] if r["n"] == 31 and r["lambda"] < 6]
+    assert len(rows) == 3
+    for r in rows:
+        cfg = O.solver_config(precond=O.PRECOND_GMG, jv_mode=O.DERIV_ANALYTIC)
+        x, res = oracle.bratu_solve(31, r["lambda"], cfg, O.krylov_params(rtol_nl=1e-13))
+        assert res.converged
+        assert abs(np.linalg.norm(x) - r["norm"]) < 1e-9 * r["norm"]
+
+
+def test_constant_forcing_converges(oracle):
+    cfg = O.solver_config(precond=O.PRECOND_GMG, mg=cheb_cfg(4))
+    x, r = oracle.bratu_solve(32, 6.0, cfg, O.krylov_params(gmres_restart=30, eta_init=1e-3,
+                                                             forcing=O.FORCING_CONSTANT))
+    assert r.converged and r.res <= 1e-8 * r.res0
+    assert r.n_lin_failures == 0
+
+
+# ---- Chandrasekhar -------------------------------------------------------------
+def test_gauss_legendre(oracle):
+    for N in (8, 24):
+        a, w = oracle.gauss_legendre_unit(N)
+        assert rel(a, VEC[f"gl_nodes_{N}"]) < 1e-15 and rel(w, VEC[f"gl_weights_{N}"]) < 1e-15
+    a, w = oracle.gauss_legendre_unit(8)
+    for d in range(16):  # test_chandrasekhar_op.cpp:36-55
+        assert abs(np.sum(w * a ** d) - 1 / (d + 1)) < 5e-14
+
+
+def test_chandra_ops_match_reference(oracle):
+    for N in (8, 24):
+        x = 1.0 + 0.2 * np.sin(0.37 * np.arange(N) + 0.5)
+        d = 0.5 * np.cos(0.23 * np.arange(N) + 0.1)
+        assert rel(oracle.chandra_eval(N, 0.8, x), VEC[f"chandra_eval_{N}"]) < 1e-15
+        assert rel(oracle.chandra_jvp(N, 0.8, x, d), VEC[f"chandra_jvp_{N}"]) < 1e-15
+        assert rel(oracle.chandra_d2(N, 0.8, x, d), VEC[f"chandra_d2_{N}"]) < 1e-15
+
+
+def test_chandra_golden_and_moment_identity(oracle):
+    gold = np.array(REFDATA["chandra_n8_c05"])
+    assert np.linalg.norm(oracle.chandra_eval(8, 0.5, gold)) < 2e-10
+    cfg = O.solver_config(method=0, precond=O.PRECOND_NONE, jv_mode=O.DERIV_ANALYTIC)
+    for method in (0, 1):
+        cfg.method = method
+        x, r = oracle.chandra_solve(8, 0.5, cfg, O.krylov_params())
+        assert r.converged and rel(x, gold) < 2e-7
+    # sum_i w_i x_i = (2/c)(1 - sqrt(1 - c)) for any rule with sum w = 1
+    for rule, form in ((0, 0), (1, 0), (1, 1)):
+        for c in (0.5, 0.9):
+            N = 64
+            x, r = oracle.chandra_solve(N, c, cfg, O.krylov_params(rtol_nl=1e-13), rule=rule,
+                                        form=form)
+            w = oracle.gauss_legendre_unit(N)[1] if rule == 0 else np.full(N, 1.0 / N)
+            assert abs(np.dot(w, x) - 2 / c * (1 - np.sqrt(1 - c))) < 1e-11
