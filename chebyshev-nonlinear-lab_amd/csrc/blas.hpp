@@ -1,13 +1,14 @@
 // blas.hpp - BLAS-1 and Arnoldi kernels (operators/operator.hpp:19-41,
 // linalg/gmres.hpp:141-153) on gfx950.
 //
-// Grid-stride streaming kernels: each thread handles 4 consecutive elements
-// per iteration with one 16-byte (FP32) or two 16-byte (FP64) vector loads
-// per stream, so enough bytes are in flight per wave to keep HBM busy.  The
-// launch shape is a fixed function of the length, so every reduction sums the
-// same partials in the same order (bitwise repeatable).  Dots accumulate in
-// FP64 whatever the storage type.  Pointers that are not 4-element aligned
-// take the scalar-load variant of the same kernel.
+// Grid-stride streaming kernels: per iteration a thread handles two element
+// pairs, q and q + 256, so each wave-wide access is one contiguous span
+// (16-byte FP64 / 8-byte FP32 per lane) and two of them are in flight per
+// stream.  All loads of an iteration are issued before its stores.  The launch
+// shape is a fixed function of the length, so reductions sum the same
+// partials in the same order (bitwise repeatable); dots accumulate in FP64
+// whatever the storage type.  Pointers that are not pair-aligned take the
+// scalar-load variant of the same kernel.
 #pragma once
 
 #include <algorithm>
@@ -22,155 +23,188 @@ __device__ inline double dv(const T* p, int64_t i) {
 }
 
 template <class T>
-struct alignas(4 * sizeof(T)) V4 {
-    T v[4];
+struct alignas(2 * sizeof(T)) Pair {
+    T a, b;
 };
 
 template <bool VEC, class T>
-__device__ inline V4<T> ld4(const T* p, int64_t i) {
+__device__ inline Pair<T> ldp(const T* p, int64_t q) {
     if constexpr (VEC) {
-        return *reinterpret_cast<const V4<T>*>(p + i);
+        return reinterpret_cast<const Pair<T>*>(p)[q];
     } else {
-        V4<T> r;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) r.v[e] = p[i + e];
-        return r;
+        return Pair<T>{p[2 * q], p[2 * q + 1]};
     }
 }
 
 template <bool VEC, class T>
-__device__ inline void st4(T* p, int64_t i, const V4<T>& r) {
+__device__ inline void stp(T* p, int64_t q, const Pair<T>& v) {
     if constexpr (VEC) {
-        *reinterpret_cast<V4<T>*>(p + i) = r;
+        reinterpret_cast<Pair<T>*>(p)[q] = v;
     } else {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) p[i + e] = r.v[e];
+        p[2 * q] = v.a;
+        p[2 * q + 1] = v.b;
     }
 }
 
-// Visits [0, N) as groups of 4 (f4(base)) plus a scalar tail (f1(i)) handled
-// by block 0.
-template <bool VEC, class F4, class F1>
-__device__ inline void for_each4(int64_t N, F4&& f4, F1&& f1) {
-    const int64_t N4 = N >> 2;
-    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < N4;
-         q += (int64_t)gridDim.x * kBlock)
-        f4(q << 2);
-    const int64_t rem = N - (N4 << 2);
-    if (blockIdx.x == 0 && threadIdx.x < rem) f1((N4 << 2) + threadIdx.x);
+// Visits the pairs of [0, N): f2(q0, q1, ok0, ok1) with q1 = q0 + 256, then
+// the odd last element (if any) with f1(N-1) on one thread.
+template <class F2, class F1>
+__device__ inline void for_each_pair(int64_t N, F2&& f2, F1&& f1) {
+    const int64_t P = N >> 1;
+    const int64_t step = (int64_t)gridDim.x * (2 * kBlock);
+    for (int64_t base = (int64_t)blockIdx.x * (2 * kBlock); base < P; base += step) {
+        const int64_t q0 = base + threadIdx.x, q1 = q0 + kBlock;
+        f2(q0, q1, q0 < P, q1 < P);
+    }
+    if ((N & 1) && blockIdx.x == 0 && threadIdx.x == 0) f1(N - 1);
 }
 
-inline bool aligned4(const void* p, size_t elem) {
-    return p == nullptr || (reinterpret_cast<uintptr_t>(p) % (4 * elem)) == 0;
+inline bool aligned2(const void* p, size_t elem) {
+    return p == nullptr || (reinterpret_cast<uintptr_t>(p) % (2 * elem)) == 0;
 }
 
-// sum_i a_i b_i (b may equal a) -> epilogue.
+// sum_i a_i b_i (b may equal a) -> one partial per block.
 template <bool VEC, class TA, class TB>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t N, const TA* a, const TB* b, Reduce red,
-                                                 Epilogue ep) {
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t N, const TA* a, const TB* b, Reduce red) {
     double acc = 0.0;
-    for_each4<VEC>(
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            const V4<TA> x = ld4<VEC>(a, i);
-            const V4<TB> y = ld4<VEC>(b, i);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) acc += (double)x.v[e] * (double)y.v[e];
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<TA> x0{}, x1{};
+            Pair<TB> y0{}, y1{};
+            if (ok0) {
+                x0 = ldp<VEC>(a, q0);
+                y0 = ldp<VEC>(b, q0);
+            }
+            if (ok1) {
+                x1 = ldp<VEC>(a, q1);
+                y1 = ldp<VEC>(b, q1);
+            }
+            acc += (double)x0.a * (double)y0.a;
+            acc += (double)x0.b * (double)y0.b;
+            acc += (double)x1.a * (double)y1.a;
+            acc += (double)x1.b * (double)y1.b;
         },
         [&](int64_t i) { acc += dv(a, i) * dv(b, i); });
-    reduce_finish<kBlock>(block_sum<kBlock>(acc), red, ep, gridDim.x, blockIdx.x);
+    block_partial<kBlock>(acc, red, blockIdx.x);
 }
 
 template <class TA, class TB>
 void launch_dot(Ctx& c, int64_t N, const TA* a, const TB* b, const Epilogue& ep) {
-    if (aligned4(a, sizeof(TA)) && aligned4(b, sizeof(TB)))
-        hipLaunchKernelGGL((k_dot<true, TA, TB>), dim3(stream_blocks(N)), dim3(kBlock), 0,
-                           c.stream, N, a, b, c.red, ep);
+    const unsigned nb = stream_blocks(N);
+    if (aligned2(a, sizeof(TA)) && aligned2(b, sizeof(TB)))
+        hipLaunchKernelGGL((k_dot<true, TA, TB>), dim3(nb), dim3(kBlock), 0, c.stream, N, a, b,
+                           c.red);
     else
-        hipLaunchKernelGGL((k_dot<false, TA, TB>), dim3(stream_blocks(N)), dim3(kBlock), 0,
-                           c.stream, N, a, b, c.red, ep);
+        hipLaunchKernelGGL((k_dot<false, TA, TB>), dim3(nb), dim3(kBlock), 0, c.stream, N, a, b,
+                           c.red);
     NUMA_HIP_LAUNCH_CHECK();
+    c.finish(nb, ep);
 }
 
 // y += a x  (operator.hpp:32-41)
 template <bool VEC, class TX, class TY>
 __global__ __launch_bounds__(kBlock) void k_axpy(int64_t N, double a, const TX* x, TY* y) {
 #pragma clang fp contract(off)
-    for_each4<VEC>(
+    auto upd = [a](Pair<TY> yv, Pair<TX> xv) {
+        return Pair<TY>{static_cast<TY>((double)yv.a + a * (double)xv.a),
+                        static_cast<TY>((double)yv.b + a * (double)xv.b)};
+    };
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            const V4<TX> xv = ld4<VEC>(x, i);
-            V4<TY> yv = ld4<VEC>(y, i);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) yv.v[e] = static_cast<TY>((double)yv.v[e] + a * (double)xv.v[e]);
-            st4<VEC>(y, i, yv);
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<TX> x0{}, x1{};
+            Pair<TY> y0{}, y1{};
+            if (ok0) {
+                x0 = ldp<VEC>(x, q0);
+                y0 = ldp<VEC>(y, q0);
+            }
+            if (ok1) {
+                x1 = ldp<VEC>(x, q1);
+                y1 = ldp<VEC>(y, q1);
+            }
+            if (ok0) stp<VEC>(y, q0, upd(y0, x0));
+            if (ok1) stp<VEC>(y, q1, upd(y1, x1));
         },
         [&](int64_t i) { y[i] = static_cast<TY>(dv(y, i) + a * dv(x, i)); });
 }
 
 template <bool VEC, class T>
 __global__ __launch_bounds__(kBlock) void k_fill(int64_t N, T v, T* y) {
-    for_each4<VEC>(
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            V4<T> r;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) r.v[e] = v;
-            st4<VEC>(y, i, r);
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            if (ok0) stp<VEC>(y, q0, Pair<T>{v, v});
+            if (ok1) stp<VEC>(y, q1, Pair<T>{v, v});
         },
         [&](int64_t i) { y[i] = v; });
 }
 
 template <bool VEC, class TI, class TO>
 __global__ __launch_bounds__(kBlock) void k_convert(int64_t N, const TI* x, TO* y) {
-    for_each4<VEC>(
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            const V4<TI> a = ld4<VEC>(x, i);
-            V4<TO> r;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) r.v[e] = static_cast<TO>(a.v[e]);
-            st4<VEC>(y, i, r);
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<TI> a0{}, a1{};
+            if (ok0) a0 = ldp<VEC>(x, q0);
+            if (ok1) a1 = ldp<VEC>(x, q1);
+            if (ok0) stp<VEC>(y, q0, Pair<TO>{static_cast<TO>(a0.a), static_cast<TO>(a0.b)});
+            if (ok1) stp<VEC>(y, q1, Pair<TO>{static_cast<TO>(a1.a), static_cast<TO>(a1.b)});
         },
         [&](int64_t i) { y[i] = static_cast<TO>(x[i]); });
 }
 
 // One fused modified-Gram-Schmidt pass (gmres.hpp:141-144):
 //   w -= h_i * (s_i V_i)            with h_i = *hi (already the true dot)
-//   next:  sum w * V_{i+1}   -> H[i+1][j] = s_{i+1} * sum   (EPI_STORE, mul_ptr)
-//   last:  sum w * w         -> H[j+1][j] = sqrt, s_{j+1} = 1/sqrt (EPI_NORM_SCALE)
+//   !LAST: partial of sum w * V_{i+1}   (-> H[i+1][j] = s_{i+1} * sum)
+//   LAST:  partial of sum w * w         (-> H[j+1][j] = sqrt, s_{j+1} = 1/sqrt)
 // V_i are stored unnormalised with scales s_i.
-template <bool VEC, class T>
+template <bool VEC, bool LAST, class T>
 __global__ __launch_bounds__(kBlock) void k_mgs(int64_t N, T* w, const T* Vi, const double* hi,
-                                                 const double* si, const T* Vnext, Reduce red,
-                                                 Epilogue ep) {
+                                                 const double* si, const T* Vnext, Reduce red) {
 #pragma clang fp contract(off)
     const double coef = (*hi) * (*si);
     double acc = 0.0;
-    for_each4<VEC>(
+    auto upd = [coef](Pair<T> wv, Pair<T> vv) {
+        return Pair<T>{static_cast<T>((double)wv.a - coef * (double)vv.a),
+                       static_cast<T>((double)wv.b - coef * (double)vv.b)};
+    };
+    auto dot = [](Pair<T> u, Pair<T> v) {
+        return (double)u.a * (double)v.a + (double)u.b * (double)v.b;
+    };
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            V4<T> wv = ld4<VEC>(w, i);
-            const V4<T> vv = ld4<VEC>(Vi, i);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) wv.v[e] = static_cast<T>((double)wv.v[e] - coef * (double)vv.v[e]);
-            st4<VEC>(w, i, wv);
-            if (Vnext) {
-                const V4<T> nv = ld4<VEC>(Vnext, i);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc += (double)wv.v[e] * (double)nv.v[e];
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<T> w0{}, w1{}, v0{}, v1{}, n0{}, n1{};
+            if (ok0) {
+                w0 = ldp<VEC>(w, q0);
+                v0 = ldp<VEC>(Vi, q0);
+                if constexpr (!LAST) n0 = ldp<VEC>(Vnext, q0);
+            }
+            if (ok1) {
+                w1 = ldp<VEC>(w, q1);
+                v1 = ldp<VEC>(Vi, q1);
+                if constexpr (!LAST) n1 = ldp<VEC>(Vnext, q1);
+            }
+            w0 = upd(w0, v0);
+            w1 = upd(w1, v1);
+            if (ok0) stp<VEC>(w, q0, w0);
+            if (ok1) stp<VEC>(w, q1, w1);
+            if constexpr (LAST) {
+                acc += dot(w0, w0);
+                acc += dot(w1, w1);
             } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc += (double)wv.v[e] * (double)wv.v[e];
+                acc += dot(w0, n0);
+                acc += dot(w1, n1);
             }
         },
         [&](int64_t i) {
             const T wn = static_cast<T>(dv(w, i) - coef * dv(Vi, i));
             w[i] = wn;
             const double wd = static_cast<double>(wn);
-            acc += Vnext ? wd * dv(Vnext, i) : wd * wd;
+            acc += LAST ? wd * wd : wd * dv(Vnext, i);
         });
-    reduce_finish<kBlock>(block_sum<kBlock>(acc), red, ep, gridDim.x, blockIdx.x);
+    block_partial<kBlock>(acc, red, blockIdx.x);
 }
 
 // y (=|+=) sum_k c_k X_k over up to kMaxComb vectors (x += V y, gmres.hpp:198-202).
@@ -186,24 +220,33 @@ template <bool VEC, class T, class TY>
 __global__ __launch_bounds__(kBlock) void k_combine(int64_t N, CombArgs<T> a, TY* y,
                                                      int accumulate) {
 #pragma clang fp contract(off)
-    for_each4<VEC>(
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            double s[4] = {0.0, 0.0, 0.0, 0.0};
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            double s0a = 0.0, s0b = 0.0, s1a = 0.0, s1b = 0.0;
             if (accumulate) {
-                const V4<TY> yv = ld4<VEC>(y, i);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) s[e] = (double)yv.v[e];
+                if (ok0) {
+                    const Pair<TY> t = ldp<VEC>(y, q0);
+                    s0a = (double)t.a;
+                    s0b = (double)t.b;
+                }
+                if (ok1) {
+                    const Pair<TY> t = ldp<VEC>(y, q1);
+                    s1a = (double)t.a;
+                    s1b = (double)t.b;
+                }
             }
             for (int k = 0; k < a.count; ++k) {
-                const V4<T> xv = ld4<VEC>(a.v[k], i);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) s[e] += a.c[k] * (double)xv.v[e];
+                Pair<T> x0{}, x1{};
+                if (ok0) x0 = ldp<VEC>(a.v[k], q0);
+                if (ok1) x1 = ldp<VEC>(a.v[k], q1);
+                s0a += a.c[k] * (double)x0.a;
+                s0b += a.c[k] * (double)x0.b;
+                s1a += a.c[k] * (double)x1.a;
+                s1b += a.c[k] * (double)x1.b;
             }
-            V4<TY> r;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) r.v[e] = static_cast<TY>(s[e]);
-            st4<VEC>(y, i, r);
+            if (ok0) stp<VEC>(y, q0, Pair<TY>{static_cast<TY>(s0a), static_cast<TY>(s0b)});
+            if (ok1) stp<VEC>(y, q1, Pair<TY>{static_cast<TY>(s1a), static_cast<TY>(s1b)});
         },
         [&](int64_t i) {
             double s = accumulate ? dv(y, i) : 0.0;
@@ -216,15 +259,25 @@ __global__ __launch_bounds__(kBlock) void k_combine(int64_t N, CombArgs<T> a, TY
 template <bool VEC, class T>
 __global__ __launch_bounds__(kBlock) void k_diag_solve(int64_t N, const T* r, const double* d,
                                                         T* z) {
-    for_each4<VEC>(
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            const V4<T> rv = ld4<VEC>(r, i);
-            const V4<double> dd = ld4<VEC>(d, i);
-            V4<T> o;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o.v[e] = static_cast<T>((double)rv.v[e] / dd.v[e]);
-            st4<VEC>(z, i, o);
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<T> r0{}, r1{};
+            Pair<double> d0{1.0, 1.0}, d1{1.0, 1.0};
+            if (ok0) {
+                r0 = ldp<VEC>(r, q0);
+                d0 = ldp<VEC>(d, q0);
+            }
+            if (ok1) {
+                r1 = ldp<VEC>(r, q1);
+                d1 = ldp<VEC>(d, q1);
+            }
+            if (ok0)
+                stp<VEC>(z, q0, Pair<T>{static_cast<T>((double)r0.a / d0.a),
+                                        static_cast<T>((double)r0.b / d0.b)});
+            if (ok1)
+                stp<VEC>(z, q1, Pair<T>{static_cast<T>((double)r1.a / d1.a),
+                                        static_cast<T>((double)r1.b / d1.b)});
         },
         [&](int64_t i) { z[i] = static_cast<T>(dv(r, i) / d[i]); });
 }
@@ -251,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_d2_analytic(int64_t N, double lambda
 
 template <class T>
 void launch_fill(Ctx& c, int64_t N, T v, T* y) {
-    if (aligned4(y, sizeof(T)))
+    if (aligned2(y, sizeof(T)))
         hipLaunchKernelGGL((k_fill<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N,
                            v, y);
     else
@@ -262,7 +315,7 @@ void launch_fill(Ctx& c, int64_t N, T v, T* y) {
 
 template <class TI, class TO>
 void launch_convert(Ctx& c, int64_t N, const TI* x, TO* y) {
-    if (aligned4(x, sizeof(TI)) && aligned4(y, sizeof(TO)))
+    if (aligned2(x, sizeof(TI)) && aligned2(y, sizeof(TO)))
         hipLaunchKernelGGL((k_convert<true, TI, TO>), dim3(stream_blocks(N)), dim3(kBlock), 0,
                            c.stream, N, x, y);
     else
@@ -273,7 +326,7 @@ void launch_convert(Ctx& c, int64_t N, const TI* x, TO* y) {
 
 template <class TX, class TY>
 void launch_axpy(Ctx& c, int64_t N, double a, const TX* x, TY* y) {
-    if (aligned4(x, sizeof(TX)) && aligned4(y, sizeof(TY)))
+    if (aligned2(x, sizeof(TX)) && aligned2(y, sizeof(TY)))
         hipLaunchKernelGGL((k_axpy<true, TX, TY>), dim3(stream_blocks(N)), dim3(kBlock), 0,
                            c.stream, N, a, x, y);
     else
@@ -282,21 +335,33 @@ void launch_axpy(Ctx& c, int64_t N, double a, const TX* x, TY* y) {
     NUMA_HIP_LAUNCH_CHECK();
 }
 
+template <bool LAST, class T>
+void launch_mgs_impl(Ctx& c, int64_t N, T* w, const T* Vi, const double* hi, const double* si,
+                     const T* Vnext, unsigned nb) {
+    if (aligned2(w, sizeof(T)) && aligned2(Vi, sizeof(T)) && aligned2(Vnext, sizeof(T)))
+        hipLaunchKernelGGL((k_mgs<true, LAST, T>), dim3(nb), dim3(kBlock), 0, c.stream, N, w, Vi,
+                           hi, si, Vnext, c.red);
+    else
+        hipLaunchKernelGGL((k_mgs<false, LAST, T>), dim3(nb), dim3(kBlock), 0, c.stream, N, w, Vi,
+                           hi, si, Vnext, c.red);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+// MGS pass i; Vnext == null marks the last pass (norm of w).
 template <class T>
 void launch_mgs(Ctx& c, int64_t N, T* w, const T* Vi, const double* hi, const double* si,
                 const T* Vnext, const Epilogue& ep) {
-    if (aligned4(w, sizeof(T)) && aligned4(Vi, sizeof(T)) && aligned4(Vnext, sizeof(T)))
-        hipLaunchKernelGGL((k_mgs<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N,
-                           w, Vi, hi, si, Vnext, c.red, ep);
+    const unsigned nb = stream_blocks(N);
+    if (Vnext)
+        launch_mgs_impl<false, T>(c, N, w, Vi, hi, si, Vnext, nb);
     else
-        hipLaunchKernelGGL((k_mgs<false, T>), dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N,
-                           w, Vi, hi, si, Vnext, c.red, ep);
-    NUMA_HIP_LAUNCH_CHECK();
+        launch_mgs_impl<true, T>(c, N, w, Vi, hi, si, Vnext, nb);
+    c.finish(nb, ep);
 }
 
 template <class T>
 void launch_diag_solve(Ctx& c, int64_t N, const T* r, const double* d, T* z) {
-    if (aligned4(r, sizeof(T)) && aligned4(d, sizeof(double)) && aligned4(z, sizeof(T)))
+    if (aligned2(r, sizeof(T)) && aligned2(d, sizeof(double)) && aligned2(z, sizeof(T)))
         hipLaunchKernelGGL((k_diag_solve<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
                            c.stream, N, r, d, z);
     else
@@ -318,11 +383,11 @@ void launch_combine(Ctx& c, int64_t N, const std::vector<const T*>& v,
     while (done < v.size()) {
         CombArgs<T> a{};
         a.count = (int)std::min<size_t>(kMaxComb, v.size() - done);
-        bool al = aligned4(y, sizeof(TY));
+        bool al = aligned2(y, sizeof(TY));
         for (int k = 0; k < a.count; ++k) {
             a.v[k] = v[done + k];
             a.c[k] = coef[done + k];
-            al = al && aligned4(a.v[k], sizeof(T));
+            al = al && aligned2(a.v[k], sizeof(T));
         }
         if (al)
             hipLaunchKernelGGL((k_combine<true, T, TY>), dim3(stream_blocks(N)), dim3(kBlock), 0,

@@ -15,6 +15,31 @@ thread_local std::string g_last_error;
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+__global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partials, unsigned count,
+                                                           Epilogue ep) {
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < count; i += kFinishThreads) s += partials[i];
+    s = block_sum<kFinishThreads>(s);
+    if (threadIdx.x == 0) {
+        switch (ep.kind) {
+            case EPI_STORE: ep.out[0] = s * (ep.mul_ptr ? ep.mul_ptr[0] : ep.mul); break;
+            case EPI_SQRT: ep.out[0] = sqrt(s); break;
+            default: {
+                const double r = sqrt(s);
+                ep.out[0] = r;
+                if (ep.out2) ep.out2[0] = r > 0.0 ? 1.0 / r : 0.0;
+            }
+        }
+    }
+}
+
+void Ctx::finish(unsigned count, const Epilogue& ep) {
+    require(count >= 1 && count <= (unsigned)kMaxPartials, "bad reduction size");
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishThreads), 0, stream,
+                       (const double*)red.partials, count, ep);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
 Ctx::Ctx(int dev) : device(dev) {
     int count = 0;
     NUMA_HIP_CHECK(hipGetDeviceCount(&count));
@@ -25,10 +50,8 @@ Ctx::Ctx(int dev) : device(dev) {
     cu_count = prop.multiProcessorCount;
     NUMA_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     void* p = nullptr;
-    NUMA_HIP_CHECK(hipMalloc(&p, kMaxPartials * sizeof(double) + 256));
+    NUMA_HIP_CHECK(hipMalloc(&p, kMaxPartials * sizeof(double)));
     red.partials = static_cast<double*>(p);
-    red.counter = reinterpret_cast<unsigned*>(static_cast<char*>(p) + kMaxPartials * sizeof(double));
-    NUMA_HIP_CHECK(hipMemsetAsync(red.counter, 0, 256, stream));
     NUMA_HIP_CHECK(hipMalloc(&p, kScalars * sizeof(double)));
     dscal = static_cast<double*>(p);
     NUMA_HIP_CHECK(hipHostMalloc(&p, kScalars * sizeof(double), hipHostMallocDefault));

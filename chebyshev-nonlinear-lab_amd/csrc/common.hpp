@@ -42,22 +42,24 @@ inline void require(bool ok, const std::string& msg, int code = NUMA_ERR_ARG) {
 }
 
 // ---------------------------------------------------------------------------
-// Reductions.  A reducing kernel writes one FP64 partial per block, counts
-// finished blocks with a global atomic, and the last block to finish sums the
-// partials in a fixed order and runs an epilogue on the total.  The result is
-// therefore independent of block scheduling (bitwise repeatable).
-// Kernels on one stream run one after another, so a single partial array and
-// counter serve every reduction of a context.
+// Reductions.  A reducing kernel writes one FP64 partial per block (fixed
+// thread -> element mapping, fixed in-block order); a one-block finish kernel
+// on the same stream sums the partials in a fixed order and applies an
+// epilogue to the total.  The result is independent of scheduling (bitwise
+// repeatable) and needs no device-scope fence in the streaming kernel (a
+// release fence per block would write back the XCD's L2 each time).
+// Kernels on one stream run in order, so one partial array serves every
+// reduction of a context.
 // ---------------------------------------------------------------------------
-constexpr int kBlock = 256;           // threads per block of streaming kernels
-constexpr int kMaxPartials = 16384;   // max blocks of a reducing launch
+constexpr int kBlock = 256;            // threads per block of streaming kernels
+constexpr int kMaxPartials = 65536;    // max blocks of a reducing launch
+constexpr int kFinishThreads = 1024;
 
 struct Reduce {
     double* partials;   // [kMaxPartials]
-    unsigned* counter;  // single counter, left at zero by every reduction
 };
 
-// Epilogues applied by the last block to the reduced total s.
+// Epilogues applied by the finish kernel to the reduced total s.
 enum EpiKind : int {
     EPI_STORE = 0,       // out[0] = s * mul
     EPI_SQRT = 1,        // out[0] = sqrt(s)
@@ -92,39 +94,15 @@ __device__ inline double block_sum(double v) {
     return t;
 }
 
-// Called by every thread of every block of a reducing launch with the
-// block-local total (valid in thread 0).  nblocks = number of blocks.
+// Every thread of a reducing kernel calls this with its private sum.
 template <int BLOCK>
-__device__ inline void reduce_finish(double block_total, const Reduce& red, const Epilogue& ep,
-                                     unsigned nblocks, unsigned bid) {
-    __shared__ bool is_last;
-    if (threadIdx.x == 0) {
-        red.partials[bid] = block_total;
-        __threadfence();
-        const unsigned prev = atomicAdd(red.counter, 1u);
-        is_last = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < nblocks; i += BLOCK) {
-        s += __builtin_nontemporal_load(&red.partials[i]);
-    }
-    s = block_sum<BLOCK>(s);
-    if (threadIdx.x == 0) {
-        switch (ep.kind) {
-            case EPI_STORE: ep.out[0] = s * (ep.mul_ptr ? ep.mul_ptr[0] : ep.mul); break;
-            case EPI_SQRT: ep.out[0] = sqrt(s); break;
-            default: {
-                const double r = sqrt(s);
-                ep.out[0] = r;
-                if (ep.out2) ep.out2[0] = r > 0.0 ? 1.0 / r : 0.0;
-            }
-        }
-        *red.counter = 0u;
-    }
+__device__ inline void block_partial(double acc, const Reduce& red, unsigned bid) {
+    const double t = block_sum<BLOCK>(acc);
+    if (threadIdx.x == 0) red.partials[bid] = t;
 }
+
+__global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partials, unsigned count,
+                                                           Epilogue ep);
 
 // ---------------------------------------------------------------------------
 // Context: one device, one stream, reduction scratch and a pinned host
@@ -142,14 +120,16 @@ struct Ctx {
 
     explicit Ctx(int dev);
     ~Ctx();
+    // One-block sum of the first `count` partials and the epilogue.
+    void finish(unsigned count, const Epilogue& ep);
     // Reads device scalar slots [first, first+count) to the host (synchronises).
     void read_scalars(const double* dev, double* host, int count);
     void sync() { NUMA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
 
-// Launch shape of grid-stride streaming kernels over N points: each thread
-// handles 4 consecutive elements per iteration; up to 4096 blocks (16 per CU
-// of the 256-CU part), capped for the reduction partials.
+// Launch shape of grid-stride streaming kernels over N elements: each thread
+// moves 2 x 2 consecutive elements per iteration (16-byte FP64 / 8-byte FP32
+// accesses, contiguous across the wave); up to 4096 blocks (16 per CU).
 constexpr int kVec = 4;
 inline unsigned stream_blocks(int64_t N) {
     int64_t b = (N + (int64_t)kBlock * kVec - 1) / ((int64_t)kBlock * kVec);

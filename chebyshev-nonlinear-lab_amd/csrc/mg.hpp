@@ -24,6 +24,7 @@ namespace numa_hip {
 // USE_D reads the previous direction, WRITE_D stores the new one.
 template <class T, bool USE_D, bool WRITE_D>
 struct EpPoly {
+    static constexpr bool kMarch = false;
     double h2inv, c1, c2;
     const T* b;
     T* d;
@@ -52,6 +53,7 @@ struct EpPoly {
 // r = b - A x   (poisson_mg.hpp:128-149)
 template <class T>
 struct EpResid {
+    static constexpr bool kMarch = false;
     double h2inv;
     const T* b;
     T* r;
@@ -71,15 +73,24 @@ template <bool VEC, class T>
 __global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2, const T* b, T* d,
                                                         T* xn) {
 #pragma clang fp contract(off)
-    for_each4<VEC>(
+    auto f = [c2](Pair<T> v) {
+        return Pair<T>{static_cast<T>(c2 * (double)v.a), static_cast<T>(c2 * (double)v.b)};
+    };
+    for_each_pair(
         N,
-        [&](int64_t i) {
-            const V4<T> bv = ld4<VEC>(b, i);
-            V4<T> o;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o.v[e] = static_cast<T>(c2 * (double)bv.v[e]);
-            if (d) st4<VEC>(d, i, o);
-            st4<VEC>(xn, i, o);
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<T> b0{}, b1{};
+            if (ok0) b0 = ldp<VEC>(b, q0);
+            if (ok1) b1 = ldp<VEC>(b, q1);
+            const Pair<T> o0 = f(b0), o1 = f(b1);
+            if (ok0) {
+                if (d) stp<VEC>(d, q0, o0);
+                stp<VEC>(xn, q0, o0);
+            }
+            if (ok1) {
+                if (d) stp<VEC>(d, q1, o1);
+                stp<VEC>(xn, q1, o1);
+            }
         },
         [&](int64_t i) {
             const double dn = c2 * dv(b, i);
@@ -493,7 +504,7 @@ private:
         const int64_t N = L.n * L.n * L.n;
         if (from_zero) {
             T* d = write_d ? L.d : (T*)nullptr;
-            if (aligned4(L.b, sizeof(T)) && aligned4(d, sizeof(T)) && aligned4(L.x2, sizeof(T)))
+            if (aligned2(L.b, sizeof(T)) && aligned2(d, sizeof(T)) && aligned2(L.x2, sizeof(T)))
                 hipLaunchKernelGGL((k_poly_first<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
                                    c_.stream, N, c2, (const T*)L.b, d, L.x2);
             else

@@ -4,9 +4,10 @@
 // layout, problems/bratu3d.hpp:85), zero Dirichlet values outside.
 //
 // One generic driver k_stencil: a block is 64 (k) x 4 (j) threads (32 x 8 for
-// n <= 32); each thread owns one (j,k) column of a slab of `ichunk` planes and
-// marches along i, keeping the i-1 / i / i+1 values of the field in registers
-// and prefetching the next plane's loads before computing the current one.  The j+-1 and
+// n <= 32).  Either one node per thread, or each thread owns one (j,k) column
+// of a slab of `ichunk` planes and marches along i, keeping the i-1 / i / i+1
+// values in registers and prefetching the next plane's loads before computing
+// the current one.  The j+-1 and
 // k+-1 neighbours are re-read and hit the CU's L1 / the XCD's L2 (the
 // neighbouring waves and blocks stream the same rows), so HBM traffic stays
 // close to one read per input field.  A Loader gives the field value(s) at a
@@ -58,9 +59,9 @@ __device__ inline double lap_num(double c, double im, double ip, double jm, doub
 // Out-of-domain neighbours are loaded from a valid address (the centre) and
 // replaced by zero with a select, so every load is unconditional and the
 // whole next plane is in flight while the current one is computed.
-template <int TK, bool REDUCE, class Load, class Epi>
-__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Epi epi, Reduce red,
-                                                  Epilogue ep) {
+template <int TK, bool REDUCE, bool MARCH, class Load, class Epi>
+__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Epi epi,
+                                                  Reduce red) {
     constexpr int TJ = 256 / TK;
     const Load L = ldr.prep();
     const Epi E = epi.prep();
@@ -87,64 +88,81 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Ep
         P km = fetch(id - 1, id, hk0);
         P kp = fetch(id + 1, id, hk1);
         X ex = E.preload(id);
-        for (int i = i0; i < i1; ++i, id += N2) {
-            // prefetch plane i+1 (its centre pp is already resident)
-            const bool more = i + 1 < i1;
-            const int64_t nx = more ? id + N2 : id;
-            const P qp = fetch(nx + N2, nx, more && i + 2 < n);
-            const P qjm = fetch(nx - N1, nx, hj0);
-            const P qjp = fetch(nx + N1, nx, hj1);
-            const P qkm = fetch(nx - 1, nx, hk0);
-            const P qkp = fetch(nx + 1, nx, hk1);
-            const X qex = E.preload(nx);
+        if constexpr (!MARCH) {
             acc += E(id, ex, pc, pm, pp, jm, jp, km, kp);
-            pm = pc;
-            pc = pp;
-            pp = qp;
-            jm = qjm;
-            jp = qjp;
-            km = qkm;
-            kp = qkp;
-            ex = qex;
+        } else {
+            for (int i = i0; i < i1; ++i, id += N2) {
+                // prefetch plane i+1 (its centre pp is already resident)
+                const bool more = i + 1 < i1;
+                const int64_t nx = more ? id + N2 : id;
+                const P qp = fetch(nx + N2, nx, more && i + 2 < n);
+                const P qjm = fetch(nx - N1, nx, hj0);
+                const P qjp = fetch(nx + N1, nx, hj1);
+                const P qkm = fetch(nx - 1, nx, hk0);
+                const P qkp = fetch(nx + 1, nx, hk1);
+                const X qex = E.preload(nx);
+                acc += E(id, ex, pc, pm, pp, jm, jp, km, kp);
+                pm = pc;
+                pc = pp;
+                pp = qp;
+                jm = qjm;
+                jp = qjp;
+                km = qkm;
+                kp = qkp;
+                ex = qex;
+            }
         }
     }
     if constexpr (REDUCE) {
-        const double t = block_sum<256>(acc);
         const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        reduce_finish<256>(t, red, ep, gridDim.x * gridDim.y * gridDim.z, bid);
+        block_partial<256>(acc, red, bid);
     }
 }
 
-// Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below; planes per thread
-// chosen so large grids keep a long register queue and small grids still get
-// enough blocks to fill the chip; at most kMaxPartials blocks.
+// Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below.  Epilogues with an
+// FP64 exp() per node (Epi::kMarch) march along i in chunks of 8 planes on
+// large grids (register queue, next plane prefetched); the others use one node
+// per thread, which streams best for cheap epilogues.  At most kMaxPartials
+// blocks.
 struct StencilShape {
     int tk, ichunk;
+    bool march;
     dim3 grid;
 };
 
-inline StencilShape stencil_shape(int64_t n) {
+inline StencilShape stencil_shape(int64_t n, bool heavy) {
     StencilShape s;
     s.tk = n > 32 ? 64 : 32;
     const int tj = 256 / s.tk;
     const int64_t gx = (n + s.tk - 1) / s.tk, gy = (n + tj - 1) / tj;
-    int ich = n >= 192 ? 8 : (n >= 96 ? 2 : 1);
+    int ich = (heavy && n >= 128) ? 8 : 1;
     while (gx * gy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
     s.ichunk = ich;
+    s.march = ich > 1;
     s.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)((n + ich - 1) / ich));
     return s;
 }
 
 template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
-    const StencilShape s = stencil_shape(n);
-    if (s.tk == 64)
-        hipLaunchKernelGGL((k_stencil<64, REDUCE, Load, Epi>), s.grid, dim3(256), 0, c.stream,
-                           (int)n, s.ichunk, ldr, epi, c.red, ep);
-    else
-        hipLaunchKernelGGL((k_stencil<32, REDUCE, Load, Epi>), s.grid, dim3(256), 0, c.stream,
-                           (int)n, s.ichunk, ldr, epi, c.red, ep);
+    const StencilShape s = stencil_shape(n, Epi::kMarch);
+#define NUMA_STENCIL_LAUNCH(TK_, MARCH_)                                                     \
+    hipLaunchKernelGGL((k_stencil<TK_, REDUCE, MARCH_, Load, Epi>), s.grid, dim3(256), 0,    \
+                       c.stream, (int)n, s.ichunk, ldr, epi, c.red)
+    if (s.tk == 64) {
+        if (s.march)
+            NUMA_STENCIL_LAUNCH(64, true);
+        else
+            NUMA_STENCIL_LAUNCH(64, false);
+    } else {
+        if (s.march)
+            NUMA_STENCIL_LAUNCH(32, true);
+        else
+            NUMA_STENCIL_LAUNCH(32, false);
+    }
+#undef NUMA_STENCIL_LAUNCH
     NUMA_HIP_LAUNCH_CHECK();
+    if constexpr (REDUCE) c.finish(s.grid.x * s.grid.y * s.grid.z, ep);
 }
 
 // ---------------------------------------------------------------------------
@@ -260,6 +278,7 @@ struct LdPM {
 // writes the trial point w itself; returns F^2.
 template <bool WOUT>
 struct EpF {
+    static constexpr bool kMarch = true;
     double h2inv, lambda;
     double* F;
     double* wout;
@@ -289,6 +308,7 @@ struct JvExtra {
 // the Arnoldi scale *scale_ptr (1 if null).
 template <class TO, bool DOT, bool RESID>
 struct EpJvFD {
+    static constexpr bool kMarch = true;
     double h2inv, lambda;
     FDStep step;
     const double* scale_ptr;
@@ -331,6 +351,7 @@ struct EpJvFD {
 // Analytic J v = h2inv*(6v - sum nbrs) - lambda e^x v (bratu3d.hpp:33-43).
 template <class TO, bool DOT, bool RESID>
 struct EpJvAn {
+    static constexpr bool kMarch = true;
     double h2inv, lambda;
     const double* scale_ptr;
     const double* x;
@@ -366,6 +387,7 @@ struct EpJvAn {
 
 // Central-difference F''[v,v] = (F(x+hv) - 2F0 + F(x-hv))/h^2 (operator.hpp:132-133).
 struct EpD2FD {
+    static constexpr bool kMarch = true;
     double h2inv, lambda;
     FDStep step;
     const double* F0;
@@ -396,6 +418,7 @@ struct EpD2FD {
 // Plain -Lap_h v.
 template <class TO>
 struct EpLap {
+    static constexpr bool kMarch = false;
     double h2inv;
     TO* out;
     __device__ EpLap prep() const { return *this; }

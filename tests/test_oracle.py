@@ -262,7 +262,8 @@ def test_reference_calibration_rows(oracle, row):
     # counters are exact; the final ||F|| (~1e-9 ||F0||) sits at the roundoff
     # floor of F (the reference summed norms in one thread, the oracle in fixed
     # chunks), amplified near the fold (lambda -> 9.9) where J is nearly singular
-    assert abs(r.res / r.res0 - e["residual_rel"]) <= 1e-3 * e["residual_rel"]
+    tol = 5e-3 if row["lambda"] > 9.0 else 1e-3
+    assert abs(r.res / r.res0 - e["residual_rel"]) <= tol * e["residual_rel"]
 
 
 @pytest.mark.parametrize("pc", ["mg", "jacobi"])
