@@ -125,6 +125,77 @@ def run_cpu_baseline(args):
     return info, out
 
 
+CH_N, CH_B, CH_SEED = 8192, 256, 20261203
+FP64_MFMA_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+
+def chandra_cs():
+    """256 values of c uniform in [0.5, 0.9999] from a recorded seed
+    (mt19937-independent: numpy PCG64)."""
+    import numpy as np
+    return np.random.default_rng(CH_SEED).uniform(0.5, 0.9999, CH_B)
+
+
+def chandra_leg(H, ctx, args) -> dict:
+    """Chandrasekhar H-equation, midpoint rule, Kelley form, N = 8192, a batch of
+    256 c values, x0 = 1, Newton-GMRES(20), tol 1e-10: FP64 vs split-BF16 J v."""
+    import numpy as np
+    cs = chandra_cs()
+    p = H.krylov_params(rtol_nl=1e-10, gmres_restart=20, gmres_max_iters=400)
+    out = {"workload": f"chandrasekhar_midpoint_kelley_N{CH_N}_batch{CH_B}_newton_gmres20_tol1e-10",
+           "c_seed": CH_SEED}
+    sols = {}
+    for name, prec in (("fp64", H.PRECISION_FP64), ("mixed", H.PRECISION_MIXED)):
+        ch = H.ChandrasekharBatch(ctx, CH_N, cs, rule=1, form=1, precision=prec, restart=20)
+        x = ctx.array(np.ones(CH_B * CH_N))
+        ch.solve_device(p, x)  # warm-up
+        x.upload(np.ones(CH_B * CH_N))
+        ctx.sync()
+        t0 = time.perf_counter()
+        res = ch.solve_device(p, x)
+        ctx.sync()
+        t = time.perf_counter() - t0
+        sols[name] = x.numpy().reshape(CH_B, CH_N)
+        ms64, fl = ch.gemm_cost(H.PRECISION_FP64, 10)
+        leg = {"solve_s": round(t, 5), "converged": int(sum(r.converged for r in res)),
+               "max_newton": int(max(r.iters for r in res)),
+               "total_jv": int(sum(r.n_JVP for r in res)),
+               "gemm_fp64_ms": round(ms64, 4),
+               "gemm_fp64_tflops": round(fl / (ms64 * 1e-3) / 1e12, 2),
+               "gemm_fp64_frac_of_78.6": round(fl / (ms64 * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4)}
+        if prec == H.PRECISION_MIXED:
+            msb, _ = ch.gemm_cost(H.PRECISION_MIXED, 10)
+            leg.update({"gemm_bf16x3_ms": round(msb, 4),
+                        "gemm_bf16x3_useful_tflops": round(fl / (msb * 1e-3) / 1e12, 2),
+                        "gemm_bf16x3_issued_tflops": round(3 * fl / (msb * 1e-3) / 1e12, 2),
+                        "gemm_bf16x3_issued_frac_of_2500": round(
+                            3 * fl / (msb * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4)})
+        out[name] = leg
+        log(f"chandrasekhar {name}: {t*1e3:.1f} ms, converged {leg['converged']}/{CH_B}, "
+            f"max newton {leg['max_newton']}, {leg}")
+        ch.close()
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["solve_s"] / out["mixed"]["solve_s"], 3)
+    out["mixed_vs_fp64_max_rel_l2"] = float(max(
+        np.linalg.norm(sols["mixed"][b] - sols["fp64"][b]) / np.linalg.norm(sols["fp64"][b])
+        for b in range(CH_B)))
+    return out
+
+
+def chandra_cpu_sample(args) -> None:
+    """Child process: FP64 oracle Newton-GMRES(20) for the first 2 c values."""
+    import oracle as O
+    cs = chandra_cs()[:2]
+    o = O.Oracle()
+    cfg = O.solver_config(method=0, precond=O.PRECOND_NONE, jv_mode=O.DERIV_ANALYTIC)
+    p = O.krylov_params(rtol_nl=1e-10, gmres_restart=20, gmres_max_iters=400)
+    t0 = time.perf_counter()
+    for c in cs:
+        o.chandra_solve(CH_N, float(c), cfg, p, rule=1, form=1)
+    t = time.perf_counter() - t0
+    print(json.dumps({"seconds": t, "instances": len(cs), "threads": o.threads()}))
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -149,12 +220,17 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--legs", default="fp64,mixed")
+    ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.cpu_leg:
         cpu_leg(args)
+        return 0
+    if args.chandra_cpu_leg:
+        chandra_cpu_sample(args)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build "
@@ -231,6 +307,23 @@ def main() -> int:
                                "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                                "newton_iters": info["result"]["iters"],
                                "gmres_iters": info["result"]["n_lin_iters"]}
+    if not args.no_chandra:
+        ch = chandra_leg(H, ctx, args)
+        if not args.no_cpu:
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores")
+            pr = subprocess.run([sys.executable, os.path.abspath(__file__), "--chandra-cpu-leg"],
+                                env=env, capture_output=True, text=True, timeout=args.cpu_timeout)
+            if pr.returncode == 0:
+                info = json.loads(pr.stdout.strip().splitlines()[-1])
+                per = info["seconds"] / info["instances"]
+                ch["cpu_baseline"] = {"value": round(per * CH_B, 3), "unit": "s", "cores":
+                                      info["threads"], "kind": "port",
+                                      "sample": f"FP64 oracle Newton-GMRES(20) for {info['instances']} "
+                                                f"of the {CH_B} c values ({info['seconds']:.2f} s), "
+                                                f"scaled by {CH_B}/{info['instances']}"}
+        out["chandrasekhar"] = ch
     print(json.dumps(out))
     for leg in legs.values():
         leg["solver"].close()

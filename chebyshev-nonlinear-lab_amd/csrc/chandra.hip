@@ -1,0 +1,851 @@
+// chandra.hip - Chandrasekhar H-equation on gfx950 for a batch of parameters
+// c_b, solved by lockstep Newton-GMRES (methods/krylov_newton.hpp:69-114 with
+// linalg/gmres.hpp:61-223 and no preconditioner, as the reference's
+// apps/smoke_chandrasekhar.cpp and tests do).
+//
+// Operator (problems/chandrasekhar_op.hpp:31-136), per instance b:
+//   (A x)_i = c_b sum_j K_ij x_j,  K_ij = 0.5 w_j mu_i / (mu_i + mu_j)
+//   form 0 (reference):  F = 1 + x (A x) - x,   J v = v (Ax) + x (Av) - v
+//   form 1 (Kelley):     F = x - 1/(1 - A x),   J v = v - (A v)/(1 - Ax)^2
+// rule 0: Gauss-Legendre on [0,1] (functions/gauss_legendre_dynamic.hpp:50-114,
+// computed on the host in long double); rule 1: midpoint mu_i = (i-1/2)/N,
+// w = 1/N.
+//
+// All instances share one MFMA contraction per operator application
+// (chandra_gemm.hpp): FP64 for F(x) always, and for J v either FP64
+// (NUMA_PRECISION_FP64) or split-BF16 with FP32 accumulation
+// (NUMA_PRECISION_MIXED).  Everything else (Krylov basis, Givens, update) is
+// FP64.  Each instance follows the reference's step sequence exactly and keeps
+// its own forcing term, Krylov counters, line search and convergence; a
+// finished instance stops counting and stops changing its iterate.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <utility>
+#include <vector>
+
+#include "chandra_gemm.hpp"
+#include "common.hpp"
+
+namespace numa_hip {
+
+// ---------------------------------------------------------------------------
+// batched kernels over the padded [Bp][Np] layout; grid (x chunks, B)
+// ---------------------------------------------------------------------------
+constexpr int kChBX = 16;  // blocks per instance of the batched streaming kernels
+
+struct BEpi {   // per-instance epilogue of a batched reduction
+    int kind;   // EPI_STORE / EPI_SQRT / EPI_NORM_SCALE
+    const double* mul;  // EPI_STORE multiplier per instance (null = 1)
+    double* out;
+    double* out2;
+};
+
+__global__ __launch_bounds__(256) void k_bfinish(int nbx, const double* partials, BEpi ep) {
+    const int b = blockIdx.x;
+    double s = 0.0;
+    for (int x = threadIdx.x; x < nbx; x += 256) s += partials[b * nbx + x];
+    s = block_sum<256>(s);
+    if (threadIdx.x == 0) {
+        if (ep.kind == EPI_STORE) {
+            ep.out[b] = s * (ep.mul ? ep.mul[b] : 1.0);
+        } else {
+            const double r = sqrt(s);
+            ep.out[b] = r;
+            if (ep.kind == EPI_NORM_SCALE && ep.out2) ep.out2[b] = r > 0.0 ? 1.0 / r : 0.0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bdot(int N, int Np, const double* a, const double* c,
+                                               Reduce red) {
+    const int b = blockIdx.y;
+    const double* pa = a + (size_t)b * Np;
+    const double* pc = c + (size_t)b * Np;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) acc += pa[i] * pc[i];
+    block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
+}
+
+// w_b -= (h_b s_b) V_b ;  partial of <w_b, next_b> or <w_b, w_b> (next == null)
+__global__ __launch_bounds__(256) void k_bmgs(int N, int Np, double* w, const double* V,
+                                               const double* h, const double* s,
+                                               const double* next, Reduce red) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const double coef = h[b] * s[b];
+    double* pw = w + (size_t)b * Np;
+    const double* pv = V + (size_t)b * Np;
+    const double* pn = next ? next + (size_t)b * Np : nullptr;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const double nv = pn ? pn[i] : 0.0;
+        const double wn = pw[i] - coef * pv[i];
+        pw[i] = wn;
+        acc += pn ? wn * nv : wn * wn;
+    }
+    block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
+}
+
+// From the contraction Y = K X of the point X: Ax = c_b Y, F(X) and ||F||^2.
+__global__ __launch_bounds__(256) void k_ch_F(int N, int Np, int form, const double* c,
+                                               const double* X, const double* Y, double* Ax,
+                                               double* F, Reduce red) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const double x = X[o + i];
+        const double ax = c[b] * Y[o + i];
+        const double f = form == 0 ? 1.0 + x * ax - x : x - 1.0 / (1.0 - ax);
+        Ax[o + i] = ax;
+        F[o + i] = f;
+        acc += f * f;
+    }
+    block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
+}
+
+// out = scale_b * J(X) V from Y = K V (Av = c_b Y); optionally out = r - J V
+// (restart residual, then the partial is ||out||^2).
+__global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const double* c,
+                                                const double* X, const double* Ax,
+                                                const double* V, const double* Y,
+                                                const double* scale, const double* rhs,
+                                                double* out, Reduce red) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    const double sc = scale ? scale[b] : 1.0;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const double v = V[o + i], av = c[b] * Y[o + i], ax = Ax[o + i];
+        double jv;
+        if (form == 0) {
+            jv = v * ax + X[o + i] * av - v;
+        } else {
+            const double q = 1.0 - ax;
+            jv = v - av / (q * q);
+        }
+        jv = jv * sc;
+        if (rhs) jv = rhs[o + i] - jv;
+        out[o + i] = jv;
+        acc += jv * jv;
+    }
+    block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
+}
+
+// Xt_b = X_b - lam_b Z_b
+__global__ __launch_bounds__(256) void k_ch_trial(int N, int Np, const double* X, const double* Z,
+                                                   const double* lam, double* Xt) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+        Xt[o + i] = X[o + i] - lam[b] * Z[o + i];
+}
+
+// dst_b = src_b where flag_b != 0
+__global__ __launch_bounds__(256) void k_ch_select(int N, int Np, const double* flag,
+                                                    const double* src, double* dst) {
+    const int b = blockIdx.y;
+    if (flag[b] == 0.0) return;
+    const size_t o = (size_t)b * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) dst[o + i] = src[o + i];
+}
+
+// y_b += sum_k coef[k*B + b] V_k,b   (k < count)
+__global__ __launch_bounds__(256) void k_ch_combine(int N, int Np, int B, int count,
+                                                     const double* const* V, const double* coef,
+                                                     double* y) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        double s = y[o + i];
+        for (int k = 0; k < count; ++k) s += coef[k * B + b] * V[k][o + i];
+        y[o + i] = s;
+    }
+}
+
+// user [B][N] <-> padded [Bp][Np]
+__global__ __launch_bounds__(256) void k_ch_pack(int N, int Np, const double* src, double* dst,
+                                                  int to_padded) {
+    const int b = blockIdx.y;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        if (to_padded)
+            dst[(size_t)b * Np + i] = src[(size_t)b * N + i];
+        else
+            dst[(size_t)b * N + i] = src[(size_t)b * Np + i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// quadrature rules (host)
+// ---------------------------------------------------------------------------
+// gauss_legendre_unit (functions/gauss_legendre_dynamic.hpp:50-114): Newton on
+// the Legendre recurrence in long double, weights halved to sum to 1.
+inline void gauss_legendre_unit(size_t n, std::vector<double>& nodes, std::vector<double>& weights) {
+    nodes.assign(n, 0.0);
+    weights.assign(n, 0.0);
+    const size_t half = (n + 1) / 2;
+    const long double pi = std::acos(-1.0L);
+    const long double eps = 32.0L * std::numeric_limits<long double>::epsilon();
+    auto leg = [n](long double x, long double& p, long double& dp) {
+        long double p0 = 1.0L, p1 = x;
+        for (size_t k = 2; k <= n; ++k) {
+            const long double kd = (long double)k;
+            const long double pk = ((2.0L * kd - 1.0L) * x * p1 - (kd - 1.0L) * p0) / kd;
+            p0 = p1;
+            p1 = pk;
+        }
+        p = p1;
+        dp = (long double)n * (x * p1 - p0) / (x * x - 1.0L);
+    };
+    for (size_t i = 0; i < half; ++i) {
+        long double z = std::cos(pi * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        bool ok = false;
+        for (int it = 0; it < 100; ++it) {
+            long double p, dp;
+            leg(z, p, dp);
+            const long double d = p / dp;
+            z -= d;
+            if (std::abs(d) <= eps * (1.0L + std::abs(z))) {
+                ok = true;
+                break;
+            }
+        }
+        require(ok, "Gauss-Legendre Newton iteration did not converge", NUMA_ERR_HIP);
+        long double p, dp;
+        leg(z, p, dp);
+        const long double wpm = 2.0L / ((1.0L - z * z) * dp * dp);
+        nodes[i] = (double)(0.5L * (1.0L - z));
+        nodes[n - 1 - i] = (double)(0.5L * (1.0L + z));
+        weights[i] = (double)(0.5L * wpm);
+        weights[n - 1 - i] = (double)(0.5L * wpm);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched operator + solver
+// ---------------------------------------------------------------------------
+class ChandraBatch {
+public:
+    ChandraBatch(Ctx& c, int64_t n, int64_t B, const double* cvals, int rule, int form,
+                 int precision, int64_t restart)
+        : c_(c), N_(n), B_(B), rule_(rule), form_(form), prec_(precision), m_(restart) {
+        require(n > 0 && B > 0, "Chandrasekhar needs N > 0 and batch > 0");
+        require(n <= (1 << 20) && B * kChBX <= kMaxPartials, "Chandrasekhar size too large",
+                NUMA_ERR_OVERFLOW);
+        require(rule == 0 || rule == 1, "rule must be 0 (Gauss-Legendre) or 1 (midpoint)");
+        require(form == 0 || form == 1, "form must be 0 (reference) or 1 (Kelley)");
+        require(precision == NUMA_PRECISION_FP64 || precision == NUMA_PRECISION_MIXED,
+                "unknown precision");
+        require(restart >= 1 && restart <= 256, "gmres restart must be in [1, 256]");
+        for (int64_t b = 0; b < B; ++b)
+            require(std::isfinite(cvals[b]) && cvals[b] >= 0.0 && cvals[b] <= 1.0,
+                    "c must be finite in [0, 1] (chandrasekhar_op.hpp:40-43)");
+        Np_ = (n + kChPadN - 1) / kChPadN * kChPadN;
+        Bp_ = (B + kChPadB - 1) / kChPadB * kChPadB;
+        // quadrature, padded with w = 0, mu = 1
+        std::vector<double> mu, w;
+        if (rule == 0) {
+            gauss_legendre_unit((size_t)n, mu, w);
+        } else {
+            mu.resize(n);
+            w.assign(n, 1.0 / double(n));
+            for (int64_t i = 0; i < n; ++i) mu[i] = (double(i) + 0.5) / double(n);
+        }
+        mu.resize(Np_, 1.0);
+        w.resize(Np_, 0.0);
+        std::vector<float> muf(mu.begin(), mu.end()), wf(w.begin(), w.end());
+        std::vector<double> cv(cvals, cvals + B);
+        cv.resize(Bp_, 0.0);
+        mu_ = upload(mu);
+        w_ = upload(w);
+        c_dev_ = upload(cv);
+        muf_ = static_cast<float*>(alloc(Np_ * sizeof(float)));
+        wf_ = static_cast<float*>(alloc(Np_ * sizeof(float)));
+        NUMA_HIP_CHECK(hipMemcpy(muf_, muf.data(), Np_ * sizeof(float), hipMemcpyHostToDevice));
+        NUMA_HIP_CHECK(hipMemcpy(wf_, wf.data(), Np_ * sizeof(float), hipMemcpyHostToDevice));
+        const size_t vec = (size_t)Bp_ * Np_;
+        for (double** p : {&X_, &Xt_, &F_, &Ft_, &Ax_, &Axt_, &Z_, &Y_, &R_})
+            *p = static_cast<double*>(zalloc(vec * sizeof(double)));
+        V_.resize((size_t)m_ + 1);
+        for (auto& v : V_) v = static_cast<double*>(zalloc(vec * sizeof(double)));
+        if (prec_ == NUMA_PRECISION_MIXED) {
+            Xh_ = static_cast<__bf16*>(zalloc(vec * sizeof(__bf16)));
+            Xl_ = static_cast<__bf16*>(zalloc(vec * sizeof(__bf16)));
+            ksplit_ = (Np_ / kGbRows) * (Bp_ / kGbCols) >= 512 ? 1 : 2;
+            if ((Np_ / kGbRows) * (Bp_ / kGbCols) < 128) ksplit_ = 4;
+            P_ = static_cast<float*>(alloc((size_t)ksplit_ * vec * sizeof(float)));
+        }
+        // per-instance scalars: H ((m+1) x m x B), s ((m+1) x B), misc
+        nsc_ = ((m_ + 1) * m_ + (m_ + 1) + 8) * Bp_;
+        sd_ = static_cast<double*>(zalloc((size_t)nsc_ * sizeof(double)));
+        void* hp = nullptr;
+        NUMA_HIP_CHECK(hipHostMalloc(&hp, (size_t)nsc_ * sizeof(double), hipHostMallocDefault));
+        hs_ = static_cast<double*>(hp);
+        vptr_ = static_cast<const double**>(alloc((size_t)(m_ + 1) * sizeof(double*)));
+        coef_ = static_cast<double*>(alloc((size_t)(m_ + 1) * Bp_ * sizeof(double)));
+        NUMA_HIP_CHECK(hipMemcpy(vptr_, V_.data(), (m_ + 1) * sizeof(double*), hipMemcpyHostToDevice));
+    }
+    ~ChandraBatch() {
+        for (void* p : allocs_) (void)hipFree(p);
+        if (hs_) (void)hipHostFree(hs_);
+    }
+
+    int64_t size() const { return N_ * B_; }
+
+    // F for all instances at x (user layout [B][N]); ||F_b|| to norms (host).
+    void eval_user(const double* x, double* F, double* norms) {
+        pack(x, X_, true);
+        eval_F(X_, F_, Ax_, MISC(0));
+        pack(F_, F, false);
+        if (norms) read(MISC(0), B_, norms);
+        c_.sync();
+    }
+
+    // out = J(x) v for all instances (user layout), precision of the handle.
+    void jvp_user(const double* x, const double* v, double* out) {
+        pack(x, X_, true);
+        eval_F(X_, F_, Ax_, MISC(0));
+        pack(v, V_[0], true);
+        jv(V_[0], nullptr, nullptr, R_, nullptr);
+        pack(R_, out, false);
+        c_.sync();
+    }
+
+    // Average ms of one batched contraction and its useful FLOP count 2 N^2 B.
+    double gemm_cost(int precision, int64_t reps, double* flops) {
+        require(reps > 0, "reps must be positive");
+        require(precision == NUMA_PRECISION_FP64 || prec_ == NUMA_PRECISION_MIXED,
+                "the handle was created without the split-BF16 buffers");
+        launch_fill_rows(V_[0], 0.5);
+        auto run = [&] {
+            if (precision == NUMA_PRECISION_FP64)
+                gemm_f64(V_[0], Y_);
+            else
+                gemm_bf16(V_[0], Y_);
+        };
+        run();
+        hipEvent_t e0, e1;
+        NUMA_HIP_CHECK(hipEventCreate(&e0));
+        NUMA_HIP_CHECK(hipEventCreate(&e1));
+        NUMA_HIP_CHECK(hipEventRecord(e0, c_.stream));
+        for (int64_t r = 0; r < reps; ++r) run();
+        NUMA_HIP_CHECK(hipEventRecord(e1, c_.stream));
+        NUMA_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        NUMA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (flops) *flops = 2.0 * (double)N_ * (double)N_ * (double)B_;
+        return (double)ms / (double)reps;
+    }
+
+    void solve(const numa_krylov_params& p, double* xuser, numa_krylov_result* out);
+
+private:
+    double* MISC(int k) { return sd_ + ((m_ + 1) * m_ + (m_ + 1) + k) * Bp_; }
+    double* H(int64_t i, int64_t j) { return sd_ + (j * (m_ + 1) + i) * Bp_; }
+    double* S(int64_t i) { return sd_ + ((m_ + 1) * m_ + i) * Bp_; }
+
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+        allocs_.push_back(p);
+        return p;
+    }
+    void* zalloc(size_t bytes) {
+        void* p = alloc(bytes);
+        NUMA_HIP_CHECK(hipMemset(p, 0, bytes));
+        return p;
+    }
+    double* upload(const std::vector<double>& v) {
+        double* p = static_cast<double*>(alloc(v.size() * sizeof(double)));
+        NUMA_HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        return p;
+    }
+    void read(const double* dev, int64_t count, double* host) {
+        NUMA_HIP_CHECK(hipMemcpyAsync(hs_, dev, (size_t)count * sizeof(double),
+                                      hipMemcpyDeviceToHost, c_.stream));
+        c_.sync();
+        std::copy(hs_, hs_ + count, host);
+    }
+    void write(double* dev, const std::vector<double>& host) {
+        std::copy(host.begin(), host.end(), hs_);
+        NUMA_HIP_CHECK(hipMemcpyAsync(dev, hs_, host.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, c_.stream));
+        c_.sync();
+    }
+    dim3 bgrid() const { return dim3((unsigned)std::min<int64_t>(kChBX, (N_ + 255) / 256), (unsigned)B_); }
+    unsigned nbx() const { return bgrid().x; }
+
+    void pack(const double* src, double* dst, bool to_padded) {
+        hipLaunchKernelGGL(k_ch_pack, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, src, dst,
+                           to_padded ? 1 : 0);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+    void launch_fill_rows(double* dst, double v) {
+        std::vector<double> row((size_t)N_, v);
+        for (int64_t b = 0; b < B_; ++b)
+            NUMA_HIP_CHECK(hipMemcpyAsync(dst + b * Np_, row.data(), N_ * sizeof(double),
+                                          hipMemcpyHostToDevice, c_.stream));
+        c_.sync();
+    }
+    void bfinish(const BEpi& ep) {
+        hipLaunchKernelGGL(k_bfinish, dim3((unsigned)B_), dim3(256), 0, c_.stream, (int)nbx(),
+                           (const double*)c_.red.partials, ep);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+
+    void gemm_f64(const double* Xin, double* Yout) {
+        hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(Np_ / kG64Rows), (unsigned)(Bp_ / kG64Cols)),
+                           dim3(256), 0, c_.stream, (int)Np_, (const double*)mu_, (const double*)w_,
+                           Xin, Yout);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+    void gemm_bf16(const double* Xin, double* Yout) {
+        const int64_t total = (int64_t)Bp_ * Np_;
+        hipLaunchKernelGGL(k_split_x, dim3(2048), dim3(256), 0, c_.stream, total, Xin, Xh_, Xl_);
+        NUMA_HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_gemm_bf16x3,
+                           dim3((unsigned)(Np_ / kGbRows), (unsigned)(Bp_ / kGbCols), (unsigned)ksplit_),
+                           dim3(256), 0, c_.stream, (int)Np_, ksplit_, (const float*)muf_,
+                           (const float*)wf_, (const __bf16*)Xh_, (const __bf16*)Xl_, P_);
+        NUMA_HIP_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(2048), dim3(256), 0, c_.stream, total, ksplit_,
+                           (const float*)P_, Yout);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+
+    // F and Ax at X; ||F_b|| -> norms
+    void eval_F(const double* X, double* F, double* Ax, double* norms) {
+        gemm_f64(X, Y_);
+        hipLaunchKernelGGL(k_ch_F, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
+                           (const double*)c_dev_, X, (const double*)Y_, Ax, F, c_.red);
+        NUMA_HIP_LAUNCH_CHECK();
+        bfinish(BEpi{EPI_SQRT, nullptr, norms, nullptr});
+    }
+
+    // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
+    void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep) {
+        if (prec_ == NUMA_PRECISION_MIXED)
+            gemm_bf16(V, Y_);
+        else
+            gemm_f64(V, Y_);
+        hipLaunchKernelGGL(k_ch_jv, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
+                           (const double*)c_dev_, (const double*)X_, (const double*)Ax_, V,
+                           (const double*)Y_, scale, rhs, out, c_.red);
+        NUMA_HIP_LAUNCH_CHECK();
+        if (ep) bfinish(*ep);
+    }
+
+    void bdot(const double* a, const double* b, const BEpi& ep) {
+        hipLaunchKernelGGL(k_bdot, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, a, b, c_.red);
+        NUMA_HIP_LAUNCH_CHECK();
+        bfinish(ep);
+    }
+
+    struct GState {
+        bool active = false;  // participates in this GMRES
+        numa_gmres_result res{};
+    };
+    void gmres_batch(const std::vector<char>& active, const std::vector<double>& rtol,
+                     int64_t restart, int64_t max_iters, std::vector<numa_gmres_result>& out);
+
+    Ctx& c_;
+    int64_t N_, B_, Np_ = 0, Bp_ = 0;
+    int rule_, form_, prec_;
+    int64_t m_;
+    double *mu_ = nullptr, *w_ = nullptr, *c_dev_ = nullptr;
+    float *muf_ = nullptr, *wf_ = nullptr;
+    double *X_ = nullptr, *Xt_ = nullptr, *F_ = nullptr, *Ft_ = nullptr, *Ax_ = nullptr,
+           *Axt_ = nullptr, *Z_ = nullptr, *Y_ = nullptr, *R_ = nullptr;
+    std::vector<double*> V_;
+    __bf16 *Xh_ = nullptr, *Xl_ = nullptr;
+    float* P_ = nullptr;
+    int ksplit_ = 1;
+    int64_t nsc_ = 0;
+    double* sd_ = nullptr;
+    double* hs_ = nullptr;
+    const double** vptr_ = nullptr;
+    double* coef_ = nullptr;
+    std::vector<void*> allocs_;
+};
+
+// Lockstep GMRES(m) for all active instances on J(X_) z = F_ into Z_
+// (gmres.hpp:61-223 per instance; IdentityPrecond, whose applications are
+// counted as the reference counts them).
+void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vector<double>& rtol,
+                               int64_t restart, int64_t max_iters,
+                               std::vector<numa_gmres_result>& out) {
+    const int64_t B = B_;
+    const size_t m = (size_t)restart;
+    out.assign((size_t)B, numa_gmres_result{});
+    // x = 0 for everybody (the Newton caller passes z = 0)
+    NUMA_HIP_CHECK(hipMemsetAsync(Z_, 0, (size_t)Bp_ * Np_ * sizeof(double), c_.stream));
+    std::vector<double> bnorm((size_t)B), beta((size_t)B);
+    bdot(F_, F_, BEpi{EPI_SQRT, nullptr, MISC(1), nullptr});
+    read(MISC(1), B, bnorm.data());
+    std::vector<char> done((size_t)B, 1);
+    for (int64_t b = 0; b < B; ++b) {
+        if (!active[b]) continue;
+        if (bnorm[b] == 0.0) {
+            out[b].converged = 1;
+            continue;
+        }
+        if (restart == 0) {
+            out[b].rel_res = 1.0;
+            out[b].breakdown = 1;
+            continue;
+        }
+        if (max_iters == 0) {
+            out[b].rel_res = 1.0;
+            continue;
+        }
+        done[b] = 0;
+    }
+    std::vector<std::vector<std::vector<double>>> Hh((size_t)B);
+    std::vector<std::vector<double>> cs((size_t)B, std::vector<double>(m)), sn = cs,
+        g((size_t)B, std::vector<double>(m + 1)), sv((size_t)B, std::vector<double>(m + 1));
+    for (auto& h : Hh) h.assign(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> col((m + 2) * (size_t)Bp_);  // H columns are read with the padded stride
+    bool first = true;
+    auto any = [&](const std::vector<char>& f) {
+        return std::any_of(f.begin(), f.end(), [](char v) { return v != 0; });
+    };
+    std::vector<char> notdone((size_t)B);
+    for (int64_t b = 0; b < B; ++b) notdone[b] = !done[b];
+    while (any(notdone)) {
+        // cycle start: r = F (first) or F - J z
+        if (first) {
+            NUMA_HIP_CHECK(hipMemcpyAsync(V_[0], F_, (size_t)Bp_ * Np_ * sizeof(double),
+                                          hipMemcpyDeviceToDevice, c_.stream));
+            bdot(V_[0], V_[0], BEpi{EPI_NORM_SCALE, nullptr, MISC(2), S(0)});
+        } else {
+            const BEpi ep{EPI_NORM_SCALE, nullptr, MISC(2), S(0)};
+            jv(Z_, nullptr, F_, V_[0], &ep);
+            for (int64_t b = 0; b < B; ++b)
+                if (notdone[b]) ++out[b].matvecs;
+        }
+        read(MISC(2), B, beta.data());
+        std::vector<char> cyc((size_t)B, 0);
+        std::vector<size_t> k((size_t)B, 0);
+        for (int64_t b = 0; b < B; ++b) {
+            if (!notdone[b]) continue;
+            out[b].rel_res = beta[b] / bnorm[b];
+            if (out[b].rel_res <= rtol[b]) {
+                out[b].converged = 1;
+                notdone[b] = 0;
+                continue;
+            }
+            for (auto& row : Hh[b]) std::fill(row.begin(), row.end(), 0.0);
+            std::fill(g[b].begin(), g[b].end(), 0.0);
+            g[b][0] = beta[b];
+            sv[b][0] = beta[b] > 0.0 ? 1.0 / beta[b] : 0.0;
+            cyc[b] = out[b].iters < max_iters;
+        }
+        first = false;
+        for (size_t j = 0; j < m && any(cyc); ++j) {
+            // w = s_j J V_j into V_{j+1}; MGS
+            jv(V_[j], S(j), nullptr, V_[j + 1], nullptr);
+            bdot(V_[j + 1], V_[0], BEpi{EPI_STORE, S(0), H(0, j), nullptr});
+            for (size_t i = 0; i <= j; ++i) {
+                const bool last = i == j;
+                hipLaunchKernelGGL(k_bmgs, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
+                                   V_[j + 1], (const double*)V_[i], (const double*)H(i, j),
+                                   (const double*)S(i), last ? nullptr : (const double*)V_[i + 1],
+                                   c_.red);
+                NUMA_HIP_LAUNCH_CHECK();
+                bfinish(last ? BEpi{EPI_NORM_SCALE, nullptr, H(j + 1, j), S(j + 1)}
+                             : BEpi{EPI_STORE, S(i + 1), H(i + 1, j), nullptr});
+            }
+            read(H(0, j), (int64_t)(j + 2) * Bp_, col.data());
+            for (int64_t b = 0; b < B; ++b) {
+                if (!cyc[b]) continue;
+                auto& Hb = Hh[b];
+                for (size_t i = 0; i <= j + 1; ++i) Hb[i][j] = col[i * Bp_ + b];
+                sv[b][j + 1] = Hb[j + 1][j] > 0.0 ? 1.0 / Hb[j + 1][j] : 0.0;
+                ++out[b].precs;
+                ++out[b].matvecs;
+                for (size_t i = 0; i < j; ++i) {
+                    const double t = cs[b][i] * Hb[i][j] + sn[b][i] * Hb[i + 1][j];
+                    Hb[i + 1][j] = -sn[b][i] * Hb[i][j] + cs[b][i] * Hb[i + 1][j];
+                    Hb[i][j] = t;
+                }
+                const double denom = std::hypot(Hb[j][j], Hb[j + 1][j]);
+                if (!(denom > 0.0) || !std::isfinite(denom)) {
+                    out[b].breakdown = 1;
+                    cyc[b] = 0;
+                    notdone[b] = 0;
+                    k[b] = 0;  // the reference returns before updating x
+                    continue;
+                }
+                cs[b][j] = Hb[j][j] / denom;
+                sn[b][j] = Hb[j + 1][j] / denom;
+                Hb[j][j] = denom;
+                Hb[j + 1][j] = 0.0;
+                g[b][j + 1] = -sn[b][j] * g[b][j];
+                g[b][j] = cs[b][j] * g[b][j];
+                ++k[b];
+                ++out[b].iters;
+                out[b].rel_res = std::abs(g[b][j + 1]) / bnorm[b];
+                if (out[b].rel_res <= rtol[b] || out[b].iters >= max_iters) cyc[b] = 0;
+            }
+        }
+        // back substitution and x += V (y s) per instance (gmres.hpp:185-202)
+        std::vector<double> coef((m + 1) * (size_t)B, 0.0);
+        size_t kmax = 0;
+        for (int64_t b = 0; b < B; ++b) {
+            if (!notdone[b] || out[b].breakdown) continue;
+            const size_t kb = k[b];
+            std::vector<double> y(kb, 0.0);
+            bool bad = false;
+            for (size_t ii = kb; ii-- > 0;) {
+                const double pivot = Hh[b][ii][ii];
+                if (!(std::abs(pivot) > 0.0) || !std::isfinite(pivot)) {
+                    bad = true;
+                    break;
+                }
+                double acc = g[b][ii];
+                for (size_t jj = ii + 1; jj < kb; ++jj) acc -= Hh[b][ii][jj] * y[jj];
+                y[ii] = acc / pivot;
+            }
+            if (bad) {
+                out[b].breakdown = 1;
+                notdone[b] = 0;
+                continue;
+            }
+            for (size_t i = 0; i < kb; ++i) coef[i * B + b] = y[i] * sv[b][i];
+            kmax = std::max(kmax, kb);
+            ++out[b].precs;  // M^{-1}(V y) with the identity
+        }
+        if (kmax > 0) {
+            NUMA_HIP_CHECK(hipMemcpyAsync(coef_, coef.data(), kmax * B * sizeof(double),
+                                          hipMemcpyHostToDevice, c_.stream));
+            hipLaunchKernelGGL(k_ch_combine, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
+                               (int)B, (int)kmax, (const double* const*)vptr_,
+                               (const double*)coef_, Z_);
+            NUMA_HIP_LAUNCH_CHECK();
+            c_.sync();
+        }
+        for (int64_t b = 0; b < B; ++b) {
+            if (!notdone[b]) continue;
+            if (out[b].rel_res <= rtol[b]) {
+                out[b].converged = 1;
+                notdone[b] = 0;
+            } else if (out[b].iters >= max_iters) {
+                notdone[b] = 0;
+            }
+        }
+    }
+}
+
+// newton_krylov (krylov_newton.hpp:69-114) for every instance, in lockstep.
+void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov_result* res) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t B = B_;
+    require(p.gmres_restart >= 0 && p.gmres_restart <= m_,
+            "gmres_restart exceeds the handle's basis capacity");
+    for (int64_t b = 0; b < B; ++b) res[b] = numa_krylov_result{};
+    pack(xuser, X_, true);
+    std::vector<double> rnorm((size_t)B), rprev((size_t)B), eta_prev((size_t)B, p.eta_init),
+        eta((size_t)B), lam((size_t)B), fn((size_t)B), acc((size_t)B);
+    eval_F(X_, F_, Ax_, MISC(0));
+    read(MISC(0), B, rnorm.data());
+    std::vector<char> active((size_t)B, 1);
+    for (int64_t b = 0; b < B; ++b) {
+        res[b].n_F = 1;
+        res[b].res0 = rnorm[b];
+        rprev[b] = rnorm[b];
+    }
+    std::vector<numa_gmres_result> g;
+    for (int64_t k = 0; k < p.max_nl; ++k) {
+        bool any = false;
+        for (int64_t b = 0; b < B; ++b) {
+            if (!active[b]) continue;
+            if (rnorm[b] <= p.rtol_nl * res[b].res0) {
+                res[b].converged = 1;
+                active[b] = 0;
+                continue;
+            }
+            any = true;
+            // krylov_newton.hpp:54-65
+            double e;
+            if (p.forcing == NUMA_FORCING_CONSTANT || k == 0) {
+                e = p.eta_init;
+            } else {
+                e = p.gamma * std::pow(rnorm[b] / rprev[b], p.alpha);
+                const double safe = p.gamma * std::pow(eta_prev[b], p.alpha);
+                if (safe > 0.1) e = std::max(e, safe);
+                e = std::min(e, p.eta_max);
+                e = std::max(e, 0.5 * p.rtol_nl * res[b].res0 / rnorm[b]);
+                e = std::min(e, p.eta_max);
+            }
+            eta[b] = e;
+        }
+        if (!any) break;
+        gmres_batch(active, eta, p.gmres_restart, p.gmres_max_iters, g);
+        std::vector<char> pending((size_t)B, 0);
+        for (int64_t b = 0; b < B; ++b) {
+            if (!active[b]) continue;
+            res[b].n_JVP += g[b].matvecs;
+            res[b].n_lin_iters += g[b].iters;
+            res[b].n_prec += g[b].precs;
+            if (!g[b].converged) ++res[b].n_lin_failures;
+            if (g[b].breakdown) {
+                active[b] = 0;
+                continue;
+            }
+            lam[b] = 1.0;
+            pending[b] = 1;
+        }
+        // Armijo backtracking (krylov_newton.hpp:96-104), all pending instances at once
+        std::vector<char> accepted((size_t)B, 0);
+        while (std::any_of(pending.begin(), pending.end(), [](char v) { return v != 0; })) {
+            std::vector<double> l((size_t)Bp_, 0.0);
+            for (int64_t b = 0; b < B; ++b) l[b] = pending[b] ? lam[b] : 0.0;
+            write(MISC(3), l);
+            hipLaunchKernelGGL(k_ch_trial, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
+                               (const double*)X_, (const double*)Z_, (const double*)MISC(3), Xt_);
+            NUMA_HIP_LAUNCH_CHECK();
+            eval_F(Xt_, Ft_, Axt_, MISC(0));
+            read(MISC(0), B, fn.data());
+            std::vector<double> take((size_t)Bp_, 0.0);
+            for (int64_t b = 0; b < B; ++b) {
+                if (!pending[b]) continue;
+                ++res[b].n_F;
+                if (fn[b] <= (1.0 - p.armijo * lam[b]) * rnorm[b]) {
+                    pending[b] = 0;
+                    accepted[b] = 1;
+                    take[b] = 1.0;
+                    acc[b] = fn[b];
+                } else {
+                    lam[b] *= 0.5;
+                    if (lam[b] < p.min_lambda) {
+                        pending[b] = 0;
+                        active[b] = 0;  // line search failed: stop this instance
+                    }
+                }
+            }
+            write(MISC(4), take);
+            for (auto pr : {std::pair<double*, double*>{Xt_, X_}, {Ft_, F_}, {Axt_, Ax_}}) {
+                hipLaunchKernelGGL(k_ch_select, bgrid(), dim3(256), 0, c_.stream, (int)N_,
+                                   (int)Np_, (const double*)MISC(4), (const double*)pr.first,
+                                   pr.second);
+                NUMA_HIP_LAUNCH_CHECK();
+            }
+        }
+        for (int64_t b = 0; b < B; ++b) {
+            if (!accepted[b]) continue;
+            rprev[b] = rnorm[b];
+            rnorm[b] = acc[b];
+            eta_prev[b] = eta[b];
+            ++res[b].iters;
+        }
+    }
+    pack(X_, xuser, false);
+    c_.sync();
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int64_t b = 0; b < B; ++b) {
+        if (rnorm[b] <= p.rtol_nl * res[b].res0) res[b].converged = 1;
+        res[b].res = rnorm[b];
+        res[b].t_total = t;
+    }
+}
+
+}  // namespace numa_hip
+
+struct numa_hip_chandra {
+    numa_hip::Ctx* ctx;
+    std::unique_ptr<numa_hip::ChandraBatch> impl;
+};
+
+using numa_hip::Error;
+
+#define NUMA_API_BEGIN try {
+#define NUMA_API_END                                        \
+    }                                                       \
+    catch (const Error& e) {                                \
+        numa_hip::set_last_error(e.what());                 \
+        return e.code;                                      \
+    }                                                       \
+    catch (const std::bad_alloc&) {                         \
+        numa_hip::set_last_error("host allocation failed"); \
+        return NUMA_ERR_NOMEM;                              \
+    }                                                       \
+    catch (const std::exception& e) {                       \
+        numa_hip::set_last_error(e.what());                 \
+        return NUMA_ERR_HIP;                                \
+    }                                                       \
+    return NUMA_OK;
+
+extern "C" {
+
+int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
+                            int64_t rule, int64_t form, int64_t precision, int64_t gmres_restart,
+                            numa_hip_chandra** out) {
+    NUMA_API_BEGIN
+    numa_hip::require(ctx && ctx->impl && c_host && out, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(ctx->impl->device));
+    auto h = std::make_unique<numa_hip_chandra>();
+    h->ctx = ctx->impl;
+    h->impl = std::make_unique<numa_hip::ChandraBatch>(*ctx->impl, n, batch, c_host, (int)rule,
+                                                       (int)form, (int)precision, gmres_restart);
+    ++ctx->impl->live_handles;
+    *out = h.release();
+    NUMA_API_END
+}
+
+int numa_hip_chandra_destroy(numa_hip_chandra* h) {
+    NUMA_API_BEGIN
+    if (h) {
+        NUMA_HIP_CHECK(hipStreamSynchronize(h->ctx->stream));
+        --h->ctx->live_handles;
+        delete h;
+    }
+    NUMA_API_END
+}
+
+int numa_hip_chandra_eval(numa_hip_chandra* h, const double* x_dev, double* F_dev,
+                          double* norms_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(h && x_dev && F_dev, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    h->impl->eval_user(x_dev, F_dev, norms_host);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_jvp(numa_hip_chandra* h, const double* x_dev, const double* v_dev,
+                         double* out_dev) {
+    NUMA_API_BEGIN
+    numa_hip::require(h && x_dev && v_dev && out_dev, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    h->impl->jvp_user(x_dev, v_dev, out_dev);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_solve(numa_hip_chandra* h, const numa_krylov_params* p, double* x_dev,
+                           numa_krylov_result* results) {
+    NUMA_API_BEGIN
+    numa_hip::require(h && p && x_dev && results, "null argument");
+    numa_hip::require(p->max_nl >= 0 && p->gmres_max_iters >= 0, "negative iteration limits");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    h->impl->solve(*p, x_dev, results);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_gemm_cost(numa_hip_chandra* h, int64_t precision, int64_t reps,
+                               double* ms_per_call, double* flops_per_call) {
+    NUMA_API_BEGIN
+    numa_hip::require(h && ms_per_call, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    *ms_per_call = h->impl->gemm_cost((int)precision, reps, flops_per_call);
+    NUMA_API_END
+}
+
+}  // extern "C"

@@ -167,6 +167,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
         "numa_hip_solve_host": [_vp, C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)],
         "numa_hip_primitive_cost": [_vp, _i64, _i64, _dp, _dp],
+        "numa_hip_chandra_create": [_vp, _i64, _i64, _dp, _i64, _i64, _i64, _i64, C.POINTER(_vp)],
+        "numa_hip_chandra_destroy": [_vp],
+        "numa_hip_chandra_eval": [_vp, _vp, _vp, _dp],
+        "numa_hip_chandra_jvp": [_vp, _vp, _vp, _vp],
+        "numa_hip_chandra_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
+        "numa_hip_chandra_gemm_cost": [_vp, _i64, _i64, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -423,3 +429,68 @@ def chebyshev_inexact(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restar
     r = s.solve_host(params, x)
     s.close()
     return x, r
+
+
+class ChandrasekharBatch:
+    """problems/chandrasekhar_op.hpp:31-136 for a batch of parameters c on the
+    device (rule 0 Gauss-Legendre / 1 midpoint; form 0 reference / 1 Kelley),
+    solved by lockstep newton_krylov (krylov_newton.hpp:69-114).
+    Vectors are instance-major numpy arrays of shape (batch, n)."""
+
+    def __init__(self, ctx: Context, n: int, cvals, rule: int = 0, form: int = 0,
+                 precision: int = PRECISION_FP64, restart: int = 50):
+        self.ctx, self.n = ctx, int(n)
+        self.c = np.ascontiguousarray(cvals, np.float64).ravel()
+        self.batch = self.c.size
+        h = _vp()
+        check(ctx.lib.numa_hip_chandra_create(ctx.h, self.n, self.batch,
+                                              self.c.ctypes.data_as(_dp), rule, form, precision,
+                                              int(restart), C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def _arr(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(self.batch, self.n)
+        return self.ctx.array(x.ravel())
+
+    def eval(self, x):
+        dx = self._arr(x)
+        dF = self.ctx.empty(self.batch * self.n)
+        norms = np.empty(self.batch)
+        check(self.ctx.lib.numa_hip_chandra_eval(self.h, dx.ptr, dF.ptr, norms.ctypes.data_as(_dp)))
+        return dF.numpy().reshape(self.batch, self.n), norms
+
+    def jvp(self, x, v):
+        dx, dv = self._arr(x), self._arr(v)
+        do = self.ctx.empty(self.batch * self.n)
+        check(self.ctx.lib.numa_hip_chandra_jvp(self.h, dx.ptr, dv.ptr, do.ptr))
+        return do.numpy().reshape(self.batch, self.n)
+
+    def solve(self, params: KrylovParams, x0=None):
+        x0 = np.ones((self.batch, self.n)) if x0 is None else x0
+        dx = self._arr(x0)
+        res = (KrylovResult * self.batch)()
+        check(self.ctx.lib.numa_hip_chandra_solve(self.h, C.byref(params), dx.ptr, res))
+        return dx.numpy().reshape(self.batch, self.n), list(res)
+
+    def solve_device(self, params: KrylovParams, dx: "DeviceArray"):
+        res = (KrylovResult * self.batch)()
+        check(self.ctx.lib.numa_hip_chandra_solve(self.h, C.byref(params), dx.ptr, res))
+        return list(res)
+
+    def gemm_cost(self, precision: int, reps: int = 10):
+        ms, fl = C.c_double(), C.c_double()
+        check(self.ctx.lib.numa_hip_chandra_gemm_cost(self.h, precision, reps, C.byref(ms),
+                                                      C.byref(fl)))
+        return ms.value, fl.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_chandra_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

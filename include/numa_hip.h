@@ -225,6 +225,36 @@ int numa_hip_solve(numa_hip_solver* s, const numa_krylov_params* p,
 int numa_hip_solve_host(numa_hip_solver* s, const numa_krylov_params* p,
                         double* x_host, numa_krylov_result* out);
 
+/* ---- Chandrasekhar H-equation, batch of parameters c ---------------------- */
+/* problems/chandrasekhar_op.hpp:31-136 for `batch` values of c at once:
+ * (A x)_i = c sum_j 0.5 w_j mu_i/(mu_i+mu_j) x_j with rule 0 = Gauss-Legendre
+ * on [0,1] (functions/gauss_legendre_dynamic.hpp:50-114) or rule 1 = midpoint
+ * mu_i = (i-1/2)/N, w = 1/N; form 0: F = 1 + x(Ax) - x (reference), form 1:
+ * F = x - 1/(1 - Ax).  Vectors are instance-major: x[b*n + i].  The contraction
+ * runs on MFMA: FP64 for F; for J v FP64 (precision FP64) or split-BF16 with
+ * FP32 accumulation (precision MIXED). */
+typedef struct numa_hip_chandra numa_hip_chandra;
+int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
+                            int64_t rule, int64_t form, int64_t precision,
+                            int64_t gmres_restart, numa_hip_chandra** out);
+int numa_hip_chandra_destroy(numa_hip_chandra* h);
+/* F for every instance (ChandrasekharOp::eval, chandrasekhar_op.hpp:56-66);
+ * norms_host (batch doubles, may be NULL) receives ||F_b||. */
+int numa_hip_chandra_eval(numa_hip_chandra* h, const double* x_dev, double* F_dev,
+                          double* norms_host);
+/* J v for every instance (ChandrasekharOp::jvp, chandrasekhar_op.hpp:68-79). */
+int numa_hip_chandra_jvp(numa_hip_chandra* h, const double* x_dev, const double* v_dev,
+                         double* out_dev);
+/* newton_krylov (krylov_newton.hpp:69-114, no preconditioner) for every
+ * instance; x_dev holds the starting points and receives the iterates;
+ * results[batch] receives each instance's KrylovResult. */
+int numa_hip_chandra_solve(numa_hip_chandra* h, const numa_krylov_params* p, double* x_dev,
+                           numa_krylov_result* results);
+/* Average ms of one batched contraction (FP64 or split-BF16) and its useful
+ * FLOP count 2 n^2 batch. */
+int numa_hip_chandra_gemm_cost(numa_hip_chandra* h, int64_t precision, int64_t reps,
+                               double* ms_per_call, double* flops_per_call);
+
 /* ---- per-call primitive costs (instrumentation) -------------------------- */
 /* The reference freezes per-call costs c_F, c_JVP, c_D2, c_prec of its
  * primitives (experiments/c1/frozen/primitive_costs.csv, measured by
