@@ -182,6 +182,102 @@ def chandra_leg(H, ctx, args) -> dict:
     return out
 
 
+LEARN_LAMBDAS = (5.0, 6.0, 6.5, 6.7)
+CONV_FLOP_PER_POINT = 2 * 27 * (1 * 24 + 4 * 24 * 24 + 24 * 1)  # 127,008
+
+
+def learned_leg(H, ctx, args, params) -> dict:
+    """Learned preconditioner.  The conv net is trained on the GPU on
+    (residual, V(2,2) correction) pairs generated by mixed Newton-FGMRES solves
+    (16^3-32^3, lambda ~ U[1, 6.7], residual-norm loss), then Newton-FGMRES(30)
+    at n^3 for lambda in {5.0, 6.0, 6.5, 6.7} runs with the network and the
+    V-cycle fallback (a step reducing the residual by less than 0.5 switches the
+    rest of that linear solve to the V-cycle) next to the V-cycle alone."""
+    import numpy as np
+    n = args.n
+    net = H.ConvNet(ctx, seed=21)
+    tcfg = H.net_train_config(pairs=args.learned_pairs, batch=4, sizes=(16, 24, 32), lr=2e-3,
+                              seed=5, mg=H.mg_cheb(levels=3), loss=H.LOSS_RESIDUAL)
+    st, hist = net.train_solver(tcfg, hist_len=100000)
+    k = max(1, len(hist) // 10)
+    out = {"train": {"pairs": int(st.pairs), "steps": int(st.steps), "solves": int(st.solves),
+                     "seconds": round(st.seconds, 3), "sizes": [16, 24, 32],
+                     "loss": "residual", "loss_first_window": round(float(hist[:k].mean()), 4),
+                     "loss_last_window": round(float(hist[-k:].mean()), 4)}}
+    log(f"learned: trained on {st.pairs} pairs in {st.seconds:.2f} s, loss "
+        f"{out['train']['loss_first_window']} -> {out['train']['loss_last_window']}")
+    mg = H.mg_cheb(levels=args.levels, degree=3, pre=2, post=2)
+    cfg_l = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_LEARNED,
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
+    cfg_g = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
+    zeros = np.zeros(n ** 3)
+    runs = []
+    for lam in LEARN_LAMBDAS:
+        row = {"lambda": lam}
+        sols = {}
+        for name, cfg in (("learned", cfg_l), ("gmg", cfg_g)):
+            s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+            if name == "learned":
+                s.set_net(net, 0.5)
+            x = ctx.empty(n ** 3)
+            x.upload(zeros)
+            s.solve(params, x)  # warm-up
+            x.upload(zeros)
+            st0 = s.learned_stats() if name == "learned" else None
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = s.solve(params, x)
+            ctx.sync()
+            t = time.perf_counter() - t0
+            row[name] = {"s": round(t, 5), "newton": int(r.iters), "fgmres": int(r.n_lin_iters),
+                         "converged": int(r.converged), "rel_res": r.res / r.res0}
+            if name == "learned":
+                st1 = s.learned_stats()
+                row[name].update({k2: st1[k2] - st0[k2] for k2 in st1})
+            sols[name] = x.numpy()
+            s.close()
+            x.free()
+        row["rel_l2_learned_vs_gmg"] = float(np.linalg.norm(sols["learned"] - sols["gmg"]) /
+                                             np.linalg.norm(sols["gmg"]))
+        log(f"  lambda {lam}: {row}")
+        runs.append(row)
+    out["solves"] = runs
+    # what the trained network gains, network only (no fallback): FGMRES
+    # residual after 8 steps on J(0) z = b at 64^3
+    gain = {}
+    gn = 64
+    for rhs in ("random", "constant"):
+        bv = (np.random.default_rng(31).normal(size=gn ** 3) if rhs == "random"
+              else -6.0 * np.ones(gn ** 3))
+        b, x0, z = ctx.array(bv), ctx.array(np.zeros(gn ** 3)), ctx.empty(gn ** 3)
+        row = {}
+        untrained = H.ConvNet(ctx, seed=21)
+        for name, pc, nt in (("trained", H.PRECOND_LEARNED, net),
+                             ("untrained", H.PRECOND_LEARNED, untrained),
+                             ("none", H.PRECOND_NONE, None), ("gmg", H.PRECOND_GMG, None)):
+            c = H.solver_config(precond=pc, precision=H.PRECISION_MIXED, flexible=1,
+                                mg=H.mg_cheb(levels=4))
+            s = H.BratuSolver(ctx, gn, 6.0, c, restart=30)
+            if nt is not None:
+                s.set_net(nt, 1.0)
+            row[name] = s.gmres(x0, b, z, 1e-14, 30, 8).rel_res
+            s.close()
+        untrained.close()
+        gain[rhs] = row
+    out["gain_64cubed_rel_res_after_8_fgmres_steps"] = gain
+    log(f"  gain: {gain}")
+    ms_inf = net.cost(n, 0, 1, 3)
+    ms_tr = net.cost(64, 1, 4, 3)
+    out["inference_ms"] = round(ms_inf, 3)
+    out["inference_useful_tflops"] = round(CONV_FLOP_PER_POINT * n ** 3 / (ms_inf * 1e-3) / 1e12, 1)
+    out["inference_frac_of_bf16_2500"] = round(out["inference_useful_tflops"] /
+                                               BF16_MFMA_PEAK_TFLOPS, 4)
+    out["train_step_64cubed_batch4_ms"] = round(ms_tr, 3)
+    net.close()
+    return out
+
+
 def chandra_cpu_sample(args) -> None:
     """Child process: FP64 oracle Newton-GMRES(20) for the first 2 c values."""
     import oracle as O
@@ -221,6 +317,8 @@ def main() -> int:
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--legs", default="fp64,mixed")
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--learned-pairs", type=int, default=4000)
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-out", default="", help=argparse.SUPPRESS)
@@ -324,6 +422,8 @@ def main() -> int:
                                                 f"of the {CH_B} c values ({info['seconds']:.2f} s), "
                                                 f"scaled by {CH_B}/{info['instances']}"}
         out["chandrasekhar"] = ch
+    if not args.no_learned:
+        out["learned"] = learned_leg(H, ctx, args, params)
     print(json.dumps(out))
     for leg in legs.values():
         leg["solver"].close()

@@ -82,22 +82,6 @@ using numa_hip::Epilogue;
 using numa_hip::Error;
 using numa_hip::require;
 
-#define NUMA_API_BEGIN try {
-#define NUMA_API_END                                        \
-    }                                                       \
-    catch (const Error& e) {                                \
-        numa_hip::set_last_error(e.what());                 \
-        return e.code;                                      \
-    }                                                       \
-    catch (const std::bad_alloc&) {                         \
-        numa_hip::set_last_error("host allocation failed"); \
-        return NUMA_ERR_NOMEM;                              \
-    }                                                       \
-    catch (const std::exception& e) {                       \
-        numa_hip::set_last_error(e.what());                 \
-        return NUMA_ERR_HIP;                                \
-    }                                                       \
-    return NUMA_OK;
 
 namespace {
 

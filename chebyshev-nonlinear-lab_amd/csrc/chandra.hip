@@ -235,7 +235,7 @@ class ChandraBatch {
 public:
     ChandraBatch(Ctx& c, int64_t n, int64_t B, const double* cvals, int rule, int form,
                  int precision, int64_t restart)
-        : c_(c), N_(n), B_(B), rule_(rule), form_(form), prec_(precision), m_(restart) {
+        : c_(c), N_(n), B_(B), form_(form), prec_(precision), m_(restart) {
         require(n > 0 && B > 0, "Chandrasekhar needs N > 0 and batch > 0");
         require(n <= (1 << 20) && B * kChBX <= kMaxPartials, "Chandrasekhar size too large",
                 NUMA_ERR_OVERFLOW);
@@ -459,7 +459,7 @@ private:
 
     Ctx& c_;
     int64_t N_, B_, Np_ = 0, Bp_ = 0;
-    int rule_, form_, prec_;
+    int form_, prec_;
     int64_t m_;
     double *mu_ = nullptr, *w_ = nullptr, *c_dev_ = nullptr;
     float *muf_ = nullptr, *wf_ = nullptr;
@@ -767,22 +767,6 @@ struct numa_hip_chandra {
 
 using numa_hip::Error;
 
-#define NUMA_API_BEGIN try {
-#define NUMA_API_END                                        \
-    }                                                       \
-    catch (const Error& e) {                                \
-        numa_hip::set_last_error(e.what());                 \
-        return e.code;                                      \
-    }                                                       \
-    catch (const std::bad_alloc&) {                         \
-        numa_hip::set_last_error("host allocation failed"); \
-        return NUMA_ERR_NOMEM;                              \
-    }                                                       \
-    catch (const std::exception& e) {                       \
-        numa_hip::set_last_error(e.what());                 \
-        return NUMA_ERR_HIP;                                \
-    }                                                       \
-    return NUMA_OK;
 
 extern "C" {
 

@@ -140,6 +140,25 @@ inline unsigned stream_blocks(int64_t N) {
 
 }  // namespace numa_hip
 
+// Body guards of every C entry point: exceptions become status codes and a
+// thread-local message.
+#define NUMA_API_BEGIN try {
+#define NUMA_API_END                                          \
+    }                                                         \
+    catch (const ::numa_hip::Error& e) {                      \
+        ::numa_hip::set_last_error(e.what());                 \
+        return e.code;                                        \
+    }                                                         \
+    catch (const std::bad_alloc&) {                           \
+        ::numa_hip::set_last_error("host allocation failed"); \
+        return NUMA_ERR_NOMEM;                                \
+    }                                                         \
+    catch (const std::exception& e) {                         \
+        ::numa_hip::set_last_error(e.what());                 \
+        return NUMA_ERR_HIP;                                  \
+    }                                                         \
+    return NUMA_OK;
+
 // Opaque C handle of a context.
 struct numa_hip_ctx {
     numa_hip::Ctx* impl;

@@ -20,25 +20,8 @@
 
 #include "blas.hpp"
 #include "mg.hpp"
+#include "solver_base.hpp"
 #include "stencil.hpp"
-
-namespace numa_hip {
-
-struct SolverBase {
-    virtual ~SolverBase() = default;
-    virtual numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
-                                          int64_t restart, int64_t max_iters, bool zero_guess) = 0;
-    virtual void solve(const numa_krylov_params& p, double* x, numa_krylov_result& out) = 0;
-    virtual int64_t size() const = 0;
-    virtual double primitive_cost(int64_t prim, int64_t reps, double* bytes) = 0;
-};
-
-}  // namespace numa_hip
-
-struct numa_hip_solver {
-    numa_hip::Ctx* ctx;
-    std::unique_ptr<numa_hip::SolverBase> impl;
-};
 
 struct numa_hip_mg {
     numa_hip::Ctx* ctx;
@@ -71,9 +54,12 @@ public:
         require(n > 0, "Bratu3D needs n > 0");
         require(n <= 2048, "grid too large", NUMA_ERR_OVERFLOW);
         require(restart > 0 && restart <= 256, "gmres restart must be in [1, 256]");
-        require(cfg.precond >= NUMA_PRECOND_NONE && cfg.precond <= NUMA_PRECOND_GMG,
+        require(cfg.precond >= NUMA_PRECOND_NONE && cfg.precond <= NUMA_PRECOND_LEARNED,
                 "unknown preconditioner");
-        if (cfg.precond == NUMA_PRECOND_GMG) mg_.reset(make_mg(c, n, cfg.mg, cfg.precision));
+        require(cfg.precond != NUMA_PRECOND_LEARNED || cfg.flexible,
+                "the learned preconditioner is nonlinear and needs FGMRES (flexible = 1)");
+        if (cfg.precond == NUMA_PRECOND_GMG || cfg.precond == NUMA_PRECOND_LEARNED)
+            mg_.reset(make_mg(c, n, cfg.mg, cfg.precision));
         x2_ = allocd();
         Fx_ = allocd();
         Fn_ = allocd();
@@ -104,6 +90,20 @@ public:
         if (hs_) (void)hipHostFree(hs_);
     }
     int64_t size() const override { return N_; }
+    int64_t grid_n() const override { return n_; }
+    void set_lambda(double lambda) override {
+        require(std::isfinite(lambda), "lambda must be finite");
+        lambda_ = lambda;
+    }
+    void set_capture(CaptureFn fn) override { capture_ = std::move(fn); }
+    void set_learned(std::shared_ptr<NetApply> net, double ratio) override {
+        require(cfg_.precond == NUMA_PRECOND_LEARNED,
+                "solver was not created with NUMA_PRECOND_LEARNED");
+        require(ratio > 0.0 && ratio <= 1.0, "fallback ratio must be in (0, 1]");
+        net_ = std::move(net);
+        ratio_ = ratio;
+    }
+    LearnedStats learned_stats() const override { return lstats_; }
 
     numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
                                   int64_t restart, int64_t max_iters, bool zero_guess) override {
@@ -302,16 +302,31 @@ private:
     // Preconditioner M^-1 in -> out; returns the result pointer (in itself
     // for no preconditioner, IdentityPrecond of gmres.hpp:37-39).
     const T* precond(const T* in, T* out) {
+        const T* res = in;
         switch (cfg_.precond) {
+            case NUMA_PRECOND_LEARNED:
+                if (net_ && use_net_) {
+                    net_->apply(n_, in, out);
+                    ++lstats_.net_applications;
+                } else {
+                    mg_->apply(in, out);
+                    ++lstats_.gmg_applications;
+                }
+                res = out;
+                break;
             case NUMA_PRECOND_GMG:
                 mg_->apply(in, out);
-                return out;
+                res = out;
+                break;
             case NUMA_PRECOND_JACOBI:
                 launch_diag_solve<T>(c_, N_, in, diag_, out);
-                return out;
+                res = out;
+                break;
             default:
-                return in;
+                break;
         }
+        if (capture_) capture_(in, res);
+        return res;
     }
 
     void d2(const double* x, const double* F0, const double* v, double* out) {
@@ -340,6 +355,11 @@ private:
     numa_solver_config cfg_;
     int64_t m_;
     std::unique_ptr<MGBase> mg_;
+    std::shared_ptr<NetApply> net_;
+    double ratio_ = 0.5;
+    bool use_net_ = false;
+    LearnedStats lstats_;
+    CaptureFn capture_;
     double *x2_ = nullptr, *Fx_ = nullptr, *Fn_ = nullptr, *z_ = nullptr, *u_ = nullptr,
            *w_ = nullptr, *r_ = nullptr, *diag_ = nullptr;
     std::vector<T*> V_, Z_;
@@ -381,6 +401,7 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
         return res;
     }
     if (zero_guess) launch_fill<double>(c_, N_, 0.0, xl);
+    use_net_ = net_ != nullptr;
 
     const bool flex = cfg_.flexible != 0;
     const size_t m = (size_t)restart;
@@ -426,6 +447,7 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
         while (k < m && res.iters < max_iters) {
             const size_t j = k;
             T* zdst = flex && !Z_.empty() ? Z_[j] : zb_;
+            const bool by_net = cfg_.precond == NUMA_PRECOND_LEARNED && net_ && use_net_;
             const T* zj = precond(V_[j], zdst);
             ++res.precs;
             // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
@@ -457,8 +479,13 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             sn[j] = Hh[j + 1][j] / denom;
             Hh[j][j] = denom;
             Hh[j + 1][j] = 0.0;
+            const double gprev = std::abs(g[j]);
             g[j + 1] = -sn[j] * g[j];
             g[j] = cs[j] * g[j];
+            if (by_net && std::abs(g[j + 1]) > ratio_ * gprev) {
+                use_net_ = false;  // the V-cycle serves the rest of this solve
+                ++lstats_.fallbacks;
+            }
             ++k;
             ++res.iters;
             k_last = k;
@@ -661,6 +688,15 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,
+                              int64_t restart) {
+    if (cfg.precision == NUMA_PRECISION_MIXED)
+        return new BratuSolver<float>(c, n, lambda, cfg, restart);
+    if (cfg.precision == NUMA_PRECISION_FP64)
+        return new BratuSolver<double>(c, n, lambda, cfg, restart);
+    throw Error(NUMA_ERR_ARG, "unknown precision");
+}
+
 }  // namespace numa_hip
 
 // ===========================================================================
@@ -668,22 +704,6 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
 // ===========================================================================
 using numa_hip::Error;
 
-#define NUMA_API_BEGIN try {
-#define NUMA_API_END                                    \
-    }                                                   \
-    catch (const Error& e) {                            \
-        numa_hip::set_last_error(e.what());             \
-        return e.code;                                  \
-    }                                                   \
-    catch (const std::bad_alloc&) {                     \
-        numa_hip::set_last_error("host allocation failed"); \
-        return NUMA_ERR_NOMEM;                          \
-    }                                                   \
-    catch (const std::exception& e) {                   \
-        numa_hip::set_last_error(e.what());             \
-        return NUMA_ERR_HIP;                            \
-    }                                                   \
-    return NUMA_OK;
 
 extern "C" {
 
@@ -739,12 +759,7 @@ int numa_hip_bratu_solver_create(numa_hip_ctx* ctx, int64_t n, double lambda,
     NUMA_HIP_CHECK(hipSetDevice(ctx->impl->device));
     auto h = std::make_unique<numa_hip_solver>();
     h->ctx = ctx->impl;
-    if (cfg->precision == NUMA_PRECISION_MIXED)
-        h->impl.reset(new numa_hip::BratuSolver<float>(*ctx->impl, n, lambda, *cfg, gmres_restart));
-    else if (cfg->precision == NUMA_PRECISION_FP64)
-        h->impl.reset(new numa_hip::BratuSolver<double>(*ctx->impl, n, lambda, *cfg, gmres_restart));
-    else
-        throw Error(NUMA_ERR_ARG, "unknown precision");
+    h->impl.reset(numa_hip::make_bratu_solver(*ctx->impl, n, lambda, *cfg, gmres_restart));
     ++ctx->impl->live_handles;
     *out = h.release();
     NUMA_API_END
@@ -788,6 +803,16 @@ int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
     numa_hip::require(s && ms_per_call, "null argument");
     NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
     *ms_per_call = s->impl->primitive_cost(prim, reps, bytes_per_call);
+    NUMA_API_END
+}
+
+int numa_hip_solver_learned_stats(const numa_hip_solver* s, int64_t* stats3) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && stats3, "null argument");
+    const numa_hip::LearnedStats st = s->impl->learned_stats();
+    stats3[0] = st.net_applications;
+    stats3[1] = st.gmg_applications;
+    stats3[2] = st.fallbacks;
     NUMA_API_END
 }
 

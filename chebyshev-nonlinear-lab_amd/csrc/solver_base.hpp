@@ -1,0 +1,58 @@
+// solver_base.hpp - the type-erased interface of the device-resident Bratu3D
+// solver, shared by the solver and the learned-preconditioner translation
+// units.
+#pragma once
+
+#include <functional>
+#include <memory>
+
+#include "common.hpp"
+
+namespace numa_hip {
+
+// A learned preconditioner z = M(r) on an n^3 grid (FP32 or FP64 vectors on
+// the context's stream).
+struct NetApply {
+    virtual ~NetApply() = default;
+    virtual void apply(int64_t n, const float* r, float* z) = 0;
+    virtual void apply(int64_t n, const double* r, double* z) = 0;
+};
+
+// Called after every preconditioner application with the device input and
+// output vectors (T = the solver's Krylov type), in stream order.
+using CaptureFn = std::function<void(const void* in, const void* out)>;
+
+struct LearnedStats {
+    int64_t net_applications = 0;  // preconditioner calls served by the network
+    int64_t gmg_applications = 0;  // calls served by the V-cycle
+    int64_t fallbacks = 0;         // linear solves switched to the V-cycle
+};
+
+struct SolverBase {
+    virtual ~SolverBase() = default;
+    virtual numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
+                                          int64_t restart, int64_t max_iters, bool zero_guess) = 0;
+    virtual void solve(const numa_krylov_params& p, double* x, numa_krylov_result& out) = 0;
+    virtual int64_t size() const = 0;
+    virtual int64_t grid_n() const = 0;
+    virtual double primitive_cost(int64_t prim, int64_t reps, double* bytes) = 0;
+    virtual void set_lambda(double lambda) = 0;
+    virtual void set_capture(CaptureFn fn) = 0;
+    // Attaches a learned preconditioner (NUMA_PRECOND_LEARNED solvers): the
+    // network serves each linear solve until a step reduces the FGMRES
+    // residual estimate by less than the factor `ratio`; the rest of that
+    // solve uses the V-cycle.
+    virtual void set_learned(std::shared_ptr<NetApply> net, double ratio) = 0;
+    virtual LearnedStats learned_stats() const = 0;
+};
+
+SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,
+                              int64_t restart);
+
+}  // namespace numa_hip
+
+// Opaque C handle of a solver.
+struct numa_hip_solver {
+    numa_hip::Ctx* ctx;
+    std::unique_ptr<numa_hip::SolverBase> impl;
+};

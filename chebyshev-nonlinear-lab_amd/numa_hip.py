@@ -30,12 +30,13 @@ ERR_NAMES = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "over
              5: "bad state"}
 
 METHOD_NEWTON, METHOD_CHEBYSHEV = 0, 1
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG, PRECOND_LEARNED = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 DERIV_FD, DERIV_ANALYTIC = 0, 1
 FORCING_EW, FORCING_CONSTANT = 0, 1
 SMOOTH_RBGS, SMOOTH_WJACOBI, SMOOTH_CHEBYSHEV = 0, 1, 2
 PRIM_F, PRIM_JVP, PRIM_D2, PRIM_PREC, PRIM_SMOOTH, PRIM_MGS = 0, 1, 2, 3, 4, 5
+LOSS_CORRECTION, LOSS_RESIDUAL = 0, 1
 
 
 class KrylovParams(C.Structure):
@@ -73,6 +74,18 @@ class MGConfig(C.Structure):
 class SolverConfig(C.Structure):
     _fields_ = [("method", _i64), ("precond", _i64), ("precision", _i64), ("flexible", _i64),
                 ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
+
+
+class NetTrainConfig(C.Structure):
+    _fields_ = [("pairs", _i64), ("batch", _i64), ("sizes", _i64 * 4), ("lambda_lo", C.c_double),
+                ("lambda_hi", C.c_double), ("noise", C.c_double), ("lr", C.c_double),
+                ("seed", _i64), ("gmres_restart", _i64), ("mg", MGConfig),
+                ("solve", KrylovParams), ("loss", _i64)]
+
+
+class NetTrainStats(C.Structure):
+    _fields_ = [("pairs", _i64), ("steps", _i64), ("solves", _i64), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("seconds", C.c_double)]
 
 
 def krylov_params(**kw) -> KrylovParams:
@@ -173,6 +186,20 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_chandra_jvp": [_vp, _vp, _vp, _vp],
         "numa_hip_chandra_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
         "numa_hip_chandra_gemm_cost": [_vp, _i64, _i64, _dp, _dp],
+        "numa_hip_net_create": [_vp, _i64, C.POINTER(_vp)],
+        "numa_hip_net_destroy": [_vp],
+        "numa_hip_net_param_count": [C.POINTER(_i64)],
+        "numa_hip_net_get_params": [_vp, _vp],
+        "numa_hip_net_set_params": [_vp, _vp],
+        "numa_hip_net_forward": [_vp, _i64, _i64, _vp, _vp],
+        "numa_hip_net_loss_grad": [_vp, _i64, _i64, _vp, _vp, _i64, _dp, _vp],
+        "numa_hip_net_train_step": [_vp, _i64, _i64, _vp, _vp, _i64, C.c_double, _dp],
+        "numa_hip_net_apply": [_vp, _i64, _vp, _vp],
+        "numa_hip_net_train_solver": [_vp, C.POINTER(NetTrainConfig), _dp, _i64,
+                                      C.POINTER(NetTrainStats)],
+        "numa_hip_net_cost": [_vp, _i64, _i64, _i64, _i64, _dp],
+        "numa_hip_solver_set_net": [_vp, _vp, C.c_double],
+        "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -389,6 +416,18 @@ class BratuSolver:
                                                x.ctypes.data_as(_dp), C.byref(r)))
         return r
 
+    def set_net(self, net: "ConvNet | None", fallback_ratio: float = 0.5) -> None:
+        """Attach a learned preconditioner (PRECOND_LEARNED solvers)."""
+        check(self.ctx.lib.numa_hip_solver_set_net(self.h, net.h if net is not None else None,
+                                                   fallback_ratio))
+        self._net = net
+
+    def learned_stats(self) -> dict:
+        s = (_i64 * 3)()
+        check(self.ctx.lib.numa_hip_solver_learned_stats(self.h, s))
+        return {"net_applications": int(s[0]), "gmg_applications": int(s[1]),
+                "fallbacks": int(s[2])}
+
     def primitive_cost(self, prim: int, reps: int = 20) -> tuple[float, float]:
         """(ms per call, algorithmic bytes per call) of one primitive."""
         ms, b = C.c_double(), C.c_double()
@@ -487,6 +526,99 @@ class ChandrasekharBatch:
     def close(self):
         if self.h and self.ctx.h:
             check(self.ctx.lib.numa_hip_chandra_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def net_train_config(pairs: int = 2000, batch: int = 4, sizes=(32, 48, 64), lambda_lo: float = 1.0,
+                     lambda_hi: float = 6.7, noise: float = 0.1, lr: float = 1e-3, seed: int = 7,
+                     gmres_restart: int = 30, mg: MGConfig | None = None,
+                     solve: KrylovParams | None = None, loss: int = LOSS_RESIDUAL) -> NetTrainConfig:
+    sz = (_i64 * 4)(*([int(v) for v in sizes] + [0] * (4 - len(sizes))))
+    return NetTrainConfig(pairs=pairs, batch=batch, sizes=sz, lambda_lo=lambda_lo,
+                          lambda_hi=lambda_hi, noise=noise, lr=lr, seed=seed,
+                          gmres_restart=gmres_restart,
+                          mg=mg if mg is not None else mg_cheb(levels=5),
+                          solve=solve if solve is not None else krylov_params(gmres_restart=30),
+                          loss=loss)
+
+
+class ConvNet:
+    """The learned preconditioner's network (numa_hip_net): 6 conv layers
+    (1-24-24-24-24-24-1, 3x3x3, GELU), bf16 MFMA on the 24->24 layers."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = _vp()
+        check(ctx.lib.numa_hip_net_create(ctx.h, int(seed), C.byref(h)))
+        self.h = h
+        cnt = _i64()
+        check(ctx.lib.numa_hip_net_param_count(C.byref(cnt)))
+        self.n_params = int(cnt.value)
+        ctx._adopt(self)
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, np.float32)
+        check(self.ctx.lib.numa_hip_net_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, w: np.ndarray) -> None:
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.n_params
+        check(self.ctx.lib.numa_hip_net_set_params(self.h, w.ctypes.data))
+
+    def forward(self, n: int, x: np.ndarray) -> np.ndarray:
+        """Raw network on a batch of grids, x of shape (B, n, n, n)."""
+        x = np.ascontiguousarray(x, np.float32)
+        B = x.size // n ** 3
+        dx = self.ctx.array(x.ravel())
+        do = self.ctx.empty(x.size, np.float32)
+        check(self.ctx.lib.numa_hip_net_forward(self.h, n, B, dx.ptr, do.ptr))
+        return do.numpy().reshape(x.shape)
+
+    def loss_grad(self, n: int, x: np.ndarray, y: np.ndarray,
+                  loss: int = LOSS_CORRECTION) -> tuple[float, np.ndarray]:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        B = x.size // n ** 3
+        dx, dy = self.ctx.array(x.ravel()), self.ctx.array(y.ravel())
+        g = np.empty(self.n_params, np.float32)
+        lv = C.c_double()
+        check(self.ctx.lib.numa_hip_net_loss_grad(self.h, n, B, dx.ptr, dy.ptr, loss, C.byref(lv),
+                                                  g.ctypes.data))
+        return lv.value, g
+
+    def train_step(self, n: int, r: "DeviceArray", z: "DeviceArray", batch: int,
+                   lr: float, loss: int = LOSS_RESIDUAL) -> float:
+        lv = C.c_double()
+        check(self.ctx.lib.numa_hip_net_train_step(self.h, n, batch, r.ptr, z.ptr, loss, lr,
+                                                   C.byref(lv)))
+        return lv.value
+
+    def apply(self, n: int, r: "DeviceArray", z: "DeviceArray") -> None:
+        check(self.ctx.lib.numa_hip_net_apply(self.h, n, r.ptr, z.ptr))
+
+    def train_solver(self, cfg: NetTrainConfig, hist_len: int = 0):
+        hist = np.zeros(max(1, hist_len))
+        st = NetTrainStats()
+        check(self.ctx.lib.numa_hip_net_train_solver(self.h, C.byref(cfg),
+                                                     hist.ctypes.data_as(_dp), hist_len,
+                                                     C.byref(st)))
+        return st, hist[:min(hist_len, st.steps)]
+
+    def cost(self, n: int, what: int = 0, batch: int = 1, reps: int = 5) -> float:
+        ms = C.c_double()
+        check(self.ctx.lib.numa_hip_net_cost(self.h, n, what, batch, reps, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_net_destroy(self.h))
         self.h = None
 
     def __del__(self):

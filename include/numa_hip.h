@@ -41,7 +41,9 @@ enum { NUMA_METHOD_NEWTON = 0, NUMA_METHOD_CHEBYSHEV = 1 };
 enum {
     NUMA_PRECOND_NONE = 0,
     NUMA_PRECOND_JACOBI = 1, /* BratuJacobi, bratu3d.hpp:104-120            */
-    NUMA_PRECOND_GMG = 2     /* geometric multigrid V-cycle, see numa_mg_config */
+    NUMA_PRECOND_GMG = 2,    /* geometric multigrid V-cycle, see numa_mg_config */
+    NUMA_PRECOND_LEARNED = 3 /* conv-net preconditioner with V-cycle fallback
+                                (no reference counterpart; needs flexible = 1) */
 };
 enum { NUMA_PRECISION_FP64 = 0, NUMA_PRECISION_MIXED = 1 };
 enum { NUMA_DERIV_FD = 0, NUMA_DERIV_ANALYTIC = 1 };
@@ -254,6 +256,84 @@ int numa_hip_chandra_solve(numa_hip_chandra* h, const numa_krylov_params* p, dou
  * FLOP count 2 n^2 batch. */
 int numa_hip_chandra_gemm_cost(numa_hip_chandra* h, int64_t precision, int64_t reps,
                                double* ms_per_call, double* flops_per_call);
+
+/* ---- learned preconditioner (no reference counterpart) -------------------- */
+/* A 3D conv net of six 3x3x3 layers, channels 1-24-24-24-24-24-1, GELU
+ * between layers, zero padding, used as z = M(r) = rho/(n+1)^2 net(r/rho),
+ * rho = rms(r), in FGMRES with the V-cycle as fallback.  The four 24->24
+ * layers run on bf16 MFMA with FP32 accumulation.  Parameters are FP32,
+ * flat: per layer W[Cout][Cin][3][3][3] then b[Cout] (63625 values). */
+typedef struct numa_hip_net numa_hip_net;
+
+/* Loss of a training pair (x, y) with network output o:
+ * CORRECTION: ||o - y||^2 / ||y||^2;
+ * RESIDUAL:   ||A7 (o - y)||^2 / ||A7 y||^2, A7 u = 6u - (sum of the six
+ *             neighbours), zero outside: the error in the operator's residual. */
+enum { NUMA_NET_LOSS_CORRECTION = 0, NUMA_NET_LOSS_RESIDUAL = 1 };
+
+/* Training on pairs generated by the solver: mixed-precision Newton-FGMRES
+ * with the V-cycle `mg` on random instances (n from `sizes`, lambda uniform,
+ * u0 = noise * U[0,1)); each preconditioner call (r, z) is one pair, with
+ * input r/rho and target (n+1)^2 z/rho.  Adam, one step per `batch` pairs. */
+typedef struct numa_net_train_config {
+    int64_t pairs;            /* training pairs to consume                       */
+    int64_t batch;            /* pairs per optimiser step (1..64)                */
+    int64_t sizes[4];         /* grid sizes drawn uniformly; 0 = unused slot     */
+    double lambda_lo;         /* lambda ~ U[lambda_lo, lambda_hi]                */
+    double lambda_hi;
+    double noise;             /* u0 = noise * U[0,1) per grid point              */
+    double lr;                /* Adam step size                                  */
+    int64_t seed;             /* mt19937_64; U[0,1) = (u64 >> 11) * 2^-53        */
+    int64_t gmres_restart;    /* FGMRES restart of the generating solves         */
+    numa_mg_config mg;        /* V-cycle whose corrections are the targets       */
+    numa_krylov_params solve; /* Newton parameters of the generating solves      */
+    int64_t loss;             /* NUMA_NET_LOSS_*; the step size decays linearly
+                                 from lr to lr/10 over the pairs                 */
+} numa_net_train_config;
+
+typedef struct numa_net_train_stats {
+    int64_t pairs;      /* pairs consumed                              */
+    int64_t steps;      /* optimiser steps                             */
+    int64_t solves;     /* generating solves                           */
+    double loss_first;  /* mean loss of the first step                 */
+    double loss_last;   /* mean loss of the last step                  */
+    double seconds;     /* wall time                                   */
+} numa_net_train_stats;
+
+int numa_hip_net_create(numa_hip_ctx* ctx, int64_t seed, numa_hip_net** out);
+/* The network stays alive while a solver uses it. */
+int numa_hip_net_destroy(numa_hip_net* net);
+int numa_hip_net_param_count(int64_t* count);
+int numa_hip_net_get_params(numa_hip_net* net, float* host);
+int numa_hip_net_set_params(numa_hip_net* net, const float* host);
+/* out = net(x) for `batch` grids of n^3 points (FP32, device, grid-major). */
+int numa_hip_net_forward(numa_hip_net* net, int64_t n, int64_t batch, const float* x_dev,
+                         float* out_dev);
+/* Mean loss over the batch (NUMA_NET_LOSS_*) of net(x_b) against y_b, and its
+ * gradient (host, 63625 floats, parameter layout); no update. */
+int numa_hip_net_loss_grad(numa_hip_net* net, int64_t n, int64_t batch, const float* x_dev,
+                           const float* y_dev, int64_t loss_kind, double* loss_host,
+                           float* grad_host);
+/* One Adam step on pairs (r_b, z_b) scaled as in training. */
+int numa_hip_net_train_step(numa_hip_net* net, int64_t n, int64_t batch, const float* r_dev,
+                            const float* z_dev, int64_t loss_kind, double lr,
+                            double* loss_host);
+/* z = M(r) on one n^3 grid. */
+int numa_hip_net_apply(numa_hip_net* net, int64_t n, const float* r_dev, float* z_dev);
+int numa_hip_net_train_solver(numa_hip_net* net, const numa_net_train_config* cfg,
+                              double* loss_hist_host, int64_t hist_len,
+                              numa_net_train_stats* stats);
+/* Average device ms of one inference (what = 0) or one training step of
+ * `batch` grids (what = 1) at n^3. */
+int numa_hip_net_cost(numa_hip_net* net, int64_t n, int64_t what, int64_t batch, int64_t reps,
+                      double* ms_per_call);
+/* Attaches the network to a NUMA_PRECOND_LEARNED solver (NULL detaches).  In
+ * each linear solve the network serves the FGMRES steps until one reduces the
+ * residual estimate by less than fallback_ratio (|g_j+1| > ratio |g_j|); the
+ * V-cycle serves the rest of that solve. */
+int numa_hip_solver_set_net(numa_hip_solver* s, numa_hip_net* net, double fallback_ratio);
+/* stats3 = {network applications, V-cycle applications, fallbacks}. */
+int numa_hip_solver_learned_stats(const numa_hip_solver* s, int64_t* stats3);
 
 /* ---- per-call primitive costs (instrumentation) -------------------------- */
 /* The reference freezes per-call costs c_F, c_JVP, c_D2, c_prec of its
