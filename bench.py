@@ -4,10 +4,14 @@
 A step is one complete nonlinear solve of Bratu3D (n = 256 interior nodes per
 axis, lambda = 6.5, u0 = 0, ||F|| <= 1e-8 ||F(u0)||) with the device-resident
 Newton-Krylov solver: FD J v, GMRES(30) / FGMRES(30), 5-level GMG V(2,2) with
-the degree-3 Chebyshev smoother.  Two legs run on the same GPU:
-  * fp64:  plain FP64 GMG-GMRES (the speed-up denominator);
+the degree-3 Chebyshev smoother.  Three legs run on the same GPU:
+  * fp64:  plain FP64 GMG-GMRES, one kernel per smoothing step (the speed-up
+           denominator);
   * mixed: FGMRES with FP32 Krylov basis / J v / V-cycle and FP64 F(x),
-           difference quotient, Newton update and Givens solve (the value).
+           difference quotient, Newton update and Givens solve, smoothing
+           sweeps temporally blocked (autotuned per level) (the value);
+  * fp64_fused: the FP64 method with the mixed leg's smoothing kernels
+           (reported next to the value).
 The timed region holds only solves, inputs resident in HBM.  Rank 0 prints
 one JSON line.  The CPU baseline is the FP64 CPU oracle (OpenMP) solving the
 same problem on the host, timed once; its solution is also the parity
@@ -46,12 +50,20 @@ def make_params(H, forcing: str):
 
 
 def leg_configs(H, levels: int):
-    mg = H.mg_cheb(levels=levels, degree=3, pre=2, post=2)
+    """fp64: the plain method (one kernel per smoothing step); mixed: FGMRES in
+    mixed precision with autotuned temporally blocked smoothing; fp64_fused: the
+    FP64 method with the same smoothing kernels as mixed (reported, so that the
+    share of the speed-up that comes from precision is visible)."""
+    plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE)
+    fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO)
     fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
-                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=mg)
+                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=plain)
     mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
-                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
-    return {"fp64": fp64, "mixed": mixed}
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=fused)
+    fp64_fused = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                                 precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
+                                 mg=fused)
+    return {"fp64": fp64, "mixed": mixed, "fp64_fused": fp64_fused}
 
 
 def time_leg(H, ctx, n, lam, cfg, params, steps, warmup):
@@ -315,7 +327,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
-    ap.add_argument("--legs", default="fp64,mixed")
+    ap.add_argument("--legs", default="fp64,mixed,fp64_fused")
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
@@ -349,7 +361,7 @@ def main() -> int:
         s, x, res, times = time_leg(H, ctx, args.n, args.lam, cfgs[name], params, args.steps,
                                     args.warmup)
         legs[name] = {"solver": s, "x": x, "res": res, "times": times,
-                      "prims": primitive_table(H, s)}
+                      "prims": primitive_table(H, s), "fusion": s.mg_fusion()}
         log(f"  primitives: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['GBs']:.0f} GB/s"
                                           for k, v in legs[name]["prims"].items()))
     main_leg = "mixed" if "mixed" in legs else list(legs)[0]
@@ -357,9 +369,14 @@ def main() -> int:
     t_main = float(np.mean(L["times"]))
     r = L["res"]
 
-    # roofline of the dominant kernel: the finest-level smoothing step
+    # roofline of the dominant kernel: the finest-level smoothing launch
     sm = L["prims"]["SMOOTH"]
-    roofline = {"bound": "hbm", "kernel": "k_stencil<EpPoly> (finest Chebyshev smoothing step)",
+    f0 = L["fusion"][0] if L["fusion"] else (H.FUSE_NONE, 0, 0)
+    kname = ("k_stencil<EpPoly> (finest Chebyshev smoothing step)" if f0[0] == H.FUSE_NONE else
+             f"k_poly_fused<{'float' if main_leg == 'mixed' else 'double'},"
+             f"{3 if f0[0] == H.FUSE_SWEEP else 6},{f0[2]}> (finest-level fused Chebyshev "
+             f"{'sweep' if f0[0] == H.FUSE_SWEEP else 'phase'}, ichunk {f0[1]})")
+    roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(sm["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(sm["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes_per_launch": sm["bytes"], "avg_launch_ms": sm["ms"]}
@@ -378,6 +395,7 @@ def main() -> int:
            "final_rel_residual": r.res / r.res0, "roofline": roofline,
            "primitives": {k: {kk: round(vv, 5) for kk, vv in v.items()}
                           for k, v in L["prims"].items()}}
+    out["mg_fusion_per_level"] = [list(f) for f in L["fusion"]]
     if "fp64" in legs and main_leg != "fp64":
         t64 = float(np.mean(legs["fp64"]["times"]))
         r64 = legs["fp64"]["res"]
@@ -385,6 +403,10 @@ def main() -> int:
         out["speedup_vs_fp64"] = round(t64 / t_main, 3)
         out["fp64_newton_iters"] = int(r64.iters)
         out["fp64_gmres_iters"] = int(r64.n_lin_iters)
+    if "fp64_fused" in legs and main_leg != "fp64_fused":
+        tf = float(np.mean(legs["fp64_fused"]["times"]))
+        out["fp64_fused_smoothing_s"] = round(tf, 6)
+        out["speedup_vs_fp64_fused_smoothing"] = round(tf / t_main, 3)
 
     if not args.no_cpu:
         info, path = run_cpu_baseline(args)

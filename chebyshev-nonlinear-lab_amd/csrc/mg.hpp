@@ -9,21 +9,27 @@
 //   * a Chebyshev semi-iteration over the exact spectrum
 //     [1 - cos(pi/(n+1)), 1 + cos(pi/(n+1))] as coarse solve, run by one
 //     workgroup with the level resident in LDS.
-// Level storage is FP64 or FP32 (T); arithmetic is FP64 per point.
+// Level storage is FP64 or FP32 (T).  The polynomial smoothing steps compute in
+// the storage precision (SmoothArith<T>); the transfers and the coarse solve
+// compute in FP64 per point.
 #pragma once
 
 #include <algorithm>
 #include <utility>
 
 #include "blas.hpp"
+#include "smooth_fused.hpp"
 #include "stencil.hpp"
+#include "tune.hpp"
 
 namespace numa_hip {
 
-// One Jacobi-type / Chebyshev step:  d <- c1 d + c2 (b - A x);  xn = x + d.
+// One Jacobi-type / Chebyshev step:  d <- c1 d + c2 (b - A x);  xn = x + d,
+// in the arithmetic SmoothArith<T> (smooth_fused.hpp).
 // USE_D reads the previous direction, WRITE_D stores the new one.
 template <class T, bool USE_D, bool WRITE_D>
 struct EpPoly {
+    using A = SmoothArith<T>;
     static constexpr bool kMarch = false;
     double h2inv, c1, c2;
     const T* b;
@@ -41,11 +47,16 @@ struct EpPoly {
     __device__ double operator()(int64_t id, const Extra& e, P1 c, P1 im, P1 ip, P1 jm, P1 jp,
                                  P1 km, P1 kp) const {
 #pragma clang fp contract(off)
-        const double res = static_cast<double>(e.b) -
-                           h2inv * lap_num(c.a, im.a, ip.a, jm.a, jp.a, km.a, kp.a);
-        const double dn = USE_D ? c1 * static_cast<double>(e.d) + c2 * res : c2 * res;
+        const A cc = static_cast<A>(c.a);
+        const A res = static_cast<A>(e.b) -
+                      static_cast<A>(h2inv) *
+                          lap_num<A>(cc, static_cast<A>(im.a), static_cast<A>(ip.a),
+                                     static_cast<A>(jm.a), static_cast<A>(jp.a),
+                                     static_cast<A>(km.a), static_cast<A>(kp.a));
+        const A dn = USE_D ? static_cast<A>(c1) * static_cast<A>(e.d) + static_cast<A>(c2) * res
+                           : static_cast<A>(c2) * res;
         if constexpr (WRITE_D) d[id] = static_cast<T>(dn);
-        xn[id] = static_cast<T>(c.a + dn);
+        xn[id] = static_cast<T>(cc + dn);
         return 0.0;
     }
 };
@@ -68,13 +79,15 @@ struct EpResid {
     }
 };
 
-// First step from x = 0:  d = c2 b;  xn = d.
+// First step from x = 0:  d = c2 b;  xn = d  (arithmetic SmoothArith<T>).
 template <bool VEC, class T>
-__global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2, const T* b, T* d,
+__global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2d, const T* b, T* d,
                                                         T* xn) {
 #pragma clang fp contract(off)
+    using A = SmoothArith<T>;
+    const A c2 = static_cast<A>(c2d);
     auto f = [c2](Pair<T> v) {
-        return Pair<T>{static_cast<T>(c2 * (double)v.a), static_cast<T>(c2 * (double)v.b)};
+        return Pair<T>{static_cast<T>(c2 * (A)v.a), static_cast<T>(c2 * (A)v.b)};
     };
     for_each_pair(
         N,
@@ -93,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2, con
             }
         },
         [&](int64_t i) {
-            const double dn = c2 * dv(b, i);
+            const A dn = c2 * static_cast<A>(b[i]);
             if (d) d[i] = static_cast<T>(dn);
             xn[i] = static_cast<T>(dn);
         });
@@ -343,6 +356,59 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
     for (int id = threadIdx.x; id < N; id += kCoarseThreads) xout[id] = static_cast<T>(xs[id]);
 }
 
+// One fused launch of `steps` polynomial smoothing steps (x == nullptr: the
+// iterate starts at zero) with R rows per thread (the region is 64 x 8R).
+// Row counts: {4, 8} for FP32, {2, 4} for FP64 storage.
+template <class T, int S, int R>
+inline void launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolySteps& cs,
+                            const T* x, const T* b, T* xo) {
+    int tk, tj, ch;
+    const dim3 g = fused_grid<S, R>(n, ichunk, tk, tj, ch);
+    hipLaunchKernelGGL((k_poly_fused<T, S, R>), g, dim3(kFusedThreads), 0, c.stream, n, ichunk, tk,
+                       tj, ch, h2inv, cs, x, b, xo);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+template <class T>
+constexpr int fused_rows_small() { return sizeof(T) == 4 ? 4 : 2; }
+
+// LDS of the fused kernel: STEPS levels x 2 planes x (8R+2) x 66 values.
+template <class T>
+constexpr bool fused_fits(int steps, int rows) {
+    return (int64_t)steps * 2 * (8 * rows + 2) * 66 * (int64_t)sizeof(T) <= 160 * 1024;
+}
+
+template <class T, int S>
+inline void launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv, const PolySteps& cs,
+                           const T* x, const T* b, T* xo) {
+    constexpr int R0 = fused_rows_small<T>();
+    if (rows == R0) {
+        launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo);
+        return;
+    }
+    if constexpr (fused_fits<T>(S, 2 * R0)) {
+        if (rows == 2 * R0) {
+            launch_fused_sr<T, S, 2 * R0>(c, n, ichunk, h2inv, cs, x, b, xo);
+            return;
+        }
+    }
+    throw Error(NUMA_ERR_ARG, "unsupported steps x rows per thread for fused smoothing");
+}
+
+template <class T>
+inline void launch_fused(Ctx& c, int n, int steps, int rows, int ichunk, double h2inv,
+                         const PolySteps& cs, const T* x, const T* b, T* xo) {
+    switch (steps) {
+        case 1: launch_fused_s<T, 1>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 2: launch_fused_s<T, 2>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 3: launch_fused_s<T, 3>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 4: launch_fused_s<T, 4>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 5: launch_fused_s<T, 5>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 6: launch_fused_s<T, 6>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        default: throw Error(NUMA_ERR_ARG, "fused smoothing supports 1..6 steps");
+    }
+}
+
 inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
 inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1.0)); }
 
@@ -356,6 +422,9 @@ struct MGBase {
     virtual double smooth_step_for_timing() = 0;
     // Algorithmic HBM bytes of one V-cycle.
     virtual double vcycle_bytes() const = 0;
+    // Smoother fusion chosen for level l (NUMA_FUSE_*; AUTO until tuned), its
+    // plane chunk and rows per thread.
+    virtual void fusion(int64_t l, int64_t* mode, int64_t* ichunk, int64_t* rows) const = 0;
 };
 
 template <class T>
@@ -369,6 +438,11 @@ public:
         require(cfg.smoother != NUMA_SMOOTH_CHEBYSHEV || (cfg.degree >= 1 && cfg.cheb_ratio > 1.0),
                 "Chebyshev smoother needs degree >= 1 and cheb_ratio > 1");
         require(n <= 2048, "grid too large", NUMA_ERR_OVERFLOW);
+        require(cfg.fuse >= NUMA_FUSE_NONE && cfg.fuse <= NUMA_FUSE_AUTO, "unknown fuse mode");
+        require(cfg.ichunk >= 0 && cfg.ichunk <= 4096, "ichunk must be in [0, 4096]");
+        require(cfg.rows == 0 || cfg.rows == fused_rows_small<T>() ||
+                    cfg.rows == 2 * fused_rows_small<T>(),
+                "rows must be 0 or the storage type's 1x / 2x rows per thread");
         // Hierarchy (poisson_mg.hpp:44-63, plus even coarsening and limits).
         int64_t m = n;
         double h2inv = (double(n) + 1.0) * (double(n) + 1.0);
@@ -397,6 +471,13 @@ public:
                 const double hi = cheb_lmax(L.n), lo = hi / cfg.cheb_ratio;
                 cheb_coefs(L, (int)cfg.degree, lo, hi, L.c1, L.c2);
             }
+            L.fuse = cfg.smoother == NUMA_SMOOTH_RBGS ? (int)NUMA_FUSE_NONE : (int)cfg.fuse;
+            L.ichunk = cfg.ichunk > 0 ? (int)cfg.ichunk : 32;
+            L.rows = cfg.rows > 0 ? (int)cfg.rows : fused_rows_small<T>();
+            if ((L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE) &&
+                cfg.smoother != NUMA_SMOOTH_RBGS)
+                require(fused_fits<T>(fused_group(L), L.rows),
+                        "fused smoothing: steps per launch x rows per thread exceed the LDS");
         }
     }
     ~MG() override {
@@ -405,12 +486,30 @@ public:
 
     int64_t levels() const override { return (int64_t)levels_.size(); }
     int64_t level_n(int64_t l) const override { return levels_[(size_t)l].n; }
+    void fusion(int64_t l, int64_t* mode, int64_t* ichunk, int64_t* rows) const override {
+        *mode = levels_[(size_t)l].fuse;
+        *ichunk = levels_[(size_t)l].ichunk;
+        *rows = levels_[(size_t)l].rows;
+    }
 
     double smooth_step_for_timing() override {
+        if (!tuned_) autotune();
         Level& L = levels_[0];
         const int64_t N = L.n * L.n * L.n;
         T* saved_b = L.b;
         L.b = L.r;
+        if (cfg_.smoother != NUMA_SMOOTH_RBGS &&
+            (L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE)) {
+            // one fused launch of the level's group of steps from a nonzero x
+            const int steps = fused_group(L);
+            PolySteps cs{};
+            for (int q = 0; q < steps; ++q) cs.c2[q] = 0.1 / (6.0 * L.h2inv);
+            launch_fused<T>(c_, (int)L.n, steps, L.rows, L.ichunk, L.h2inv, cs, (const T*)L.x,
+                            (const T*)L.b, L.x2);
+            std::swap(L.x, L.x2);
+            L.b = saved_b;
+            return (double)N * sizeof(T) * 3.0;  // read x, b; write x
+        }
         if (cfg_.smoother == NUMA_SMOOTH_RBGS) {
             rbgs_sweep(L, 0);
             L.b = saved_b;
@@ -429,7 +528,8 @@ public:
     double vcycle_bytes() const override {
         // per level: smoothing steps (first step from zero reads b, writes
         // d, x), residual (x, b -> r), restriction (r -> b_c), prolongation
-        // (x_c, x -> x), coarsest solve (b -> x); plus the output copy.
+        // (x_c, x -> x), coarsest solve (b -> x); plus the output copy.  A
+        // fused launch reads x and b and writes x (from zero: reads b only).
         const double s = sizeof(T);
         double total = 0.0;
         for (size_t l = 0; l < levels_.size(); ++l) {
@@ -440,6 +540,16 @@ public:
             }
             const int per_sweep = cfg_.smoother == NUMA_SMOOTH_CHEBYSHEV ? (int)cfg_.degree : 1;
             const int steps = (int)(cfg_.pre + cfg_.post) * per_sweep;
+            const int fz = levels_[l].fuse;
+            if (cfg_.smoother != NUMA_SMOOTH_RBGS && (fz == NUMA_FUSE_SWEEP || fz == NUMA_FUSE_PHASE) &&
+                per_sweep <= kFusedMaxSteps) {
+                const int g = fused_group(levels_[l]);
+                const int lp = (int)((cfg_.pre * per_sweep + g - 1) / g);
+                const int lq = (int)((cfg_.post * per_sweep + g - 1) / g);
+                total += N * s * (3.0 * (lp + lq) - (lp > 0 ? 1.0 : 0.0));
+                total += N * s * (3.0 + 1.125 + 2.125);
+                continue;
+            }
             double step_bytes = cfg_.smoother == NUMA_SMOOTH_CHEBYSHEV ? 5.0 : 3.0;
             if (cfg_.smoother == NUMA_SMOOTH_RBGS) step_bytes = 4.0;
             total += N * s * (steps * step_bytes - (cfg_.pre > 0 ? step_bytes - 3.0 : 0.0));
@@ -453,6 +563,7 @@ public:
 
     // z = V-cycle(r) with zero initial guess (poisson_mg.hpp:69-80).
     void apply(const void* rv, void* zv) override {
+        if (!tuned_) autotune();
         const T* r = static_cast<const T*>(rv);
         T* z = static_cast<T*>(zv);
         Level& F = levels_[0];
@@ -473,6 +584,9 @@ private:
         T* r = nullptr;
         T* d = nullptr;
         std::vector<double> c1, c2;
+        int fuse = NUMA_FUSE_NONE;  // NUMA_FUSE_*
+        int ichunk = 32;
+        int rows = 4;               // rows per thread of fused launches
     };
 
     T* alloc(int64_t N) {
@@ -551,6 +665,9 @@ private:
             if (from_zero) launch_fill<T>(c_, N, T(0), L.x);
             return;
         }
+        if (L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE) {
+            if (smooth_fused(L, sweeps, from_zero)) return;
+        }
         for (int s = 0; s < sweeps; ++s) {
             if (cfg_.smoother == NUMA_SMOOTH_WJACOBI) {
                 poly_step(L, 0.0, cfg_.omega / (6.0 * L.h2inv), false, false, from_zero && s == 0);
@@ -560,6 +677,118 @@ private:
                     poly_step(L, L.c1[k], L.c2[k], k > 0, k + 1 < deg, from_zero && s == 0 && k == 0);
             }
         }
+    }
+
+    // Steps per fused launch of a level.
+    int fused_group(const Level& L) const {
+        const int per = cfg_.smoother == NUMA_SMOOTH_WJACOBI ? 1 : (int)cfg_.degree;
+        return L.fuse == NUMA_FUSE_SWEEP ? per : (kFusedMaxSteps / per) * per;
+    }
+
+    // Fused form of `sweeps` polynomial sweeps: the step sequence of the
+    // sweeps (the first step of each sweep starts a new direction, c1 = 0) in
+    // launches of one sweep or of as many whole sweeps as fit in 6 steps.
+    // Returns false if a sweep is longer than a fused launch can hold.
+    bool smooth_fused(Level& L, int sweeps, bool from_zero) {
+        const bool wj = cfg_.smoother == NUMA_SMOOTH_WJACOBI;
+        const int per = wj ? 1 : (int)cfg_.degree;
+        if (per > kFusedMaxSteps) return false;
+        std::vector<double> c1s, c2s;
+        for (int s = 0; s < sweeps; ++s)
+            for (int k = 0; k < per; ++k) {
+                c1s.push_back(wj || k == 0 ? 0.0 : L.c1[(size_t)k]);
+                c2s.push_back(wj ? cfg_.omega / (6.0 * L.h2inv) : L.c2[(size_t)k]);
+            }
+        const int total = (int)c1s.size();
+        const int group = fused_group(L);
+        for (int st = 0; st < total; st += group) {
+            const int cnt = std::min(group, total - st);
+            PolySteps cs{};
+            for (int q = 0; q < cnt; ++q) {
+                cs.c1[q] = c1s[(size_t)(st + q)];
+                cs.c2[q] = c2s[(size_t)(st + q)];
+            }
+            launch_fused<T>(c_, (int)L.n, cnt, L.rows, L.ichunk, L.h2inv, cs,
+                            (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x,
+                            (const T*)L.b, L.x2);
+            std::swap(L.x, L.x2);
+        }
+        return true;
+    }
+
+    // NUMA_FUSE_AUTO: per level, time one pre-smoothing phase (from zero, on
+    // the level's own buffers with b = 1) for every fusion choice and plane
+    // chunk, keep the fastest, and remember it in the tune cache.
+    void autotune() {
+        tuned_ = true;
+        const std::string dev = TuneCache::device_key(c_.device);
+        for (size_t l = 0; l + 1 < levels_.size(); ++l) {
+            Level& L = levels_[l];
+            if (L.fuse != NUMA_FUSE_AUTO) continue;
+            std::ostringstream key;
+            key << dev << "|mg-smooth|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
+                << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
+            std::string val;
+            if (TuneCache::get().lookup(key.str(), val)) {
+                int m = 0, ic = 0, rw = 0;
+                constexpr int R0 = fused_rows_small<T>();
+                if (std::sscanf(val.c_str(), "%d,%d,%d", &m, &ic, &rw) == 3 && m >= NUMA_FUSE_NONE &&
+                    m <= NUMA_FUSE_PHASE && ic > 0 && ic <= 4096 && (rw == R0 || rw == 2 * R0)) {
+                    L.fuse = m;
+                    L.ichunk = ic;
+                    L.rows = rw;
+                    continue;
+                }
+            }
+            const int64_t N = L.n * L.n * L.n;
+            T* saved_b = L.b;
+            L.b = L.r;
+            launch_fill<T>(c_, N, T(1), L.r);
+            struct Cand {
+                int mode, ichunk, rows;
+            };
+            constexpr int R0 = fused_rows_small<T>();
+            std::vector<Cand> cands{{NUMA_FUSE_NONE, 32, R0}};
+            for (int rw : {R0, 2 * R0})
+                for (int ic : {16, 32, 64})
+                    for (int md : {(int)NUMA_FUSE_SWEEP, (int)NUMA_FUSE_PHASE}) {
+                        if (md == NUMA_FUSE_PHASE && cfg_.pre <= 1) continue;
+                        L.fuse = md;
+                        if (fused_fits<T>(fused_group(L), rw)) cands.push_back({md, ic, rw});
+                    }
+            hipEvent_t e0, e1;
+            NUMA_HIP_CHECK(hipEventCreate(&e0));
+            NUMA_HIP_CHECK(hipEventCreate(&e1));
+            double best = 1e300;
+            Cand pick = cands[0];
+            for (const Cand& cd : cands) {
+                L.fuse = cd.mode;
+                L.ichunk = cd.ichunk;
+                L.rows = cd.rows;
+                smooth(L, (int)cfg_.pre, 0, true);  // warm-up
+                NUMA_HIP_CHECK(hipEventRecord(e0, c_.stream));
+                for (int rep = 0; rep < 3; ++rep) smooth(L, (int)cfg_.pre, 0, true);
+                NUMA_HIP_CHECK(hipEventRecord(e1, c_.stream));
+                NUMA_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                NUMA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    pick = cd;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            L.b = saved_b;
+            L.fuse = pick.mode;
+            L.ichunk = pick.ichunk;
+            L.rows = pick.rows;
+            std::ostringstream v;
+            v << pick.mode << "," << pick.ichunk << "," << pick.rows;
+            TuneCache::get().store(key.str(), v.str());
+        }
+        for (Level& L : levels_)
+            if (L.fuse == NUMA_FUSE_AUTO) L.fuse = NUMA_FUSE_NONE;  // coarsest level
     }
 
     void coarse_solve(Level& L) {
@@ -619,6 +848,7 @@ private:
     numa_mg_config cfg_;
     std::vector<Level> levels_;
     std::vector<void*> allocs_;
+    bool tuned_ = false;
 };
 
 inline MGBase* make_mg(Ctx& c, int64_t n, const numa_mg_config& cfg, int64_t precision) {

@@ -104,6 +104,7 @@ public:
         ratio_ = ratio;
     }
     LearnedStats learned_stats() const override { return lstats_; }
+    const MGBase* multigrid() const override { return mg_.get(); }
 
     numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
                                   int64_t restart, int64_t max_iters, bool zero_guess) override {
@@ -803,6 +804,27 @@ int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
     numa_hip::require(s && ms_per_call, "null argument");
     NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
     *ms_per_call = s->impl->primitive_cost(prim, reps, bytes_per_call);
+    NUMA_API_END
+}
+
+int numa_hip_mg_fusion(const numa_hip_mg* mg, int64_t* mode_host, int64_t* ichunk_host,
+                       int64_t* rows_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(mg && mode_host && ichunk_host && rows_host, "null argument");
+    for (int64_t l = 0; l < mg->impl->levels(); ++l)
+        mg->impl->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
+    NUMA_API_END
+}
+
+int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t* mode_host,
+                              int64_t* ichunk_host, int64_t* rows_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && levels, "null argument");
+    const numa_hip::MGBase* mg = s->impl->multigrid();
+    *levels = mg ? mg->levels() : 0;
+    if (mg && mode_host && ichunk_host && rows_host)
+        for (int64_t l = 0; l < mg->levels(); ++l)
+            mg->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
     NUMA_API_END
 }
 

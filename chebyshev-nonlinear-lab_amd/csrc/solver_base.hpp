@@ -7,6 +7,7 @@
 #include <memory>
 
 #include "common.hpp"
+#include "mg.hpp"
 
 namespace numa_hip {
 
@@ -44,6 +45,8 @@ struct SolverBase {
     // solve uses the V-cycle.
     virtual void set_learned(std::shared_ptr<NetApply> net, double ratio) = 0;
     virtual LearnedStats learned_stats() const = 0;
+    // The multigrid preconditioner (null without one).
+    virtual const MGBase* multigrid() const = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

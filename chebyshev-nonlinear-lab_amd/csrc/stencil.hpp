@@ -37,11 +37,12 @@ __device__ inline double ld(const T* p, int64_t i) {
     return static_cast<double>(p[i]);
 }
 
-// -Lap_h numerator in the reference's order: 6c - im - ip - jm - jp - km - kp.
-__device__ inline double lap_num(double c, double im, double ip, double jm, double jp, double km,
-                                 double kp) {
+// -Lap_h numerator in the reference's order: 6c - im - ip - jm - jp - km - kp
+// (in the arithmetic type A).
+template <class A>
+__device__ inline A lap_num(A c, A im, A ip, A jm, A jp, A km, A kp) {
 #pragma clang fp contract(off)
-    double s = 6.0 * c;
+    A s = A(6) * c;
     s -= im;
     s -= ip;
     s -= jm;

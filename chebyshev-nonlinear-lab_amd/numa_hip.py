@@ -35,6 +35,7 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 DERIV_FD, DERIV_ANALYTIC = 0, 1
 FORCING_EW, FORCING_CONSTANT = 0, 1
 SMOOTH_RBGS, SMOOTH_WJACOBI, SMOOTH_CHEBYSHEV = 0, 1, 2
+FUSE_NONE, FUSE_SWEEP, FUSE_PHASE, FUSE_AUTO = 0, 1, 2, 3
 PRIM_F, PRIM_JVP, PRIM_D2, PRIM_PREC, PRIM_SMOOTH, PRIM_MGS = 0, 1, 2, 3, 4, 5
 LOSS_CORRECTION, LOSS_RESIDUAL = 0, 1
 
@@ -68,7 +69,8 @@ class GmresResult(C.Structure):
 class MGConfig(C.Structure):
     _fields_ = [("smoother", _i64), ("pre", _i64), ("post", _i64), ("degree", _i64),
                 ("max_levels", _i64), ("min_coarse_n", _i64), ("coarse_iters", _i64),
-                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double)]
+                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double),
+                ("fuse", _i64), ("ichunk", _i64), ("rows", _i64)]
 
 
 class SolverConfig(C.Structure):
@@ -112,18 +114,22 @@ def mg_config(**kw) -> MGConfig:
 
 
 def mg_cheb(levels: int = 5, degree: int = 3, pre: int = 2, post: int = 2,
-            cheb_ratio: float = 4.0, coarse_iters: int = 24) -> MGConfig:
+            cheb_ratio: float = 4.0, coarse_iters: int = 24, fuse: int = FUSE_NONE,
+            ichunk: int = 0, rows: int = 0) -> MGConfig:
     """5-level even-n V(2,2) with the degree-3 Chebyshev smoother (256^3 config)."""
     return mg_config(smoother=SMOOTH_CHEBYSHEV, pre=pre, post=post, degree=degree,
                      max_levels=levels, min_coarse_n=2, coarse_iters=coarse_iters,
-                     even_coarsening=1, cheb_ratio=cheb_ratio)
+                     even_coarsening=1, cheb_ratio=cheb_ratio, fuse=fuse, ichunk=ichunk,
+                     rows=rows)
 
 
 def mg_wjacobi(levels: int = 0, pre: int = 2, post: int = 2, omega: float = 6.0 / 7.0,
-               coarse_iters: int = 24) -> MGConfig:
+               coarse_iters: int = 24, fuse: int = FUSE_NONE, ichunk: int = 0,
+               rows: int = 0) -> MGConfig:
     """Even-n V(2,2) with the weighted-Jacobi smoother (32^3 CPU config)."""
     return mg_config(smoother=SMOOTH_WJACOBI, pre=pre, post=post, max_levels=levels,
-                     min_coarse_n=2, coarse_iters=coarse_iters, even_coarsening=1, omega=omega)
+                     min_coarse_n=2, coarse_iters=coarse_iters, even_coarsening=1, omega=omega,
+                     fuse=fuse, ichunk=ichunk, rows=rows)
 
 
 def solver_config(method=METHOD_NEWTON, precond=PRECOND_GMG, precision=PRECISION_FP64,
@@ -200,6 +206,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_net_cost": [_vp, _i64, _i64, _i64, _i64, _dp],
         "numa_hip_solver_set_net": [_vp, _vp, C.c_double],
         "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
+        "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
+        "numa_hip_solver_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                                      C.POINTER(_i64)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -374,6 +383,13 @@ class PoissonMG:
     def __call__(self, r: DeviceArray, z: DeviceArray) -> None:
         check(self.ctx.lib.numa_hip_mg_apply(self.h, r.ptr, z.ptr))
 
+    def fusion(self) -> list[tuple[int, int, int]]:
+        """(NUMA_FUSE_* mode, plane chunk, rows per thread) per level."""
+        L = self.levels()
+        m, ic, rw = (_i64 * L)(), (_i64 * L)(), (_i64 * L)()
+        check(self.ctx.lib.numa_hip_mg_fusion(self.h, m, ic, rw))
+        return [(int(m[i]), int(ic[i]), int(rw[i])) for i in range(L)]
+
     def close(self):
         if self.h and self.ctx.h:
             check(self.ctx.lib.numa_hip_mg_destroy(self.h))
@@ -421,6 +437,15 @@ class BratuSolver:
         check(self.ctx.lib.numa_hip_solver_set_net(self.h, net.h if net is not None else None,
                                                    fallback_ratio))
         self._net = net
+
+    def mg_fusion(self) -> list[tuple[int, int, int]]:
+        """(NUMA_FUSE_* mode, plane chunk, rows per thread) per multigrid level."""
+        L = _i64()
+        check(self.ctx.lib.numa_hip_solver_mg_fusion(self.h, C.byref(L), None, None, None))
+        k = max(1, L.value)
+        m, ic, rw = (_i64 * k)(), (_i64 * k)(), (_i64 * k)()
+        check(self.ctx.lib.numa_hip_solver_mg_fusion(self.h, C.byref(L), m, ic, rw))
+        return [(int(m[i]), int(ic[i]), int(rw[i])) for i in range(L.value)]
 
     def learned_stats(self) -> dict:
         s = (_i64 * 3)()

@@ -119,7 +119,25 @@ typedef struct numa_mg_config {
     int64_t even_coarsening; /* 1: coarsen any n to n/2; 0: only odd n (reference) */
     double omega;            /* weighted-Jacobi damping                            */
     double cheb_ratio;       /* Chebyshev smoothing interval [lmax/ratio, lmax]    */
+    int64_t fuse;            /* NUMA_FUSE_*: smoothing steps per kernel launch     */
+    int64_t ichunk;          /* planes per workgroup of fused launches (0 = 32)    */
+    int64_t rows;            /* rows per thread of fused launches: 0 = 4 (FP32) /
+                                2 (FP64), or twice that                            */
 } numa_mg_config;
+
+/* Kernel fusion of the polynomial (weighted-Jacobi / Chebyshev) smoothers.
+ * Results are bit for bit the same for every choice. */
+enum {
+    NUMA_FUSE_NONE = 0,  /* one launch per step                                   */
+    NUMA_FUSE_SWEEP = 1, /* one launch per sweep (the degree's steps)             */
+    NUMA_FUSE_PHASE = 2, /* one launch for all pre (post) smoothing steps, <= 6   */
+    NUMA_FUSE_AUTO = 3   /* per level, the fastest of the above and of the plane
+                            chunks {16, 32, 64} and rows per thread, timed on the
+                            device at the first
+                            V-cycle and cached in a JSON file
+                            ($NUMA_HIP_TUNE_CACHE, default
+                            ~/.cache/numa_hip/tune.json)                          */
+};
 
 /* Solver configuration: what the reference selects by template arguments and
  * provider types, plus the precision / flexibility additions. */
@@ -204,6 +222,11 @@ int numa_hip_mg_levels(const numa_hip_mg* mg, int64_t* levels, int64_t* sizes_ho
  * NUMA_PRECISION_FP64 and float* for NUMA_PRECISION_MIXED. */
 int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev);
 int numa_hip_mg_destroy(numa_hip_mg* mg);
+/* Per level: the smoother fusion in use (NUMA_FUSE_*; NUMA_FUSE_AUTO until the
+ * first V-cycle has tuned it), its plane chunk and rows per thread (the
+ * fused region is 64 x 8 rows).  Arrays of levels entries. */
+int numa_hip_mg_fusion(const numa_hip_mg* mg, int64_t* mode_host, int64_t* ichunk_host,
+                       int64_t* rows_host);
 
 /* ---- device-resident solvers on Bratu3D ---------------------------------- */
 int numa_hip_bratu_solver_create(numa_hip_ctx* ctx, int64_t n, double lambda,
@@ -332,6 +355,11 @@ int numa_hip_net_cost(numa_hip_net* net, int64_t n, int64_t what, int64_t batch,
  * residual estimate by less than fallback_ratio (|g_j+1| > ratio |g_j|); the
  * V-cycle serves the rest of that solve. */
 int numa_hip_solver_set_net(numa_hip_solver* s, numa_hip_net* net, double fallback_ratio);
+/* The solver's multigrid levels and, per level, the smoother fusion, plane
+ * chunk and rows (as numa_hip_mg_fusion); the arrays may be NULL to query the
+ * count. */
+int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t* mode_host,
+                              int64_t* ichunk_host, int64_t* rows_host);
 /* stats3 = {network applications, V-cycle applications, fallbacks}. */
 int numa_hip_solver_learned_stats(const numa_hip_solver* s, int64_t* stats3);
 

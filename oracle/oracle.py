@@ -45,7 +45,8 @@ class GmresResult(C.Structure):
 class MGConfig(C.Structure):
     _fields_ = [("smoother", _i64), ("pre", _i64), ("post", _i64), ("degree", _i64),
                 ("max_levels", _i64), ("min_coarse_n", _i64), ("coarse_iters", _i64),
-                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double)]
+                ("even_coarsening", _i64), ("omega", C.c_double), ("cheb_ratio", C.c_double),
+                ("fuse", _i64), ("ichunk", _i64), ("rows", _i64)]
 
 
 class SolverConfig(C.Structure):

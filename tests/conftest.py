@@ -2,6 +2,7 @@
 HIP library's Python bindings, and the `gpu` marker."""
 import os
 import sys
+import tempfile
 
 import pytest
 
@@ -12,6 +13,10 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 for p in (PKG, ORACLE_DIR):
     if p not in sys.path:
         sys.path.insert(0, p)
+
+# autotuner cache of this test session (read by the library at its first use)
+os.environ.setdefault("NUMA_HIP_TUNE_CACHE",
+                      os.path.join(tempfile.gettempdir(), f"numa_hip_tune_test_{os.getpid()}.json"))
 
 
 def pytest_configure(config):

@@ -1,0 +1,110 @@
+"""Temporally blocked smoothing (one launch per sweep or per pre/post phase)
+against one launch per step: the V-cycle output is bit for bit the same for
+every fusion mode, plane chunk, precision, smoother and degree, on grid sizes
+that leave partial tiles and chunks; the autotuned choice is recorded in the
+JSON tune cache; a solve with autotuned fusion equals the unfused solve."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import numa_hip as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _vcycle(ctx, n, cfg, precision, r):
+    dt = np.float32 if precision == H.PRECISION_MIXED else np.float64
+    mg = H.PoissonMG(ctx, n, cfg, precision)
+    dz = ctx.empty(n ** 3, dt)
+    mg(ctx.array(r.astype(dt)), dz)
+    out = dz.numpy()
+    fus = mg.fusion()
+    mg.close()
+    return out, fus
+
+
+CASES = [
+    # (smoother factory kwargs, n)
+    (dict(kind="cheb", degree=3, pre=2, post=2), 16),
+    (dict(kind="cheb", degree=3, pre=2, post=2), 45),
+    (dict(kind="cheb", degree=3, pre=2, post=2), 64),
+    (dict(kind="cheb", degree=3, pre=2, post=2), 130),
+    (dict(kind="cheb", degree=2, pre=1, post=3), 40),
+    (dict(kind="cheb", degree=4, pre=1, post=1), 50),
+    (dict(kind="cheb", degree=6, pre=1, post=1), 36),
+    (dict(kind="wj", pre=2, post=2), 48),
+    (dict(kind="wj", pre=3, post=1), 33),
+]
+
+
+def _cfg(spec, fuse, ichunk, rows=0):
+    if spec["kind"] == "cheb":
+        return H.mg_cheb(levels=4, degree=spec["degree"], pre=spec["pre"], post=spec["post"],
+                         fuse=fuse, ichunk=ichunk, rows=rows)
+    return H.mg_wjacobi(levels=4, pre=spec["pre"], post=spec["post"], fuse=fuse, ichunk=ichunk,
+                        rows=rows)
+
+
+@pytest.mark.parametrize("precision", [H.PRECISION_MIXED, H.PRECISION_FP64])
+@pytest.mark.parametrize("spec,n", CASES)
+def test_fused_smoothing_is_bitwise_unfused(ctx, spec, n, precision):
+    rng = np.random.default_rng(n)
+    r = rng.normal(size=n ** 3)
+    ref, fus = _vcycle(ctx, n, _cfg(spec, H.FUSE_NONE, 0), precision, r)
+    assert all(m == H.FUSE_NONE for m, _, _ in fus)
+    r0 = 4 if precision == H.PRECISION_MIXED else 2
+    per = spec.get("degree", 1)
+    for fuse in (H.FUSE_SWEEP, H.FUSE_PHASE):
+        group = per if fuse == H.FUSE_SWEEP else (6 // per) * per
+        for ichunk, rows in ((16, r0), (32, r0), (64, 2 * r0), (7, 2 * r0), (32, 2 * r0)):
+            if rows == 2 * r0 and group > 4:  # more than 160 KB of LDS: rejected
+                with pytest.raises(H.NumaError):
+                    _vcycle(ctx, n, _cfg(spec, fuse, ichunk, rows), precision, r)
+                continue
+            out, fus = _vcycle(ctx, n, _cfg(spec, fuse, ichunk, rows), precision, r)
+            assert fus[0] == (fuse, ichunk, rows)
+            np.testing.assert_array_equal(out, ref,
+                                          err_msg=f"fuse={fuse} ichunk={ichunk} rows={rows}")
+
+
+def test_autotune_records_choice(ctx):
+    path = os.environ["NUMA_HIP_TUNE_CACHE"]
+    n = 96
+    r = np.random.default_rng(1).normal(size=n ** 3)
+    ref, _ = _vcycle(ctx, n, H.mg_cheb(levels=4), H.PRECISION_MIXED, r)
+    out, fus = _vcycle(ctx, n, H.mg_cheb(levels=4, fuse=H.FUSE_AUTO), H.PRECISION_MIXED, r)
+    np.testing.assert_array_equal(out, ref)
+    assert all(m in (H.FUSE_NONE, H.FUSE_SWEEP, H.FUSE_PHASE) for m, _, _ in fus), fus
+    with open(path) as f:
+        cache = json.load(f)
+    keys = [k for k in cache if "|mg-smooth|f32|n=96|" in k]
+    assert keys, cache
+    mode, ichunk, rows = (int(v) for v in cache[keys[0]].split(","))
+    assert (mode, ichunk, rows) == fus[0]
+    # a new hierarchy of the same shape takes the cached choice
+    out2, fus2 = _vcycle(ctx, n, H.mg_cheb(levels=4, fuse=H.FUSE_AUTO), H.PRECISION_MIXED, r)
+    assert fus2 == fus
+    np.testing.assert_array_equal(out2, ref)
+
+
+@pytest.mark.parametrize("precision", [H.PRECISION_MIXED, H.PRECISION_FP64])
+def test_solve_with_autotuned_fusion_matches_unfused(ctx, precision):
+    n, lam = 48, 6.0
+    p = H.krylov_params(gmres_restart=30)
+    sols, results = [], []
+    for fuse in (H.FUSE_NONE, H.FUSE_AUTO):
+        cfg = H.solver_config(precond=H.PRECOND_GMG, precision=precision,
+                              flexible=1 if precision == H.PRECISION_MIXED else 0,
+                              mg=H.mg_cheb(levels=4, fuse=fuse))
+        s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+        x = np.zeros(n ** 3)
+        res = s.solve_host(p, x)
+        sols.append(x)
+        results.append((res.iters, res.n_lin_iters, res.n_JVP, res.res))
+        if fuse == H.FUSE_AUTO:
+            assert all(m != H.FUSE_AUTO for m, _, _ in s.mg_fusion())
+        s.close()
+    assert results[0] == results[1]
+    np.testing.assert_array_equal(sols[0], sols[1])
