@@ -331,6 +331,8 @@ def main() -> int:
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
+    ap.add_argument("--only", choices=["chandra", "learned"], default=None,
+                    help="run only this leg and print its JSON (for profiling)")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-out", default="", help=argparse.SUPPRESS)
@@ -354,6 +356,12 @@ def main() -> int:
     H.load()
     ctx = H.Context(0)
     params = make_params(H, args.forcing)
+    if args.only:
+        part = chandra_leg(H, ctx, args) if args.only == "chandra" else learned_leg(H, ctx, args,
+                                                                                      params)
+        print(json.dumps({args.only: part}))
+        ctx.close()
+        return 0
     cfgs = leg_configs(H, args.levels)
     legs = {}
     for name in args.legs.split(","):
