@@ -290,6 +290,56 @@ def learned_leg(H, ctx, args, params) -> dict:
     return out
 
 
+SWEEP_SEED, SWEEP_COUNT, SWEEP_SIZES = 20261203, 96, (32, 48, 64, 96)
+
+
+def sweep_leg(H, ctx, args, params) -> dict:
+    """Batched sweep: 96 Bratu3D instances, n from {32,48,64,96} and lambda ~
+    U[1.0, 6.7] drawn with std::mt19937_64(20261203), u0 = 0.1 U[0,1) from
+    mt19937_64(seed*7919 + 13 + i); mixed Newton-FGMRES(30) + GMG (the main
+    leg's configuration, autotuned fused smoothing).  One eager worker (the
+    instances one after another) next to W workers with captured Arnoldi
+    steps; each configuration is run once untimed (allocation, tuning,
+    capture) and once timed."""
+    import numpy as np
+    n, lam = H.sweep_instances(SWEEP_SEED, SWEEP_COUNT, sizes=SWEEP_SIZES)
+    x0 = [H.uniform_host(SWEEP_SEED * 7919 + 13 + i, int(n[i]) ** 3, 0.1)
+          for i in range(SWEEP_COUNT)]
+    xs = [ctx.array(x) for x in x0]
+    cfg = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                          precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
+                          mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO))
+    out = {"workload": f"bratu3d_sweep_{SWEEP_COUNT}_instances_n{'-'.join(map(str, SWEEP_SIZES))}"
+                       f"_lambda1.0-6.7_u0_0.1noise_mixed_newton_fgmres30_gmg",
+           "seed": SWEEP_SEED, "sizes": {str(k): int((n == k).sum()) for k in SWEEP_SIZES}}
+    sols = None
+    for workers, graphs in ((1, False), (4, True), (8, True)):
+        sw = H.Sweep(ctx, cfg, workers=workers, restart=30)
+        for rep_ in range(2):
+            for x, v in zip(xs, x0):
+                x.upload(v)
+            res, sec = sw.run(n, lam, xs, params, graphs=graphs)
+        sw.close()
+        key = f"workers{workers}_{'graphs' if graphs else 'eager'}"
+        out[key] = {"seconds": round(sec, 4), "solves_per_s": round(SWEEP_COUNT / sec, 1),
+                    "converged": int(sum(r.converged for r in res)),
+                    "newton_total": int(sum(r.iters for r in res)),
+                    "fgmres_total": int(sum(r.n_lin_iters for r in res))}
+        cur = [x.numpy() for x in xs]
+        if sols is None:
+            sols = cur
+        else:
+            out[key]["max_abs_diff_vs_workers1"] = float(max(np.max(np.abs(a - b))
+                                                             for a, b in zip(cur, sols)))
+        log(f"sweep {key}: {out[key]}")
+    best = max((v["solves_per_s"], k) for k, v in out.items() if isinstance(v, dict) and
+               "solves_per_s" in v)
+    out["best"] = {"config": best[1], "solves_per_s": best[0]}
+    for x in xs:
+        x.free()
+    return out
+
+
 def chandra_cpu_sample(args) -> None:
     """Child process: FP64 oracle Newton-GMRES(20) for the first 2 c values."""
     import oracle as O
@@ -331,7 +381,8 @@ def main() -> int:
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
-    ap.add_argument("--only", choices=["chandra", "learned"], default=None,
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--only", choices=["chandra", "learned", "sweep"], default=None,
                     help="run only this leg and print its JSON (for profiling)")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
@@ -357,8 +408,9 @@ def main() -> int:
     ctx = H.Context(0)
     params = make_params(H, args.forcing)
     if args.only:
-        part = chandra_leg(H, ctx, args) if args.only == "chandra" else learned_leg(H, ctx, args,
-                                                                                      params)
+        part = {"chandra": lambda: chandra_leg(H, ctx, args),
+                "learned": lambda: learned_leg(H, ctx, args, params),
+                "sweep": lambda: sweep_leg(H, ctx, args, params)}[args.only]()
         print(json.dumps({args.only: part}))
         ctx.close()
         return 0
@@ -452,6 +504,8 @@ def main() -> int:
                                                 f"of the {CH_B} c values ({info['seconds']:.2f} s), "
                                                 f"scaled by {CH_B}/{info['instances']}"}
         out["chandrasekhar"] = ch
+    if not args.no_sweep:
+        out["sweep"] = sweep_leg(H, ctx, args, params)
     if not args.no_learned:
         out["learned"] = learned_leg(H, ctx, args, params)
     print(json.dumps(out))

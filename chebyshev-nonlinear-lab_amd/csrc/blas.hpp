@@ -284,18 +284,22 @@ __global__ __launch_bounds__(kBlock) void k_diag_solve(int64_t N, const T* r, co
 
 namespace {
 // d = 6 h2inv - lambda e^u   (bratu3d.hpp:59-69)
-__global__ __launch_bounds__(kBlock) void k_jacobi_diag(int64_t N, double h2inv, double lambda,
-                                                         const double* u, double* d) {
+__global__ __launch_bounds__(kBlock) void k_jacobi_diag(int64_t N, double h2inv, double lambda_v,
+                                                        const double* lam_ptr, const double* u,
+                                                        double* d) {
 #pragma clang fp contract(off)
+    const double lambda = lam_ptr ? *lam_ptr : lambda_v;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * kBlock)
         d[i] = 6.0 * h2inv - lambda * exp(u[i]);
 }
 
 // out = -lambda e^u v v   (bratu3d.hpp:46-56)
-__global__ __launch_bounds__(kBlock) void k_d2_analytic(int64_t N, double lambda, const double* u,
-                                                         const double* v, double* out) {
+__global__ __launch_bounds__(kBlock) void k_d2_analytic(int64_t N, double lambda_v,
+                                                        const double* lam_ptr, const double* u,
+                                                        const double* v, double* out) {
 #pragma clang fp contract(off)
+    const double lambda = lam_ptr ? *lam_ptr : lambda_v;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * kBlock)
         out[i] = -lambda * exp(u[i]) * v[i] * v[i];

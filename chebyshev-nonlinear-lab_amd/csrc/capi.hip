@@ -220,7 +220,8 @@ int numa_hip_bratu_d2(numa_hip_ctx* ctx, int64_t n, double lambda, const double*
     require(u_dev && v_dev && out_dev, "null device pointer");
     const int64_t N = n * n * n;
     hipLaunchKernelGGL(numa_hip::k_d2_analytic, dim3(numa_hip::stream_blocks(N)),
-                       dim3(numa_hip::kBlock), 0, c.stream, N, lambda, u_dev, v_dev, out_dev);
+                       dim3(numa_hip::kBlock), 0, c.stream, N, lambda, (const double*)nullptr,
+                       u_dev, v_dev, out_dev);
     NUMA_HIP_LAUNCH_CHECK();
     NUMA_API_END
 }
@@ -234,7 +235,7 @@ int numa_hip_bratu_jacobi_diag(numa_hip_ctx* ctx, int64_t n, double lambda, cons
     const int64_t N = n * n * n;
     hipLaunchKernelGGL(numa_hip::k_jacobi_diag, dim3(numa_hip::stream_blocks(N)),
                        dim3(numa_hip::kBlock), 0, c.stream, N, (n + 1.0) * (n + 1.0), lambda,
-                       u_dev, d_dev);
+                       (const double*)nullptr, u_dev, d_dev);
     NUMA_HIP_LAUNCH_CHECK();
     NUMA_API_END
 }

@@ -607,18 +607,22 @@ __global__ __launch_bounds__(256) void k_wgrad_mid(int n, int nsamples, const bf
     }
 }
 
-// Sum of per-workgroup partials in workgroup order, scattered into the flat
-// gradient.  MODE 0: first layer ([co][28] -> W0, b0), MODE 1: last layer
-// ([ci][28] -> W5, b5), MODE 2: mid layer ([3][9][24][24] -> W[co][ci][tap]),
-// MODE 3: mid layer bias ([24] -> b).
+// Sum of per-workgroup partials, scattered into the flat gradient.  One wave
+// per output: lane l adds the partials l, l+64, ... in order, then a fixed
+// shuffle tree combines the lanes (deterministic).  MODE 0: first layer
+// ([co][28] -> W0, b0), MODE 1: last layer ([ci][28] -> W5, b5), MODE 2: mid
+// layer ([3][9][24][24] -> W[co][ci][tap]), MODE 3: mid layer bias ([24] -> b).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_reduce_partials(int G, const float* __restrict__ part,
                                                          float* __restrict__ grad) {
     constexpr int len = MODE == 2 ? 27 * kC * kC : (MODE == 3 ? kC : kC * 28);
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= len) return;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e >= len) return;  // uniform per wave
     float s = 0.0f;
-    for (int w = 0; w < G; ++w) s += part[(size_t)w * len + e];
+    for (int w = lane; w < G; w += 64) s += part[(size_t)w * len + e];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane != 0) return;
     if (MODE == 0 || MODE == 1) {
         const int c = e / 28, q = e % 28;
         if (q < 27)
@@ -633,6 +637,10 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int G, const float* __r
     } else {
         grad[e] = s;
     }
+}
+template <int MODE>
+constexpr int reduce_blocks() {
+    return ((MODE == 2 ? 27 * kC * kC : (MODE == 3 ? kC : kC * 28)) + 3) / 4;
 }
 
 // ---------------------------------------------------------------------------

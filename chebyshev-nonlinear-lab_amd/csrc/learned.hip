@@ -255,6 +255,9 @@ void ConvNet::loss_grad(int64_t n, int B, const float* x, const float* z, bool n
     const int nblk = sum_blocks(N3);
     require(2 * B * nblk <= kMaxPartials, "batch too large for the reduction scratch");
     const hipStream_t s = c_.stream;
+    // weight-gradient workgroups: one per tile up to kWgradWorkers (multiple of 8)
+    const int64_t wtiles = cnn::Tiles<>((int)n, B).total;
+    const int WG = (int)std::min<int64_t>(kWgradWorkers, (wtiles + 7) / 8 * 8);
     const float* xs = ones_();
     const float* ys = ones_();
     if (normalized) {
@@ -299,11 +302,11 @@ void ConvNet::loss_grad(int64_t n, int B, const float* x, const float* z, bool n
                            ni, (const float*)e_.as<float>(), (const float*)coef_(), g6_.as<float>());
     NUMA_HIP_LAUNCH_CHECK();
     // last layer
-    hipLaunchKernelGGL(cnn::k_wgrad_last, dim3(kWgradWorkers), dim3(256), 0, s, ni, B,
+    hipLaunchKernelGGL(cnn::k_wgrad_last, dim3(WG), dim3(256), 0, s, ni, B,
                        (const bf16x8*)act_[4].as<bf16x8>(), (const float*)g6_.as<float>(),
                        part_.as<float>());
-    hipLaunchKernelGGL(cnn::k_reduce_partials<1>, dim3(3), dim3(256), 0, s, kWgradWorkers,
-                       (const float*)part_.as<float>(), G(cnn::kW5));
+    hipLaunchKernelGGL(cnn::k_reduce_partials<1>, dim3(cnn::reduce_blocks<1>()), dim3(256), 0, s,
+                       WG, (const float*)part_.as<float>(), G(cnn::kW5));
     hipLaunchKernelGGL(cnn::k_bwd_last, dim3((unsigned)((N3 + 255) / 256), B), dim3(256), 0, s, ni,
                        (const float*)g6_.as<float>(), (const float*)P(cnn::kW5),
                        (const bf16x8*)pre_[4].as<bf16x8>(), ga_.as<bf16x8>());
@@ -311,14 +314,13 @@ void ConvNet::loss_grad(int64_t n, int B, const float* x, const float* z, bool n
     bf16x8* g = ga_.as<bf16x8>();
     bf16x8* gn = gb_.as<bf16x8>();
     for (int l = 3; l >= 0; --l) {
-        hipLaunchKernelGGL(cnn::k_wgrad_mid, dim3(kWgradWorkers, 3), dim3(256), 0, s, ni, B,
+        hipLaunchKernelGGL(cnn::k_wgrad_mid, dim3(WG, 3), dim3(256), 0, s, ni, B,
                            (const bf16x8*)g, (const bf16x8*)act_[l].as<bf16x8>(), part_.as<float>(),
                            bpart_.as<float>());
-        hipLaunchKernelGGL(cnn::k_reduce_partials<2>,
-                           dim3((unsigned)((27 * cnn::kC * cnn::kC + 255) / 256)), dim3(256), 0, s,
-                           kWgradWorkers, (const float*)part_.as<float>(), G(cnn::mid_w(l)));
-        hipLaunchKernelGGL(cnn::k_reduce_partials<3>, dim3(1), dim3(256), 0, s, kWgradWorkers,
-                           (const float*)bpart_.as<float>(), G(cnn::mid_b(l)));
+        hipLaunchKernelGGL(cnn::k_reduce_partials<2>, dim3(cnn::reduce_blocks<2>()), dim3(256), 0,
+                           s, WG, (const float*)part_.as<float>(), G(cnn::mid_w(l)));
+        hipLaunchKernelGGL(cnn::k_reduce_partials<3>, dim3(cnn::reduce_blocks<3>()), dim3(256), 0,
+                           s, WG, (const float*)bpart_.as<float>(), G(cnn::mid_b(l)));
         hipLaunchKernelGGL(cnn::k_conv24<cnn::CE_BWD>, dim3(cnn::kConvWorkers),
                            dim3(cnn::kConvThreads), 0, s, ni, B, (const bf16x8*)g,
                            (const bf16x8*)wpkT_[l].as<bf16x8>(), (const float*)nullptr,
@@ -326,10 +328,10 @@ void ConvNet::loss_grad(int64_t n, int B, const float* x, const float* z, bool n
         NUMA_HIP_LAUNCH_CHECK();
         std::swap(g, gn);
     }
-    hipLaunchKernelGGL(cnn::k_wgrad_first<float>, dim3(kWgradWorkers), dim3(256), 0, s, ni, B, x, xs,
+    hipLaunchKernelGGL(cnn::k_wgrad_first<float>, dim3(WG), dim3(256), 0, s, ni, B, x, xs,
                        (const bf16x8*)g, part_.as<float>());
-    hipLaunchKernelGGL(cnn::k_reduce_partials<0>, dim3(3), dim3(256), 0, s, kWgradWorkers,
-                       (const float*)part_.as<float>(), G(cnn::kW0));
+    hipLaunchKernelGGL(cnn::k_reduce_partials<0>, dim3(cnn::reduce_blocks<0>()), dim3(256), 0, s,
+                       WG, (const float*)part_.as<float>(), G(cnn::kW0));
     NUMA_HIP_LAUNCH_CHECK();
 }
 

@@ -425,6 +425,9 @@ struct MGBase {
     // Smoother fusion chosen for level l (NUMA_FUSE_*; AUTO until tuned), its
     // plane chunk and rows per thread.
     virtual void fusion(int64_t l, int64_t* mode, int64_t* ichunk, int64_t* rows) const = 0;
+    // Finishes set-up that needs eager device work (the autotuner's timing), so
+    // that apply() only launches kernels (and can be captured in a graph).
+    virtual void prepare() = 0;
 };
 
 template <class T>
@@ -486,6 +489,9 @@ public:
 
     int64_t levels() const override { return (int64_t)levels_.size(); }
     int64_t level_n(int64_t l) const override { return levels_[(size_t)l].n; }
+    void prepare() override {
+        if (!tuned_) autotune();
+    }
     void fusion(int64_t l, int64_t* mode, int64_t* ichunk, int64_t* rows) const override {
         *mode = levels_[(size_t)l].fuse;
         *ichunk = levels_[(size_t)l].ichunk;

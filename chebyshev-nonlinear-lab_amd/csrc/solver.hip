@@ -16,7 +16,9 @@
 // stay FP64.
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <memory>
+#include <tuple>
 
 #include "blas.hpp"
 #include "mg.hpp"
@@ -84,8 +86,10 @@ public:
         sd_ = static_cast<double*>(p);
         NUMA_HIP_CHECK(hipHostMalloc(&p, (size_t)nsc_ * sizeof(double), hipHostMallocDefault));
         hs_ = static_cast<double*>(p);
+        launch_fill<double>(c_, 1, lambda_, MISC(S_LAMBDA));
     }
     ~BratuSolver() override {
+        drop_graphs();
         for (void* p : allocs_) (void)hipFree(p);
         if (hs_) (void)hipHostFree(hs_);
     }
@@ -94,7 +98,14 @@ public:
     void set_lambda(double lambda) override {
         require(std::isfinite(lambda), "lambda must be finite");
         lambda_ = lambda;
+        launch_fill<double>(c_, 1, lambda_, MISC(S_LAMBDA));
     }
+    void set_graphs(bool on) override {
+        if (on && !xa_) xa_ = allocd();
+        if (!on) drop_graphs();
+        graphs_ = on;
+    }
+    int64_t graphs_cached() const override { return (int64_t)gcache_.size(); }
     void set_capture(CaptureFn fn) override { capture_ = std::move(fn); }
     void set_learned(std::shared_ptr<NetApply> net, double ratio) override {
         require(cfg_.precond == NUMA_PRECOND_LEARNED,
@@ -144,13 +155,13 @@ public:
                                          MISC(S_XNORM), MISC(S_VNORM)};
                         launch_stencil<true>(c_, n_, LdXpHV<T>{x2_, zb_, stp, 0.0},
                                              EpJvFD<T, true, false>{h2inv_, lambda_, stp, S(0), Fx_,
-                                                                    V_[1], V_[0], nullptr, 0.0, 1.0},
+                                                                    V_[1], V_[0], nullptr, 0.0, 1.0, lam_dev()},
                                              Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
                         return N * (8.0 + 8.0 + 3.0 * st);  // x, F0, v, out, V0
                     }
                     launch_stencil<true>(c_, n_, LdVec<T>{zb_},
                                          EpJvAn<T, true, false>{h2inv_, lambda_, S(0), x2_, V_[1],
-                                                                V_[0], nullptr, 1.0},
+                                                                V_[0], nullptr, 1.0, lam_dev()},
                                          Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
                     return N * (8.0 + 3.0 * st);
                 }
@@ -191,7 +202,10 @@ public:
 
 private:
     // misc scalar slots
-    enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_TMP, kMisc = 8 };
+    enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_TMP, S_LAMBDA, kMisc = 8 };
+    // lambda as a device scalar: the kernels read it, so captured graphs stay
+    // valid across set_lambda
+    const double* lam_dev() { return MISC(S_LAMBDA); }
     double* H(int64_t i, int64_t j) { return sd_ + j * (m_ + 1) + i; }
     double* S(int64_t i) { return sd_ + (m_ + 1) * m_ + i; }
     double* RHS(int64_t i) { return sd_ + (m_ + 1) * m_ + (m_ + 1) + i; }
@@ -235,14 +249,14 @@ private:
                 double* wout) {
         const Epilogue ep{EPI_SQRT, 1.0, nullptr, MISC(S_FNORM), nullptr};
         if (!z && !u)
-            launch_stencil<true>(c_, n_, LdVec<double>{x}, EpF<false>{h2inv_, lambda_, F, nullptr},
+            launch_stencil<true>(c_, n_, LdVec<double>{x}, EpF<false>{h2inv_, lambda_, F, nullptr, lam_dev()},
                                  ep);
         else if (!u)
             launch_stencil<true>(c_, n_, LdStep<double, true, false>{x, z, a, nullptr, 0.0},
-                                 EpF<true>{h2inv_, lambda_, F, wout}, ep);
+                                 EpF<true>{h2inv_, lambda_, F, wout, lam_dev()}, ep);
         else
             launch_stencil<true>(c_, n_, LdStep<double, true, true>{x, z, a, u, b},
-                                 EpF<true>{h2inv_, lambda_, F, wout}, ep);
+                                 EpF<true>{h2inv_, lambda_, F, wout, lam_dev()}, ep);
     }
 
     // Everything that depends only on the Newton point x: ||x|| for the FD
@@ -251,7 +265,7 @@ private:
         if (cfg_.jv_mode == NUMA_DERIV_FD || cfg_.d2_mode == NUMA_DERIV_FD) norm(x, MISC(S_XNORM));
         if (cfg_.precond == NUMA_PRECOND_JACOBI) {
             hipLaunchKernelGGL(k_jacobi_diag, dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
-                               N_, h2inv_, lambda_, x, diag_);
+                               N_, h2inv_, lambda_, lam_dev(), x, diag_);
             NUMA_HIP_LAUNCH_CHECK();
         }
     }
@@ -270,32 +284,32 @@ private:
             if (dotv)
                 launch_stencil<true>(c_, n_, L,
                                      EpJvFD<T, true, false>{h2inv_, lambda_, st, scale, F0, out,
-                                                            dotv, nullptr, 0.0, 1.0},
+                                                            dotv, nullptr, 0.0, 1.0, lam_dev()},
                                      ep);
             else if (b)
                 launch_stencil<true>(c_, n_, L,
                                      EpJvFD<T, false, true>{h2inv_, lambda_, st, scale, F0, out,
-                                                            nullptr, b, 0.0, 1.0},
+                                                            nullptr, b, 0.0, 1.0, lam_dev()},
                                      ep);
             else
                 launch_stencil<false>(c_, n_, L,
                                       EpJvFD<T, false, false>{h2inv_, lambda_, st, scale, F0, out,
-                                                              nullptr, nullptr, 0.0, 1.0},
+                                                              nullptr, nullptr, 0.0, 1.0, lam_dev()},
                                       ep);
         } else {
             const LdVec<TV> L{v};
             if (dotv)
                 launch_stencil<true>(
                     c_, n_, L,
-                    EpJvAn<T, true, false>{h2inv_, lambda_, scale, x, out, dotv, nullptr, 1.0}, ep);
+                    EpJvAn<T, true, false>{h2inv_, lambda_, scale, x, out, dotv, nullptr, 1.0, lam_dev()}, ep);
             else if (b)
                 launch_stencil<true>(
                     c_, n_, L,
-                    EpJvAn<T, false, true>{h2inv_, lambda_, scale, x, out, nullptr, b, 1.0}, ep);
+                    EpJvAn<T, false, true>{h2inv_, lambda_, scale, x, out, nullptr, b, 1.0, lam_dev()}, ep);
             else
                 launch_stencil<false>(
                     c_, n_, L,
-                    EpJvAn<T, false, false>{h2inv_, lambda_, scale, x, out, nullptr, nullptr, 1.0},
+                    EpJvAn<T, false, false>{h2inv_, lambda_, scale, x, out, nullptr, nullptr, 1.0, lam_dev()},
                     ep);
         }
     }
@@ -336,10 +350,10 @@ private:
             const FDStep st{std::pow(std::numeric_limits<double>::epsilon(), 0.25), MISC(S_XNORM),
                             MISC(S_VNORM)};
             launch_stencil<false>(c_, n_, LdPM<double>{x, v, st, 0.0},
-                                  EpD2FD{h2inv_, lambda_, st, F0, out, 0.0}, Epilogue{});
+                                  EpD2FD{h2inv_, lambda_, st, F0, out, 0.0, lam_dev()}, Epilogue{});
         } else {
             hipLaunchKernelGGL(k_d2_analytic, dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
-                               N_, lambda_, x, v, out);
+                               N_, lambda_, lam_dev(), x, v, out);
             NUMA_HIP_LAUNCH_CHECK();
         }
     }
@@ -356,6 +370,49 @@ private:
     numa_solver_config cfg_;
     int64_t m_;
     std::unique_ptr<MGBase> mg_;
+    // captured Arnoldi steps: (step, x0, F0) -> executable graph
+    bool graphs_ = false;
+    double* xa_ = nullptr;  // iterate buffer of graph mode (fixed pointers)
+    std::map<std::tuple<size_t, const void*, const void*>, hipGraphExec_t> gcache_;
+    void drop_graphs() {
+        for (auto& kv : gcache_) (void)hipGraphExecDestroy(kv.second);
+        gcache_.clear();
+    }
+    bool in_solve_ = false;  // graphs serve solve() (fixed iterate buffers) only
+    bool graph_ok() const {
+        return graphs_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_;
+    }
+    // Runs `body` (kernel launches only) for Arnoldi step j, from the graph
+    // captured for (j, x0, F0) or by capturing it now.
+    template <class F>
+    void run_step(size_t j, const double* x0, const double* F0, F&& body) {
+        if (!graph_ok() || !in_solve_) {
+            body();
+            return;
+        }
+        const auto key = std::make_tuple(j, (const void*)x0, (const void*)F0);
+        auto it = gcache_.find(key);
+        if (it == gcache_.end()) {
+            if (mg_) mg_->prepare();
+            NUMA_HIP_CHECK(hipStreamBeginCapture(c_.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c_.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            NUMA_HIP_CHECK(hipStreamEndCapture(c_.stream, &g));
+            hipGraphExec_t ge = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            NUMA_HIP_CHECK(e);
+            it = gcache_.emplace(key, ge).first;
+        }
+        NUMA_HIP_CHECK(hipGraphLaunch(it->second, c_.stream));
+    }
     std::shared_ptr<NetApply> net_;
     double ratio_ = 0.5;
     bool use_net_ = false;
@@ -449,19 +506,22 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             const size_t j = k;
             T* zdst = flex && !Z_.empty() ? Z_[j] : zb_;
             const bool by_net = cfg_.precond == NUMA_PRECOND_LEARNED && net_ && use_net_;
-            const T* zj = precond(V_[j], zdst);
+            run_step(j, x0, F0, [&]() {
+                const T* zj = precond(V_[j], zdst);
+                // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
+                jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
+                      Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
+                for (size_t i = 0; i <= j; ++i) {
+                    const bool last = i == j;
+                    const Epilogue ep =
+                        last ? Epilogue{EPI_NORM_SCALE, 1.0, nullptr, H(j + 1, j), S(j + 1)}
+                             : Epilogue{EPI_STORE, 1.0, S(i + 1), H(i + 1, j), nullptr};
+                    launch_mgs<T>(c_, N_, V_[j + 1], V_[i], H(i, j), S(i),
+                                  last ? (const T*)nullptr : (const T*)V_[i + 1], ep);
+                }
+            });
             ++res.precs;
-            // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
-            jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
-                  Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
             ++res.matvecs;
-            for (size_t i = 0; i <= j; ++i) {
-                const bool last = i == j;
-                const Epilogue ep = last ? Epilogue{EPI_NORM_SCALE, 1.0, nullptr, H(j + 1, j), S(j + 1)}
-                                         : Epilogue{EPI_STORE, 1.0, S(i + 1), H(i + 1, j), nullptr};
-                launch_mgs<T>(c_, N_, V_[j + 1], V_[i], H(i, j), S(i),
-                              last ? (const T*)nullptr : (const T*)V_[i + 1], ep);
-            }
             readn(H(0, j), (int64_t)j + 2, col.data());
             for (size_t i = 0; i <= j + 1; ++i) Hh[i][j] = col[i];
             s[j + 1] = Hh[j + 1][j] > 0.0 ? 1.0 / Hh[j + 1][j] : 0.0;
@@ -611,8 +671,16 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     const bool rec = cheb && p.recycle;
     require(p.gmres_restart >= 0 && p.gmres_restart <= m_,
             "gmres_restart exceeds the solver's basis capacity");
-    double* xc = xdev;  // current iterate
-    double* xt = x2_;   // trial point
+    // graph mode iterates in the solver's own buffers (fixed graph arguments)
+    const bool own = graph_ok();
+    if (own) launch_convert<double, double>(c_, N_, xdev, xa_);
+    struct InSolve {
+        bool& f;
+        ~InSolve() { f = false; }
+    } in_solve_guard{in_solve_};
+    in_solve_ = own;
+    double* xc = own ? xa_ : xdev;  // current iterate
+    double* xt = x2_;               // trial point
     double* Fc = Fx_;
     double* Ft = Fn_;
     eval_F(xc, nullptr, 0.0, nullptr, 0.0, Fc, nullptr);
@@ -678,7 +746,11 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     }
     if (rnorm <= p.rtol_nl * out.res0) out.converged = 1;
     out.res = rnorm;
-    if (xc != xdev) {
+    if (own) {
+        launch_convert<double, double>(c_, N_, xc, xdev);
+        xa_ = xc;  // the pair {xa_, x2_} is kept, in either order
+        x2_ = xt;
+    } else if (xc != xdev) {
         launch_convert<double, double>(c_, N_, xc, xdev);
         x2_ = xc;  // the internal buffer now holds the previous trial storage
     }
@@ -825,6 +897,22 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
     if (mg && mode_host && ichunk_host && rows_host)
         for (int64_t l = 0; l < mg->levels(); ++l)
             mg->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
+    NUMA_API_END
+}
+
+int numa_hip_solver_set_graphs(numa_hip_solver* s, int64_t enable) {
+    NUMA_API_BEGIN
+    numa_hip::require(s != nullptr, "null solver");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    NUMA_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    s->impl->set_graphs(enable != 0);
+    NUMA_API_END
+}
+
+int numa_hip_solver_graphs_cached(const numa_hip_solver* s, int64_t* count) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && count, "null argument");
+    *count = s->impl->graphs_cached();
     NUMA_API_END
 }
 

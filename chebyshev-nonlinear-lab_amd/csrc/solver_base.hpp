@@ -47,6 +47,11 @@ struct SolverBase {
     virtual LearnedStats learned_stats() const = 0;
     // The multigrid preconditioner (null without one).
     virtual const MGBase* multigrid() const = 0;
+    // Replay each Arnoldi step of solve() from a captured hipGraph (one graph
+    // per step index and iterate buffers, captured at first use).  Solvers with
+    // the learned preconditioner or a capture hook run eagerly.
+    virtual void set_graphs(bool on) = 0;
+    virtual int64_t graphs_cached() const = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

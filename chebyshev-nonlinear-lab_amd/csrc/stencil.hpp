@@ -275,6 +275,10 @@ struct LdPM {
 // ---------------------------------------------------------------------------
 // Epilogues
 // ---------------------------------------------------------------------------
+// Every Bratu epilogue takes lambda by value or, when lam_ptr is set, from a
+// device scalar read once per thread in prep() (a solver's lambda, so that a
+// captured graph stays valid when lambda changes).
+//
 // F = h2inv*(6w - sum nbrs) - lambda e^w (bratu3d.hpp:21-31); with WOUT also
 // writes the trial point w itself; returns F^2.
 template <bool WOUT>
@@ -283,7 +287,12 @@ struct EpF {
     double h2inv, lambda;
     double* F;
     double* wout;
-    __device__ EpF prep() const { return *this; }
+    const double* lam_ptr = nullptr;
+    __device__ EpF prep() const {
+        EpF r = *this;
+        if (lam_ptr) r.lambda = *lam_ptr;
+        return r;
+    }
     __device__ NoExtra preload(int64_t) const { return {}; }
     __device__ double operator()(int64_t id, NoExtra, P1 c, P1 im, P1 ip, P1 jm, P1 jp, P1 km,
                                  P1 kp) const {
@@ -318,10 +327,12 @@ struct EpJvFD {
     const TO* dotv;
     const double* b;
     double h, scale;
+    const double* lam_ptr = nullptr;
     __device__ EpJvFD prep() const {
         EpJvFD r = *this;
         r.h = step.h();
         r.scale = scale_ptr ? *scale_ptr : 1.0;
+        if (lam_ptr) r.lambda = *lam_ptr;
         return r;
     }
     __device__ JvExtra<TO> preload(int64_t id) const {
@@ -360,9 +371,11 @@ struct EpJvAn {
     const TO* dotv;
     const double* b;
     double scale;
+    const double* lam_ptr = nullptr;
     __device__ EpJvAn prep() const {
         EpJvAn r = *this;
         r.scale = scale_ptr ? *scale_ptr : 1.0;
+        if (lam_ptr) r.lambda = *lam_ptr;
         return r;
     }
     __device__ JvExtra<TO> preload(int64_t id) const {
@@ -394,9 +407,11 @@ struct EpD2FD {
     const double* F0;
     double* out;
     double h;
+    const double* lam_ptr = nullptr;
     __device__ EpD2FD prep() const {
         EpD2FD r = *this;
         r.h = step.h();
+        if (lam_ptr) r.lambda = *lam_ptr;
         return r;
     }
     __device__ double preload(int64_t id) const { return F0[id]; }

@@ -207,6 +207,15 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_solver_set_net": [_vp, _vp, C.c_double],
         "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
         "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
+        "numa_hip_solver_set_graphs": [_vp, _i64],
+        "numa_hip_solver_graphs_cached": [_vp, C.POINTER(_i64)],
+        "numa_hip_sweep_create": [_vp, _i64, C.POINTER(SolverConfig), _i64, C.POINTER(_vp)],
+        "numa_hip_sweep_destroy": [_vp],
+        "numa_hip_sweep_run": [_vp, _i64, C.POINTER(_i64), _dp, C.POINTER(_vp),
+                               C.POINTER(KrylovParams), _i64, C.POINTER(KrylovResult), _dp],
+        "numa_hip_sweep_instances": [_i64, _i64, C.POINTER(_i64), _i64, C.c_double, C.c_double,
+                                     C.POINTER(_i64), _dp],
+        "numa_hip_uniform_host": [_i64, _i64, C.c_double, _dp],
         "numa_hip_solver_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                                       C.POINTER(_i64)],
     }
@@ -438,6 +447,14 @@ class BratuSolver:
                                                    fallback_ratio))
         self._net = net
 
+    def set_graphs(self, on: bool = True) -> None:
+        check(self.ctx.lib.numa_hip_solver_set_graphs(self.h, 1 if on else 0))
+
+    def graphs_cached(self) -> int:
+        c = _i64()
+        check(self.ctx.lib.numa_hip_solver_graphs_cached(self.h, C.byref(c)))
+        return int(c.value)
+
     def mg_fusion(self) -> list[tuple[int, int, int]]:
         """(NUMA_FUSE_* mode, plane chunk, rows per thread) per multigrid level."""
         L = _i64()
@@ -644,6 +661,66 @@ class ConvNet:
     def close(self):
         if self.h and self.ctx.h:
             check(self.ctx.lib.numa_hip_net_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def uniform_host(seed: int, count: int, scale: float = 1.0) -> np.ndarray:
+    """scale * U[0,1) from std::mt19937_64(seed), U = (u64 >> 11) 2^-53 (host)."""
+    L = load()
+    out = np.empty(int(count))
+    check(L.numa_hip_uniform_host(int(seed), int(count), float(scale), out.ctypes.data_as(_dp)))
+    return out
+
+
+def sweep_instances(seed: int, count: int, sizes=(32, 48, 64, 96), lambda_lo: float = 1.0,
+                    lambda_hi: float = 6.7) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded (n, lambda) of a sweep (std::mt19937_64, see include/numa_hip.h)."""
+    L = load()
+    sz = np.ascontiguousarray(sizes, np.int64)
+    n = np.empty(int(count), np.int64)
+    lam = np.empty(int(count))
+    check(L.numa_hip_sweep_instances(int(seed), int(count),
+                                     sz.ctypes.data_as(C.POINTER(_i64)), sz.size, lambda_lo,
+                                     lambda_hi, n.ctypes.data_as(C.POINTER(_i64)),
+                                     lam.ctypes.data_as(_dp)))
+    return n, lam
+
+
+class Sweep:
+    """Batched sweep over Bratu3D instances (numa_hip_sweep): `workers`
+    streams with per-size solvers, driven by host threads."""
+
+    def __init__(self, ctx: Context, cfg: SolverConfig, workers: int = 8, restart: int = 30):
+        self.ctx, self.cfg = ctx, cfg
+        h = _vp()
+        check(ctx.lib.numa_hip_sweep_create(ctx.h, int(workers), C.byref(cfg), int(restart),
+                                            C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def run(self, ns, lambdas, xs: list, params: KrylovParams, graphs: bool = True):
+        """Solves instance i from xs[i] (DeviceArray, overwritten); returns
+        (list of KrylovResult, wall seconds)."""
+        count = len(xs)
+        n = np.ascontiguousarray(ns, np.int64)
+        lam = np.ascontiguousarray(lambdas, np.float64)
+        ptrs = (_vp * max(1, count))(*[x.ptr for x in xs])
+        res = (KrylovResult * max(1, count))()
+        sec = C.c_double()
+        check(self.ctx.lib.numa_hip_sweep_run(self.h, count, n.ctypes.data_as(C.POINTER(_i64)),
+                                              lam.ctypes.data_as(_dp), ptrs, C.byref(params),
+                                              1 if graphs else 0, res, C.byref(sec)))
+        return list(res)[:count], sec.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_sweep_destroy(self.h))
         self.h = None
 
     def __del__(self):

@@ -363,6 +363,37 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
 /* stats3 = {network applications, V-cycle applications, fallbacks}. */
 int numa_hip_solver_learned_stats(const numa_hip_solver* s, int64_t* stats3);
 
+/* ---- Krylov-step graphs and the batched sweep (no reference counterpart) -- */
+/* enable != 0: solve() replays each Arnoldi step (preconditioner, J z, MGS
+ * passes) from a hipGraph captured at its first use (per step index and
+ * iterate buffer); the iterate is copied into the solver's own buffers for
+ * the solve.  Results are identical to the eager path.  Solvers with the
+ * learned preconditioner run eagerly. */
+int numa_hip_solver_set_graphs(numa_hip_solver* s, int64_t enable);
+int numa_hip_solver_graphs_cached(const numa_hip_solver* s, int64_t* count);
+
+/* A sweep over many instances (grid n_i, lambda_i) with one configuration:
+ * `workers` streams of the context's device, each with its own solver per
+ * grid size (reused across instances), driven by as many host threads; the
+ * instances, largest first, go to the next free worker. */
+typedef struct numa_hip_sweep numa_hip_sweep;
+int numa_hip_sweep_create(numa_hip_ctx* ctx, int64_t workers, const numa_solver_config* cfg,
+                          int64_t gmres_restart, numa_hip_sweep** out);
+int numa_hip_sweep_destroy(numa_hip_sweep* h);
+/* Solves instance i from x_dev[i] (device, FP64, n_i^3, overwritten by the
+ * solution); results[i] receives its KrylovResult; *seconds (may be NULL) the
+ * wall time of the run.  graphs != 0 replays captured Arnoldi steps. */
+int numa_hip_sweep_run(numa_hip_sweep* h, int64_t count, const int64_t* n, const double* lambda,
+                       double* const* x_dev, const numa_krylov_params* p, int64_t graphs,
+                       numa_krylov_result* results, double* seconds);
+/* Seeded instances: n_out[i] from sizes (u64 % nsizes), lambda_out[i] ~
+ * U[lambda_lo, lambda_hi], std::mt19937_64(seed), U[0,1) = (u64 >> 11) 2^-53. */
+int numa_hip_sweep_instances(int64_t seed, int64_t count, const int64_t* sizes, int64_t nsizes,
+                             double lambda_lo, double lambda_hi, int64_t* n_out,
+                             double* lambda_out);
+/* out[i] = scale * U[0,1) from std::mt19937_64(seed) (host only). */
+int numa_hip_uniform_host(int64_t seed, int64_t count, double scale, double* out);
+
 /* ---- per-call primitive costs (instrumentation) -------------------------- */
 /* The reference freezes per-call costs c_F, c_JVP, c_D2, c_prec of its
  * primitives (experiments/c1/frozen/primitive_costs.csv, measured by

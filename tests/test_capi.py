@@ -3,6 +3,7 @@ entry point include/numa_hip.h declares (no device calls)."""
 import ctypes
 import os
 
+import numpy as np
 import pytest
 
 import numa_hip as H
@@ -62,3 +63,25 @@ def test_pod_layouts_match_the_c_compiler(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_uniform_host_is_mt19937_64():
+    """std::mt19937_64 with the default seed 5489 gives 9981545732273789042 as
+    its 10000th output (C++ standard, [rand.predef]); U = (u64 >> 11) 2^-53."""
+    u = H.uniform_host(5489, 10000, 1.0)
+    assert u[-1] == (9981545732273789042 >> 11) * 2.0 ** -53
+    assert np.all((u >= 0.0) & (u < 1.0))
+    v = H.uniform_host(5489, 10000, 0.1)
+    np.testing.assert_array_equal(v, 0.1 * u)
+
+
+def test_sweep_instances_are_seeded():
+    n1, l1 = H.sweep_instances(20261203, 96)
+    n2, l2 = H.sweep_instances(20261203, 96)
+    np.testing.assert_array_equal(n1, n2)
+    np.testing.assert_array_equal(l1, l2)
+    assert set(n1.tolist()) <= {32, 48, 64, 96} and len(set(n1.tolist())) == 4
+    assert l1.min() >= 1.0 and l1.max() <= 6.7
+    # first draw: size index = u64 % 4 of the first output of mt19937_64(seed)
+    n3, _ = H.sweep_instances(5489, 1, sizes=(10, 11, 12, 13))
+    assert n3[0] == (10, 11, 12, 13)[14514284786278117030 % 4]
