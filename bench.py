@@ -66,9 +66,11 @@ def leg_configs(H, levels: int):
     return {"fp64": fp64, "mixed": mixed, "fp64_fused": fp64_fused}
 
 
-def time_leg(H, ctx, n, lam, cfg, params, steps, warmup):
+def time_leg(H, ctx, n, lam, cfg, params, steps, warmup, graphs=False):
     import numpy as np
     s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+    if graphs:
+        s.set_graphs(True)
     x = ctx.empty(n ** 3)
     zeros = np.zeros(n ** 3)
     res = None
@@ -378,6 +380,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--legs", default="fp64,mixed,fp64_fused")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the Arnoldi steps of the mixed and fp64_fused legs from hipGraphs")
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
@@ -419,7 +423,7 @@ def main() -> int:
     for name in args.legs.split(","):
         log(f"leg {name}: n={args.n} lambda={args.lam}")
         s, x, res, times = time_leg(H, ctx, args.n, args.lam, cfgs[name], params, args.steps,
-                                    args.warmup)
+                                    args.warmup, graphs=args.graphs and name != "fp64")
         legs[name] = {"solver": s, "x": x, "res": res, "times": times,
                       "prims": primitive_table(H, s), "fusion": s.mg_fusion()}
         log(f"  primitives: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['GBs']:.0f} GB/s"

@@ -90,6 +90,11 @@ public:
     }
     ~BratuSolver() override {
         drop_graphs();
+        for (auto& t : tpend_) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        for (hipEvent_t e : evpool_) (void)hipEventDestroy(e);
         for (void* p : allocs_) (void)hipFree(p);
         if (hs_) (void)hipHostFree(hs_);
     }
@@ -106,6 +111,18 @@ public:
         graphs_ = on;
     }
     int64_t graphs_cached() const override { return (int64_t)gcache_.size(); }
+    void set_timing(bool on) override {
+        c_.sync();
+        collect_timers();
+        timing_ = on;
+    }
+    numa_timers timers(bool reset) override {
+        c_.sync();
+        collect_timers();
+        const numa_timers t = tacc_;
+        if (reset) tacc_ = numa_timers{};
+        return t;
+    }
     void set_capture(CaptureFn fn) override { capture_ = std::move(fn); }
     void set_learned(std::shared_ptr<NetApply> net, double ratio) override {
         require(cfg_.precond == NUMA_PRECOND_LEARNED,
@@ -123,6 +140,7 @@ public:
         eval_F(x0, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
         auto r = gmres(x0, Fx_, b, z, rtol, restart, max_iters, zero_guess, nullptr, false);
         c_.sync();
+        collect_timers();
         return r;
     }
 
@@ -380,7 +398,58 @@ private:
     }
     bool in_solve_ = false;  // graphs serve solve() (fixed iterate buffers) only
     bool graph_ok() const {
-        return graphs_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_;
+        return graphs_ && !timing_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_;
+    }
+    // ---- timers (operators/timing.hpp:16-108): device time of each primitive
+    // call, from a pair of HIP events on the solver stream, summed after the
+    // solve's final synchronisation.
+    enum { T_F = 0, T_JVP, T_D2, T_PREC, T_SETUP };
+    bool timing_ = false;
+    numa_timers tacc_{};
+    struct TPair {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> evpool_;
+    std::vector<TPair> tpend_;
+    hipEvent_t ev_get() {
+        if (evpool_.empty()) {
+            hipEvent_t e = nullptr;
+            NUMA_HIP_CHECK(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = evpool_.back();
+        evpool_.pop_back();
+        return e;
+    }
+    template <class F>
+    void timed(int cat, F&& f) {
+        if (!timing_) {
+            f();
+            return;
+        }
+        hipEvent_t a = ev_get(), b = ev_get();
+        NUMA_HIP_CHECK(hipEventRecord(a, c_.stream));
+        f();
+        NUMA_HIP_CHECK(hipEventRecord(b, c_.stream));
+        tpend_.push_back({cat, a, b});
+    }
+    void collect_timers() {  // the stream has been synchronised
+        for (const TPair& t : tpend_) {
+            float ms = 0.0f;
+            NUMA_HIP_CHECK(hipEventElapsedTime(&ms, t.a, t.b));
+            const double sec = 1e-3 * (double)ms;
+            switch (t.cat) {
+                case T_F: tacc_.t_F += sec; break;
+                case T_JVP: tacc_.t_JVP += sec; break;
+                case T_D2: tacc_.t_D2 += sec; break;
+                case T_PREC: tacc_.t_prec += sec; break;
+                default: tacc_.t_setup_dynamic += sec; break;
+            }
+            evpool_.push_back(t.a);
+            evpool_.push_back(t.b);
+        }
+        tpend_.clear();
     }
     // Runs `body` (kernel launches only) for Arnoldi step j, from the graph
     // captured for (j, x0, F0) or by capturing it now.
@@ -484,8 +553,10 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             norm(b, MISC(S_BETA), S(0));
             first = false;
         } else {
-            jv<double>(x0, F0, xl, V_[0], nullptr, nullptr, b,
-                       Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)});
+            timed(T_JVP, [&]() {
+                jv<double>(x0, F0, xl, V_[0], nullptr, nullptr, b,
+                           Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)});
+            });
             ++res.matvecs;
         }
         beta = read1(MISC(S_BETA));
@@ -507,10 +578,13 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             T* zdst = flex && !Z_.empty() ? Z_[j] : zb_;
             const bool by_net = cfg_.precond == NUMA_PRECOND_LEARNED && net_ && use_net_;
             run_step(j, x0, F0, [&]() {
-                const T* zj = precond(V_[j], zdst);
+                const T* zj = nullptr;
+                timed(T_PREC, [&]() { zj = precond(V_[j], zdst); });
                 // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
-                jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
-                      Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
+                timed(T_JVP, [&]() {
+                    jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
+                          Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
+                });
                 for (size_t i = 0; i <= j; ++i) {
                     const bool last = i == j;
                     const Epilogue ep =
@@ -572,7 +646,8 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
         } else {
             for (size_t i = 0; i < k; ++i) vecs[i] = V_[i];
             launch_combine<T, T>(c_, N_, vecs, coef, comb_, false);
-            const T* zq = precond(comb_, zb_);
+            const T* zq = nullptr;
+            timed(T_PREC, [&]() { zq = precond(comb_, zb_); });
             ++res.precs;
             launch_axpy<T, double>(c_, N_, 1.0, zq, xl);
         }
@@ -654,7 +729,8 @@ numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double*
         ck[j] = y[j] * saved_s_[j];
     }
     launch_combine<T, T>(c_, N_, vk, ck, comb_, false);
-    const T* px = precond(comb_, zb_);
+    const T* px = nullptr;
+    timed(T_PREC, [&]() { px = precond(comb_, zb_); });
     launch_convert<T, double>(c_, N_, px, xl);
     auto result = gmres(x0, F0, b, xl, rtol, restart, max_iters, false, r_, false);
     ++result.precs;
@@ -683,7 +759,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     double* xt = x2_;               // trial point
     double* Fc = Fx_;
     double* Ft = Fn_;
-    eval_F(xc, nullptr, 0.0, nullptr, 0.0, Fc, nullptr);
+    timed(T_F, [&]() { eval_F(xc, nullptr, 0.0, nullptr, 0.0, Fc, nullptr); });
     ++out.n_F;
     double rnorm = read1(MISC(S_FNORM));
     out.res0 = rnorm;
@@ -695,7 +771,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         }
         const double eta1 = forcing(p, k, rnorm, rprev, eta_prev, out.res0);
         const double eta2 = std::min(p.eta2_max, p.eta2_factor * eta1);
-        point_setup(xc);
+        timed(T_SETUP, [&]() { point_setup(xc); });
         const auto g1 = gmres(xc, Fc, Fc, z_, eta1, p.gmres_restart, p.gmres_max_iters, true,
                               nullptr, rec);
         out.n_JVP += g1.matvecs;
@@ -706,7 +782,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         bool ok = false;
         double fn = 0.0;
         if (cheb) {
-            d2(xc, Fc, z_, w_);
+            timed(T_D2, [&]() { d2(xc, Fc, z_, w_); });
             ++out.n_D2;
             const auto g2 = rec ? gmres_recycled(xc, Fc, w_, u_, eta2, p.gmres_restart,
                                                  p.gmres_max_iters)
@@ -717,7 +793,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
             out.n_prec += g2.precs;
             if (!g2.converged) ++out.n_lin_failures;
             if (!g2.breakdown) {
-                eval_F(xc, z_, 1.0, u_, 0.5, Ft, xt);
+                timed(T_F, [&]() { eval_F(xc, z_, 1.0, u_, 0.5, Ft, xt); });
                 ++out.n_F;
                 fn = read1(MISC(S_FNORM));
                 if (fn < rnorm) ok = true;
@@ -726,7 +802,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         if (!ok) {
             double lam = 1.0;
             while (lam >= p.min_lambda) {
-                eval_F(xc, z_, lam, nullptr, 0.0, Ft, xt);
+                timed(T_F, [&]() { eval_F(xc, z_, lam, nullptr, 0.0, Ft, xt); });
                 ++out.n_F;
                 fn = read1(MISC(S_FNORM));
                 if (fn <= (1.0 - p.armijo * lam) * rnorm) {
@@ -758,6 +834,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     Fx_ = Fc;
     Fn_ = Ft;
     c_.sync();
+    collect_timers();
     out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -913,6 +990,22 @@ int numa_hip_solver_graphs_cached(const numa_hip_solver* s, int64_t* count) {
     NUMA_API_BEGIN
     numa_hip::require(s && count, "null argument");
     *count = s->impl->graphs_cached();
+    NUMA_API_END
+}
+
+int numa_hip_solver_set_timing(numa_hip_solver* s, int64_t enable) {
+    NUMA_API_BEGIN
+    numa_hip::require(s != nullptr, "null solver");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    s->impl->set_timing(enable != 0);
+    NUMA_API_END
+}
+
+int numa_hip_solver_timers(numa_hip_solver* s, numa_timers* out, int64_t reset) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && out, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *out = s->impl->timers(reset != 0);
     NUMA_API_END
 }
 

@@ -52,6 +52,10 @@ struct SolverBase {
     // the learned preconditioner or a capture hook run eagerly.
     virtual void set_graphs(bool on) = 0;
     virtual int64_t graphs_cached() const = 0;
+    // Device time per primitive (Timed / TimedPrecond, operators/timing.hpp);
+    // timing runs the Arnoldi steps eagerly.
+    virtual void set_timing(bool on) = 0;
+    virtual numa_timers timers(bool reset) = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

@@ -78,6 +78,12 @@ class SolverConfig(C.Structure):
                 ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
 
 
+class Timers(C.Structure):
+    """numa::op::Timers (operators/timing.hpp:16-24): seconds per primitive."""
+    _fields_ = [("t_F", C.c_double), ("t_JVP", C.c_double), ("t_D2", C.c_double),
+                ("t_prec", C.c_double), ("t_setup_dynamic", C.c_double)]
+
+
 class NetTrainConfig(C.Structure):
     _fields_ = [("pairs", _i64), ("batch", _i64), ("sizes", _i64 * 4), ("lambda_lo", C.c_double),
                 ("lambda_hi", C.c_double), ("noise", C.c_double), ("lr", C.c_double),
@@ -208,6 +214,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
         "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
         "numa_hip_solver_set_graphs": [_vp, _i64],
+        "numa_hip_solver_set_timing": [_vp, _i64],
+        "numa_hip_solver_timers": [_vp, C.POINTER(Timers), _i64],
         "numa_hip_solver_graphs_cached": [_vp, C.POINTER(_i64)],
         "numa_hip_sweep_create": [_vp, _i64, C.POINTER(SolverConfig), _i64, C.POINTER(_vp)],
         "numa_hip_sweep_destroy": [_vp],
@@ -446,6 +454,14 @@ class BratuSolver:
         check(self.ctx.lib.numa_hip_solver_set_net(self.h, net.h if net is not None else None,
                                                    fallback_ratio))
         self._net = net
+
+    def set_timing(self, on: bool = True) -> None:
+        check(self.ctx.lib.numa_hip_solver_set_timing(self.h, 1 if on else 0))
+
+    def timers(self, reset: bool = False) -> Timers:
+        t = Timers()
+        check(self.ctx.lib.numa_hip_solver_timers(self.h, C.byref(t), 1 if reset else 0))
+        return t
 
     def set_graphs(self, on: bool = True) -> None:
         check(self.ctx.lib.numa_hip_solver_set_graphs(self.h, 1 if on else 0))

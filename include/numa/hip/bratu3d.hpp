@@ -132,12 +132,26 @@ struct GmresResult {
     bool breakdown = false;
 };
 
+// operators/timing.hpp:16-24: seconds per primitive.  On the device each is
+// the GPU time of the primitive's kernels (HIP events on the solver stream).
+struct Timers {
+    double t_F = 0.0;
+    double t_JVP = 0.0;
+    double t_D2 = 0.0;
+    double t_prec = 0.0;
+    double t_setup_dynamic = 0.0;
+    void reset() { *this = Timers{}; }
+};
+
 // Derivative and precision options of the device solver.
 struct SolverOptions {
     int precision = NUMA_PRECISION_FP64;
     bool flexible = false;              // FGMRES
     int jv_mode = NUMA_DERIV_ANALYTIC;  // Bratu3D::jvp as in the reference
     int d2_mode = NUMA_DERIV_ANALYTIC;  // Bratu3D::d2 as in the reference
+    // If set, the solve adds its per-primitive times to *timers, as the
+    // reference's Timed<Op> / TimedPrecond wrappers do (timing.hpp:37-108).
+    Timers* timers = nullptr;
 };
 
 // problems/bratu3d.hpp:12-100 on the device.  eval / jvp / d2 stage host
@@ -229,7 +243,7 @@ inline numa_solver_config base_config(int method, const SolverOptions& o) {
 }
 
 inline KrylovResult solve(const Bratu3D& f, DVec x, const KrylovParams& p,
-                          const numa_solver_config& cfg) {
+                          const numa_solver_config& cfg, Timers* timers = nullptr) {
     if (x.size() != f.size()) throw std::invalid_argument("x has the wrong size");
     numa_hip_solver* s = nullptr;
     check(numa_hip_bratu_solver_create(f.context().get(), (int64_t)f.grid(), f.lambda(), &cfg,
@@ -237,7 +251,17 @@ inline KrylovResult solve(const Bratu3D& f, DVec x, const KrylovParams& p,
     std::unique_ptr<numa_hip_solver, int (*)(numa_hip_solver*)> guard(s, numa_hip_solver_destroy);
     const numa_krylov_params cp = p.c();
     numa_krylov_result r{};
+    if (timers) check(numa_hip_solver_set_timing(s, 1));
     check(numa_hip_solve_host(s, &cp, x.data(), &r));
+    if (timers) {
+        numa_timers t{};
+        check(numa_hip_solver_timers(s, &t, 0));
+        timers->t_F += t.t_F;
+        timers->t_JVP += t.t_JVP;
+        timers->t_D2 += t.t_D2;
+        timers->t_prec += t.t_prec;
+        timers->t_setup_dynamic += t.t_setup_dynamic;
+    }
     KrylovResult out;
     out.x = std::move(x);
     out.converged = r.converged != 0;
@@ -260,7 +284,8 @@ inline KrylovResult solve(const Bratu3D& f, DVec x, const KrylovParams& p,
 template <class PrecondProvider = NoPrecond>
 KrylovResult newton_krylov(const Bratu3D& f, DVec x, const KrylovParams& p = {},
                            const PrecondProvider& pc = {}, const SolverOptions& o = {}) {
-    return detail::solve(f, std::move(x), p, pc.apply(detail::base_config(NUMA_METHOD_NEWTON, o)));
+    return detail::solve(f, std::move(x), p, pc.apply(detail::base_config(NUMA_METHOD_NEWTON, o)),
+                         o.timers);
 }
 
 // methods/krylov_newton.hpp:116-118
@@ -268,7 +293,7 @@ template <class PrecondProvider = NoPrecond>
 KrylovResult chebyshev_inexact(const Bratu3D& f, DVec x, const KrylovParams& p = {},
                                const PrecondProvider& pc = {}, const SolverOptions& o = {}) {
     return detail::solve(f, std::move(x), p,
-                         pc.apply(detail::base_config(NUMA_METHOD_CHEBYSHEV, o)));
+                         pc.apply(detail::base_config(NUMA_METHOD_CHEBYSHEV, o)), o.timers);
 }
 
 // linalg/gmres.hpp:61-65 with apply(v, out) = J(x0) v and the provider's

@@ -363,6 +363,26 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
 /* stats3 = {network applications, V-cycle applications, fallbacks}. */
 int numa_hip_solver_learned_stats(const numa_hip_solver* s, int64_t* stats3);
 
+/* ---- timers (operators/timing.hpp:16-108) ------------------------------- */
+/* Mirrors numa::op::Timers: seconds spent in F, J v, F'' and the
+ * preconditioner's applications and per-point set-up (Provider::make; here the
+ * Jacobi diagonal and the ||x|| of the FD step).  On the device each is the
+ * GPU time of the call's kernels (a pair of HIP events on the solver stream),
+ * accumulated over solves until reset. */
+typedef struct numa_timers {
+    double t_F;
+    double t_JVP;
+    double t_D2;
+    double t_prec;
+    double t_setup_dynamic;
+} numa_timers;
+/* enable != 0 times every primitive call of the following solves (Timed /
+ * TimedPrecond around the operator and the provider); the Arnoldi steps then
+ * run eagerly, without graphs. */
+int numa_hip_solver_set_timing(numa_hip_solver* s, int64_t enable);
+/* Accumulated timers; reset != 0 zeroes them afterwards. */
+int numa_hip_solver_timers(numa_hip_solver* s, numa_timers* out, int64_t reset);
+
 /* ---- Krylov-step graphs and the batched sweep (no reference counterpart) -- */
 /* enable != 0: solve() replays each Arnoldi step (preconditioner, J z, MGS
  * passes) from a hipGraph captured at its first use (per step index and

@@ -98,6 +98,21 @@ int main() {
         const auto ch = chebyshev_inexact(f, x0, KrylovParams{}, PoissonMGProvider(16));
         CHECK(ch.iters == 4 && ch.n_JVP == 20 && ch.n_D2 == 4 && ch.n_prec == 28);
     }
+    {  // Timed / TimedPrecond (operators/timing.hpp:37-108): per-primitive times
+        Bratu3D f(ctx, 16, 1.0);
+        const DVec x0(f.size(), 0.0);
+        Timers t;
+        SolverOptions o;
+        o.timers = &t;
+        const auto nk = newton_krylov(f, x0, KrylovParams{}, PoissonMGProvider(16), o);
+        CHECK(nk.converged && nk.iters == 4 && nk.n_JVP == 11 && nk.n_prec == 15);
+        CHECK(t.t_F > 0.0 && t.t_JVP > 0.0 && t.t_prec > 0.0 && t.t_D2 == 0.0);
+        CHECK(t.t_F + t.t_JVP + t.t_prec + t.t_setup_dynamic <= nk.t_total);
+        const Timers after_nk = t;
+        const auto ch = chebyshev_inexact(f, x0, KrylovParams{}, PoissonMGProvider(16), o);
+        CHECK(ch.converged && ch.n_D2 == 4);
+        CHECK(t.t_D2 > 0.0 && t.t_F > after_nk.t_F && t.t_prec > after_nk.t_prec);
+    }
     {  // misuse is reported as std::invalid_argument
         bool thrown = false;
         try {

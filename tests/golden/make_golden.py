@@ -59,6 +59,25 @@ def calibration_rows():
     return out
 
 
+def csv_rows(per_group: int = 4):
+    """Raw result lines of the reference's campaign CSV (instrument/csv.hpp:57-118
+    format, 43 columns; the calibration file appends two more): header and a
+    few lines per (problem, method, precond), kept as text."""
+    path = os.path.join(REF, "experiments/c1/internal_calibration/internal_calibration_raw.csv")
+    lines = open(path).read().splitlines()
+    header = ",".join(lines[0].split(",")[:43])
+    out, count = [], {}
+    for line in lines[1:]:
+        f = line.split(",")
+        key = (f[1], f[7], f[8])
+        if count.get(key, 0) >= per_group:
+            continue
+        count[key] = count.get(key, 0) + 1
+        out.append(",".join(f[:43]))
+    return {"source": "experiments/c1/internal_calibration/internal_calibration_raw.csv",
+            "header": header, "lines": out}
+
+
 def start_norms():
     """Lower-branch solution norms ||u*(lambda)|| (natural-mode continuation)."""
     path = os.path.join(REF, "experiments/c1/starts/frozen/B1_start_hashes.csv")
@@ -82,6 +101,10 @@ def spectral_gaps():
 
 
 def main():
+    with open(os.path.join(HERE, "reference_csv_rows.json"), "w") as f:
+        json.dump(csv_rows(), f, indent=1)
+    if "--csv-only" in sys.argv:
+        return
     ref = O.Reference()
     data = {
         "calibration_n31": calibration_rows(),
