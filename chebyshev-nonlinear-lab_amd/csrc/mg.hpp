@@ -357,8 +357,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
 }
 
 // One fused launch of `steps` polynomial smoothing steps (x == nullptr: the
-// iterate starts at zero) with R rows per thread (the region is 64 x 8R).
-// Row counts: {4, 8} for FP32, {2, 4} for FP64 storage.
+// iterate starts at zero) with R rows per thread (the region is 64 x 16R).
+// Row counts: {2, 4} for FP32, {1, 2} for FP64 storage.
 template <class T, int S, int R>
 inline void launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolySteps& cs,
                             const T* x, const T* b, T* xo) {
@@ -370,12 +370,13 @@ inline void launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolyS
 }
 
 template <class T>
-constexpr int fused_rows_small() { return sizeof(T) == 4 ? 4 : 2; }
+constexpr int fused_rows_small() { return sizeof(T) == 4 ? 2 : 1; }
 
-// LDS of the fused kernel: STEPS levels x 2 planes x (8R+2) x 66 values.
+// LDS of the fused kernel: STEPS levels x 2 planes x (16R+2) x 66 values.
 template <class T>
 constexpr bool fused_fits(int steps, int rows) {
-    return (int64_t)steps * 2 * (8 * rows + 2) * 66 * (int64_t)sizeof(T) <= 160 * 1024;
+    return (int64_t)steps * 2 * ((kFusedThreads / 64) * rows + 2) * 66 * (int64_t)sizeof(T) <=
+           160 * 1024;
 }
 
 template <class T, int S>

@@ -8,9 +8,10 @@
 // storage precision) in the same operation order, x and d rounded to T after
 // every step, so the result is bit for bit the one of STEPS separate launches.
 //
-// Work decomposition.  A workgroup of 512 threads (8 waves) owns a region of
-// 64 (k, one lane each) x 8R (j, R rows per thread) columns and a chunk of
-// planes along i.  It marches over the planes: at plane q it loads x and b,
+// Work decomposition.  A workgroup of 1024 threads (16 waves) owns a region
+// of 64 (k, one lane each) x 16R (j, R rows per thread) columns and a chunk of
+// planes along i.  Few rows per thread keep the register rings small, so two
+// such workgroups (32 waves) fit a CU and keep enough loads in flight.  It marches over the planes: at plane q it loads x and b,
 // then step s advances the column to x_{s+1} at plane q-1-s.  The i-1 / i / i+1
 // values of every level live in registers, the j and k neighbours of the plane
 // being advanced in a 2-plane LDS ring per level (planes padded with a zero
@@ -34,7 +35,7 @@ template <class T>
 using SmoothArith = std::conditional_t<sizeof(T) == 4, float, double>;
 
 constexpr int kFusedMaxSteps = 6;
-constexpr int kFusedThreads = 512;
+constexpr int kFusedThreads = 1024;
 
 struct PolySteps {
     double c1[kFusedMaxSteps];
@@ -43,7 +44,7 @@ struct PolySteps {
 
 template <int STEPS, int R>
 struct FusedGeom {
-    static constexpr int kW = 64, kH = 8 * R;
+    static constexpr int kW = 64, kH = (kFusedThreads / 64) * R;
     static constexpr int kOutW = kW - 2 * STEPS, kOutH = kH - 2 * STEPS;
     static_assert(kOutW > 0 && kOutH > 0, "halo larger than the region");
 };
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kFusedThreads) void k_poly_fused(int n, int ichunk,
     int jrow[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        jrow[r] = ty + 8 * r;
+        jrow[r] = ty + (kFusedThreads / 64) * r;
         const int j = j0 + jrow[r];
         col_in[r] = (unsigned)j < (unsigned)n && (unsigned)k < (unsigned)n;
     }

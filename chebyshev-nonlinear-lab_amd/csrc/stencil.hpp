@@ -60,17 +60,31 @@ __device__ inline A lap_num(A c, A im, A ip, A jm, A jp, A km, A kp) {
 // Out-of-domain neighbours are loaded from a valid address (the centre) and
 // replaced by zero with a select, so every load is unconditional and the
 // whole next plane is in flight while the current one is computed.
+//
+// Tiles (gx k-tiles x gy j-tiles x gz i-chunks, k fastest) are dealt to the 8
+// XCDs in contiguous runs: the hardware hands blocks to the XCDs round robin,
+// so block b takes tile (b % 8) * (G / 8) + b / 8 of a 1-D grid of G (a
+// multiple of 8) blocks.  The tiles an XCD runs at the same time are then
+// neighbours in j and k and their halo rows are L2 hits (with the natural
+// order, j-neighbouring tiles sit on different XCDs and every halo row is
+// fetched from HBM again).  A reduction's partials keep the natural tile
+// order, so sums are the same as with a 3-D grid.
 template <int TK, bool REDUCE, bool MARCH, class Load, class Epi>
-__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Epi epi,
-                                                  Reduce red) {
+__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int gy, int gz,
+                                                  Load ldr, Epi epi, Reduce red) {
     constexpr int TJ = 256 / TK;
+    const unsigned G8 = gridDim.x >> 3;
+    const unsigned lin = (blockIdx.x & 7u) * G8 + (blockIdx.x >> 3);
+    if (lin >= (unsigned)(gx * gy * gz)) return;  // padding block, uniform
+    const int bx = (int)(lin % (unsigned)gx), by = (int)((lin / (unsigned)gx) % (unsigned)gy);
+    const int bz = (int)(lin / (unsigned)(gx * gy));
     const Load L = ldr.prep();
     const Epi E = epi.prep();
     using P = decltype(L.make(L.load(int64_t(0))));
     using X = decltype(E.preload(int64_t(0)));
     const int tk = threadIdx.x % TK, tj = threadIdx.x / TK;
-    const int k = blockIdx.x * TK + tk, j = blockIdx.y * TJ + tj;
-    const int i0 = blockIdx.z * ichunk;
+    const int k = bx * TK + tk, j = by * TJ + tj;
+    const int i0 = bz * ichunk;
     const int i1 = min(n, i0 + ichunk);
     double acc = 0.0;
     if (k < n && j < n && i0 < n) {
@@ -114,10 +128,7 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Ep
             }
         }
     }
-    if constexpr (REDUCE) {
-        const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        block_partial<256>(acc, red, bid);
-    }
+    if constexpr (REDUCE) block_partial<256>(acc, red, lin);
 }
 
 // Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below.  Epilogues with an
@@ -128,7 +139,8 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, Load ldr, Ep
 struct StencilShape {
     int tk, ichunk;
     bool march;
-    dim3 grid;
+    int gx, gy, gz;   // tiles
+    unsigned blocks;  // launched blocks: tiles rounded up to a multiple of 8
 };
 
 inline StencilShape stencil_shape(int64_t n, bool heavy) {
@@ -140,7 +152,11 @@ inline StencilShape stencil_shape(int64_t n, bool heavy) {
     while (gx * gy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
     s.ichunk = ich;
     s.march = ich > 1;
-    s.grid = dim3((unsigned)gx, (unsigned)gy, (unsigned)((n + ich - 1) / ich));
+    s.gx = (int)gx;
+    s.gy = (int)gy;
+    s.gz = (int)((n + ich - 1) / ich);
+    const int64_t tiles = gx * gy * (int64_t)s.gz;
+    s.blocks = (unsigned)((tiles + 7) / 8 * 8);
     return s;
 }
 
@@ -148,8 +164,8 @@ template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
     const StencilShape s = stencil_shape(n, Epi::kMarch);
 #define NUMA_STENCIL_LAUNCH(TK_, MARCH_)                                                     \
-    hipLaunchKernelGGL((k_stencil<TK_, REDUCE, MARCH_, Load, Epi>), s.grid, dim3(256), 0,    \
-                       c.stream, (int)n, s.ichunk, ldr, epi, c.red)
+    hipLaunchKernelGGL((k_stencil<TK_, REDUCE, MARCH_, Load, Epi>), dim3(s.blocks), dim3(256), \
+                       0, c.stream, (int)n, s.ichunk, s.gx, s.gy, s.gz, ldr, epi, c.red)
     if (s.tk == 64) {
         if (s.march)
             NUMA_STENCIL_LAUNCH(64, true);
@@ -163,7 +179,7 @@ void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Ep
     }
 #undef NUMA_STENCIL_LAUNCH
     NUMA_HIP_LAUNCH_CHECK();
-    if constexpr (REDUCE) c.finish(s.grid.x * s.grid.y * s.grid.z, ep);
+    if constexpr (REDUCE) c.finish((unsigned)(s.gx * s.gy * s.gz), ep);
 }
 
 // ---------------------------------------------------------------------------

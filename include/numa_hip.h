@@ -121,8 +121,8 @@ typedef struct numa_mg_config {
     double cheb_ratio;       /* Chebyshev smoothing interval [lmax/ratio, lmax]    */
     int64_t fuse;            /* NUMA_FUSE_*: smoothing steps per kernel launch     */
     int64_t ichunk;          /* planes per workgroup of fused launches (0 = 32)    */
-    int64_t rows;            /* rows per thread of fused launches: 0 = 4 (FP32) /
-                                2 (FP64), or twice that                            */
+    int64_t rows;            /* rows per thread of fused launches (region 64 x 16
+                                rows): 0 = 2 (FP32) / 1 (FP64), or twice that      */
 } numa_mg_config;
 
 /* Kernel fusion of the polynomial (weighted-Jacobi / Chebyshev) smoothers.
@@ -224,7 +224,7 @@ int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev);
 int numa_hip_mg_destroy(numa_hip_mg* mg);
 /* Per level: the smoother fusion in use (NUMA_FUSE_*; NUMA_FUSE_AUTO until the
  * first V-cycle has tuned it), its plane chunk and rows per thread (the
- * fused region is 64 x 8 rows).  Arrays of levels entries. */
+ * fused region is 64 x 16 rows).  Arrays of levels entries. */
 int numa_hip_mg_fusion(const numa_hip_mg* mg, int64_t* mode_host, int64_t* ichunk_host,
                        int64_t* rows_host);
 

@@ -54,7 +54,7 @@ def test_fused_smoothing_is_bitwise_unfused(ctx, spec, n, precision):
     r = rng.normal(size=n ** 3)
     ref, fus = _vcycle(ctx, n, _cfg(spec, H.FUSE_NONE, 0), precision, r)
     assert all(m == H.FUSE_NONE for m, _, _ in fus)
-    r0 = 4 if precision == H.PRECISION_MIXED else 2
+    r0 = 2 if precision == H.PRECISION_MIXED else 1
     per = spec.get("degree", 1)
     for fuse in (H.FUSE_SWEEP, H.FUSE_PHASE):
         group = per if fuse == H.FUSE_SWEEP else (6 // per) * per
