@@ -255,11 +255,14 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
 }
 
 // Coarse solve by one workgroup with the level in LDS (x starts at zero).
-//   mode 0: Chebyshev semi-iteration, `iters` steps over [lo, hi] of D^-1 A.
+//   mode 0: Chebyshev semi-iteration, `iters` steps over [lo, hi] of D^-1 A
+//           (iterates double-buffered in LDS: one barrier per step).
 //   mode 1: `iters` RB-GS sweeps with alternating order (poisson_mg.hpp:270-274).
 //   n == 1: direct x = b/(6 h2inv) (poisson_mg.hpp:267-269).
-constexpr int kCoarseThreads = 512;
-constexpr int kCoarseMaxPts = 8;  // points per thread: n^3 <= 4096
+// 1024 threads, up to 4 points each; the neighbour masks of a thread's points
+// are computed once.
+constexpr int kCoarseThreads = 1024;
+constexpr int kCoarseMaxPts = 4;  // points per thread: n^3 <= 4096
 constexpr int kCoarseMaxN3 = kCoarseThreads * kCoarseMaxPts;
 
 template <class T>
@@ -267,27 +270,30 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
                                                                 int iters, double lo, double hi,
                                                                 const T* b, T* xout) {
 #pragma clang fp contract(off)
-    __shared__ double xs[kCoarseMaxN3];
+    __shared__ double xs[2][kCoarseMaxN3];
     const int N1 = n, N2 = n * n, N = N2 * n;
     if (n == 1) {
         if (threadIdx.x == 0) xout[0] = static_cast<T>(dv(b, 0) / (6.0 * h2inv));
         return;
     }
-    double bl[kCoarseMaxPts], dl[kCoarseMaxPts], xl[kCoarseMaxPts];
+    double bl[kCoarseMaxPts], dl[kCoarseMaxPts];
+    unsigned fl[kCoarseMaxPts];  // bits: i-1, i+1, j-1, j+1, k-1, k+1 inside; bit 6: colour
 #pragma unroll
     for (int p = 0; p < kCoarseMaxPts; ++p) {
         const int id = threadIdx.x + p * kCoarseThreads;
         bl[p] = id < N ? dv(b, id) : 0.0;
         dl[p] = 0.0;
-        xl[p] = 0.0;
+        fl[p] = 0;
+        if (id < N) {
+            const int k = id % N1, j = (id / N1) % N1, i = id / N2;
+            fl[p] = (i > 0 ? 1u : 0u) | (i + 1 < n ? 2u : 0u) | (j > 0 ? 4u : 0u) |
+                    (j + 1 < n ? 8u : 0u) | (k > 0 ? 16u : 0u) | (k + 1 < n ? 32u : 0u) |
+                    (((i + j + k) & 1) ? 64u : 0u);
+        }
     }
-    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xs[id] = 0.0;
+    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xs[0][id] = 0.0;
     __syncthreads();
-    auto nbr = [&](int id, int& i, int& j, int& k) {
-        k = id % N1;
-        j = (id / N1) % N1;
-        i = id / N2;
-    };
+    int cur = 0;
     if (mode == 0) {
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
         const double sigma = theta / delta;
@@ -301,59 +307,55 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
                 c2 = 2.0 * rho_new / delta * dinv;
                 rho = rho_new;
             }
+            const double* xr = xs[cur];
+            double* xw = xs[cur ^ 1];
 #pragma unroll
             for (int p = 0; p < kCoarseMaxPts; ++p) {
                 const int id = threadIdx.x + p * kCoarseThreads;
                 if (id < N) {
-                    int i, j, k;
-                    nbr(id, i, j, k);
-                    const double c = xs[id];
-                    const double s = lap_num(c, i > 0 ? xs[id - N2] : 0.0,
-                                             i + 1 < n ? xs[id + N2] : 0.0,
-                                             j > 0 ? xs[id - N1] : 0.0,
-                                             j + 1 < n ? xs[id + N1] : 0.0,
-                                             k > 0 ? xs[id - 1] : 0.0, k + 1 < n ? xs[id + 1] : 0.0);
+                    const unsigned f = fl[p];
+                    const double c = xr[id];
+                    const double s = lap_num(c, (f & 1u) ? xr[id - N2] : 0.0,
+                                             (f & 2u) ? xr[id + N2] : 0.0,
+                                             (f & 4u) ? xr[id - N1] : 0.0,
+                                             (f & 8u) ? xr[id + N1] : 0.0,
+                                             (f & 16u) ? xr[id - 1] : 0.0,
+                                             (f & 32u) ? xr[id + 1] : 0.0);
                     const double res = bl[p] - h2inv * s;
                     const double dn = it > 0 ? c1 * dl[p] + c2 * res : c2 * res;
                     dl[p] = dn;
-                    xl[p] = c + dn;
+                    xw[id] = c + dn;
                 }
             }
             __syncthreads();
-#pragma unroll
-            for (int p = 0; p < kCoarseMaxPts; ++p) {
-                const int id = threadIdx.x + p * kCoarseThreads;
-                if (id < N) xs[id] = xl[p];
-            }
-            __syncthreads();
+            cur ^= 1;
         }
     } else {
         const double inv = 1.0 / h2inv;
+        double* xr = xs[0];
         for (int sweep = 0; sweep < iters; ++sweep) {
             const int order = sweep % 2;
             for (int pass = 0; pass < 2; ++pass) {
-                const int color = order == 0 ? pass : 1 - pass;
+                const unsigned color = (unsigned)(order == 0 ? pass : 1 - pass);
 #pragma unroll
                 for (int p = 0; p < kCoarseMaxPts; ++p) {
                     const int id = threadIdx.x + p * kCoarseThreads;
-                    if (id >= N) continue;
-                    int i, j, k;
-                    nbr(id, i, j, k);
-                    if (((i + j + k) & 1) != color) continue;
+                    const unsigned f = fl[p];
+                    if (id >= N || ((f >> 6) & 1u) != color) continue;
                     double s = bl[p] * inv;
-                    if (i > 0) s += xs[id - N2];
-                    if (i + 1 < n) s += xs[id + N2];
-                    if (j > 0) s += xs[id - N1];
-                    if (j + 1 < n) s += xs[id + N1];
-                    if (k > 0) s += xs[id - 1];
-                    if (k + 1 < n) s += xs[id + 1];
-                    xs[id] = s / 6.0;
+                    if (f & 1u) s += xr[id - N2];
+                    if (f & 2u) s += xr[id + N2];
+                    if (f & 4u) s += xr[id - N1];
+                    if (f & 8u) s += xr[id + N1];
+                    if (f & 16u) s += xr[id - 1];
+                    if (f & 32u) s += xr[id + 1];
+                    xr[id] = s / 6.0;
                 }
                 __syncthreads();
             }
         }
     }
-    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xout[id] = static_cast<T>(xs[id]);
+    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xout[id] = static_cast<T>(xs[cur][id]);
 }
 
 // One fused launch of `steps` polynomial smoothing steps (x == nullptr: the
@@ -565,7 +567,9 @@ public:
             total += N * s * (2.0 + 0.125); // prolongation
         }
         const double N0 = (double)levels_[0].n * levels_[0].n * levels_[0].n;
-        return total + 2.0 * N0 * s;
+        // output copy, unless the last post-smoothing launch writes the output
+        const bool direct = cfg_.smoother != NUMA_SMOOTH_RBGS && cfg_.post > 0 && levels_.size() > 1;
+        return total + (direct ? 0.0 : 2.0 * N0 * s);
     }
 
     // z = V-cycle(r) with zero initial guess (poisson_mg.hpp:69-80).
@@ -575,9 +579,13 @@ public:
         T* z = static_cast<T*>(zv);
         Level& F = levels_[0];
         F.b = const_cast<T*>(r);
+        // the finest level's last post-smoothing launch writes z directly
+        // (not when z is r: that launch still reads b)
+        final_dst_ = z != r ? z : nullptr;
+        final_done_ = false;
         cycle(0);
-        const int64_t N = F.n * F.n * F.n;
-        launch_convert<T, T>(c_, N, F.x, z);
+        if (!final_done_) launch_convert<T, T>(c_, F.n * F.n * F.n, F.x, z);
+        final_dst_ = nullptr;
         F.b = nullptr;
     }
 
@@ -620,34 +628,37 @@ private:
         }
     }
 
-    // One step d <- c1 d + c2 (b - A x), x <- x + d (x double-buffered).
-    void poly_step(Level& L, double c1, double c2, bool use_d, bool write_d, bool from_zero) {
+    // One step d <- c1 d + c2 (b - A x), x <- x + d (x double-buffered; with
+    // dst the new x goes to dst and the level's buffers are left as they are).
+    void poly_step(Level& L, double c1, double c2, bool use_d, bool write_d, bool from_zero,
+                   T* dst = nullptr) {
         const int64_t N = L.n * L.n * L.n;
+        T* const xo = dst ? dst : L.x2;
         if (from_zero) {
             T* d = write_d ? L.d : (T*)nullptr;
-            if (aligned2(L.b, sizeof(T)) && aligned2(d, sizeof(T)) && aligned2(L.x2, sizeof(T)))
+            if (aligned2(L.b, sizeof(T)) && aligned2(d, sizeof(T)) && aligned2(xo, sizeof(T)))
                 hipLaunchKernelGGL((k_poly_first<true, T>), dim3(stream_blocks(N)), dim3(kBlock), 0,
-                                   c_.stream, N, c2, (const T*)L.b, d, L.x2);
+                                   c_.stream, N, c2, (const T*)L.b, d, xo);
             else
                 hipLaunchKernelGGL((k_poly_first<false, T>), dim3(stream_blocks(N)), dim3(kBlock),
-                                   0, c_.stream, N, c2, (const T*)L.b, d, L.x2);
+                                   0, c_.stream, N, c2, (const T*)L.b, d, xo);
             NUMA_HIP_LAUNCH_CHECK();
         } else {
             const LdVec<T> ld{L.x};
             if (use_d && write_d)
-                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, true>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, true>{L.h2inv, c1, c2, L.b, L.d, xo},
                                       Epilogue{});
             else if (use_d)
-                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, false>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, true, false>{L.h2inv, c1, c2, L.b, L.d, xo},
                                       Epilogue{});
             else if (write_d)
-                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, true>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, true>{L.h2inv, c1, c2, L.b, L.d, xo},
                                       Epilogue{});
             else
-                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, false>{L.h2inv, c1, c2, L.b, L.d, L.x2},
+                launch_stencil<false>(c_, L.n, ld, EpPoly<T, false, false>{L.h2inv, c1, c2, L.b, L.d, xo},
                                       Epilogue{});
         }
-        std::swap(L.x, L.x2);
+        if (!dst) std::swap(L.x, L.x2);
     }
 
     void rbgs_sweep(Level& L, int order) {
@@ -660,8 +671,10 @@ private:
         }
     }
 
-    // `sweeps` smoothing sweeps; if from_zero the iterate starts at 0.
-    void smooth(Level& L, int sweeps, int order, bool from_zero) {
+    // `sweeps` smoothing sweeps; if from_zero the iterate starts at 0.  With
+    // last_dst (polynomial smoothers only) the final launch writes last_dst
+    // and final_done_ is set.
+    void smooth(Level& L, int sweeps, int order, bool from_zero, T* last_dst = nullptr) {
         const int64_t N = L.n * L.n * L.n;
         if (cfg_.smoother == NUMA_SMOOTH_RBGS) {
             if (from_zero) launch_fill<T>(c_, N, T(0), L.x);
@@ -673,15 +686,23 @@ private:
             return;
         }
         if (L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE) {
-            if (smooth_fused(L, sweeps, from_zero)) return;
+            if (smooth_fused(L, sweeps, from_zero, last_dst)) return;
         }
         for (int s = 0; s < sweeps; ++s) {
+            const bool last_sweep = s + 1 == sweeps;
             if (cfg_.smoother == NUMA_SMOOTH_WJACOBI) {
-                poly_step(L, 0.0, cfg_.omega / (6.0 * L.h2inv), false, false, from_zero && s == 0);
+                T* dst = last_sweep ? last_dst : nullptr;
+                poly_step(L, 0.0, cfg_.omega / (6.0 * L.h2inv), false, false, from_zero && s == 0,
+                          dst);
+                if (dst) final_done_ = true;
             } else {
                 const int deg = (int)cfg_.degree;
-                for (int k = 0; k < deg; ++k)
-                    poly_step(L, L.c1[k], L.c2[k], k > 0, k + 1 < deg, from_zero && s == 0 && k == 0);
+                for (int k = 0; k < deg; ++k) {
+                    T* dst = last_sweep && k + 1 == deg ? last_dst : nullptr;
+                    poly_step(L, L.c1[k], L.c2[k], k > 0, k + 1 < deg, from_zero && s == 0 && k == 0,
+                              dst);
+                    if (dst) final_done_ = true;
+                }
             }
         }
     }
@@ -696,7 +717,7 @@ private:
     // sweeps (the first step of each sweep starts a new direction, c1 = 0) in
     // launches of one sweep or of as many whole sweeps as fit in 6 steps.
     // Returns false if a sweep is longer than a fused launch can hold.
-    bool smooth_fused(Level& L, int sweeps, bool from_zero) {
+    bool smooth_fused(Level& L, int sweeps, bool from_zero, T* last_dst = nullptr) {
         const bool wj = cfg_.smoother == NUMA_SMOOTH_WJACOBI;
         const int per = wj ? 1 : (int)cfg_.degree;
         if (per > kFusedMaxSteps) return false;
@@ -715,10 +736,14 @@ private:
                 cs.c1[q] = c1s[(size_t)(st + q)];
                 cs.c2[q] = c2s[(size_t)(st + q)];
             }
+            T* dst = st + cnt >= total ? last_dst : nullptr;
             launch_fused<T>(c_, (int)L.n, cnt, L.rows, L.ichunk, L.h2inv, cs,
                             (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x,
-                            (const T*)L.b, L.x2);
-            std::swap(L.x, L.x2);
+                            (const T*)L.b, dst ? dst : L.x2);
+            if (dst)
+                final_done_ = true;
+            else
+                std::swap(L.x, L.x2);
         }
         return true;
     }
@@ -848,7 +873,7 @@ private:
             NUMA_HIP_LAUNCH_CHECK();
             (void)NF;
         }
-        smooth(L, (int)cfg_.post, 1, false);
+        smooth(L, (int)cfg_.post, 1, false, l == 0 ? final_dst_ : nullptr);
     }
 
     Ctx& c_;
@@ -856,6 +881,8 @@ private:
     std::vector<Level> levels_;
     std::vector<void*> allocs_;
     bool tuned_ = false;
+    T* final_dst_ = nullptr;  // output of the V-cycle being applied
+    bool final_done_ = false;
 };
 
 inline MGBase* make_mg(Ctx& c, int64_t n, const numa_mg_config& cfg, int64_t precision) {
