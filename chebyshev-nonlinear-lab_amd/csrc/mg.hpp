@@ -164,6 +164,140 @@ __global__ __launch_bounds__(256) void k_restrict(int nf, int nc, const T* r, T*
     bc[((int64_t)I * nc + J) * nc + K] = static_cast<T>(sum);
 }
 
+// Residual and restriction in one pass: bc = R (b - A x) with the values of
+// an EpResid launch followed by k_restrict, bit for bit (r rounded to T at
+// every fine point, each coarse node's 27 terms added in k_restrict's order),
+// without writing r.  A workgroup owns 32 (K) x 16 (J) coarse columns and a
+// chunk of `ich` coarse planes and marches over the fine planes 2 Ib .. 2 Ie:
+// x of the padded fine region (67 x 35 points) in registers at i-1 / i / i+1
+// and in one LDS plane for the j / k neighbours, r of the fine region (65 x 33)
+// in LDS; then every thread adds the 9 terms of that plane to its coarse
+// node(s): an odd plane 2I+1 is the middle plane of node I, an even plane 2I
+// the last plane of node I-1 and the first of node I.
+constexpr int kRRThreads = 512, kRRK = 32, kRRJ = 16;
+constexpr int kRRFW = 2 * kRRK + 1, kRRFH = 2 * kRRJ + 1;  // fine region of r
+constexpr int kRRPW = kRRFW + 2, kRRPH = kRRFH + 2;        // padded region of x
+constexpr int kRRSlots = (kRRPW * kRRPH + kRRThreads - 1) / kRRThreads;
+
+template <class T>
+__global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, int ich,
+                                                               int tiles_k, int tiles_j, int chunks,
+                                                               double h2inv,
+                                                               const T* __restrict__ x,
+                                                               const T* __restrict__ b,
+                                                               T* __restrict__ bc) {
+#pragma clang fp contract(off)
+    __shared__ T xpl[kRRPH * kRRPW];
+    __shared__ T rpl[kRRFH * kRRFW];
+    // XCD-aware block -> tile (every 8th block walks consecutive tiles)
+    const int G8 = gridDim.x >> 3;
+    const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
+    if (lin >= tiles_k * tiles_j * chunks) return;  // uniform per block
+    const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ch = lin / (tiles_k * tiles_j);
+    const int K0 = tk * kRRK, J0 = tj * kRRJ;
+    const int Ib = ch * ich, Ie = min(nc, Ib + ich);
+    const int64_t F1 = nf, F2 = F1 * F1;
+    const int tid = threadIdx.x;
+
+    // slots of the padded region: fine (j, k) offset (-1 outside the domain)
+    // and the r index of interior slots (-1 for the halo ring)
+    int64_t off[kRRSlots];
+    int ridx[kRRSlots];
+#pragma unroll
+    for (int s = 0; s < kRRSlots; ++s) {
+        const int e = tid + s * kRRThreads;
+        const int row = e / kRRPW, col = e % kRRPW;
+        const int fj = 2 * J0 - 1 + row, fk = 2 * K0 - 1 + col;
+        const bool valid = e < kRRPW * kRRPH;
+        off[s] = valid && fj >= 0 && fj < nf && fk >= 0 && fk < nf ? (int64_t)fj * F1 + fk : -1;
+        ridx[s] = valid && row >= 1 && row <= kRRFH && col >= 1 && col <= kRRFW
+                      ? (row - 1) * kRRFW + (col - 1)
+                      : -1;
+    }
+    auto load_x = [&](int q, T (&v)[kRRSlots]) {
+#pragma unroll
+        for (int s = 0; s < kRRSlots; ++s)
+            v[s] = off[s] >= 0 && q >= 0 && q < nf ? x[(int64_t)q * F2 + off[s]] : T(0);
+    };
+    auto load_b = [&](int q, T (&v)[kRRSlots]) {
+#pragma unroll
+        for (int s = 0; s < kRRSlots; ++s)
+            v[s] = off[s] >= 0 && ridx[s] >= 0 && q >= 0 && q < nf ? b[(int64_t)q * F2 + off[s]]
+                                                                   : T(0);
+    };
+
+    // coarse node of this thread
+    const int cK = K0 + (tid % kRRK), cJ = J0 + tid / kRRK;
+    const bool active = cK < nc && cJ < nc;
+    const int lr = 2 * (cJ - J0), lc = 2 * (cK - K0);  // its first fine row / column in rpl
+    const double w[3] = {0.25, 0.5, 0.25};
+    auto add_terms = [&](double& acc, int a) {
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            if (2 * cJ + bb >= nf) continue;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                if (2 * cK + cc >= nf) continue;
+                acc += w[a] * w[bb] * w[cc] *
+                       static_cast<double>(rpl[(lr + bb) * kRRFW + lc + cc]);
+            }
+        }
+    };
+    auto store = [&](int I, double acc) {
+        if (active) bc[((int64_t)I * nc + cJ) * nc + cK] = static_cast<T>(acc);
+    };
+
+    const int fb = 2 * Ib, fe = min(2 * Ie, nf - 1);
+    T xm[kRRSlots], xc[kRRSlots], xp[kRRSlots], nx[kRRSlots], bv[kRRSlots], nb[kRRSlots];
+    load_x(fb - 1, xm);
+    load_x(fb, xc);
+    load_x(fb + 1, xp);
+    load_b(fb, bv);
+    double acc = 0.0;
+    for (int fi = fb; fi <= fe; ++fi) {
+#pragma unroll
+        for (int s = 0; s < kRRSlots; ++s) {
+            const int e = tid + s * kRRThreads;
+            if (e < kRRPW * kRRPH) xpl[e] = xc[s];
+        }
+        if (fi < fe) {  // next planes in flight during this one
+            load_x(fi + 2, nx);
+            load_b(fi + 1, nb);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kRRSlots; ++s) {
+            if (ridx[s] < 0) continue;
+            const int e = tid + s * kRRThreads;
+            const double sum =
+                lap_num<double>(static_cast<double>(xc[s]), static_cast<double>(xm[s]),
+                                static_cast<double>(xp[s]), static_cast<double>(xpl[e - kRRPW]),
+                                static_cast<double>(xpl[e + kRRPW]),
+                                static_cast<double>(xpl[e - 1]), static_cast<double>(xpl[e + 1]));
+            rpl[ridx[s]] = static_cast<T>(static_cast<double>(bv[s]) - h2inv * sum);
+        }
+        __syncthreads();
+        if ((fi & 1) == 0) {
+            if (fi > fb) {  // last plane of node fi/2 - 1
+                add_terms(acc, 2);
+                store(fi / 2 - 1, acc);
+            }
+            acc = 0.0;
+            if (fi / 2 < Ie) add_terms(acc, 0);  // first plane of node fi/2
+        } else {
+            add_terms(acc, 1);
+        }
+#pragma unroll
+        for (int s = 0; s < kRRSlots; ++s) {
+            xm[s] = xc[s];
+            xc[s] = xp[s];
+            xp[s] = nx[s];
+            bv[s] = nb[s];
+        }
+    }
+    if (fe & 1) store((fe - 1) / 2, acc);  // nf even: the last node has no plane 2I+2
+}
+
 // 1D trilinear terms of a fine index (poisson_mg.hpp:191-220).
 __device__ inline int interp_terms(int f, int nc, int ci[2], double wi[2]) {
     if (f & 1) {
@@ -259,8 +393,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
 //           (iterates double-buffered in LDS: one barrier per step).
 //   mode 1: `iters` RB-GS sweeps with alternating order (poisson_mg.hpp:270-274).
 //   n == 1: direct x = b/(6 h2inv) (poisson_mg.hpp:267-269).
-// 1024 threads, up to 4 points each; the neighbour masks of a thread's points
-// are computed once.
+// n <= 16: thread t owns the column (j, k) = t mod 256 at the planes
+// i = 4 (t / 256) + p, p < 4.  Its own values and the i-neighbours inside the
+// column live in registers (the same doubles as the LDS copy); the other
+// neighbours come from LDS.
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseMaxPts = 4;  // points per thread: n^3 <= 4096
 constexpr int kCoarseMaxN3 = kCoarseThreads * kCoarseMaxPts;
@@ -276,20 +412,19 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
         if (threadIdx.x == 0) xout[0] = static_cast<T>(dv(b, 0) / (6.0 * h2inv));
         return;
     }
-    double bl[kCoarseMaxPts], dl[kCoarseMaxPts];
-    unsigned fl[kCoarseMaxPts];  // bits: i-1, i+1, j-1, j+1, k-1, k+1 inside; bit 6: colour
+    const int jk = threadIdx.x & 255, i0 = 4 * (threadIdx.x >> 8);
+    const int j = jk / N1, k = jk % N1;
+    const bool col = jk < N2;
+    double bl[kCoarseMaxPts], dl[kCoarseMaxPts], xr[kCoarseMaxPts], xn[kCoarseMaxPts];
+    bool in[kCoarseMaxPts];
+    // neighbour masks shared by the column: j-1, j+1, k-1, k+1 inside
+    const bool hjm = j > 0, hjp = j + 1 < n, hkm = k > 0, hkp = k + 1 < n;
 #pragma unroll
     for (int p = 0; p < kCoarseMaxPts; ++p) {
-        const int id = threadIdx.x + p * kCoarseThreads;
-        bl[p] = id < N ? dv(b, id) : 0.0;
-        dl[p] = 0.0;
-        fl[p] = 0;
-        if (id < N) {
-            const int k = id % N1, j = (id / N1) % N1, i = id / N2;
-            fl[p] = (i > 0 ? 1u : 0u) | (i + 1 < n ? 2u : 0u) | (j > 0 ? 4u : 0u) |
-                    (j + 1 < n ? 8u : 0u) | (k > 0 ? 16u : 0u) | (k + 1 < n ? 32u : 0u) |
-                    (((i + j + k) & 1) ? 64u : 0u);
-        }
+        const int i = i0 + p;
+        in[p] = col && i < n;
+        bl[p] = in[p] ? dv(b, i * N2 + jk) : 0.0;
+        dl[p] = xr[p] = xn[p] = 0.0;
     }
     for (int id = threadIdx.x; id < N; id += kCoarseThreads) xs[0][id] = 0.0;
     __syncthreads();
@@ -307,55 +442,59 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
                 c2 = 2.0 * rho_new / delta * dinv;
                 rho = rho_new;
             }
-            const double* xr = xs[cur];
+            const double* xo = xs[cur];
             double* xw = xs[cur ^ 1];
 #pragma unroll
             for (int p = 0; p < kCoarseMaxPts; ++p) {
-                const int id = threadIdx.x + p * kCoarseThreads;
-                if (id < N) {
-                    const unsigned f = fl[p];
-                    const double c = xr[id];
-                    const double s = lap_num(c, (f & 1u) ? xr[id - N2] : 0.0,
-                                             (f & 2u) ? xr[id + N2] : 0.0,
-                                             (f & 4u) ? xr[id - N1] : 0.0,
-                                             (f & 8u) ? xr[id + N1] : 0.0,
-                                             (f & 16u) ? xr[id - 1] : 0.0,
-                                             (f & 32u) ? xr[id + 1] : 0.0);
-                    const double res = bl[p] - h2inv * s;
-                    const double dn = it > 0 ? c1 * dl[p] + c2 * res : c2 * res;
-                    dl[p] = dn;
-                    xw[id] = c + dn;
-                }
+                if (!in[p]) continue;
+                const int i = i0 + p, id = i * N2 + jk;
+                const double c = xr[p];
+                const double im = p > 0 ? xr[p - 1] : (i > 0 ? xo[id - N2] : 0.0);
+                const double ip =
+                    i + 1 < n ? (p + 1 < kCoarseMaxPts ? xr[p + 1] : xo[id + N2]) : 0.0;
+                const double s = lap_num(c, im, ip, hjm ? xo[id - N1] : 0.0,
+                                         hjp ? xo[id + N1] : 0.0, hkm ? xo[id - 1] : 0.0,
+                                         hkp ? xo[id + 1] : 0.0);
+                const double res = bl[p] - h2inv * s;
+                const double dn = it > 0 ? c1 * dl[p] + c2 * res : c2 * res;
+                dl[p] = dn;
+                xn[p] = c + dn;
+                xw[id] = xn[p];
             }
+#pragma unroll
+            for (int p = 0; p < kCoarseMaxPts; ++p) xr[p] = xn[p];
             __syncthreads();
             cur ^= 1;
         }
     } else {
         const double inv = 1.0 / h2inv;
-        double* xr = xs[0];
+        double* xo = xs[0];
         for (int sweep = 0; sweep < iters; ++sweep) {
             const int order = sweep % 2;
             for (int pass = 0; pass < 2; ++pass) {
-                const unsigned color = (unsigned)(order == 0 ? pass : 1 - pass);
+                const int color = order == 0 ? pass : 1 - pass;
 #pragma unroll
                 for (int p = 0; p < kCoarseMaxPts; ++p) {
-                    const int id = threadIdx.x + p * kCoarseThreads;
-                    const unsigned f = fl[p];
-                    if (id >= N || ((f >> 6) & 1u) != color) continue;
+                    const int i = i0 + p;
+                    if (!in[p] || ((i + j + k) & 1) != color) continue;
+                    const int id = i * N2 + jk;
                     double s = bl[p] * inv;
-                    if (f & 1u) s += xr[id - N2];
-                    if (f & 2u) s += xr[id + N2];
-                    if (f & 4u) s += xr[id - N1];
-                    if (f & 8u) s += xr[id + N1];
-                    if (f & 16u) s += xr[id - 1];
-                    if (f & 32u) s += xr[id + 1];
-                    xr[id] = s / 6.0;
+                    if (i > 0) s += p > 0 ? xr[p - 1] : xo[id - N2];
+                    if (i + 1 < n) s += p + 1 < kCoarseMaxPts ? xr[p + 1] : xo[id + N2];
+                    if (hjm) s += xo[id - N1];
+                    if (hjp) s += xo[id + N1];
+                    if (hkm) s += xo[id - 1];
+                    if (hkp) s += xo[id + 1];
+                    xr[p] = s / 6.0;
+                    xo[id] = xr[p];
                 }
                 __syncthreads();
             }
         }
     }
-    for (int id = threadIdx.x; id < N; id += kCoarseThreads) xout[id] = static_cast<T>(xs[cur][id]);
+#pragma unroll
+    for (int p = 0; p < kCoarseMaxPts; ++p)
+        if (in[p]) xout[(i0 + p) * N2 + jk] = static_cast<T>(xr[p]);
 }
 
 // One fused launch of `steps` polynomial smoothing steps (x == nullptr: the
@@ -431,6 +570,10 @@ struct MGBase {
     // Finishes set-up that needs eager device work (the autotuner's timing), so
     // that apply() only launches kernels (and can be captured in a graph).
     virtual void prepare() = 0;
+    // Coarse planes per workgroup of the fused residual / restriction launch
+    // on every level: > 0 that chunk, 0 the separate residual and restriction
+    // launches, < 0 the default choice.
+    virtual void set_resid_restrict(int64_t ich) = 0;
 };
 
 template <class T>
@@ -485,6 +628,12 @@ public:
                 require(fused_fits<T>(fused_group(L), L.rows),
                         "fused smoothing: steps per launch x rows per thread exceed the LDS");
         }
+        set_resid_restrict(-1);
+    }
+    void set_resid_restrict(int64_t ich) override {
+        require(ich <= 1024, "resid/restrict chunk must be <= 1024");
+        for (size_t l = 0; l + 1 < levels_.size(); ++l)
+            levels_[l].rr_ich = ich < 0 ? rr_chunk(levels_[l + 1].n) : (int)ich;
     }
     ~MG() override {
         for (void* p : allocs_) (void)hipFree(p);
@@ -556,14 +705,17 @@ public:
                 const int lp = (int)((cfg_.pre * per_sweep + g - 1) / g);
                 const int lq = (int)((cfg_.post * per_sweep + g - 1) / g);
                 total += N * s * (3.0 * (lp + lq) - (lp > 0 ? 1.0 : 0.0));
-                total += N * s * (3.0 + 1.125 + 2.125);
+                total += N * s * (levels_[l].rr_ich > 0 ? 2.125 : 3.0 + 1.125);
+                total += N * s * 2.125;  // prolongation
                 continue;
             }
             double step_bytes = cfg_.smoother == NUMA_SMOOTH_CHEBYSHEV ? 5.0 : 3.0;
             if (cfg_.smoother == NUMA_SMOOTH_RBGS) step_bytes = 4.0;
             total += N * s * (steps * step_bytes - (cfg_.pre > 0 ? step_bytes - 3.0 : 0.0));
-            total += N * s * 3.0;          // residual
-            total += N * s * (1.0 + 0.125); // restriction
+            if (levels_[l].rr_ich > 0)
+                total += N * s * (2.0 + 0.125);  // fused residual / restriction
+            else
+                total += N * s * (3.0 + 1.0 + 0.125);  // residual; restriction
             total += N * s * (2.0 + 0.125); // prolongation
         }
         const double N0 = (double)levels_[0].n * levels_[0].n * levels_[0].n;
@@ -602,6 +754,7 @@ private:
         int fuse = NUMA_FUSE_NONE;  // NUMA_FUSE_*
         int ichunk = 32;
         int rows = 4;               // rows per thread of fused launches
+        int rr_ich = 0;             // coarse planes per workgroup of k_resid_restrict (0: off)
     };
 
     T* alloc(int64_t N) {
@@ -847,6 +1000,37 @@ private:
         for (int k = 0; k < deg; ++k) poly_step(L, c1[k], c2[k], k > 0, k + 1 < deg, k == 0);
     }
 
+    // b_C = R (b_L - A x_L): residual (poisson_mg.hpp:128-149) and restriction
+    // (:152-188) in one launch, or the two launches when L.rr_ich is 0.
+    void resid_restrict(Level& L, Level& C) {
+        if (L.rr_ich <= 0) {
+            launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, EpResid<T>{L.h2inv, L.b, L.r},
+                                  Epilogue{});
+            hipLaunchKernelGGL((k_restrict<T>),
+                               dim3((unsigned)((C.n + 63) / 64), (unsigned)((C.n + 3) / 4),
+                                    (unsigned)C.n),
+                               dim3(256), 0, c_.stream, (int)L.n, (int)C.n, (const T*)L.r, C.b);
+            NUMA_HIP_LAUNCH_CHECK();
+            return;
+        }
+        const int tiles_k = (int)((C.n + kRRK - 1) / kRRK), tiles_j = (int)((C.n + kRRJ - 1) / kRRJ);
+        const int chunks = (int)((C.n + L.rr_ich - 1) / L.rr_ich);
+        const int64_t total = (int64_t)tiles_k * tiles_j * chunks;
+        hipLaunchKernelGGL((k_resid_restrict<T>), dim3((unsigned)((total + 7) / 8 * 8)),
+                           dim3(kRRThreads), 0, c_.stream, (int)L.n, (int)C.n, L.rr_ich, tiles_k,
+                           tiles_j, chunks, L.h2inv, (const T*)L.x, (const T*)L.b, C.b);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+
+    // Coarse planes per workgroup of the fused residual / restriction: the
+    // largest of 8, 4, 2 that still gives 1024 workgroups.
+    static int rr_chunk(int64_t nc) {
+        const int64_t tiles = ((nc + kRRK - 1) / kRRK) * ((nc + kRRJ - 1) / kRRJ);
+        int ich = 8;
+        while (ich > 2 && tiles * ((nc + ich - 1) / ich) < 1024) ich /= 2;
+        return ich;
+    }
+
     // poisson_mg.hpp:262-294
     void cycle(size_t l) {
         Level& L = levels_[l];
@@ -855,14 +1039,8 @@ private:
             return;
         }
         smooth(L, (int)cfg_.pre, 0, true);
-        launch_stencil<false>(c_, L.n, LdVec<T>{L.x}, EpResid<T>{L.h2inv, L.b, L.r}, Epilogue{});
         Level& C = levels_[l + 1];
-        const int64_t NC = C.n * C.n * C.n, NF = L.n * L.n * L.n;
-        hipLaunchKernelGGL((k_restrict<T>),
-                           dim3((unsigned)((C.n + 63) / 64), (unsigned)((C.n + 3) / 4), (unsigned)C.n),
-                           dim3(256), 0, c_.stream, (int)L.n, (int)C.n, (const T*)L.r, C.b);
-        NUMA_HIP_LAUNCH_CHECK();
-        (void)NC;
+        resid_restrict(L, C);
         cycle(l + 1);
         {
             const int64_t ncell = (L.n + 1) / 2;
@@ -871,7 +1049,6 @@ private:
                                     (unsigned)ncell),
                                dim3(256), 0, c_.stream, (int)L.n, (int)C.n, (const T*)C.x, L.x);
             NUMA_HIP_LAUNCH_CHECK();
-            (void)NF;
         }
         smooth(L, (int)cfg_.post, 1, false, l == 0 ? final_dst_ : nullptr);
     }

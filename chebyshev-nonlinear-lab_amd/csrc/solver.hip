@@ -965,6 +965,13 @@ int numa_hip_mg_fusion(const numa_hip_mg* mg, int64_t* mode_host, int64_t* ichun
     NUMA_API_END
 }
 
+int numa_hip_mg_set_resid_restrict(numa_hip_mg* mg, int64_t ich) {
+    NUMA_API_BEGIN
+    numa_hip::require(mg != nullptr, "null argument");
+    mg->impl->set_resid_restrict(ich);
+    NUMA_API_END
+}
+
 int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t* mode_host,
                               int64_t* ichunk_host, int64_t* rows_host) {
     NUMA_API_BEGIN

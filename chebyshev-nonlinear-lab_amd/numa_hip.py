@@ -213,6 +213,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_solver_set_net": [_vp, _vp, C.c_double],
         "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
         "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
+        "numa_hip_mg_set_resid_restrict": [_vp, _i64],
         "numa_hip_solver_set_graphs": [_vp, _i64],
         "numa_hip_solver_set_timing": [_vp, _i64],
         "numa_hip_solver_timers": [_vp, C.POINTER(Timers), _i64],
@@ -406,6 +407,11 @@ class PoissonMG:
         m, ic, rw = (_i64 * L)(), (_i64 * L)(), (_i64 * L)()
         check(self.ctx.lib.numa_hip_mg_fusion(self.h, m, ic, rw))
         return [(int(m[i]), int(ic[i]), int(rw[i])) for i in range(L)]
+
+    def set_resid_restrict(self, ich: int) -> None:
+        """Residual + restriction as one launch with `ich` coarse planes per
+        workgroup (> 0), the default chunk (< 0) or two launches (0)."""
+        check(self.ctx.lib.numa_hip_mg_set_resid_restrict(self.h, int(ich)))
 
     def close(self):
         if self.h and self.ctx.h:

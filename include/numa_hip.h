@@ -227,6 +227,11 @@ int numa_hip_mg_destroy(numa_hip_mg* mg);
  * fused region is 64 x 16 rows).  Arrays of levels entries. */
 int numa_hip_mg_fusion(const numa_hip_mg* mg, int64_t* mode_host, int64_t* ichunk_host,
                        int64_t* rows_host);
+/* Residual and restriction of every level in one launch (ich > 0 coarse
+ * planes per workgroup; ich < 0 the default chunk, the state after create) or
+ * as two launches (ich = 0).  The values are the same bit for bit either way.
+ * Tuning / testing knob; no counterpart in the reference. */
+int numa_hip_mg_set_resid_restrict(numa_hip_mg* mg, int64_t ich);
 
 /* ---- device-resident solvers on Bratu3D ---------------------------------- */
 int numa_hip_bratu_solver_create(numa_hip_ctx* ctx, int64_t n, double lambda,
