@@ -131,6 +131,123 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int 
     if constexpr (REDUCE) block_partial<256>(acc, red, lin);
 }
 
+// LDS-tiled form for large grids.  A block of 256 threads owns a 64 (k) x
+// 4*RW (j) tile (rows tj, tj + 4, ... of each thread) and a chunk of planes,
+// and marches along i loading every node once: the own nodes' loads run three
+// planes ahead of the plane being computed and the halo ring's two (the ring
+// of 2 rows of 64 and 2 columns of 4*RW nodes is loaded by the first threads,
+// one node each), so two planes of HBM traffic are in flight per block.  The
+// plane's values go to a double-buffered LDS plane with the ring, where the
+// j / k neighbours are read; the i neighbours stay in registers.  One barrier
+// per plane.  Same Loader / Epilogue interface, arithmetic and outputs as
+// k_stencil (only a reduction's summation order differs).
+template <int RW, bool REDUCE, class Load, class Epi>
+__global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, int gy, int gz,
+                                                      Load ldr, Epi epi, Reduce red) {
+    constexpr int TK = 64, TJ = 4 * RW;
+    const unsigned G8 = gridDim.x >> 3;
+    const unsigned lin = (blockIdx.x & 7u) * G8 + (blockIdx.x >> 3);
+    if (lin >= (unsigned)(gx * gy * gz)) return;  // padding block, uniform
+    const int bx = (int)(lin % (unsigned)gx), by = (int)((lin / (unsigned)gx) % (unsigned)gy);
+    const int bz = (int)(lin / (unsigned)(gx * gy));
+    const Load L = ldr.prep();
+    const Epi E = epi.prep();
+    using R = decltype(L.load(int64_t(0)));
+    using P = decltype(L.make(L.load(int64_t(0))));
+    using X = decltype(E.preload(int64_t(0)));
+    __shared__ P buf[2][TJ + 2][TK + 2];
+
+    const int tk = threadIdx.x & 63, tj = threadIdx.x >> 6;
+    const int k0 = bx * TK, j0 = by * TJ, k = k0 + tk;
+    const int i0 = bz * ichunk, i1 = min(n, i0 + ichunk);
+    const int64_t N1 = n, N2 = N1 * N1;
+    bool own[RW];
+    int64_t col[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        const int j = j0 + tj + 4 * r;
+        own[r] = j < n && k < n;
+        col[r] = own[r] ? (int64_t)j * N1 + k : 0;
+    }
+    // halo node of this thread: threads 0..127 the rows j0-1 and j0+TJ,
+    // threads 128..128+2TJ-1 the columns k0-1 and k0+TK
+    int hr = -1, hc = 0, hj = -1, hk = -1;
+    if (threadIdx.x < 128) {
+        hr = threadIdx.x < 64 ? 0 : TJ + 1;
+        hc = tk + 1;
+        hj = threadIdx.x < 64 ? j0 - 1 : j0 + TJ;
+        hk = k;
+    } else if (threadIdx.x < 128 + 2 * TJ) {
+        const int t = threadIdx.x - 128, side = t / TJ;
+        hr = t % TJ + 1;
+        hc = side ? TK + 1 : 0;
+        hj = j0 + t % TJ;
+        hk = side ? k0 + TK : k0 - 1;
+    }
+    const bool hown = hr >= 0 && hj >= 0 && hj < n && hk >= 0 && hk < n;
+    const int64_t hcol = hown ? (int64_t)hj * N1 + hk : 0;
+    // loads at invalid nodes go to node 0 and their values are replaced by zero
+    auto own_ok = [&](int r, int q) { return own[r] && q >= 0 && q < n; };
+    auto own_id = [&](int r, int q) -> int64_t {
+        return own_ok(r, q) ? (int64_t)q * N2 + col[r] : 0;
+    };
+    auto halo_id = [&](int q) -> int64_t {
+        return hown && q >= 0 && q < n ? (int64_t)q * N2 + hcol : 0;
+    };
+
+    P pm[RW], pc[RW];
+    R q1[RW], q2[RW], q3[RW];  // own raw values of planes i+1, i+2, i+3
+    X e0[RW], e1[RW], e2[RW];  // epilogue loads of planes i, i+1, i+2
+    R h0 = L.load(halo_id(i0)), h1 = L.load(halo_id(i0 + 1)), h2 = h1;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+        const R a = L.load(own_id(r, i0 - 1)), b = L.load(own_id(r, i0));
+        q1[r] = L.load(own_id(r, i0 + 1));
+        q2[r] = L.load(own_id(r, i0 + 2));
+        q3[r] = q2[r];
+        e0[r] = E.preload(own_id(r, i0));
+        e1[r] = E.preload(own_id(r, i0 + 1));
+        e2[r] = e1[r];
+        pm[r] = own_ok(r, i0 - 1) ? L.make(a) : P::zero();
+        pc[r] = own_ok(r, i0) ? L.make(b) : P::zero();
+    }
+    double acc = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        P(&B)[TJ + 2][TK + 2] = buf[i & 1];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) B[tj + 4 * r + 1][tk + 1] = pc[r];
+        if (hr >= 0) B[hr][hc] = hown ? L.make(h0) : P::zero();
+        if (i + 3 <= i1) {
+#pragma unroll
+            for (int r = 0; r < RW; ++r) q3[r] = L.load(own_id(r, i + 3));
+        }
+        if (i + 2 < i1) {
+#pragma unroll
+            for (int r = 0; r < RW; ++r) e2[r] = E.preload(own_id(r, i + 2));
+            h2 = L.load(halo_id(i + 2));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const P pp = own_ok(r, i + 1) ? L.make(q1[r]) : P::zero();
+            if (own[r]) {
+                const int row = tj + 4 * r + 1;
+                acc += E((int64_t)i * N2 + col[r], e0[r], pc[r], pm[r], pp, B[row - 1][tk + 1],
+                         B[row + 1][tk + 1], B[row][tk], B[row][tk + 2]);
+            }
+            pm[r] = pc[r];
+            pc[r] = pp;
+            q1[r] = q2[r];
+            q2[r] = q3[r];
+            e0[r] = e1[r];
+            e1[r] = e2[r];
+        }
+        h0 = h1;
+        h1 = h2;
+    }
+    if constexpr (REDUCE) block_partial<256>(acc, red, lin);
+}
+
 // Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below.  Epilogues with an
 // FP64 exp() per node (Epi::kMarch) march along i in chunks of 8 planes on
 // large grids (register queue, next plane prefetched); the others use one node
@@ -160,8 +277,39 @@ inline StencilShape stencil_shape(int64_t n, bool heavy) {
     return s;
 }
 
+// Grids from kLdsMinN up take the LDS-tiled form (64 x 8 tiles; 16 planes per
+// chunk, fewer where that leaves under 2048 blocks).
+constexpr int64_t kLdsMinN = 128;
+constexpr int kLdsRows = 2;
+
+inline StencilShape stencil_shape_lds(int64_t n) {
+    StencilShape s;
+    s.tk = 64;
+    const int64_t gx = (n + 63) / 64, gy = (n + 4 * kLdsRows - 1) / (4 * kLdsRows);
+    int ich = 16;
+    while (ich > 4 && gx * gy * ((n + ich - 1) / ich) < 2048) ich /= 2;
+    while (gx * gy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
+    s.ichunk = ich;
+    s.march = true;
+    s.gx = (int)gx;
+    s.gy = (int)gy;
+    s.gz = (int)((n + ich - 1) / ich);
+    const int64_t tiles = gx * gy * (int64_t)s.gz;
+    s.blocks = (unsigned)((tiles + 7) / 8 * 8);
+    return s;
+}
+
 template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
+    if (n >= kLdsMinN) {
+        const StencilShape s = stencil_shape_lds(n);
+        hipLaunchKernelGGL((k_stencil_lds<kLdsRows, REDUCE, Load, Epi>), dim3(s.blocks),
+                           dim3(256), 0, c.stream, (int)n, s.ichunk, s.gx, s.gy, s.gz, ldr, epi,
+                           c.red);
+        NUMA_HIP_LAUNCH_CHECK();
+        if constexpr (REDUCE) c.finish((unsigned)(s.gx * s.gy * s.gz), ep);
+        return;
+    }
     const StencilShape s = stencil_shape(n, Epi::kMarch);
 #define NUMA_STENCIL_LAUNCH(TK_, MARCH_)                                                     \
     hipLaunchKernelGGL((k_stencil<TK_, REDUCE, MARCH_, Load, Epi>), dim3(s.blocks), dim3(256), \
