@@ -3,6 +3,8 @@
 // deterministic "last block reduces" protocol and launch-shape helpers.
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -103,6 +105,37 @@ __device__ inline void block_partial(double acc, const Reduce& red, unsigned bid
 
 __global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partials, unsigned count,
                                                            Epilogue ep);
+
+// Plane chunk of an i-marching launch over `tiles` tiles per chunk row, with
+// `halo` extra planes per chunk and `slots` blocks resident at once: the
+// chunk in [lo, hi] minimising waves * (chunk + halo), the time model of
+// blocks that each march their chunk (fewer, longer waves of blocks beat a
+// last wave that leaves most of the device idle).
+inline int fill_chunk(int64_t n, int64_t tiles, int64_t slots, int halo, int lo, int hi) {
+    int best = hi;
+    double best_cost = 1e300;
+    for (int ich = lo; ich <= hi; ++ich) {
+        const int64_t chunks = (n + ich - 1) / ich;
+        const int64_t waves = (tiles * chunks + slots - 1) / std::max<int64_t>(1, slots);
+        const int64_t planes = std::min<int64_t>(ich, n);
+        const double cost = (double)waves * (double)(planes + halo);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = ich;
+        }
+    }
+    return best;
+}
+
+// Resident blocks of `kernel` per CU at `threads` threads per block.
+template <class K>
+inline int blocks_per_cu(K kernel, int threads) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess ||
+        nb < 1)
+        nb = 1;
+    return nb;
+}
 
 // ---------------------------------------------------------------------------
 // Context: one device, one stream, reduction scratch and a pinned host

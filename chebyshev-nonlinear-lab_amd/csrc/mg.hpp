@@ -551,6 +551,37 @@ inline void launch_fused(Ctx& c, int n, int steps, int rows, int ichunk, double 
     }
 }
 
+// Plane chunk of a fused launch (steps, rows) on an n^3 level that fills the
+// device in whole waves of resident workgroups (fill_chunk with the
+// kernel's occupancy; the 2*steps halo planes of a chunk are recomputed).
+template <class T, int S, int R>
+inline int fused_fill_chunk_sr(int n, int cu_count) {
+    using G = FusedGeom<S, R>;
+    static const int per_cu = blocks_per_cu(k_poly_fused<T, S, R>, kFusedThreads);
+    const int64_t tiles = (int64_t)((n + G::kOutW - 1) / G::kOutW) * ((n + G::kOutH - 1) / G::kOutH);
+    return fill_chunk(n, tiles, (int64_t)per_cu * cu_count, 2 * S, 4, 128);
+}
+
+template <class T, int S>
+inline int fused_fill_chunk_s(int n, int rows, int cu_count) {
+    constexpr int R0 = fused_rows_small<T>();
+    if constexpr (fused_fits<T>(S, 2 * R0))
+        if (rows == 2 * R0) return fused_fill_chunk_sr<T, S, 2 * R0>(n, cu_count);
+    return fused_fill_chunk_sr<T, S, R0>(n, cu_count);
+}
+
+template <class T>
+inline int fused_fill_chunk(int n, int steps, int rows, int cu_count) {
+    switch (steps) {
+        case 1: return fused_fill_chunk_s<T, 1>(n, rows, cu_count);
+        case 2: return fused_fill_chunk_s<T, 2>(n, rows, cu_count);
+        case 3: return fused_fill_chunk_s<T, 3>(n, rows, cu_count);
+        case 4: return fused_fill_chunk_s<T, 4>(n, rows, cu_count);
+        case 5: return fused_fill_chunk_s<T, 5>(n, rows, cu_count);
+        default: return fused_fill_chunk_s<T, 6>(n, rows, cu_count);
+    }
+}
+
 inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
 inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1.0)); }
 
@@ -624,9 +655,12 @@ public:
             L.ichunk = cfg.ichunk > 0 ? (int)cfg.ichunk : 32;
             L.rows = cfg.rows > 0 ? (int)cfg.rows : fused_rows_small<T>();
             if ((L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE) &&
-                cfg.smoother != NUMA_SMOOTH_RBGS)
+                cfg.smoother != NUMA_SMOOTH_RBGS) {
                 require(fused_fits<T>(fused_group(L), L.rows),
                         "fused smoothing: steps per launch x rows per thread exceed the LDS");
+                if (cfg.ichunk == 0)
+                    L.ichunk = fused_fill_chunk<T>((int)L.n, fused_group(L), L.rows, c.cu_count);
+            }
         }
         set_resid_restrict(-1);
     }
@@ -911,7 +945,7 @@ private:
             Level& L = levels_[l];
             if (L.fuse != NUMA_FUSE_AUTO) continue;
             std::ostringstream key;
-            key << dev << "|mg-smooth|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
+            key << dev << "|mg-smooth-v2|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
                 << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
             std::string val;
             if (TuneCache::get().lookup(key.str(), val)) {
@@ -935,12 +969,17 @@ private:
             constexpr int R0 = fused_rows_small<T>();
             std::vector<Cand> cands{{NUMA_FUSE_NONE, 32, R0}};
             for (int rw : {R0, 2 * R0})
-                for (int ic : {16, 32, 64})
-                    for (int md : {(int)NUMA_FUSE_SWEEP, (int)NUMA_FUSE_PHASE}) {
-                        if (md == NUMA_FUSE_PHASE && cfg_.pre <= 1) continue;
-                        L.fuse = md;
-                        if (fused_fits<T>(fused_group(L), rw)) cands.push_back({md, ic, rw});
-                    }
+                for (int md : {(int)NUMA_FUSE_SWEEP, (int)NUMA_FUSE_PHASE}) {
+                    if (md == NUMA_FUSE_PHASE && cfg_.pre <= 1) continue;
+                    L.fuse = md;
+                    const int g = fused_group(L);
+                    if (!fused_fits<T>(g, rw)) continue;
+                    // the chunk that fills the device in whole waves, and powers of two
+                    const int fill = fused_fill_chunk<T>((int)L.n, g, rw, c_.cu_count);
+                    cands.push_back({md, fill, rw});
+                    for (int ic : {16, 32, 64})
+                        if (ic != fill) cands.push_back({md, ic, rw});
+                }
             hipEvent_t e0, e1;
             NUMA_HIP_CHECK(hipEventCreate(&e0));
             NUMA_HIP_CHECK(hipEventCreate(&e1));

@@ -277,17 +277,17 @@ inline StencilShape stencil_shape(int64_t n, bool heavy) {
     return s;
 }
 
-// Grids from kLdsMinN up take the LDS-tiled form (64 x 8 tiles; 16 planes per
-// chunk, fewer where that leaves under 2048 blocks).
+// Grids from kLdsMinN up take the LDS-tiled form: 64 x 8 tiles, and the plane
+// chunk that fills the device in whole waves of resident blocks (fill_chunk,
+// with the kernel's occupancy).
 constexpr int64_t kLdsMinN = 128;
 constexpr int kLdsRows = 2;
 
-inline StencilShape stencil_shape_lds(int64_t n) {
+inline StencilShape stencil_shape_lds(int64_t n, int64_t slots) {
     StencilShape s;
     s.tk = 64;
     const int64_t gx = (n + 63) / 64, gy = (n + 4 * kLdsRows - 1) / (4 * kLdsRows);
-    int ich = 16;
-    while (ich > 4 && gx * gy * ((n + ich - 1) / ich) < 2048) ich /= 2;
+    int ich = fill_chunk(n, gx * gy, slots, 2, 4, 64);
     while (gx * gy * ((n + ich - 1) / ich) > kMaxPartials) ich *= 2;
     s.ichunk = ich;
     s.march = true;
@@ -302,7 +302,8 @@ inline StencilShape stencil_shape_lds(int64_t n) {
 template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
     if (n >= kLdsMinN) {
-        const StencilShape s = stencil_shape_lds(n);
+        static const int per_cu = blocks_per_cu(k_stencil_lds<kLdsRows, REDUCE, Load, Epi>, 256);
+        const StencilShape s = stencil_shape_lds(n, (int64_t)per_cu * c.cu_count);
         hipLaunchKernelGGL((k_stencil_lds<kLdsRows, REDUCE, Load, Epi>), dim3(s.blocks),
                            dim3(256), 0, c.stream, (int)n, s.ichunk, s.gx, s.gy, s.gz, ldr, epi,
                            c.red);
