@@ -90,6 +90,34 @@ def time_leg(H, ctx, n, lam, cfg, params, steps, warmup, graphs=False):
     return s, x, res, times
 
 
+PROFILE_MD = os.path.join(ROOT, "profiles", "bratu256_lambda6.5_newton_fgmres_gmg_kernels.md")
+
+
+def fused_grid_threads(n: int, steps: int, rows: int, ichunk: int) -> int:
+    """Threads of a k_poly_fused launch (64 x 16*rows regions less 2*steps
+    halo per axis, plane chunks, blocks rounded up to a multiple of 8)."""
+    out_w, out_h = 64 - 2 * steps, 16 * rows - 2 * steps
+    tiles = -(-n // out_w) * -(-n // out_h) * -(-n // ichunk)
+    return -(-tiles // 8) * 8 * 1024
+
+
+def measured_traffic(kernel: str, n: int, ichunk: int, grid: int):
+    """HBM bytes per launch of the dominant kernel from the committed
+    FETCH_SIZE / WRITE_SIZE profile (calibrated with a copy kernel), if that
+    profile is of the same kernel, grid size and launch shape; else None."""
+    try:
+        with open(PROFILE_MD) as f:
+            for line in f:
+                if line.startswith("traffic-json:"):
+                    t = json.loads(line.split(":", 1)[1])
+                    if (t["kernel"] == kernel and t["n"] == n and t["ichunk"] == ichunk
+                            and t["grid_threads"] == grid):
+                        return int(t["traffic_bytes"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def primitive_table(H, s, reps=20):
     import ctypes as C
     names = {"F": H.PRIM_F, "JVP": H.PRIM_JVP, "PREC": H.PRIM_PREC, "SMOOTH": H.PRIM_SMOOTH,
@@ -436,13 +464,22 @@ def main() -> int:
     # roofline of the dominant kernel: the finest-level smoothing launch
     sm = L["prims"]["SMOOTH"]
     f0 = L["fusion"][0] if L["fusion"] else (H.FUSE_NONE, 0, 0)
-    kname = ("k_stencil<EpPoly> (finest Chebyshev smoothing step)" if f0[0] == H.FUSE_NONE else
-             f"k_poly_fused<{'float' if main_leg == 'mixed' else 'double'},"
-             f"{3 if f0[0] == H.FUSE_SWEEP else 6},{f0[2]}> (finest-level fused Chebyshev "
-             f"{'sweep' if f0[0] == H.FUSE_SWEEP else 'phase'}, ichunk {f0[1]})")
+    traffic = None
+    if f0[0] == H.FUSE_NONE:
+        kname = "k_stencil_lds<EpPoly> (finest Chebyshev smoothing step)"
+    else:
+        steps = 3 if f0[0] == H.FUSE_SWEEP else 6
+        kbase = f"k_poly_fused<{'float' if main_leg == 'mixed' else 'double'},{steps},{f0[2]}>"
+        kname = (f"{kbase} (finest-level fused Chebyshev "
+                 f"{'sweep' if f0[0] == H.FUSE_SWEEP else 'phase'}, ichunk {f0[1]})")
+        traffic = measured_traffic(kbase, args.n, int(f0[1]),
+                                   fused_grid_threads(args.n, steps, int(f0[2]), int(f0[1])))
     roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(sm["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(sm["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(sm["GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": ("profiles/" + os.path.basename(PROFILE_MD) + " (FETCH_SIZE + "
+                                   "WRITE_SIZE per launch, copy-kernel calibrated)")
+                if traffic is not None else None,
                 "algorithmic_bytes_per_launch": sm["bytes"], "avg_launch_ms": sm["ms"]}
 
     out = {"metric": METRIC, "value": round(t_main, 6), "unit": "s", "n_gpus": 1,
