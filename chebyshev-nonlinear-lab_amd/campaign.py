@@ -99,7 +99,7 @@ def solve_row(ctx, n: int, lam: float, method: str = "nk", precond: str = "mg",
               family: str = "B1", repetition: int = 1, seed: int = 0,
               commit: str | None = None) -> tuple[dict, np.ndarray]:
     """Solves Bratu3D on the device as run_campaign.cpp:344-475 solves one
-    instance (method nk | cheb | rec, precond none | jacobi | mg; by default
+    instance (method nk | cheb | rec | tf2018, precond none | jacobi | mg; by default
     the operator's analytic J v and F'' and the reference PoissonMG, as the
     campaign runs them) and returns its CSV row and the solution.  t_setup is
     the solver (provider) construction plus the per-point set-up; times are
@@ -107,7 +107,11 @@ def solve_row(ctx, n: int, lam: float, method: str = "nk", precond: str = "mg",
     p = (H.KrylovParams.from_buffer_copy(params) if params is not None
          else H.krylov_params())
     p.recycle = 1 if method == "rec" else 0
-    cfg = H.solver_config(method=H.METHOD_NEWTON if method == "nk" else H.METHOD_CHEBYSHEV,
+    methods = {"nk": H.METHOD_NEWTON, "cheb": H.METHOD_CHEBYSHEV, "rec": H.METHOD_CHEBYSHEV,
+               "tf2018": H.METHOD_TF2018}
+    if method not in methods:
+        raise ValueError(f"unknown method {method!r} (nk | cheb | rec | tf2018)")
+    cfg = H.solver_config(method=methods[method],
                           precond=_PRECOND[precond], precision=precision,
                           flexible=1 if precision == H.PRECISION_MIXED else 0,
                           jv_mode=jv_mode, d2_mode=d2_mode,

@@ -294,6 +294,34 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_diag(int64_t N, double h2inv,
         d[i] = 6.0 * h2inv - lambda * exp(u[i]);
 }
 
+// out = x + a d  (y = x; axpy(a, d, y), operator.hpp:32-41)
+__global__ __launch_bounds__(kBlock) void k_axpy_to(int64_t N, const double* x, double a,
+                                                    const double* d, double* out) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock)
+        out[i] = x[i] + a * d[i];
+}
+
+// out = s x
+__global__ __launch_bounds__(kBlock) void k_scale_to(int64_t N, double s, const double* x,
+                                                     double* out) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock)
+        out[i] = s * x[i];
+}
+
+// out = s ((a - b) / h)  (difference quotient, then scaled; in place allowed)
+__global__ __launch_bounds__(kBlock) void k_diffq(int64_t N, const double* a, const double* b,
+                                                  double h, double s, double* out) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock) {
+        const double q = (a[i] - b[i]) / h;
+        out[i] = s * q;
+    }
+}
+
 // out = -lambda e^u v v   (bratu3d.hpp:46-56)
 __global__ __launch_bounds__(kBlock) void k_d2_analytic(int64_t N, double lambda_v,
                                                         const double* lam_ptr, const double* u,
@@ -325,6 +353,26 @@ void launch_convert(Ctx& c, int64_t N, const TI* x, TO* y) {
     else
         hipLaunchKernelGGL((k_convert<false, TI, TO>), dim3(stream_blocks(N)), dim3(kBlock), 0,
                            c.stream, N, x, y);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+inline void launch_axpy_to(Ctx& c, int64_t N, const double* x, double a, const double* d,
+                           double* out) {
+    hipLaunchKernelGGL(k_axpy_to, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, x, a, d,
+                       out);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+inline void launch_scale_to(Ctx& c, int64_t N, double s, const double* x, double* out) {
+    hipLaunchKernelGGL(k_scale_to, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, s, x,
+                       out);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+inline void launch_diffq(Ctx& c, int64_t N, const double* a, const double* b, double h, double s,
+                         double* out) {
+    hipLaunchKernelGGL(k_diffq, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, a, b, h, s,
+                       out);
     NUMA_HIP_LAUNCH_CHECK();
 }
 

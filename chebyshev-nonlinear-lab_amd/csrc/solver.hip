@@ -60,6 +60,8 @@ public:
                 "unknown preconditioner");
         require(cfg.precond != NUMA_PRECOND_LEARNED || cfg.flexible,
                 "the learned preconditioner is nonlinear and needs FGMRES (flexible = 1)");
+        require(cfg.method >= NUMA_METHOD_NEWTON && cfg.method <= NUMA_METHOD_TF2018,
+                "unknown method");
         if (cfg.precond == NUMA_PRECOND_GMG || cfg.precond == NUMA_PRECOND_LEARNED)
             mg_.reset(make_mg(c, n, cfg.mg, cfg.precision));
         x2_ = allocd();
@@ -70,6 +72,10 @@ public:
         w_ = allocd();
         r_ = allocd();
         if (cfg.precond == NUMA_PRECOND_JACOBI) diag_ = allocd();
+        if (cfg.method == NUMA_METHOD_TF2018) {
+            tfa_ = allocd();
+            tfb_ = allocd();
+        }
         V_.resize((size_t)m_ + 1);
         for (auto& v : V_) v = alloct();
         if (cfg.flexible && cfg.precond != NUMA_PRECOND_NONE) {
@@ -115,6 +121,12 @@ public:
         c_.sync();
         collect_timers();
         timing_ = on;
+    }
+    double tf2018_curvature(const double* x, const double* d, double* out) override {
+        require(n_ > 0, "empty problem");
+        const double h = tf_curvature(x, nullptr, d, out, 1.0, x2_, Fn_);
+        c_.sync();
+        return h;
     }
     numa_timers timers(bool reset) override {
         c_.sync();
@@ -220,7 +232,8 @@ public:
 
 private:
     // misc scalar slots
-    enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_TMP, S_LAMBDA, kMisc = 8 };
+    enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_TMP, S_LAMBDA, S_DNORM, S_XHNORM,
+           kMisc = 10 };
     // lambda as a device scalar: the kernels read it, so captured graphs stay
     // valid across set_lambda
     const double* lam_dev() { return MISC(S_LAMBDA); }
@@ -362,6 +375,65 @@ private:
         return res;
     }
 
+    // out = J(x) v in FP64 whatever the Krylov precision; F0 = F(x) and
+    // xnorm = ||x|| (device slot) are what the finite-difference form uses
+    // (operator.hpp:101-114); the analytic form is bratu3d.hpp:33-43.
+    void jv64(const double* x, const double* F0, const double* xnorm, const double* v,
+              double* out) {
+        if (cfg_.jv_mode == NUMA_DERIV_FD) {
+            norm(v, MISC(S_VNORM));
+            const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), xnorm,
+                            MISC(S_VNORM)};
+            launch_stencil<false>(c_, n_, LdXpHV<double>{x, v, st, 0.0},
+                                  EpJvFD<double, false, false>{h2inv_, lambda_, st, nullptr, F0,
+                                                               out, nullptr, nullptr, 0.0, 1.0,
+                                                               lam_dev()},
+                                  Epilogue{});
+        } else {
+            launch_stencil<false>(c_, n_, LdVec<double>{v},
+                                  EpJvAn<double, false, false>{h2inv_, lambda_, nullptr, x, out,
+                                                               nullptr, nullptr, 1.0, lam_dev()},
+                                  Epilogue{});
+        }
+    }
+
+    // tensor_free_curvature_2018 (methods/tf_chebyshev_2018.hpp:23-51):
+    // out = s [J(x + h d) d - J(x) d] / h with h = clamp(sqrt(eps) ||x|| / ||d||,
+    // 2 eps, 0.5) (2 eps if d = 0); F0 = F(x) for the finite-difference J v
+    // (computed here if null).  xh, Fh: scratch for the shifted point and its
+    // F.  J v products go to tfa_ / tfb_; out may be tfa_.  Returns h.
+    double tf_curvature(const double* x, const double* F0, const double* d, double* out,
+                        double s, double* xh, double* Fh) {
+        const double eps = std::numeric_limits<double>::epsilon();
+        norm(d, MISC(S_DNORM));
+        norm(x, MISC(S_XNORM));
+        double nrm[2];
+        readn(MISC(S_XNORM), 1, nrm);
+        readn(MISC(S_DNORM), 1, nrm + 1);
+        const double nd = nrm[1];
+        const double h = !(nd > 0.0) ? 2.0 * eps
+                                     : std::clamp(std::sqrt(eps) * nrm[0] / nd, 2.0 * eps, 0.5);
+        if (!tfa_) {
+            tfa_ = allocd();
+            tfb_ = allocd();
+        }
+        launch_axpy_to(c_, N_, x, h, d, xh);  // xh = x + h d
+        const bool fd = cfg_.jv_mode == NUMA_DERIV_FD;
+        const double* F0x = F0;
+        if (fd && !F0) {  // F(x) in Fh, consumed by the first J v before Fh = F(xh)
+            eval_F(x, nullptr, 0.0, nullptr, 0.0, Fh, nullptr);
+            F0x = Fh;
+        }
+        jv64(x, F0x, MISC(S_XNORM), d, tfa_);
+        if (fd) {
+            eval_F(xh, nullptr, 0.0, nullptr, 0.0, Fh, nullptr);
+            norm(xh, MISC(S_XHNORM));
+        }
+        jv64(xh, Fh, MISC(S_XHNORM), d, tfb_);
+        launch_diffq(c_, N_, tfb_, tfa_, h, s, out);
+        return h;
+    }
+
     void d2(const double* x, const double* F0, const double* v, double* out) {
         if (cfg_.d2_mode == NUMA_DERIV_FD) {
             norm(v, MISC(S_VNORM));
@@ -489,6 +561,7 @@ private:
     CaptureFn capture_;
     double *x2_ = nullptr, *Fx_ = nullptr, *Fn_ = nullptr, *z_ = nullptr, *u_ = nullptr,
            *w_ = nullptr, *r_ = nullptr, *diag_ = nullptr;
+    double *tfa_ = nullptr, *tfb_ = nullptr;  // J v products of the TF-2018 curvature
     std::vector<T*> V_, Z_;
     T* zb_ = nullptr;
     T* comb_ = nullptr;
@@ -744,6 +817,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     const auto t0 = std::chrono::steady_clock::now();
     out = numa_krylov_result{};
     const bool cheb = cfg_.method == NUMA_METHOD_CHEBYSHEV;
+    const bool tf = cfg_.method == NUMA_METHOD_TF2018;
     const bool rec = cheb && p.recycle;
     require(p.gmres_restart >= 0 && p.gmres_restart <= m_,
             "gmres_restart exceeds the solver's basis capacity");
@@ -764,7 +838,50 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
     double rnorm = read1(MISC(S_FNORM));
     out.res0 = rnorm;
     double rprev = rnorm, eta_prev = p.eta_init;
-    for (int64_t k = 0; k < p.max_nl; ++k) {
+    // tensor-free Chebyshev 2018 (methods/tf_chebyshev_2018.hpp:53-132):
+    // J z = F (eta1), d1 = -z, J d2 = -C d1 / 2 (eta2), x+ = (x + d1) + d2
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (int64_t k = 0; tf && k < p.max_nl; ++k) {
+        if (rnorm <= p.rtol_nl * out.res0) {
+            out.converged = 1;
+            break;
+        }
+        const double eta1 = std::clamp(std::min(1e-8, rnorm * rnorm), 2.0 * eps, 1e-8);
+        const double eta2 = std::clamp(std::min(1e-8, rnorm), 2.0 * eps, 1e-8);
+        timed(T_SETUP, [&]() { point_setup(xc); });
+        const auto g1 = gmres(xc, Fc, Fc, z_, eta1, p.gmres_restart, p.gmres_max_iters, true,
+                              nullptr, false);
+        out.n_JVP += g1.matvecs;
+        out.n_lin_iters += g1.iters;
+        out.n_prec += g1.precs;
+        if (!g1.converged) {
+            ++out.n_lin_failures;
+            break;
+        }
+        if (g1.breakdown) break;
+        launch_scale_to(c_, N_, -1.0, z_, w_);  // d1
+        timed(T_JVP, [&]() { tf_curvature(xc, Fc, w_, tfa_, -0.5, xt, Ft); });
+        out.n_JVP += 2;
+        const auto g2 = gmres(xc, Fc, tfa_, u_, eta2, p.gmres_restart, p.gmres_max_iters, true,
+                              nullptr, false);
+        out.n_JVP += g2.matvecs;
+        out.n_lin_iters += g2.iters;
+        out.n_prec += g2.precs;
+        if (!g2.converged) {
+            ++out.n_lin_failures;
+            break;
+        }
+        if (g2.breakdown) break;
+        // (x - 1 z) - (-1) d2 = (x + d1) + d2, and F there
+        timed(T_F, [&]() { eval_F(xc, z_, 1.0, u_, -1.0, Ft, xt); });
+        ++out.n_F;
+        rnorm = read1(MISC(S_FNORM));
+        std::swap(xc, xt);
+        std::swap(Fc, Ft);
+        ++out.iters;
+        if (!std::isfinite(rnorm)) break;
+    }
+    for (int64_t k = 0; !tf && k < p.max_nl; ++k) {
         if (rnorm <= p.rtol_nl * out.res0) {
             out.converged = 1;
             break;
@@ -981,6 +1098,17 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
     if (mg && mode_host && ichunk_host && rows_host)
         for (int64_t l = 0; l < mg->levels(); ++l)
             mg->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
+    NUMA_API_END
+}
+
+int numa_hip_tf2018_curvature(numa_hip_solver* s, const double* x_dev, const double* d_dev,
+                              double* out_dev, double* h_used) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && x_dev && d_dev && out_dev, "null argument");
+    numa_hip::require(out_dev != x_dev && out_dev != d_dev, "out must not alias x or d");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    const double h = s->impl->tf2018_curvature(x_dev, d_dev, out_dev);
+    if (h_used) *h_used = h;
     NUMA_API_END
 }
 

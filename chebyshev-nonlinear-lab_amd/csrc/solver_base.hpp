@@ -56,6 +56,9 @@ struct SolverBase {
     // timing runs the Arnoldi steps eagerly.
     virtual void set_timing(bool on) = 0;
     virtual numa_timers timers(bool reset) = 0;
+    // tensor_free_curvature_2018 at (x, d) into out (device FP64 vectors,
+    // out distinct from x and d); returns the step h.
+    virtual double tf2018_curvature(const double* x, const double* d, double* out) = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

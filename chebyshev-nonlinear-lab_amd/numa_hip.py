@@ -29,7 +29,7 @@ NUMA_OK = 0
 ERR_NAMES = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "overflow",
              5: "bad state"}
 
-METHOD_NEWTON, METHOD_CHEBYSHEV = 0, 1
+METHOD_NEWTON, METHOD_CHEBYSHEV, METHOD_TF2018 = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG, PRECOND_LEARNED = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 DERIV_FD, DERIV_ANALYTIC = 0, 1
@@ -215,6 +215,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
         "numa_hip_mg_set_resid_restrict": [_vp, _i64],
         "numa_hip_solver_set_graphs": [_vp, _i64],
+        "numa_hip_tf2018_curvature": [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)],
         "numa_hip_solver_set_timing": [_vp, _i64],
         "numa_hip_solver_timers": [_vp, C.POINTER(Timers), _i64],
         "numa_hip_solver_graphs_cached": [_vp, C.POINTER(_i64)],
@@ -472,6 +473,13 @@ class BratuSolver:
     def set_graphs(self, on: bool = True) -> None:
         check(self.ctx.lib.numa_hip_solver_set_graphs(self.h, 1 if on else 0))
 
+    def tf2018_curvature(self, x: "DeviceArray", d: "DeviceArray", out: "DeviceArray") -> float:
+        """tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) on device
+        FP64 vectors; returns the step h."""
+        h = C.c_double()
+        check(self.ctx.lib.numa_hip_tf2018_curvature(self.h, x.ptr, d.ptr, out.ptr, C.byref(h)))
+        return h.value
+
     def graphs_cached(self) -> int:
         c = _i64()
         check(self.ctx.lib.numa_hip_solver_graphs_cached(self.h, C.byref(c)))
@@ -526,6 +534,18 @@ def chebyshev_inexact(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restar
     """numa::op::chebyshev_inexact (krylov_newton.hpp:116-189) on the device."""
     cfg = cfg if cfg is not None else solver_config()
     cfg.method = METHOD_CHEBYSHEV
+    params = params if params is not None else krylov_params()
+    s = BratuSolver(ctx, n, lam, cfg, restart or max(1, params.gmres_restart))
+    x = np.array(x0, np.float64)
+    r = s.solve_host(params, x)
+    s.close()
+    return x, r
+
+
+def tf_chebyshev_2018(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restart=None):
+    """numa::op::tf_chebyshev_2018 (tf_chebyshev_2018.hpp:53-132) on the device."""
+    cfg = cfg if cfg is not None else solver_config()
+    cfg.method = METHOD_TF2018
     params = params if params is not None else krylov_params()
     s = BratuSolver(ctx, n, lam, cfg, restart or max(1, params.gmres_restart))
     x = np.array(x0, np.float64)

@@ -15,6 +15,8 @@
 //   numa::hip::KrylovParams / KrylovResult   methods/krylov_newton.hpp:24-49
 //   numa::hip::GmresResult      linalg/gmres.hpp:14-21
 //   numa::hip::newton_krylov / chebyshev_inexact  krylov_newton.hpp:69-189
+//   numa::hip::tf_chebyshev_2018 / tensor_free_curvature_2018
+//                               methods/tf_chebyshev_2018.hpp:23-132
 //   numa::hip::gmres            linalg/gmres.hpp:61-65 (apply = J(x0) v)
 // The solves run entirely in HBM; host vectors are staged once per call.
 // Misuse throws std::invalid_argument / std::overflow_error as the reference
@@ -294,6 +296,33 @@ KrylovResult chebyshev_inexact(const Bratu3D& f, DVec x, const KrylovParams& p =
                                const PrecondProvider& pc = {}, const SolverOptions& o = {}) {
     return detail::solve(f, std::move(x), p,
                          pc.apply(detail::base_config(NUMA_METHOD_CHEBYSHEV, o)), o.timers);
+}
+
+// methods/tf_chebyshev_2018.hpp:53-57
+template <class PrecondProvider = NoPrecond>
+KrylovResult tf_chebyshev_2018(const Bratu3D& f, DVec x, const KrylovParams& p = {},
+                               const PrecondProvider& pc = {}, const SolverOptions& o = {}) {
+    return detail::solve(f, std::move(x), p,
+                         pc.apply(detail::base_config(NUMA_METHOD_TF2018, o)), o.timers);
+}
+
+// methods/tf_chebyshev_2018.hpp:31-51: out = [J(x + h d) d - J(x) d] / h with
+// the operator's analytic J v (o.jv_mode selects the finite-difference one).
+inline void tensor_free_curvature_2018(const Bratu3D& f, const DVec& x, const DVec& d1, DVec& out,
+                                       double* h_used = nullptr, const SolverOptions& o = {}) {
+    const std::size_t N = f.size();
+    if (x.size() != N || d1.size() != N) throw std::invalid_argument("vector size mismatch");
+    const numa_solver_config cfg = NoPrecond{}.apply(detail::base_config(NUMA_METHOD_TF2018, o));
+    numa_hip_solver* s = nullptr;
+    check(numa_hip_bratu_solver_create(f.context().get(), (int64_t)f.grid(), f.lambda(), &cfg, 1,
+                                       &s));
+    std::unique_ptr<numa_hip_solver, int (*)(numa_hip_solver*)> guard(s, numa_hip_solver_destroy);
+    DeviceVec dx(f.context(), N), dd(f.context(), N), dout(f.context(), N);
+    dx.upload(x);
+    dd.upload(d1);
+    check(numa_hip_tf2018_curvature(s, dx.data(), dd.data(), dout.data(), h_used));
+    out.resize(N);
+    dout.download(out);
 }
 
 // linalg/gmres.hpp:61-65 with apply(v, out) = J(x0) v and the provider's

@@ -37,7 +37,7 @@ enum {
 };
 
 /* ---- enumerations used in the POD configs ------------------------------ */
-enum { NUMA_METHOD_NEWTON = 0, NUMA_METHOD_CHEBYSHEV = 1 };
+enum { NUMA_METHOD_NEWTON = 0, NUMA_METHOD_CHEBYSHEV = 1, NUMA_METHOD_TF2018 = 2 };
 enum {
     NUMA_PRECOND_NONE = 0,
     NUMA_PRECOND_JACOBI = 1, /* BratuJacobi, bratu3d.hpp:104-120            */
@@ -387,6 +387,17 @@ typedef struct numa_timers {
 int numa_hip_solver_set_timing(numa_hip_solver* s, int64_t enable);
 /* Accumulated timers; reset != 0 zeroes them afterwards. */
 int numa_hip_solver_timers(numa_hip_solver* s, numa_timers* out, int64_t reset);
+
+/* tensor_free_curvature_2018 (methods/tf_chebyshev_2018.hpp:31-51): out =
+ * [J(x + h d) d - J(x) d] / h with h = clamp(sqrt(eps) ||x|| / ||d||, 2 eps,
+ * 0.5) (2 eps if d = 0), J v as the solver's jv_mode (analytic, bratu3d.hpp:
+ * 33-43, or the forward difference of operator.hpp:101-114 at each base
+ * point).  x, d, out: device FP64 vectors of size n^3, out distinct from x
+ * and d; h_used (host, may be NULL) receives h.  The TF-2018 outer method is
+ * cfg->method = NUMA_METHOD_TF2018 in numa_hip_solve (tf_chebyshev_2018.hpp:
+ * 53-132). */
+int numa_hip_tf2018_curvature(numa_hip_solver* s, const double* x_dev, const double* d_dev,
+                              double* out_dev, double* h_used);
 
 /* ---- Krylov-step graphs and the batched sweep (no reference counterpart) -- */
 /* enable != 0: solve() replays each Arnoldi step (preconditioner, J z, MGS

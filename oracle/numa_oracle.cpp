@@ -20,6 +20,10 @@
 //       brute force, breakdown cases; FGMRES == GMRES for linear M.
 //   newton_krylov / chebyshev      - against _ref counts and iterates; the
 //       reference-held continuation norms (B1_start_hashes.csv).
+//   tf2018 curvature / solve       - against _ref (tensor_free_curvature_2018,
+//       tf_chebyshev_2018) counts and iterates; curvature vs analytic F''
+//       (test_tf_chebyshev_2018.cpp:19-33); exact quadratic: for F quadratic
+//       the curvature is F''[d,d] to rounding.
 //   chandrasekhar                  - N=8 golden vector (test_chandrasekhar_op.cpp:106-115),
 //       moment identity sum w x = (2/c)(1 - sqrt(1-c)), GL monomial exactness.
 
@@ -661,6 +665,7 @@ double forcing(const numa_krylov_params& p, size_t k, double rnorm, double rprev
 // and the preconditioner bound at x.
 struct Problem {
     size_t n;
+    bool jvp_needs_F = false;  // J v uses F at its base point (finite differences)
     std::function<void(const DVec&, DVec&)> eval;
     std::function<void(const DVec& x, const DVec& Fx, double xnorm, const DVec& v, DVec& out)> jvp;
     std::function<void(const DVec& x, const DVec& Fx, double xnorm, const DVec& v, DVec& out)> d2;
@@ -748,6 +753,95 @@ void outer_solve(const Problem& P, DVec& x, const numa_krylov_params& p, bool ch
     out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Tensor-free Chebyshev of Eustaquio, Ribeiro and Dumett (2018) with
+// alpha = 0, methods/tf_chebyshev_2018.hpp:23-132: d1 = -z from J z = F,
+// curvature C d1 = [J(x + h d1) d1 - J(x) d1] / h, J d2 = -C d1 / 2,
+// x+ = x + d1 + d2.
+double tf2018_step_length(const DVec& x, const DVec& d1) {  // :23-29
+    const double eps = std::numeric_limits<double>::epsilon();
+    const double nd = norm2(d1);
+    if (!(nd > 0.0)) return 2.0 * eps;
+    const double raw = std::sqrt(eps) * norm2(x) / nd;
+    return std::clamp(raw, 2.0 * eps, 0.5);
+}
+
+// :31-51.  J v at the shifted point x + h d1 takes that point's own F and
+// norm (the finite-difference J v evaluates F at its base point).
+void tf2018_curvature(const Problem& P, const DVec& x, const DVec& Fx, double xnorm,
+                      const DVec& d1, DVec& out, double* h_used) {
+    const size_t n = x.size();
+    const double h = tf2018_step_length(x, d1);
+    DVec xh = x;
+    axpy(h, d1, xh);
+    DVec Fh(n), j0(n), j1(n);
+    if (P.jvp_needs_F) P.eval(xh, Fh);
+    P.jvp(x, Fx, xnorm, d1, j0);
+    P.jvp(xh, Fh, norm2(xh), d1, j1);
+    out.resize(n);
+    for (size_t i = 0; i < n; ++i) out[i] = (j1[i] - j0[i]) / h;
+    if (h_used) *h_used = h;
+}
+
+void tf2018_solve(const Problem& P, DVec& x, const numa_krylov_params& p, bool flexible,
+                  numa_krylov_result& out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t n = P.n;
+    out = numa_krylov_result{};
+    DVec Fx(n), z(n), d1(n), d2(n), curv(n), rhs2(n);
+    P.eval(x, Fx);
+    ++out.n_F;
+    double rnorm = norm2(Fx);
+    out.res0 = rnorm;
+    const double eps = std::numeric_limits<double>::epsilon();
+    for (size_t k = 0; k < (size_t)p.max_nl; ++k) {
+        if (rnorm <= p.rtol_nl * out.res0) {
+            out.converged = 1;
+            break;
+        }
+        const double eta1 = std::clamp(std::min(1e-8, rnorm * rnorm), 2.0 * eps, 1e-8);
+        const double eta2 = std::clamp(std::min(1e-8, rnorm), 2.0 * eps, 1e-8);
+        const ApplyFn M = P.make_precond(x);
+        const double xnorm = norm2(x);
+        ApplyFn apply = [&](const DVec& v, DVec& o) { P.jvp(x, Fx, xnorm, v, o); };
+        z.assign(n, 0.0);
+        const auto g1 = gmres(apply, M, Fx, z, eta1, (size_t)p.gmres_restart,
+                              (size_t)p.gmres_max_iters, true, flexible);
+        out.n_JVP += g1.matvecs;
+        out.n_lin_iters += g1.iters;
+        out.n_prec += g1.precs;
+        if (!g1.converged) {
+            ++out.n_lin_failures;
+            break;
+        }
+        if (g1.breakdown) break;
+        for (size_t i = 0; i < n; ++i) d1[i] = -z[i];
+        tf2018_curvature(P, x, Fx, xnorm, d1, curv, nullptr);
+        out.n_JVP += 2;
+        for (size_t i = 0; i < n; ++i) rhs2[i] = -0.5 * curv[i];
+        d2.assign(n, 0.0);
+        const auto g2 = gmres(apply, M, rhs2, d2, eta2, (size_t)p.gmres_restart,
+                              (size_t)p.gmres_max_iters, true, flexible);
+        out.n_JVP += g2.matvecs;
+        out.n_lin_iters += g2.iters;
+        out.n_prec += g2.precs;
+        if (!g2.converged) {
+            ++out.n_lin_failures;
+            break;
+        }
+        if (g2.breakdown) break;
+        axpy(1.0, d1, x);
+        axpy(1.0, d2, x);
+        P.eval(x, Fx);
+        ++out.n_F;
+        rnorm = norm2(Fx);
+        ++out.iters;
+        if (!std::isfinite(rnorm)) break;
+    }
+    if (rnorm <= p.rtol_nl * out.res0) out.converged = 1;
+    out.res = rnorm;
+    out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 ApplyFn make_bratu_precond(const Bratu& f, const numa_solver_config& cfg,
                            std::shared_ptr<MG> mg, const DVec& x) {
     if (cfg.precond == NUMA_PRECOND_GMG) {
@@ -776,6 +870,7 @@ Problem bratu_problem(const Bratu& f, const numa_solver_config& cfg) {
         o.resize(x.size());
         f.eval(x.data(), o.data());
     };
+    P.jvp_needs_F = cfg.jv_mode == NUMA_DERIV_FD;
     if (cfg.jv_mode == NUMA_DERIV_FD)
         P.jvp = [&f](const DVec& x, const DVec& Fx, double xn, const DVec& v, DVec& o) {
             o.resize(x.size());
@@ -1019,8 +1114,26 @@ int orc_bratu_solve(int64_t n, double lambda, const numa_solver_config* cfg,
     Bratu f((long)n, lambda);
     Problem P = bratu_problem(f, *cfg);
     DVec X(x, x + f.size());
-    outer_solve(P, X, *p, cfg->method == NUMA_METHOD_CHEBYSHEV, cfg->flexible != 0, *out);
+    if (cfg->method == NUMA_METHOD_TF2018)
+        tf2018_solve(P, X, *p, cfg->flexible != 0, *out);
+    else
+        outer_solve(P, X, *p, cfg->method == NUMA_METHOD_CHEBYSHEV, cfg->flexible != 0, *out);
     std::copy(X.begin(), X.end(), x);
+    return NUMA_OK;
+}
+
+int orc_tf2018_curvature(int64_t n, double lambda, int64_t jv_mode, const double* x,
+                         const double* d, double* out, double* h_used) {
+    if (n <= 0 || !x || !d || !out) return NUMA_ERR_ARG;
+    Bratu f((long)n, lambda);
+    numa_solver_config cfg{};
+    cfg.jv_mode = jv_mode;
+    Problem P = bratu_problem(f, cfg);
+    const DVec X(x, x + f.size()), D(d, d + f.size());
+    DVec Fx(f.size()), O;
+    if (P.jvp_needs_F) P.eval(X, Fx);
+    tf2018_curvature(P, X, Fx, norm2(X), D, O, h_used);
+    std::copy(O.begin(), O.end(), out);
     return NUMA_OK;
 }
 

@@ -51,9 +51,14 @@ int orc_poisson_gmres(int64_t n, const numa_mg_config* mg, int64_t use_mg,
                       const double* b, double* x, double rtol, int64_t restart,
                       int64_t max_iters, numa_gmres_result* out);
 
-/* Newton-Krylov / inexact Chebyshev on Bratu3D (methods/krylov_newton.hpp) */
+/* Newton-Krylov / inexact Chebyshev on Bratu3D (methods/krylov_newton.hpp)
+ * or tensor-free Chebyshev 2018 (methods/tf_chebyshev_2018.hpp), cfg->method */
 int orc_bratu_solve(int64_t n, double lambda, const numa_solver_config* cfg,
                     const numa_krylov_params* p, double* x, numa_krylov_result* out);
+/* tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) on Bratu3D with
+ * the J v of jv_mode (NUMA_DERIV_ANALYTIC or NUMA_DERIV_FD). */
+int orc_tf2018_curvature(int64_t n, double lambda, int64_t jv_mode, const double* x,
+                         const double* d, double* out, double* h_used);
 
 /* Chandrasekhar H-equation (problems/chandrasekhar_op.hpp and the midpoint /
  * Kelley form).  rule: 0 = Gauss-Legendre on [0,1], 1 = midpoint.

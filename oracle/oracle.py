@@ -54,7 +54,7 @@ class SolverConfig(C.Structure):
                 ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
 
 
-METHOD_NEWTON, METHOD_CHEBYSHEV = 0, 1
+METHOD_NEWTON, METHOD_CHEBYSHEV, METHOD_TF2018 = 0, 1, 2
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_GMG = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 DERIV_FD, DERIV_ANALYTIC = 0, 1
@@ -127,6 +127,7 @@ class Oracle:
                                         _i64, _i64, C.POINTER(GmresResult)]
         L.orc_bratu_solve.argtypes = [_i64, C.c_double, C.POINTER(SolverConfig),
                                       C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)]
+        L.orc_tf2018_curvature.argtypes = [_i64, C.c_double, _i64, _dp, _dp, _dp, _dp]
         L.orc_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
         L.orc_chandra_eval.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp]
         L.orc_chandra_jvp.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp, _dp]
@@ -220,6 +221,18 @@ class Oracle:
         assert rc == 0
         return x, res
 
+    def tf2018_curvature(self, n, lam, x, d, fd=0):
+        """tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) with the
+        analytic (fd = 0) or finite-difference (fd = 1) J v; returns (C d, h)."""
+        x = np.ascontiguousarray(x, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        out = np.empty(n ** 3)
+        h = C.c_double()
+        mode = DERIV_FD if fd else DERIV_ANALYTIC
+        assert self.lib.orc_tf2018_curvature(n, lam, mode, _ptr(x), _ptr(d), _ptr(out),
+                                             C.byref(h)) == 0
+        return out, h.value
+
     # Chandrasekhar -------------------------------------------------------------
     def gauss_legendre_unit(self, n):
         a, b = np.empty(n), np.empty(n)
@@ -277,6 +290,7 @@ class Reference:
         L.ref_bratu_gmres.argtypes = [_i64, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                       C.c_double, _i64, _i64, _dp]
         L.ref_bratu_solve.argtypes = [_i64, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ref_tf2018_curvature.argtypes = [_i64, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
         L.ref_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
         L.ref_chandra_eval.argtypes = [_i64, C.c_double, _dp, _dp]
         L.ref_chandra_jvp.argtypes = [_i64, C.c_double, _dp, _dp, _dp]
@@ -318,6 +332,14 @@ class Reference:
         res = np.zeros(10)
         self.lib.ref_bratu_solve(n, lam, method, precond, fd, _ptr(pv), _ptr(x), _ptr(res))
         return x, _result_dict(res)
+
+    def tf2018_curvature(self, n, lam, x, d, fd=0):
+        x = np.ascontiguousarray(x, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        out = np.empty(n ** 3)
+        h = C.c_double()
+        self.lib.ref_tf2018_curvature(n, lam, fd, _ptr(x), _ptr(d), _ptr(out), C.byref(h))
+        return out, h.value
 
     def gauss_legendre_unit(self, n):
         a, b = np.empty(n), np.empty(n)

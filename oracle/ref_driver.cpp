@@ -10,6 +10,7 @@
 #include <numa/linalg/gmres.hpp>
 #include <numa/linalg/poisson_mg.hpp>
 #include <numa/methods/krylov_newton.hpp>
+#include <numa/methods/tf_chebyshev_2018.hpp>
 #include <numa/operators/operator.hpp>
 #include <numa/problems/bratu3d.hpp>
 #include <numa/problems/chandrasekhar_op.hpp>
@@ -70,6 +71,7 @@ void from_result(const KrylovResult& r, double* o) {
 
 template <class Op, class Prov>
 KrylovResult run(const Op& f, const DVec& x0, const KrylovParams& p, const Prov& pc, int method) {
+    if (method == 2) return tf_chebyshev_2018(f, x0, p, pc);
     return method == 1 ? chebyshev_inexact(f, x0, p, pc) : newton_krylov(f, x0, p, pc);
 }
 
@@ -167,7 +169,21 @@ void ref_bratu_gmres(int64_t n, double lambda, int fd, int precond, const double
     res[5] = g.breakdown;
 }
 
-// method 0 newton_krylov / 1 chebyshev_inexact; fd: 0 analytic, 1 FD Jv and F''.
+// tensor_free_curvature_2018 with the analytic (fd = 0) or FD (fd = 1) J v.
+void ref_tf2018_curvature(int64_t n, double lambda, int fd, const double* x, const double* d,
+                          double* out, double* h_used) {
+    Bratu3D f((std::size_t)n, lambda);
+    const int64_t N = n * n * n;
+    DVec o;
+    if (fd)
+        tensor_free_curvature_2018(FDBratu{f}, vec(x, N), vec(d, N), o, h_used);
+    else
+        tensor_free_curvature_2018(f, vec(x, N), vec(d, N), o, h_used);
+    put(o, out);
+}
+
+// method 0 newton_krylov / 1 chebyshev_inexact / 2 tf_chebyshev_2018;
+// fd: 0 analytic, 1 FD Jv and F''.
 void ref_bratu_solve(int64_t n, double lambda, int method, int precond, int fd,
                      const double* params, double* x, double* result) {
     Bratu3D f((std::size_t)n, lambda);

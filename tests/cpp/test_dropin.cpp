@@ -113,6 +113,27 @@ int main() {
         CHECK(ch.converged && ch.n_D2 == 4);
         CHECK(t.t_D2 > 0.0 && t.t_F > after_nk.t_F && t.t_prec > after_nk.t_prec);
     }
+    {  // test_tf_chebyshev_2018.cpp:19-48: curvature vs F'', TF-2018 converges
+        Bratu3D f(ctx, 4, 1.0);
+        DVec x(f.size()), d(f.size());
+        for (std::size_t i = 0; i < f.size(); ++i) {
+            x[i] = 0.1 + 0.03 * std::sin(0.2 * static_cast<double>(i));
+            d[i] = 0.5 * std::cos(0.17 * static_cast<double>(i));
+        }
+        DVec tensor_free, exact;
+        double h = 0.0;
+        tensor_free_curvature_2018(f, x, d, tensor_free, &h);
+        f.d2(x, d, exact);
+        CHECK(h > 0.0 && h <= 0.5);
+        CHECK(relative_error(tensor_free, exact) < 5e-6);
+        KrylovParams p;
+        p.rtol_nl = 1e-8;
+        p.max_nl = 20;
+        p.gmres_restart = 30;
+        p.gmres_max_iters = 300;
+        const auto r = tf_chebyshev_2018(f, DVec(f.size(), 0.0), p, BratuJacobi{f});
+        CHECK(r.converged && r.n_D2 == 0 && r.n_JVP > 0);
+    }
     {  // misuse is reported as std::invalid_argument
         bool thrown = false;
         try {

@@ -100,6 +100,11 @@ def spectral_gaps():
     return out
 
 
+# (n, lambda, precond 0 none / 1 Jacobi / 2 PoissonMG, fd) of the TF-2018 solve
+# fixtures; (4, 1.0, 1, 0) is test_tf_chebyshev_2018.cpp:36-48
+TF_SOLVES = ((4, 1.0, 1, 0), (8, 3.0, 1, 0), (15, 2.0, 2, 0), (8, 1.0, 1, 1), (7, 6.0, 0, 0))
+
+
 def main():
     with open(os.path.join(HERE, "reference_csv_rows.json"), "w") as f:
         json.dump(csv_rows(), f, indent=1)
@@ -154,6 +159,24 @@ def main():
         arrays[tag] = xs
         arrays[tag + "_counts"] = np.array([r["iters"], r["n_F"], r["n_JVP"], r["n_D2"],
                                             r["n_lin_iters"], r["n_prec"]], np.float64)
+    # tensor-free Chebyshev 2018: curvature on the input of
+    # test_tf_chebyshev_2018.cpp:20-25 and solves (method 2)
+    for n, lam in ((4, 1.0), (8, 3.0)):
+        i = np.arange(n ** 3, dtype=np.float64)
+        x = 0.1 + 0.03 * np.sin(0.2 * i)
+        d = 0.5 * np.cos(0.17 * i)
+        for fd in (0, 1):
+            c, h = ref.tf2018_curvature(n, lam, x, d, fd)
+            arrays[f"tf2018_curv_n{n}_fd{fd}"] = c
+            arrays[f"tf2018_curv_n{n}_fd{fd}_h"] = np.array([h])
+    for n, lam, pc, fd in TF_SOLVES:
+        p = O.krylov_params(rtol_nl=1e-8, max_nl=20, gmres_restart=30, gmres_max_iters=300)
+        xs, r = ref.bratu_solve(n, lam, 2, pc, fd, p)
+        tag = f"tf2018_n{n}_l{lam}_pc{pc}_fd{fd}"
+        arrays[tag] = xs
+        arrays[tag + "_counts"] = np.array([r["converged"], r["iters"], r["n_F"], r["n_JVP"],
+                                            r["n_D2"], r["n_lin_iters"], r["n_lin_failures"],
+                                            r["n_prec"]], np.float64)
     for N in (8, 24):
         nodes, weights = ref.gauss_legendre_unit(N)
         arrays[f"gl_nodes_{N}"] = nodes

@@ -334,3 +334,68 @@ def test_chandra_golden_and_moment_identity(oracle):
                                         form=form)
             w = oracle.gauss_legendre_unit(N)[1] if rule == 0 else np.full(N, 1.0 / N)
             assert abs(np.dot(w, x) - 2 / c * (1 - np.sqrt(1 - c))) < 1e-11
+
+
+# ---- tensor-free Chebyshev 2018 ------------------------------------------------
+def _tf_input(n):
+    # test_tf_chebyshev_2018.cpp:20-25
+    i = np.arange(n ** 3, dtype=np.float64)
+    return 0.1 + 0.03 * np.sin(0.2 * i), 0.5 * np.cos(0.17 * i)
+
+
+def _rel_err(a, b):  # test_tf_chebyshev_2018.cpp:11-15
+    return np.linalg.norm(a - b) / (1.0 + np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("n,lam", [(4, 1.0), (8, 3.0)])
+@pytest.mark.parametrize("fd", [0, 1])
+def test_tf2018_curvature_matches_reference(oracle, n, lam, fd):
+    x, d = _tf_input(n)
+    c, h = oracle.tf2018_curvature(n, lam, x, d, fd)
+    tag = f"tf2018_curv_n{n}_fd{fd}"
+    assert h == VEC[tag + "_h"][0]
+    # (J(x+hd)d - J(x)d)/h cancels ~8 digits: the J v products agree bit for
+    # bit, so the quotient does too up to the ulp of the J v values / h
+    assert _rel_err(c, VEC[tag]) < 1e-12
+
+
+def test_tf2018_curvature_approximates_d2(oracle):
+    # test_tf_chebyshev_2018.cpp:19-33: 0 < h <= 0.5, ||C d - F''[d,d]|| small.
+    # (With the finite-difference J v the quotient is noise: each J v carries
+    # ~sqrt(eps) error and h ~ sqrt(eps) ||x|| / ||d||; the reference behaves
+    # the same, pinned bit for bit above.)
+    n, lam = 4, 1.0
+    x, d = _tf_input(n)
+    c, h = oracle.tf2018_curvature(n, lam, x, d, 0)
+    assert 0.0 < h <= 0.5
+    assert _rel_err(c, oracle.bratu_d2(n, lam, x, d)) < 5e-6
+
+
+def test_tf2018_curvature_of_a_linear_operator_is_zero(oracle):
+    # lambda = 0: J does not depend on x, so J(x + h d) d - J(x) d is exactly 0;
+    # d = 0 takes the step 2 eps (tf_chebyshev_2018.hpp:26)
+    n = 6
+    x, d = _tf_input(n)
+    c, h = oracle.tf2018_curvature(n, 0.0, x, d, 0)
+    assert np.all(c == 0.0) and 0.0 < h <= 0.5
+    c0, h0 = oracle.tf2018_curvature(n, 1.0, x, np.zeros(n ** 3), 0)
+    assert h0 == 2.0 * np.finfo(float).eps and np.all(c0 == 0.0)
+
+
+TF_SOLVES = [(4, 1.0, 1, 0), (8, 3.0, 1, 0), (15, 2.0, 2, 0), (8, 1.0, 1, 1), (7, 6.0, 0, 0)]
+
+
+@pytest.mark.parametrize("n,lam,pc,fd", TF_SOLVES)
+def test_tf2018_solves_match_reference(oracle, n, lam, pc, fd):
+    cfg = O.solver_config(method=O.METHOD_TF2018, precond=pc,
+                          jv_mode=O.DERIV_FD if fd else O.DERIV_ANALYTIC,
+                          d2_mode=O.DERIV_ANALYTIC, mg=O.mg_reference())
+    p = O.krylov_params(rtol_nl=1e-8, max_nl=20, gmres_restart=30, gmres_max_iters=300)
+    x, r = oracle.bratu_solve(n, lam, cfg, p)
+    tag = f"tf2018_n{n}_l{lam}_pc{pc}_fd{fd}"
+    c = VEC[tag + "_counts"]
+    got = (r.converged, r.iters, r.n_F, r.n_JVP, r.n_D2, r.n_lin_iters, r.n_lin_failures, r.n_prec)
+    assert got == tuple(int(v) for v in c)
+    assert rel(x, VEC[tag]) < 1e-12
+    if n == 4:  # test_tf_chebyshev_2018.cpp:36-48
+        assert r.converged and r.n_D2 == 0 and r.n_JVP > 0
