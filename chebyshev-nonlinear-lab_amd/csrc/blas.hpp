@@ -311,6 +311,23 @@ __global__ __launch_bounds__(kBlock) void k_scale_to(int64_t N, double s, const 
         out[i] = s * x[i];
 }
 
+// out = x / s
+__global__ __launch_bounds__(kBlock) void k_div_to(int64_t N, const double* x, double s,
+                                                   double* out) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock)
+        out[i] = x[i] / s;
+}
+
+// q_i = 1 + 0.1 sin(0.6180339887498948 (i + 1)): the inverse iteration's start
+// (analysis/spectral.hpp:51-55)
+__global__ __launch_bounds__(kBlock) void k_eig_start(int64_t N, double* q) {
+#pragma clang fp contract(off)
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * kBlock)
+        q[i] = 1.0 + 0.1 * sin(0.6180339887498948 * static_cast<double>(i + 1));
+}
+
 // out = s ((a - b) / h)  (difference quotient, then scaled; in place allowed)
 __global__ __launch_bounds__(kBlock) void k_diffq(int64_t N, const double* a, const double* b,
                                                   double h, double s, double* out) {
@@ -366,6 +383,16 @@ inline void launch_axpy_to(Ctx& c, int64_t N, const double* x, double a, const d
 inline void launch_scale_to(Ctx& c, int64_t N, double s, const double* x, double* out) {
     hipLaunchKernelGGL(k_scale_to, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, s, x,
                        out);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+inline void launch_div_to(Ctx& c, int64_t N, const double* x, double s, double* out) {
+    hipLaunchKernelGGL(k_div_to, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, x, s, out);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
+inline void launch_eig_start(Ctx& c, int64_t N, double* q) {
+    hipLaunchKernelGGL(k_eig_start, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, q);
     NUMA_HIP_LAUNCH_CHECK();
 }
 

@@ -122,6 +122,76 @@ public:
         collect_timers();
         timing_ = on;
     }
+    // analysis/spectral.hpp:34-115 on the device: GMRES on J(x) with the
+    // configured preconditioner, FP64 J v, dots and norms on device scalars.
+    numa_eigen_result eigen_smallest(const double* x, const numa_eigen_params& p,
+                                     double* vec) override {
+        require(sizeof(T) == 8, "the spectral estimator needs an FP64 solver");
+        numa_eigen_result r{};
+        r.eigenvalue = std::numeric_limits<double>::quiet_NaN();
+        r.residual_rel = std::numeric_limits<double>::infinity();
+        if (!(p.eigen_rtol > 0.0) || !(p.linear_rtol > 0.0) || p.max_outer <= 0 ||
+            p.restart <= 0 || p.max_linear <= 0)
+            return r;
+        require(p.restart <= m_, "eigen restart exceeds the solver's basis capacity");
+        point_setup(x);
+        norm(x, MISC(S_XNORM));
+        if (cfg_.jv_mode == NUMA_DERIV_FD) eval_F(x, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
+        double* q = w_;
+        double* y = z_;
+        double* jq = u_;
+        double* res = r_;
+        launch_eig_start(c_, N_, q);
+        norm(q, MISC(S_TMP));
+        const double n0 = read1(MISC(S_TMP));
+        if (!(n0 > 0.0) || !std::isfinite(n0)) return r;
+        launch_div_to(c_, N_, q, n0, q);
+        double previous = std::numeric_limits<double>::quiet_NaN();
+        for (int64_t outer = 0; outer < p.max_outer; ++outer) {
+            const auto lin = gmres(x, Fx_, q, y, p.linear_rtol, p.restart, p.max_linear, true,
+                                   nullptr, false);
+            r.linear_iterations += lin.iters;
+            r.linear_matvecs += lin.matvecs;
+            r.preconditioner_applications += lin.precs;
+            if (!lin.converged || lin.breakdown) {
+                ++r.linear_failures;
+                break;
+            }
+            norm(y, MISC(S_TMP));
+            const double yn = read1(MISC(S_TMP));
+            if (!(yn > std::numeric_limits<double>::min()) || !std::isfinite(yn)) {
+                ++r.linear_failures;
+                break;
+            }
+            launch_div_to(c_, N_, y, yn, q);
+            jv64(x, Fx_, MISC(S_XNORM), q, jq);
+            launch_dot<double, double>(c_, N_, q, jq,
+                                       Epilogue{EPI_STORE, 1.0, nullptr, MISC(S_TMP), nullptr});
+            const double theta = read1(MISC(S_TMP));
+            if (!std::isfinite(theta)) break;
+            launch_axpy_to(c_, N_, jq, -theta, q, res);  // J q - theta q
+            norm(res, MISC(S_TMP));
+            const double denom = std::max(1.0, std::abs(theta));
+            const double rr = read1(MISC(S_TMP)) / denom;
+            r.eigenvalue = theta;
+            r.residual_rel = rr;
+            r.outer_iterations = outer + 1;
+            if (vec) launch_convert<double, double>(c_, N_, q, vec);
+            const double change = std::isfinite(previous)
+                                      ? std::abs(theta - previous) / denom
+                                      : std::numeric_limits<double>::infinity();
+            if (rr <= p.eigen_rtol &&
+                (!std::isfinite(previous) || change <= 10.0 * p.eigen_rtol)) {
+                r.converged = 1;
+                break;
+            }
+            previous = theta;
+        }
+        c_.sync();
+        collect_timers();
+        return r;
+    }
+
     double tf2018_curvature(const double* x, const double* d, double* out) override {
         require(n_ > 0, "empty problem");
         const double h = tf_curvature(x, nullptr, d, out, 1.0, x2_, Fn_);
@@ -1098,6 +1168,15 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
     if (mg && mode_host && ichunk_host && rows_host)
         for (int64_t l = 0; l < mg->levels(); ++l)
             mg->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
+    NUMA_API_END
+}
+
+int numa_hip_eigen_smallest(numa_hip_solver* s, const double* x_dev, const numa_eigen_params* p,
+                            numa_eigen_result* out, double* vec_dev) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && x_dev && p && out, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *out = s->impl->eigen_smallest(x_dev, *p, vec_dev);
     NUMA_API_END
 }
 

@@ -59,6 +59,10 @@ struct SolverBase {
     // tensor_free_curvature_2018 at (x, d) into out (device FP64 vectors,
     // out distinct from x and d); returns the step h.
     virtual double tf2018_curvature(const double* x, const double* d, double* out) = 0;
+    // inverse_smallest_eigenpair of J(x) (analysis/spectral.hpp:34-115);
+    // vec (device, may be null) receives the last iterate.
+    virtual numa_eigen_result eigen_smallest(const double* x, const numa_eigen_params& p,
+                                             double* vec) = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

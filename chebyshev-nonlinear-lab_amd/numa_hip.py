@@ -78,6 +78,62 @@ class SolverConfig(C.Structure):
                 ("jv_mode", _i64), ("d2_mode", _i64), ("mg", MGConfig)]
 
 
+class EigenResult(C.Structure):
+    """numa::analysis::EigenEstimate (analysis/spectral.hpp:22-32) without the vector."""
+    _fields_ = [("converged", _i64), ("eigenvalue", C.c_double), ("residual_rel", C.c_double),
+                ("outer_iterations", _i64), ("linear_iterations", _i64),
+                ("linear_matvecs", _i64), ("preconditioner_applications", _i64),
+                ("linear_failures", _i64)]
+
+
+class EigenParams(C.Structure):
+    _fields_ = [("eigen_rtol", C.c_double), ("linear_rtol", C.c_double), ("max_outer", _i64),
+                ("restart", _i64), ("max_linear", _i64)]
+
+
+class ContinuationParams(C.Structure):
+    _fields_ = [("lambda_start", C.c_double), ("lambda_stop", C.c_double), ("step", C.c_double),
+                ("min_step", C.c_double), ("spectral_stop", C.c_double),
+                ("eigen_rtol", C.c_double), ("linear_rtol", C.c_double)]
+
+
+class ContinuationPoint(C.Structure):
+    """One CSV row of apps/calibrate_b6.cpp:128-137."""
+    _fields_ = [("lambda_", C.c_double), ("spectral_gap", C.c_double),
+                ("eigen_residual", C.c_double), ("continuation_iters", _i64), ("n_F", _i64),
+                ("n_JVP", _i64), ("n_prec", _i64)]
+
+
+def eigen_params(eigen_rtol=1e-7, linear_rtol=1e-10, max_outer=30, restart=100,
+                 max_linear=2000) -> EigenParams:
+    """inverse_smallest_eigenpair defaults (analysis/spectral.hpp:34-42)."""
+    return EigenParams(eigen_rtol, linear_rtol, max_outer, restart, max_linear)
+
+
+def continuation_params(**kw) -> ContinuationParams:
+    """calibrate_b6 defaults (apps/calibrate_b6.cpp:30-40)."""
+    d = dict(lambda_start=0.0, lambda_stop=9.8, step=0.25, min_step=1e-4, spectral_stop=5e-2,
+             eigen_rtol=1e-7, linear_rtol=1e-10)
+    d.update(kw)
+    return ContinuationParams(**d)
+
+
+def corrector_params() -> "KrylovParams":
+    """Newton-Krylov parameters of the continuation corrector (calibrate_b6.cpp:181-187)."""
+    return krylov_params(rtol_nl=1e-11, max_nl=100, eta_init=1e-8, eta_max=1e-6,
+                         gmres_restart=100, gmres_max_iters=2000)
+
+
+FOLD_CSV_HEADER = "lambda,spectral_gap,eigen_residual,continuation_iters,n_F,n_JVP,n_prec"
+
+
+def fold_csv_line(p: "ContinuationPoint") -> str:
+    """A calibrate_b6 CSV row (calibrate_b6.cpp:128-137: default stream format,
+    6 significant digits)."""
+    return "%g,%g,%g,%d,%d,%d,%d" % (p.lambda_, p.spectral_gap, p.eigen_residual,
+                                     p.continuation_iters, p.n_F, p.n_JVP, p.n_prec)
+
+
 class Timers(C.Structure):
     """numa::op::Timers (operators/timing.hpp:16-24): seconds per primitive."""
     _fields_ = [("t_F", C.c_double), ("t_JVP", C.c_double), ("t_D2", C.c_double),
@@ -216,6 +272,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_mg_set_resid_restrict": [_vp, _i64],
         "numa_hip_solver_set_graphs": [_vp, _i64],
         "numa_hip_tf2018_curvature": [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)],
+        "numa_hip_eigen_smallest": [_vp, _vp, C.POINTER(EigenParams), C.POINTER(EigenResult),
+                                    _vp],
+        "numa_hip_continuation": [_vp, _i64, C.POINTER(SolverConfig),
+                                  C.POINTER(ContinuationParams), C.POINTER(KrylovParams),
+                                  C.POINTER(ContinuationPoint), _i64, C.POINTER(_i64),
+                                  C.POINTER(_i64), C.POINTER(C.c_double), _vp],
         "numa_hip_solver_set_timing": [_vp, _i64],
         "numa_hip_solver_timers": [_vp, C.POINTER(Timers), _i64],
         "numa_hip_solver_graphs_cached": [_vp, C.POINTER(_i64)],
@@ -473,6 +535,15 @@ class BratuSolver:
     def set_graphs(self, on: bool = True) -> None:
         check(self.ctx.lib.numa_hip_solver_set_graphs(self.h, 1 if on else 0))
 
+    def eigen_smallest(self, x: "DeviceArray", params: EigenParams | None = None,
+                       vec: "DeviceArray | None" = None) -> EigenResult:
+        """inverse_smallest_eigenpair (analysis/spectral.hpp:34-115) of J(x)."""
+        r = EigenResult()
+        p = params if params is not None else eigen_params()
+        check(self.ctx.lib.numa_hip_eigen_smallest(self.h, x.ptr, C.byref(p), C.byref(r),
+                                                   vec.ptr if vec is not None else None))
+        return r
+
     def tf2018_curvature(self, x: "DeviceArray", d: "DeviceArray", out: "DeviceArray") -> float:
         """tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) on device
         FP64 vectors; returns the step h."""
@@ -540,6 +611,25 @@ def chebyshev_inexact(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restar
     r = s.solve_host(params, x)
     s.close()
     return x, r
+
+
+def continuation(ctx, n: int, cfg=None, cp: ContinuationParams | None = None, nk=None,
+                 max_points: int = 512, x_out: "DeviceArray | None" = None):
+    """apps/calibrate_b6.cpp:173-298 on the device: returns (points, status,
+    fold proxy); status 0 ok, 3 corrector failed below min_step, 4 spectral
+    estimate failed.  cfg defaults to calibrate_b6's solver (analytic J v,
+    PoissonMG(n))."""
+    if cfg is None:
+        cfg = solver_config(precond=PRECOND_GMG, jv_mode=DERIV_ANALYTIC, mg=mg_reference())
+    cp = cp if cp is not None else continuation_params()
+    nk = nk if nk is not None else corrector_params()
+    pts = (ContinuationPoint * max_points)()
+    count, status, proxy = _i64(), _i64(), C.c_double()
+    check(ctx.lib.numa_hip_continuation(ctx.h, int(n), C.byref(cfg), C.byref(cp), C.byref(nk),
+                                        pts, max_points, C.byref(count), C.byref(status),
+                                        C.byref(proxy),
+                                        x_out.ptr if x_out is not None else None))
+    return list(pts[:min(count.value, max_points)]), int(status.value), proxy.value
 
 
 def tf_chebyshev_2018(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restart=None):

@@ -17,6 +17,9 @@
 //   numa::hip::newton_krylov / chebyshev_inexact  krylov_newton.hpp:69-189
 //   numa::hip::tf_chebyshev_2018 / tensor_free_curvature_2018
 //                               methods/tf_chebyshev_2018.hpp:23-132
+//   numa::hip::EigenEstimate / inverse_smallest_eigenpair
+//                               analysis/spectral.hpp:22-115 (apply = J(x) v,
+//                               precond = the provider's preconditioner)
 //   numa::hip::gmres            linalg/gmres.hpp:61-65 (apply = J(x0) v)
 // The solves run entirely in HBM; host vectors are staged once per call.
 // Misuse throws std::invalid_argument / std::overflow_error as the reference
@@ -323,6 +326,56 @@ inline void tensor_free_curvature_2018(const Bratu3D& f, const DVec& x, const DV
     check(numa_hip_tf2018_curvature(s, dx.data(), dd.data(), dout.data(), h_used));
     out.resize(N);
     dout.download(out);
+}
+
+// analysis/spectral.hpp:22-32
+struct EigenEstimate {
+    bool converged = false;
+    double eigenvalue = 0.0;
+    double residual_rel = 0.0;
+    std::size_t outer_iterations = 0;
+    std::size_t linear_iterations = 0;
+    std::size_t linear_matvecs = 0;
+    std::size_t preconditioner_applications = 0;
+    std::size_t linear_failures = 0;
+    DVec vector;
+};
+
+// analysis/spectral.hpp:34-42 with apply(v, out) = J(x) v (the operator's J v,
+// or o.jv_mode's) and the provider's preconditioner; FP64.
+template <class PrecondProvider = NoPrecond>
+EigenEstimate inverse_smallest_eigenpair(const Bratu3D& f, const DVec& x,
+                                         const PrecondProvider& pc = {},
+                                         double eigen_rtol = 1e-7, double linear_rtol = 1e-10,
+                                         std::size_t max_outer = 30, std::size_t restart = 100,
+                                         std::size_t max_linear = 2000,
+                                         const SolverOptions& o = {}) {
+    const std::size_t N = f.size();
+    if (x.size() != N) throw std::invalid_argument("x has the wrong size");
+    numa_solver_config cfg = pc.apply(detail::base_config(NUMA_METHOD_NEWTON, o));
+    cfg.precision = NUMA_PRECISION_FP64;
+    cfg.flexible = 0;
+    numa_hip_solver* s = nullptr;
+    check(numa_hip_bratu_solver_create(f.context().get(), (int64_t)f.grid(), f.lambda(), &cfg,
+                                       (int64_t)std::max<std::size_t>(1, restart), &s));
+    std::unique_ptr<numa_hip_solver, int (*)(numa_hip_solver*)> guard(s, numa_hip_solver_destroy);
+    DeviceVec dx(f.context(), N), dv(f.context(), N);
+    dx.upload(x);
+    const numa_eigen_params p{eigen_rtol, linear_rtol, (int64_t)max_outer, (int64_t)restart,
+                              (int64_t)max_linear};
+    numa_eigen_result r{};
+    check(numa_hip_eigen_smallest(s, dx.data(), &p, &r, dv.data()));
+    EigenEstimate e;
+    e.converged = r.converged != 0;
+    e.eigenvalue = r.eigenvalue;
+    e.residual_rel = r.residual_rel;
+    e.outer_iterations = (std::size_t)r.outer_iterations;
+    e.linear_iterations = (std::size_t)r.linear_iterations;
+    e.linear_matvecs = (std::size_t)r.linear_matvecs;
+    e.preconditioner_applications = (std::size_t)r.preconditioner_applications;
+    e.linear_failures = (std::size_t)r.linear_failures;
+    if (r.outer_iterations > 0) dv.download(e.vector);
+    return e;
 }
 
 // linalg/gmres.hpp:61-65 with apply(v, out) = J(x0) v and the provider's

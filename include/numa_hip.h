@@ -255,6 +255,81 @@ int numa_hip_solve(numa_hip_solver* s, const numa_krylov_params* p,
 int numa_hip_solve_host(numa_hip_solver* s, const numa_krylov_params* p,
                         double* x_host, numa_krylov_result* out);
 
+/* ---- spectral estimator and continuation (C1 fold calibration) ---------- */
+/* Mirrors numa::analysis::EigenEstimate (analysis/spectral.hpp:22-32); the
+ * vector is returned separately. */
+typedef struct numa_eigen_result {
+    int64_t converged;
+    double eigenvalue;
+    double residual_rel;
+    int64_t outer_iterations;
+    int64_t linear_iterations;
+    int64_t linear_matvecs;
+    int64_t preconditioner_applications;
+    int64_t linear_failures;
+} numa_eigen_result;
+
+/* Arguments of inverse_smallest_eigenpair (analysis/spectral.hpp:34-42):
+ * defaults 1e-7, 1e-10, 30, 100, 2000. */
+typedef struct numa_eigen_params {
+    double eigen_rtol;
+    double linear_rtol;
+    int64_t max_outer;
+    int64_t restart;
+    int64_t max_linear;
+} numa_eigen_params;
+
+/* Smallest eigenpair of the Bratu Jacobian J(x) by inverse iteration with
+ * GMRES preconditioned by the solver's preconditioner
+ * (inverse_smallest_eigenpair, analysis/spectral.hpp:34-115: start
+ * q_i = 1 + 0.1 sin(0.6180339887498948 (i+1)), Rayleigh quotient, residual
+ * ||J q - theta q|| / max(1, |theta|)).  J v is the solver's jv_mode in FP64;
+ * the solver must be FP64 with a basis of at least p->restart vectors.
+ * x: device FP64; vec_dev (may be NULL): the last iterate q. */
+int numa_hip_eigen_smallest(numa_hip_solver* s, const double* x_dev, const numa_eigen_params* p,
+                            numa_eigen_result* out, double* vec_dev);
+
+/* Continuation arguments of apps/calibrate_b6.cpp:30-40 (defaults: 0, 9.8,
+ * 0.25, 1e-4, 5e-2, 1e-7, 1e-10). */
+typedef struct numa_continuation_params {
+    double lambda_start;
+    double lambda_stop;
+    double step;
+    double min_step;
+    double spectral_stop;
+    double eigen_rtol;
+    double linear_rtol;
+} numa_continuation_params;
+
+/* One accepted point (the CSV row of calibrate_b6.cpp:128-137). */
+typedef struct numa_continuation_point {
+    double lambda;
+    double spectral_gap;
+    double eigen_residual;
+    int64_t continuation_iters;
+    int64_t n_F;
+    int64_t n_JVP;
+    int64_t n_prec;
+} numa_continuation_point;
+
+/* Natural-parameter continuation of the lower Bratu branch with a spectral
+ * stop (apps/calibrate_b6.cpp:173-298): Newton-Krylov corrector from the
+ * previous point (nk: the corrector's parameters, calibrate_b6.cpp:181-187:
+ * rtol_nl 1e-11, max_nl 100, eta 1e-8..1e-6, GMRES(100), 2000), step halved
+ * on corrector failure down to min_step, grown by 1.5 after easy points, the
+ * smallest eigenvalue of J at every accepted point (30 outer, GMRES(100),
+ * 2000), stop when it drops to spectral_stop.  cfg: the solver (FP64;
+ * calibrate_b6 uses the analytic J v and PoissonMG(n), i.e. NUMA_PRECOND_GMG
+ * with the reference multigrid).  Writes up to max_points points, *count,
+ * *status (0 ok; 3 corrector failed below min_step; 4 spectral estimate
+ * failed, the app's exit codes), *fold_proxy (the gap^2 linear fit over the
+ * last 4 points, calibrate_b6.cpp:139-171; NaN if not admissible) and, if
+ * x_out_dev is not NULL, the last accepted solution. */
+int numa_hip_continuation(numa_hip_ctx* ctx, int64_t n, const numa_solver_config* cfg,
+                          const numa_continuation_params* cp, const numa_krylov_params* nk,
+                          numa_continuation_point* points, int64_t max_points, int64_t* count,
+                          int64_t* status, double* fold_proxy, double* x_out_dev);
+
 /* ---- Chandrasekhar H-equation, batch of parameters c ---------------------- */
 /* problems/chandrasekhar_op.hpp:31-136 for `batch` values of c at once:
  * (A x)_i = c sum_j 0.5 w_j mu_i/(mu_i+mu_j) x_j with rule 0 = Gauss-Legendre

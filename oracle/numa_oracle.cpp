@@ -20,6 +20,10 @@
 //       brute force, breakdown cases; FGMRES == GMRES for linear M.
 //   newton_krylov / chebyshev      - against _ref counts and iterates; the
 //       reference-held continuation norms (B1_start_hashes.csv).
+//   smallest eigenpair             - against _ref (inverse_smallest_eigenpair)
+//       and the closed form 12 (n+1)^2 sin^2(pi / (2(n+1))) at lambda = 0.
+//   continuation (calibrate_b6)    - the reference's recorded fold rows
+//       (fold_calibration/bratu_n31_coarse.csv): gaps and counts.
 //   tf2018 curvature / solve       - against _ref (tensor_free_curvature_2018,
 //       tf_chebyshev_2018) counts and iterates; curvature vs analytic F''
 //       (test_tf_chebyshev_2018.cpp:19-33); exact quadratic: for F quadratic
@@ -842,6 +846,64 @@ void tf2018_solve(const Problem& P, DVec& x, const numa_krylov_params& p, bool f
     out.t_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// ---------------------------------------------------------------------------
+// Smallest eigenpair by inverse iteration.  analysis/spectral.hpp:34-115.
+// ---------------------------------------------------------------------------
+numa_eigen_result inverse_smallest_eigenpair(const ApplyFn& apply, const ApplyFn& precond,
+                                             size_t n, const numa_eigen_params& p, DVec* vec) {
+    numa_eigen_result r{};
+    r.eigenvalue = std::numeric_limits<double>::quiet_NaN();
+    r.residual_rel = std::numeric_limits<double>::infinity();
+    if (n == 0 || !(p.eigen_rtol > 0.0) || !(p.linear_rtol > 0.0) || p.max_outer <= 0 ||
+        p.restart <= 0 || p.max_linear <= 0)
+        return r;
+    DVec q(n);
+    for (size_t i = 0; i < n; ++i)
+        q[i] = 1.0 + 0.1 * std::sin(0.6180339887498948 * static_cast<double>(i + 1));
+    const double n0 = norm2(q);
+    if (!(n0 > 0.0) || !std::isfinite(n0)) return r;
+    for (double& v : q) v /= n0;
+    double previous = std::numeric_limits<double>::quiet_NaN();
+    DVec y(n), jq(n), res(n);
+    for (int64_t outer = 0; outer < p.max_outer; ++outer) {
+        y.assign(n, 0.0);
+        const auto lin = gmres(apply, precond, q, y, p.linear_rtol, (size_t)p.restart,
+                               (size_t)p.max_linear, true, false);
+        r.linear_iterations += lin.iters;
+        r.linear_matvecs += lin.matvecs;
+        r.preconditioner_applications += lin.precs;
+        if (!lin.converged || lin.breakdown) {
+            ++r.linear_failures;
+            return r;
+        }
+        const double yn = norm2(y);
+        if (!(yn > std::numeric_limits<double>::min()) || !std::isfinite(yn)) {
+            ++r.linear_failures;
+            return r;
+        }
+        for (size_t i = 0; i < n; ++i) q[i] = y[i] / yn;
+        apply(q, jq);
+        const double theta = dot(q, jq);
+        if (!std::isfinite(theta)) return r;
+        res = jq;
+        axpy(-theta, q, res);
+        const double denom = std::max(1.0, std::abs(theta));
+        const double rr = norm2(res) / denom;
+        r.eigenvalue = theta;
+        r.residual_rel = rr;
+        r.outer_iterations = outer + 1;
+        if (vec) *vec = q;
+        const double change = std::isfinite(previous) ? std::abs(theta - previous) / denom
+                                                      : std::numeric_limits<double>::infinity();
+        if (rr <= p.eigen_rtol && (!std::isfinite(previous) || change <= 10.0 * p.eigen_rtol)) {
+            r.converged = 1;
+            return r;
+        }
+        previous = theta;
+    }
+    return r;
+}
+
 ApplyFn make_bratu_precond(const Bratu& f, const numa_solver_config& cfg,
                            std::shared_ptr<MG> mg, const DVec& x) {
     if (cfg.precond == NUMA_PRECOND_GMG) {
@@ -1119,6 +1181,99 @@ int orc_bratu_solve(int64_t n, double lambda, const numa_solver_config* cfg,
     else
         outer_solve(P, X, *p, cfg->method == NUMA_METHOD_CHEBYSHEV, cfg->flexible != 0, *out);
     std::copy(X.begin(), X.end(), x);
+    return NUMA_OK;
+}
+
+int orc_bratu_eigen_smallest(int64_t n, double lambda, const numa_solver_config* cfg,
+                             const double* x, const numa_eigen_params* p, numa_eigen_result* out,
+                             double* vec) {
+    if (n <= 0 || !cfg || !x || !p || !out) return NUMA_ERR_ARG;
+    Bratu f((long)n, lambda);
+    Problem P = bratu_problem(f, *cfg);
+    const DVec X(x, x + f.size());
+    DVec Fx(f.size());
+    if (P.jvp_needs_F) P.eval(X, Fx);
+    const double xn = norm2(X);
+    ApplyFn apply = [&](const DVec& v, DVec& o) { P.jvp(X, Fx, xn, v, o); };
+    DVec q;
+    *out = inverse_smallest_eigenpair(apply, P.make_precond(X), f.size(), *p, &q);
+    if (vec && q.size() == (size_t)f.size()) std::copy(q.begin(), q.end(), vec);
+    return NUMA_OK;
+}
+
+// Natural-parameter continuation with the spectral stop, apps/calibrate_b6.cpp:
+// 139-171 (fold proxy) and 173-298 (loop).
+int orc_bratu_continuation(int64_t n, const numa_solver_config* cfg,
+                           const numa_continuation_params* cp, const numa_krylov_params* nk,
+                           numa_continuation_point* points, int64_t max_points, int64_t* count,
+                           int64_t* status, double* fold_proxy, double* x_out) {
+    if (n <= 0 || !cfg || !cp || !nk || !count || !status || !fold_proxy) return NUMA_ERR_ARG;
+    const size_t N = (size_t)(n * n * n);
+    DVec previous(N, 0.0);
+    double lam = cp->lambda_start, step = cp->step;
+    std::vector<numa_continuation_point> acc;
+    *status = 0;
+    // one multigrid hierarchy for every lambda (PoissonMGProvider, :179)
+    numa_solver_config c = *cfg;
+    std::shared_ptr<MG> mg;
+    if (c.precond == NUMA_PRECOND_GMG) mg = std::make_shared<MG>((long)n, c.mg);
+    while (lam <= cp->lambda_stop + 1e-14) {
+        Bratu f((long)n, lam);
+        Problem P = bratu_problem(f, c);
+        if (mg) P.make_precond = [&f, c, mg](const DVec& x) { return make_bratu_precond(f, c, mg, x); };
+        DVec x = previous;
+        numa_krylov_result sol{};
+        outer_solve(P, x, *nk, false, c.flexible != 0, sol);
+        if (!sol.converged) {
+            step *= 0.5;
+            if (step < cp->min_step) {
+                *status = 3;
+                break;
+            }
+            lam = acc.empty() ? cp->lambda_start : acc.back().lambda + step;
+            continue;
+        }
+        DVec Fx(N);
+        if (P.jvp_needs_F) P.eval(x, Fx);
+        const double xn = norm2(x);
+        ApplyFn apply = [&](const DVec& v, DVec& o) { P.jvp(x, Fx, xn, v, o); };
+        const numa_eigen_params ep{cp->eigen_rtol, cp->linear_rtol, 30, 100, 2000};
+        const numa_eigen_result eig =
+            inverse_smallest_eigenpair(apply, P.make_precond(x), N, ep, nullptr);
+        if (!eig.converged || !std::isfinite(eig.eigenvalue) || eig.eigenvalue <= 0.0) {
+            *status = 4;
+            break;
+        }
+        acc.push_back({lam, eig.eigenvalue, eig.residual_rel, sol.iters, sol.n_F,
+                       sol.n_JVP + eig.linear_matvecs, sol.n_prec + eig.preconditioner_applications});
+        previous = x;
+        if (eig.eigenvalue <= cp->spectral_stop) break;
+        if (sol.iters <= 5 && step < cp->step) step = std::min(cp->step, 1.5 * step);
+        lam += step;
+    }
+    *count = (int64_t)acc.size();
+    for (size_t i = 0; i < acc.size() && (int64_t)i < max_points; ++i) points[i] = acc[i];
+    // fold proxy: linear fit of gap^2 over the last four points (:139-171)
+    *fold_proxy = std::numeric_limits<double>::quiet_NaN();
+    if (acc.size() >= 4) {
+        const size_t first = acc.size() - 4;
+        double sx = 0.0, sy = 0.0, sxx = 0.0, sxy = 0.0;
+        for (size_t i = first; i < acc.size(); ++i) {
+            const double xv = acc[i].lambda, yv = acc[i].spectral_gap * acc[i].spectral_gap;
+            sx += xv;
+            sy += yv;
+            sxx += xv * xv;
+            sxy += xv * yv;
+        }
+        const double cnt = 4.0, den = cnt * sxx - sx * sx;
+        if (std::abs(den) >= 1e-14) {
+            const double slope = (cnt * sxy - sx * sy) / den;
+            const double icpt = (sy - slope * sx) / cnt;
+            const double root = -icpt / slope;
+            if (slope < 0.0 && std::isfinite(root) && root > acc.back().lambda) *fold_proxy = root;
+        }
+    }
+    if (x_out) std::copy(previous.begin(), previous.end(), x_out);
     return NUMA_OK;
 }
 

@@ -55,6 +55,16 @@ int orc_poisson_gmres(int64_t n, const numa_mg_config* mg, int64_t use_mg,
  * or tensor-free Chebyshev 2018 (methods/tf_chebyshev_2018.hpp), cfg->method */
 int orc_bratu_solve(int64_t n, double lambda, const numa_solver_config* cfg,
                     const numa_krylov_params* p, double* x, numa_krylov_result* out);
+/* inverse_smallest_eigenpair (analysis/spectral.hpp:34-115) of J(x) with the
+ * preconditioner of cfg; vec (may be NULL) receives the last iterate. */
+int orc_bratu_eigen_smallest(int64_t n, double lambda, const numa_solver_config* cfg,
+                             const double* x, const numa_eigen_params* p, numa_eigen_result* out,
+                             double* vec);
+/* calibrate_b6 continuation (apps/calibrate_b6.cpp:139-298). */
+int orc_bratu_continuation(int64_t n, const numa_solver_config* cfg,
+                           const numa_continuation_params* cp, const numa_krylov_params* nk,
+                           numa_continuation_point* points, int64_t max_points, int64_t* count,
+                           int64_t* status, double* fold_proxy, double* x_out);
 /* tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) on Bratu3D with
  * the J v of jv_mode (NUMA_DERIV_ANALYTIC or NUMA_DERIV_FD). */
 int orc_tf2018_curvature(int64_t n, double lambda, int64_t jv_mode, const double* x,

@@ -42,6 +42,50 @@ class GmresResult(C.Structure):
                 ("converged", _i64), ("breakdown", _i64)]
 
 
+class EigenResult(C.Structure):
+    _fields_ = [("converged", _i64), ("eigenvalue", C.c_double), ("residual_rel", C.c_double),
+                ("outer_iterations", _i64), ("linear_iterations", _i64),
+                ("linear_matvecs", _i64), ("preconditioner_applications", _i64),
+                ("linear_failures", _i64)]
+
+
+class EigenParams(C.Structure):
+    _fields_ = [("eigen_rtol", C.c_double), ("linear_rtol", C.c_double), ("max_outer", _i64),
+                ("restart", _i64), ("max_linear", _i64)]
+
+
+class ContinuationParams(C.Structure):
+    _fields_ = [("lambda_start", C.c_double), ("lambda_stop", C.c_double), ("step", C.c_double),
+                ("min_step", C.c_double), ("spectral_stop", C.c_double),
+                ("eigen_rtol", C.c_double), ("linear_rtol", C.c_double)]
+
+
+class ContinuationPoint(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("spectral_gap", C.c_double),
+                ("eigen_residual", C.c_double), ("continuation_iters", _i64), ("n_F", _i64),
+                ("n_JVP", _i64), ("n_prec", _i64)]
+
+
+def eigen_params(eigen_rtol=1e-7, linear_rtol=1e-10, max_outer=30, restart=100,
+                 max_linear=2000) -> EigenParams:
+    """analysis/spectral.hpp:34-42 defaults."""
+    return EigenParams(eigen_rtol, linear_rtol, max_outer, restart, max_linear)
+
+
+def continuation_params(**kw) -> ContinuationParams:
+    """apps/calibrate_b6.cpp:30-40 defaults."""
+    d = dict(lambda_start=0.0, lambda_stop=9.8, step=0.25, min_step=1e-4, spectral_stop=5e-2,
+             eigen_rtol=1e-7, linear_rtol=1e-10)
+    d.update(kw)
+    return ContinuationParams(**d)
+
+
+def corrector_params() -> "KrylovParams":
+    """The continuation corrector's Newton-Krylov parameters (calibrate_b6.cpp:181-187)."""
+    return krylov_params(rtol_nl=1e-11, max_nl=100, eta_init=1e-8, eta_max=1e-6,
+                         gmres_restart=100, gmres_max_iters=2000)
+
+
 class MGConfig(C.Structure):
     _fields_ = [("smoother", _i64), ("pre", _i64), ("post", _i64), ("degree", _i64),
                 ("max_levels", _i64), ("min_coarse_n", _i64), ("coarse_iters", _i64),
@@ -128,6 +172,15 @@ class Oracle:
         L.orc_bratu_solve.argtypes = [_i64, C.c_double, C.POINTER(SolverConfig),
                                       C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)]
         L.orc_tf2018_curvature.argtypes = [_i64, C.c_double, _i64, _dp, _dp, _dp, _dp]
+        L.orc_bratu_eigen_smallest.argtypes = [_i64, C.c_double, C.POINTER(SolverConfig), _dp,
+                                               C.POINTER(EigenParams), C.POINTER(EigenResult),
+                                               _dp]
+        L.orc_bratu_continuation.argtypes = [_i64, C.POINTER(SolverConfig),
+                                             C.POINTER(ContinuationParams),
+                                             C.POINTER(KrylovParams),
+                                             C.POINTER(ContinuationPoint), _i64,
+                                             C.POINTER(_i64), C.POINTER(_i64),
+                                             C.POINTER(C.c_double), _dp]
         L.orc_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
         L.orc_chandra_eval.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp]
         L.orc_chandra_jvp.argtypes = [_i64, C.c_double, _i64, _i64, _dp, _dp, _dp]
@@ -221,6 +274,26 @@ class Oracle:
         assert rc == 0
         return x, res
 
+    def bratu_eigen_smallest(self, n, lam, cfg, x, params=None):
+        """inverse_smallest_eigenpair (analysis/spectral.hpp:34-115) of J(x)."""
+        x = np.ascontiguousarray(x, np.float64)
+        p = params if params is not None else eigen_params()
+        res, vec = EigenResult(), np.zeros(n ** 3)
+        assert self.lib.orc_bratu_eigen_smallest(n, lam, C.byref(cfg), _ptr(x), C.byref(p),
+                                                 C.byref(res), _ptr(vec)) == 0
+        return res, vec
+
+    def bratu_continuation(self, n, cfg, cp, nk=None, max_points=512):
+        """apps/calibrate_b6.cpp:173-298; returns (points, status, fold proxy, last x)."""
+        nk = nk if nk is not None else corrector_params()
+        pts = (ContinuationPoint * max_points)()
+        count, status, proxy = _i64(), _i64(), C.c_double()
+        x = np.zeros(n ** 3)
+        assert self.lib.orc_bratu_continuation(n, C.byref(cfg), C.byref(cp), C.byref(nk), pts,
+                                               max_points, C.byref(count), C.byref(status),
+                                               C.byref(proxy), _ptr(x)) == 0
+        return list(pts[:min(count.value, max_points)]), int(status.value), proxy.value, x
+
     def tf2018_curvature(self, n, lam, x, d, fd=0):
         """tensor_free_curvature_2018 (tf_chebyshev_2018.hpp:31-51) with the
         analytic (fd = 0) or finite-difference (fd = 1) J v; returns (C d, h)."""
@@ -291,6 +364,8 @@ class Reference:
                                       C.c_double, _i64, _i64, _dp]
         L.ref_bratu_solve.argtypes = [_i64, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.ref_tf2018_curvature.argtypes = [_i64, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+        L.ref_bratu_eigen_smallest.argtypes = [_i64, C.c_double, _dp, C.c_double, C.c_double,
+                                               _i64, _i64, _i64, _dp, _dp]
         L.ref_gauss_legendre_unit.argtypes = [_i64, _dp, _dp]
         L.ref_chandra_eval.argtypes = [_i64, C.c_double, _dp, _dp]
         L.ref_chandra_jvp.argtypes = [_i64, C.c_double, _dp, _dp, _dp]
@@ -332,6 +407,18 @@ class Reference:
         res = np.zeros(10)
         self.lib.ref_bratu_solve(n, lam, method, precond, fd, _ptr(pv), _ptr(x), _ptr(res))
         return x, _result_dict(res)
+
+    def bratu_eigen_smallest(self, n, lam, x, eigen_rtol=1e-7, linear_rtol=1e-10, max_outer=30,
+                             restart=100, max_linear=2000):
+        x = np.ascontiguousarray(x, np.float64)
+        res, vec = np.zeros(8), np.zeros(n ** 3)
+        self.lib.ref_bratu_eigen_smallest(n, lam, _ptr(x), eigen_rtol, linear_rtol, max_outer,
+                                          restart, max_linear, _ptr(res), _ptr(vec))
+        return dict(converged=bool(res[0]), eigenvalue=float(res[1]),
+                    residual_rel=float(res[2]), outer_iterations=int(res[3]),
+                    linear_iterations=int(res[4]), linear_matvecs=int(res[5]),
+                    preconditioner_applications=int(res[6]),
+                    linear_failures=int(res[7])), vec
 
     def tf2018_curvature(self, n, lam, x, d, fd=0):
         x = np.ascontiguousarray(x, np.float64)

@@ -6,6 +6,7 @@
 // (oracle/numa_oracle.cpp) against the reference's own arithmetic and to write
 // the fixtures under tests/golden/.  Nothing here is a product path.
 
+#include <numa/analysis/spectral.hpp>
 #include <numa/functions/gauss_legendre_dynamic.hpp>
 #include <numa/linalg/gmres.hpp>
 #include <numa/linalg/poisson_mg.hpp>
@@ -167,6 +168,30 @@ void ref_bratu_gmres(int64_t n, double lambda, int fd, int precond, const double
     res[3] = g.rel_res;
     res[4] = g.converged;
     res[5] = g.breakdown;
+}
+
+// inverse_smallest_eigenpair of the analytic J(x) preconditioned by
+// PoissonMG(n), as calibrate_b6.cpp:240-248 calls it.  res: converged,
+// eigenvalue, residual_rel, outer, linear iterations, matvecs, precs, failures.
+void ref_bratu_eigen_smallest(int64_t n, double lambda, const double* x, double eigen_rtol,
+                              double linear_rtol, int64_t max_outer, int64_t restart,
+                              int64_t max_linear, double* res, double* vec_out) {
+    Bratu3D f((std::size_t)n, lambda);
+    const DVec X = vec(x, n * n * n);
+    PoissonMG mg((std::size_t)n);
+    auto apply = [&](const DVec& v, DVec& o) { f.jvp(X, v, o); };
+    const auto e = numa::analysis::inverse_smallest_eigenpair(
+        apply, mg, f.size(), eigen_rtol, linear_rtol, (std::size_t)max_outer,
+        (std::size_t)restart, (std::size_t)max_linear);
+    res[0] = e.converged;
+    res[1] = e.eigenvalue;
+    res[2] = e.residual_rel;
+    res[3] = (double)e.outer_iterations;
+    res[4] = (double)e.linear_iterations;
+    res[5] = (double)e.linear_matvecs;
+    res[6] = (double)e.preconditioner_applications;
+    res[7] = (double)e.linear_failures;
+    if (e.vector.size() == f.size()) put(e.vector, vec_out);
 }
 
 // tensor_free_curvature_2018 with the analytic (fd = 0) or FD (fd = 1) J v.

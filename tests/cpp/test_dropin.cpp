@@ -134,6 +134,14 @@ int main() {
         const auto r = tf_chebyshev_2018(f, DVec(f.size(), 0.0), p, BratuJacobi{f});
         CHECK(r.converged && r.n_D2 == 0 && r.n_JVP > 0);
     }
+    {  // analysis/spectral.hpp: smallest eigenvalue of J at lambda = 0 is
+       // 12 (n+1)^2 sin^2(pi / (2(n+1))) (the fold CSVs' first gap, 29.585 at n = 31)
+        Bratu3D f(ctx, 31, 0.0);
+        const auto e = inverse_smallest_eigenpair(f, DVec(f.size(), 0.0), PoissonMGProvider(31));
+        const double s = std::sin(3.14159265358979323846 / 64.0);
+        CHECK(e.converged && std::abs(e.eigenvalue - 12.0 * 1024.0 * s * s) < 1e-7);
+        CHECK(e.vector.size() == f.size() && e.residual_rel <= 1e-7);
+    }
     {  // misuse is reported as std::invalid_argument
         bool thrown = false;
         try {

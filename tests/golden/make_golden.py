@@ -7,7 +7,8 @@ Two kinds of fixtures:
   * reference-held data: numbers copied from the reference's own data files
     (experiments/c1/internal_calibration/internal_calibration_raw.csv,
     experiments/c1/starts/frozen/B1_start_hashes.csv,
-    experiments/c1/fold_calibration/bratu_n*_coarse.csv,
+    experiments/c1/fold_calibration/bratu_n*_coarse.csv (spectral gaps, and every
+    row of the n = 31 continuation),
     docs/research/R1_OPERATOR_INTERFACE.md:113-121,
     tests/test_chandrasekhar_op.cpp:106-115);
   * vectors computed by the reference's own header library compiled into
@@ -89,6 +90,21 @@ def start_norms():
     return out
 
 
+def fold_rows(n: int = 31, kind: str = "coarse"):
+    """experiments/c1/fold_calibration/bratu_n{n}_{kind}.csv: every row (the
+    calibrate_b6 continuation, apps/calibrate_b6.cpp:128-137 format)."""
+    path = f"{REF}/experiments/c1/fold_calibration/bratu_n{n}_{kind}.csv"
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            out.append({"lambda": float(r["lambda"]), "spectral_gap": float(r["spectral_gap"]),
+                        "eigen_residual": float(r["eigen_residual"]),
+                        "continuation_iters": int(r["continuation_iters"]),
+                        "n_F": int(r["n_F"]), "n_JVP": int(r["n_JVP"]),
+                        "n_prec": int(r["n_prec"]), "mode": r["continuation_mode"]})
+    return out
+
+
 def spectral_gaps():
     out = []
     for n in (31, 63, 127):
@@ -115,6 +131,7 @@ def main():
         "calibration_n31": calibration_rows(),
         "start_norms": start_norms(),
         "spectral_gaps": spectral_gaps(),
+        "fold_n31_coarse": fold_rows(31, "coarse"),
         # docs/research/R1_OPERATOR_INTERFACE.md:113-121 (n = 16, lambda = 1, u0 = 0)
         "r1_smoke_n16": {
             "mg": {"nk": [4, 5, 11, 0, 15], "cheb": [4, 5, 20, 4, 28], "rec": [4, 5, 18, 4, 30]},
@@ -177,6 +194,15 @@ def main():
         arrays[tag + "_counts"] = np.array([r["converged"], r["iters"], r["n_F"], r["n_JVP"],
                                             r["n_D2"], r["n_lin_iters"], r["n_lin_failures"],
                                             r["n_prec"]], np.float64)
+    # inverse_smallest_eigenpair of J(x) with PoissonMG(n) (calibrate_b6.cpp:240-248)
+    for n, lam in ((15, 3.0), (7, 0.0)):
+        e, v = ref.bratu_eigen_smallest(n, lam, O.sin_point(n ** 3, 0.3))
+        tag = f"eig_n{n}_l{lam}"
+        arrays[tag] = v
+        arrays[tag + "_res"] = np.array([e["converged"], e["eigenvalue"], e["residual_rel"],
+                                         e["outer_iterations"], e["linear_iterations"],
+                                         e["linear_matvecs"], e["preconditioner_applications"],
+                                         e["linear_failures"]], np.float64)
     for N in (8, 24):
         nodes, weights = ref.gauss_legendre_unit(N)
         arrays[f"gl_nodes_{N}"] = nodes

@@ -399,3 +399,49 @@ def test_tf2018_solves_match_reference(oracle, n, lam, pc, fd):
     assert rel(x, VEC[tag]) < 1e-12
     if n == 4:  # test_tf_chebyshev_2018.cpp:36-48
         assert r.converged and r.n_D2 == 0 and r.n_JVP > 0
+
+
+# ---- smallest eigenpair and continuation (C1 fold calibration) -----------------
+@pytest.mark.parametrize("n,lam", [(15, 3.0), (7, 0.0)])
+def test_eigen_matches_reference(oracle, n, lam):
+    cfg = O.solver_config(precond=O.PRECOND_GMG, jv_mode=O.DERIV_ANALYTIC, mg=O.mg_reference())
+    e, v = oracle.bratu_eigen_smallest(n, lam, cfg, O.sin_point(n ** 3, 0.3))
+    r = VEC[f"eig_n{n}_l{lam}_res"]
+    assert (e.converged, e.outer_iterations, e.linear_iterations, e.linear_matvecs,
+            e.preconditioner_applications, e.linear_failures) == tuple(int(t) for t in r[[0, 3, 4, 5, 6, 7]])
+    assert abs(e.eigenvalue - r[1]) <= 1e-13 * abs(r[1])
+    assert abs(e.residual_rel - r[2]) <= 1e-6 * r[2]
+    assert rel(v, VEC[f"eig_n{n}_l{lam}"]) < 1e-9
+
+
+@pytest.mark.parametrize("n", [7, 15, 31])
+def test_eigen_closed_form_at_lambda0(oracle, n):
+    # lambda = 0: J = (n+1)^2 A, smallest eigenvalue 12 (n+1)^2 sin^2(pi / (2(n+1)))
+    # (29.585039 at n = 31, the fold CSVs' first gap)
+    cfg = O.solver_config(precond=O.PRECOND_GMG, jv_mode=O.DERIV_ANALYTIC, mg=O.mg_reference())
+    e, v = oracle.bratu_eigen_smallest(n, 0.0, cfg, np.zeros(n ** 3))
+    mu = 12.0 * (n + 1) ** 2 * np.sin(np.pi / (2 * (n + 1))) ** 2
+    assert e.converged and abs(e.eigenvalue - mu) <= 1e-9 * mu
+    k = np.arange(1, n + 1) * np.pi / (n + 1)
+    phi = np.einsum("i,j,k->ijk", np.sin(k), np.sin(k), np.sin(k)).ravel()
+    assert abs(abs(np.dot(v, phi)) / np.linalg.norm(phi) - 1.0) < 1e-6
+
+
+def test_continuation_reproduces_reference_fold_rows(oracle):
+    # every natural-continuation row of fold_calibration/bratu_n31_coarse.csv
+    # up to lambda 9.8 (40 points, step halving near the fold): lambda and gap
+    # to their 6 printed digits, the counters exactly
+    rows = [r for r in REFDATA["fold_n31_coarse"] if r["mode"] == "natural"]
+    cfg = O.solver_config(precond=O.PRECOND_GMG, jv_mode=O.DERIV_ANALYTIC, mg=O.mg_reference())
+    pts, status, proxy, x = oracle.bratu_continuation(31, cfg,
+                                                      O.continuation_params(lambda_stop=9.8))
+    assert status == 0 and len(pts) == 40
+    for p, r in zip(pts, rows):
+        assert abs(p.lambda_ - r["lambda"]) <= 5e-6 * max(1.0, r["lambda"])
+        assert abs(p.spectral_gap - r["spectral_gap"]) <= 5e-6 * r["spectral_gap"]
+        assert abs(p.eigen_residual - r["eigen_residual"]) <= 5e-3 * r["eigen_residual"]
+        assert (p.continuation_iters, p.n_F, p.n_JVP, p.n_prec) == \
+            (r["continuation_iters"], r["n_F"], r["n_JVP"], r["n_prec"])
+    # the gap^2 fit over the last 4 points lies past the last point
+    assert pts[-1].lambda_ < proxy < 10.5
+    assert np.all(np.isfinite(x)) and x.max() > 0
