@@ -274,6 +274,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_tf2018_curvature": [_vp, _vp, _vp, _vp, C.POINTER(C.c_double)],
         "numa_hip_eigen_smallest": [_vp, _vp, C.POINTER(EigenParams), C.POINTER(EigenResult),
                                     _vp],
+        "numa_hip_start_write": [C.c_char_p, _vp, _i64],
+        "numa_hip_start_read": [C.c_char_p, _i64, _vp],
+        "numa_hip_sha256": [_vp, _i64, C.c_char_p],
+        "numa_hip_sha256_file": [C.c_char_p, C.c_char_p],
+        "numa_hip_perturb_host": [_vp, _i64, C.c_double, C.c_uint64, _vp],
+        "numa_hip_prolong3d": [_vp, _vp, _i64, _vp],
         "numa_hip_continuation": [_vp, _i64, C.POINTER(SolverConfig),
                                   C.POINTER(ContinuationParams), C.POINTER(KrylovParams),
                                   C.POINTER(ContinuationPoint), _i64, C.POINTER(_i64),
@@ -630,6 +636,50 @@ def continuation(ctx, n: int, cfg=None, cp: ContinuationParams | None = None, nk
                                         C.byref(proxy),
                                         x_out.ptr if x_out is not None else None))
     return list(pts[:min(count.value, max_points)]), int(status.value), proxy.value
+
+
+# ---- start vectors (C1 campaign files) ---------------------------------------
+def write_start(path: str, x: np.ndarray) -> None:
+    """One value per line, 17 significant digits (materialize_coarse_start.cpp:18-43)."""
+    x = np.ascontiguousarray(x, np.float64)
+    check(load().numa_hip_start_write(os.fsencode(path), x.ctypes.data, x.size))
+
+
+def read_start(path: str, expected: int) -> np.ndarray:
+    """load_vector (run_campaign.cpp:287-302): finite values, exactly `expected`."""
+    x = np.empty(int(expected))
+    check(load().numa_hip_start_read(os.fsencode(path), int(expected), x.ctypes.data))
+    return x
+
+
+def sha256_bytes(data: bytes) -> str:
+    out = C.create_string_buffer(65)
+    buf = C.create_string_buffer(bytes(data), len(data))
+    check(load().numa_hip_sha256(buf, len(data), out))
+    return out.value.decode()
+
+
+def sha256_file(path: str) -> str:
+    """The *_sha256 columns of the frozen start manifests (experiments/c1/starts/frozen)."""
+    out = C.create_string_buffer(65)
+    check(load().numa_hip_sha256_file(os.fsencode(path), out))
+    return out.value.decode()
+
+
+def perturb(ref: np.ndarray, rho: float, seed: int) -> np.ndarray:
+    """perturb (run_campaign.cpp:304-327): ref + rho ||ref|| d / ||d||, d ~ N(0, 1)
+    from std::mt19937_64(seed * 7919 + 13)."""
+    ref = np.ascontiguousarray(ref, np.float64)
+    out = np.empty_like(ref)
+    check(load().numa_hip_perturb_host(ref.ctypes.data, ref.size, float(rho), int(seed),
+                                       out.ctypes.data))
+    return out
+
+
+def prolong3d(ctx: "Context", coarse: "DeviceArray", nc: int, fine: "DeviceArray") -> None:
+    """B1 coarse start (materialize_coarse_start.cpp:60-167) on the device:
+    fine (size (2 nc + 1)^3) = trilinear prolongation of coarse (nc^3)."""
+    check(ctx.lib.numa_hip_prolong3d(ctx.h, coarse.ptr, int(nc), fine.ptr))
 
 
 def tf_chebyshev_2018(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restart=None):

@@ -330,6 +330,27 @@ int numa_hip_continuation(numa_hip_ctx* ctx, int64_t n, const numa_solver_config
                           numa_continuation_point* points, int64_t max_points, int64_t* count,
                           int64_t* status, double* fold_proxy, double* x_out_dev);
 
+/* ---- start vectors of the C1 campaign ------------------------------------- */
+/* One value per line with 17 significant digits (materialize_coarse_start.cpp:
+ * 18-43 writes with std::setprecision(max_digits10)). */
+int numa_hip_start_write(const char* path, const double* x_host, int64_t count);
+/* Whitespace-separated values to the end of the file, all finite, exactly
+ * `expected` of them, else NUMA_ERR_ARG (load_vector, run_campaign.cpp:287-302). */
+int numa_hip_start_read(const char* path, int64_t expected, double* x_host);
+/* SHA-256 (FIPS 180-4) as 64 lower-case hex digits + NUL in hex_out[65]: the
+ * sha256 columns of the frozen start manifests (experiments/c1/starts/frozen). */
+int numa_hip_sha256(const void* data, int64_t bytes, char* hex_out);
+int numa_hip_sha256_file(const char* path, char* hex_out);
+/* Perturbed start out = ref + rho ||ref|| d / ||d||, d_i ~ N(0, 1) from
+ * std::mt19937_64(seed * 7919 + 13) and std::normal_distribution<double>
+ * (perturb, run_campaign.cpp:304-327; norms summed in index order).  Host. */
+int numa_hip_perturb_host(const double* ref_host, int64_t count, double rho, uint64_t seed,
+                          double* out_host);
+/* B1 coarse start: fine = P coarse on the device, fine size 2 nc + 1
+ * (prolong3d, experiments/c1/starts/materialize_coarse_start.cpp:60-167). */
+int numa_hip_prolong3d(numa_hip_ctx* ctx, const double* coarse_dev, int64_t nc,
+                       double* fine_dev);
+
 /* ---- Chandrasekhar H-equation, batch of parameters c ---------------------- */
 /* problems/chandrasekhar_op.hpp:31-136 for `batch` values of c at once:
  * (A x)_i = c sum_j 0.5 w_j mu_i/(mu_i+mu_j) x_j with rule 0 = Gauss-Legendre

@@ -465,3 +465,36 @@ def sin_point(n: int, scale: float) -> np.ndarray:
     (tests/test_operators.cpp:19-23)."""
     i = np.arange(n, dtype=np.float64)
     return scale * np.sin(0.37 * i + 0.5)
+
+
+def prolong3d(coarse: np.ndarray, nc: int) -> np.ndarray:
+    """B1 coarse start, experiments/c1/starts/materialize_coarse_start.cpp:60-167:
+    fine (nf = 2 nc + 1)^3 from coarse nc^3; per axis an odd fine index takes
+    coarse (f-1)/2 with weight 1, an even f = 2I takes I-1 (I >= 1) and I
+    (I < nc) with 1/2; the present terms summed in (i, j, k) order."""
+    nf = 2 * nc + 1
+    c = np.asarray(coarse, np.float64).reshape(nc, nc, nc)
+
+    def terms(f):
+        if f % 2 == 1:
+            return [((f - 1) // 2, 1.0)]
+        r = f // 2
+        out = []
+        if r >= 1:
+            out.append((r - 1, 0.5))
+        if r < nc:
+            out.append((r, 0.5))
+        return out
+
+    T = [terms(f) for f in range(nf)]
+    fine = np.zeros((nf, nf, nf))
+    for i in range(nf):
+        for j in range(nf):
+            for k in range(nf):
+                s = 0.0
+                for a, wa in T[i]:
+                    for b, wb in T[j]:
+                        for cc, wc in T[k]:
+                            s += wa * wb * wc * c[a, b, cc]
+                fine[i, j, k] = s
+    return fine.ravel()

@@ -445,3 +445,27 @@ def test_continuation_reproduces_reference_fold_rows(oracle):
     # the gap^2 fit over the last 4 points lies past the last point
     assert pts[-1].lambda_ < proxy < 10.5
     assert np.all(np.isfinite(x)) and x.max() > 0
+
+
+# ---- B1 coarse start: prolong3d --------------------------------------------------
+@pytest.mark.parametrize("nc", [1, 3, 5])
+def test_prolong3d_is_the_tensor_product_interpolation(nc):
+    # materialize_coarse_start.cpp:60-167 against the Kronecker product of the
+    # 1-D interpolation matrix (a different evaluation of the same operator)
+    nf = 2 * nc + 1
+    P1 = np.zeros((nf, nc))
+    for f in range(nf):
+        if f % 2 == 1:
+            P1[f, (f - 1) // 2] = 1.0
+        else:
+            if f // 2 >= 1:
+                P1[f, f // 2 - 1] = 0.5
+            if f // 2 < nc:
+                P1[f, f // 2] = 0.5
+    P = np.kron(np.kron(P1, P1), P1)
+    c = np.random.default_rng(nc).normal(size=nc ** 3)
+    fine = O.prolong3d(c, nc)
+    assert rel(fine, P @ c) < 1e-15
+    # injection at the odd (coarse) nodes
+    F = fine.reshape(nf, nf, nf)
+    assert np.array_equal(F[1::2, 1::2, 1::2].ravel(), c)
