@@ -1067,12 +1067,12 @@ private:
     }
 
     // Coarse planes per workgroup of the fused residual / restriction: the
-    // largest of 8, 4, 2 that still gives 1024 workgroups.
-    static int rr_chunk(int64_t nc) {
+    // chunk that fills the device in whole waves of resident workgroups (a
+    // chunk of ich coarse planes marches 2 ich + 1 fine planes).
+    int rr_chunk(int64_t nc) const {
+        static const int per_cu = blocks_per_cu(k_resid_restrict<T>, kRRThreads);
         const int64_t tiles = ((nc + kRRK - 1) / kRRK) * ((nc + kRRJ - 1) / kRRJ);
-        int ich = 8;
-        while (ich > 2 && tiles * ((nc + ich - 1) / ich) < 1024) ich /= 2;
-        return ich;
+        return fill_chunk(nc, tiles, (int64_t)per_cu * c_.cu_count, 1, 1, 64);
     }
 
     // poisson_mg.hpp:262-294
