@@ -6,8 +6,9 @@
  * include/numa/...) plus the new components the GPU path adds (even-n
  * multigrid, weighted-Jacobi and Chebyshev smoothers, FGMRES, constant
  * forcing, midpoint / Kelley-form Chandrasekhar).  Only tests/, the smoke()
- * check and bench.py's cpu_baseline leg may load it, and only as the checker
- * or the timed CPU baseline, never as a product path.
+ * check and bench.py's cpu_baseline leg may load it, and only as the reference
+ * the GPU results are compared with or the timed CPU baseline, never as a
+ * product path.
  *
  * Parity pins for every function are listed in the header comment of
  * numa_oracle.cpp.

@@ -2,7 +2,8 @@
 header library compiled into oracle/_ref/libnuma_ref.so.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker / timed CPU baseline.  It is never
+cpu_baseline leg of bench.py, as the reference results are compared with or the
+timed CPU baseline.  It is never
 a product path.
 """
 from __future__ import annotations
