@@ -582,6 +582,62 @@ inline int fused_fill_chunk(int n, int steps, int rows, int cu_count) {
     }
 }
 
+// Exact solve of the coarsest level (n <= 16) by the discrete sine transform:
+// the 7-point -Lap_h with zero Dirichlet values is diagonal in the basis
+// S_pk = sin(pi (p+1)(k+1)/(n+1)) along each axis, with eigenvalues
+// h2inv (c_p + c_q + c_r), c_p = 2 - 2 cos(pi (p+1)/(n+1)), and S S = (n+1)/2 I,
+// so x = (2/(n+1))^3 S3 diag(1 / (h2inv (c_p + c_q + c_r))) S3 b.  One
+// workgroup: three passes of n-point transforms along k, j, i (4 threads per
+// line, every 4th output each), the scaling, three passes back; FP64, the
+// iterate double-buffered in LDS.
+template <class T>
+__global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv, const T* b, T* xout) {
+#pragma clang fp contract(off)
+    __shared__ double u[2][4096];
+    __shared__ double S[16][16];
+    __shared__ double c1[16];
+    const int N1 = n, N2 = n * n, N = N2 * n, t = threadIdx.x;
+    const double th = M_PI / (double)(n + 1);
+    if (t < N2) S[t / n][t % n] = sin(th * (double)((t / n + 1) * (t % n + 1)));
+    if (t < n) c1[t] = 2.0 - 2.0 * cos(th * (double)(t + 1));
+    for (int id = t; id < N; id += 1024) u[0][id] = static_cast<double>(b[id]);
+    __syncthreads();
+    int cur = 0;
+    // one pass along the axis of stride st; line l = (a, c) starts at a*A + c*C
+    auto pass = [&](int st, int A, int C) {
+        const int l = t >> 2, qd = t & 3;
+        if (l < N2) {
+            const int base = (l / n) * A + (l % n) * C;
+            const double* in = u[cur];
+            double* out = u[cur ^ 1];
+            double v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = k < n ? in[base + k * st] : 0.0;
+            for (int p = qd; p < n; p += 4) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) s += S[p][k < n ? k : 0] * v[k];
+                out[base + p * st] = s;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    };
+    pass(1, N2, N1);
+    pass(N1, N2, 1);
+    pass(N2, N1, 1);
+    const double f = 8.0 / ((double)(n + 1) * (double)(n + 1) * (double)(n + 1));
+    for (int id = t; id < N; id += 1024) {
+        const int i = id / N2, j = (id / N1) % N1, k = id % N1;
+        u[cur][id] = f * (u[cur][id] / (h2inv * (c1[i] + c1[j] + c1[k])));
+    }
+    __syncthreads();
+    pass(1, N2, N1);
+    pass(N1, N2, 1);
+    pass(N2, N1, 1);
+    for (int id = t; id < N; id += 1024) xout[id] = static_cast<T>(u[cur][id]);
+}
+
 inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
 inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1.0)); }
 
@@ -662,6 +718,8 @@ public:
                     L.ichunk = fused_fill_chunk<T>((int)L.n, fused_group(L), L.rows, c.cu_count);
             }
         }
+        require(cfg.coarse_iters >= 0 || levels_.back().n <= 16,
+                "the exact coarse solve (coarse_iters < 0) needs a coarsest level of at most 16^3");
         set_resid_restrict(-1);
     }
     void set_resid_restrict(int64_t ich) override {
@@ -1023,6 +1081,12 @@ private:
     void coarse_solve(Level& L) {
         const int64_t N = L.n * L.n * L.n;
         const bool rb = cfg_.smoother == NUMA_SMOOTH_RBGS;
+        if (cfg_.coarse_iters < 0) {  // exact (sine-transform) solve, n <= 16
+            hipLaunchKernelGGL((k_coarse_dst<T>), dim3(1), dim3(1024), 0, c_.stream, (int)L.n,
+                               L.h2inv, (const T*)L.b, L.x);
+            NUMA_HIP_LAUNCH_CHECK();
+            return;
+        }
         if (N <= kCoarseMaxN3) {
             const double hi = cheb_lmax(L.n), lo = cheb_lmin(L.n);
             hipLaunchKernelGGL((k_coarse_lds<T>), dim3(1), dim3(kCoarseThreads), 0, c_.stream,

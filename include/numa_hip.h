@@ -115,7 +115,8 @@ typedef struct numa_mg_config {
     int64_t degree;          /* Chebyshev smoother degree (matvecs per sweep)      */
     int64_t max_levels;      /* 0 = no limit                                       */
     int64_t min_coarse_n;    /* stop coarsening once n <= min_coarse_n             */
-    int64_t coarse_iters;    /* coarsest level: RB-GS sweeps or Chebyshev degree   */
+    int64_t coarse_iters;    /* coarsest level: RB-GS sweeps or Chebyshev degree;
+                                < 0: exact solve by sine transforms (n <= 16)      */
     int64_t even_coarsening; /* 1: coarsen any n to n/2; 0: only odd n (reference) */
     double omega;            /* weighted-Jacobi damping                            */
     double cheb_ratio;       /* Chebyshev smoothing interval [lmax/ratio, lmax]    */

@@ -410,7 +410,48 @@ struct MG {
         }
     }
 
+    // Exact solve by the discrete sine transform (the GPU's coarse_iters < 0):
+    // x = (2/(n+1))^3 S3 diag(1 / (h2inv (c_p + c_q + c_r))) S3 b with
+    // S_pk = sin(pi (p+1)(k+1)/(n+1)), c_p = 2 - 2 cos(pi (p+1)/(n+1)).
+    static void coarse_dst(Level& L) {
+        const long n = L.n, N2 = n * n, N = N2 * n;
+        const double th = M_PI / (double)(n + 1);
+        std::vector<double> S((size_t)(n * n)), c((size_t)n), u(L.b.begin(), L.b.begin() + N),
+            w((size_t)N);
+        for (long p = 0; p < n; ++p) {
+            c[(size_t)p] = 2.0 - 2.0 * std::cos(th * (double)(p + 1));
+            for (long k = 0; k < n; ++k) S[(size_t)(p * n + k)] = std::sin(th * (double)((p + 1) * (k + 1)));
+        }
+        auto pass = [&](long st, long A, long C) {
+            for (long l = 0; l < N2; ++l) {
+                const long base = (l / n) * A + (l % n) * C;
+                for (long p = 0; p < n; ++p) {
+                    double s = 0.0;
+                    for (long k = 0; k < n; ++k) s += S[(size_t)(p * n + k)] * u[(size_t)(base + k * st)];
+                    w[(size_t)(base + p * st)] = s;
+                }
+            }
+            u.swap(w);
+        };
+        pass(1, N2, n);
+        pass(n, N2, 1);
+        pass(N2, n, 1);
+        const double f = 8.0 / ((double)(n + 1) * (double)(n + 1) * (double)(n + 1));
+        for (long id = 0; id < N; ++id) {
+            const long i = id / N2, j = (id / n) % n, k = id % n;
+            u[(size_t)id] = f * (u[(size_t)id] / (L.h2inv * (c[(size_t)i] + c[(size_t)j] + c[(size_t)k])));
+        }
+        pass(1, N2, n);
+        pass(n, N2, 1);
+        pass(N2, n, 1);
+        std::copy(u.begin(), u.end(), L.x.begin());
+    }
+
     void coarse_solve(Level& L) const {
+        if (cfg.coarse_iters < 0) {
+            coarse_dst(L);
+            return;
+        }
         if (L.n == 1) {  // poisson_mg.hpp:267-269
             L.x[0] = L.b[0] / (6.0 * L.h2inv);
             return;

@@ -17,12 +17,15 @@ def cfgs():
         "wjacobi": H.mg_wjacobi(),
         "cheb3": H.mg_cheb(levels=5),
         "cheb3_full": H.mg_cheb(levels=0),
+        "cheb3_exact": H.mg_cheb(levels=5, coarse_iters=-1),
+        "wj_exact": H.mg_wjacobi(levels=3, coarse_iters=-1),
     }
 
 
 CASES = [("reference", 7), ("reference", 15), ("reference", 16), ("reference", 31),
          ("wjacobi", 16), ("wjacobi", 32), ("wjacobi", 33), ("cheb3", 16), ("cheb3", 32),
-         ("cheb3", 64), ("cheb3", 48), ("cheb3_full", 32)]
+         ("cheb3", 64), ("cheb3", 48), ("cheb3_full", 32), ("cheb3_exact", 64),
+         ("cheb3_exact", 256), ("wj_exact", 40)]
 
 
 def _to_oracle(cfg):
@@ -44,7 +47,8 @@ def test_vcycle_fp64(ctx, oracle, name, n):
     assert rel(dz.numpy(), ref) < 1e-12
 
 
-@pytest.mark.parametrize("name,n", [("wjacobi", 32), ("cheb3", 32), ("cheb3", 64)])
+@pytest.mark.parametrize("name,n", [("wjacobi", 32), ("cheb3", 32), ("cheb3", 64),
+                                    ("cheb3_exact", 64)])
 def test_vcycle_fp32(ctx, oracle, name, n):
     import numa_hip as H
     import oracle as O
@@ -61,3 +65,26 @@ def test_levels_reference_rule(ctx):
     import numa_hip as H
     assert H.PoissonMG(ctx, 15).sizes() == [15, 7, 3, 1]  # test_operators.cpp:193-194
     assert H.PoissonMG(ctx, 256, H.mg_cheb(levels=5)).sizes() == [256, 128, 64, 32, 16]
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 16])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_exact_coarse_solve_inverts_the_operator(ctx, oracle, n, precision):
+    # one level with coarse_iters < 0: the V-cycle is the sine-transform solve,
+    # so A z = r to rounding
+    import numa_hip as H
+    dt = np.float64 if precision == H.PRECISION_FP64 else np.float32
+    cfg = H.mg_cheb(levels=1, coarse_iters=-1)
+    r = np.random.default_rng(n).normal(size=n ** 3)
+    mg = H.PoissonMG(ctx, n, cfg, precision)
+    dz = ctx.empty(n ** 3, dt)
+    mg(ctx.array(r.astype(dt)), dz)
+    z = dz.numpy().astype(np.float64)
+    assert rel(oracle.poisson_apply(n, z), r.astype(dt).astype(np.float64)) < (
+        1e-13 if precision == H.PRECISION_FP64 else 1e-6)
+
+
+def test_exact_coarse_solve_rejects_large_coarsest_level(ctx):
+    import numa_hip as H
+    with pytest.raises(H.NumaError):
+        H.PoissonMG(ctx, 64, H.mg_cheb(levels=2, coarse_iters=-1))

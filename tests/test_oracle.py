@@ -469,3 +469,18 @@ def test_prolong3d_is_the_tensor_product_interpolation(nc):
     # injection at the odd (coarse) nodes
     F = fine.reshape(nf, nf, nf)
     assert np.array_equal(F[1::2, 1::2, 1::2].ravel(), c)
+
+
+
+# ---- exact coarse solve (sine transform) -----------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 5, 16])
+def test_exact_coarse_solve_is_the_inverse(oracle, n):
+    # a one-level hierarchy with coarse_iters < 0 applies A^-1: A z = r to
+    # rounding, and z matches a dense solve on small grids
+    cfg = O.mg_config(smoother=2, max_levels=1, coarse_iters=-1, even_coarsening=1)
+    r = np.random.default_rng(n).normal(size=n ** 3)
+    z = oracle.mg_apply(n, r, cfg)
+    assert rel(oracle.poisson_apply(n, z), r) < 1e-13
+    if n <= 5:
+        A = np.column_stack([oracle.poisson_apply(n, e) for e in np.eye(n ** 3)])
+        assert rel(z, np.linalg.solve(A, r)) < 1e-12
