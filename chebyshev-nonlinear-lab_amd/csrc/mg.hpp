@@ -513,17 +513,53 @@ inline void launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolyS
 template <class T>
 constexpr int fused_rows_small() { return sizeof(T) == 4 ? 2 : 1; }
 
-// LDS of the fused kernel: STEPS levels x 2 planes x (16R+2) x 66 values.
+// Box form of a fused launch: rows = -shape selects the output box
+// (i x j x k) of k_poly_box; shapes 1..kBoxShapes.
+constexpr int kBoxShapes = 3;
+constexpr int box_edge_i(int shape) { return shape == 1 ? 8 : 4; }
+constexpr int box_edge_j(int shape) { return shape == 3 ? 4 : 8; }
+constexpr int box_edge_k(int) { return 16; }
+
+// LDS of the fused kernel: STEPS levels x 2 planes x (16R+2) x 66 values;
+// of the box form: 2 copies of the box grown by STEPS on every side.
 template <class T>
 constexpr bool fused_fits(int steps, int rows) {
+    if (rows < 0) {  // box form: up to 3 steps (one sweep of degree <= 3)
+        const int sh = -rows;
+        if (sh > kBoxShapes || steps > 3) return false;
+        return 2 * (int64_t)(box_edge_i(sh) + 2 * steps) * (box_edge_j(sh) + 2 * steps) *
+                   (box_edge_k(sh) + 2 * steps) * (int64_t)sizeof(T) <=
+               160 * 1024;
+    }
     return (int64_t)steps * 2 * ((kFusedThreads / 64) * rows + 2) * 66 * (int64_t)sizeof(T) <=
            160 * 1024;
+}
+
+template <class T, int S, int SH>
+inline void launch_box_ss(Ctx& c, int n, double h2inv, const PolySteps& cs, const T* x, const T* b,
+                          T* xo) {
+    constexpr int EI = box_edge_i(SH), EJ = box_edge_j(SH), EK = box_edge_k(SH);
+    if constexpr (fused_fits<T>(S, -SH)) {
+        int tk, tj, total;
+        const dim3 g = box_grid<EI, EJ, EK>(n, tk, tj, total);
+        hipLaunchKernelGGL((k_poly_box<T, S, EI, EJ, EK>), g, dim3(kBoxThreads), 0, c.stream, n,
+                           tk, tj, total, h2inv, cs, x, b, xo);
+        NUMA_HIP_LAUNCH_CHECK();
+    } else {
+        throw Error(NUMA_ERR_ARG, "box smoothing: steps per launch x box exceed the LDS");
+    }
 }
 
 template <class T, int S>
 inline void launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv, const PolySteps& cs,
                            const T* x, const T* b, T* xo) {
     constexpr int R0 = fused_rows_small<T>();
+    switch (rows) {
+        case -1: launch_box_ss<T, S, 1>(c, n, h2inv, cs, x, b, xo); return;
+        case -2: launch_box_ss<T, S, 2>(c, n, h2inv, cs, x, b, xo); return;
+        case -3: launch_box_ss<T, S, 3>(c, n, h2inv, cs, x, b, xo); return;
+        default: break;
+    }
     if (rows == R0) {
         launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo);
         return;
@@ -677,8 +713,10 @@ public:
         require(cfg.fuse >= NUMA_FUSE_NONE && cfg.fuse <= NUMA_FUSE_AUTO, "unknown fuse mode");
         require(cfg.ichunk >= 0 && cfg.ichunk <= 4096, "ichunk must be in [0, 4096]");
         require(cfg.rows == 0 || cfg.rows == fused_rows_small<T>() ||
-                    cfg.rows == 2 * fused_rows_small<T>(),
-                "rows must be 0 or the storage type's 1x / 2x rows per thread");
+                    cfg.rows == 2 * fused_rows_small<T>() ||
+                    (cfg.rows < 0 && cfg.rows >= -kBoxShapes),
+                "rows must be 0, the storage type's 1x / 2x rows per thread or a box shape "
+                "(-1 .. -3)");
         // Hierarchy (poisson_mg.hpp:44-63, plus even coarsening and limits).
         int64_t m = n;
         double h2inv = (double(n) + 1.0) * (double(n) + 1.0);
@@ -709,12 +747,14 @@ public:
             }
             L.fuse = cfg.smoother == NUMA_SMOOTH_RBGS ? (int)NUMA_FUSE_NONE : (int)cfg.fuse;
             L.ichunk = cfg.ichunk > 0 ? (int)cfg.ichunk : 32;
-            L.rows = cfg.rows > 0 ? (int)cfg.rows : fused_rows_small<T>();
+            L.rows = cfg.rows != 0 ? (int)cfg.rows : fused_rows_small<T>();
+            if (L.rows < 0) L.ichunk = 0;  // box form: no plane chunk
             if ((L.fuse == NUMA_FUSE_SWEEP || L.fuse == NUMA_FUSE_PHASE) &&
                 cfg.smoother != NUMA_SMOOTH_RBGS) {
                 require(fused_fits<T>(fused_group(L), L.rows),
-                        "fused smoothing: steps per launch x rows per thread exceed the LDS");
-                if (cfg.ichunk == 0)
+                        "fused smoothing: steps per launch x rows per thread (or box) exceed the "
+                        "LDS");
+                if (cfg.ichunk == 0 && L.rows > 0)
                     L.ichunk = fused_fill_chunk<T>((int)L.n, fused_group(L), L.rows, c.cu_count);
             }
         }
@@ -1003,18 +1043,21 @@ private:
             Level& L = levels_[l];
             if (L.fuse != NUMA_FUSE_AUTO) continue;
             std::ostringstream key;
-            key << dev << "|mg-smooth-v2|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
+            key << dev << "|mg-smooth-v3|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
                 << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
             std::string val;
             if (TuneCache::get().lookup(key.str(), val)) {
                 int m = 0, ic = 0, rw = 0;
                 constexpr int R0 = fused_rows_small<T>();
-                if (std::sscanf(val.c_str(), "%d,%d,%d", &m, &ic, &rw) == 3 && m >= NUMA_FUSE_NONE &&
-                    m <= NUMA_FUSE_PHASE && ic > 0 && ic <= 4096 && (rw == R0 || rw == 2 * R0)) {
+                const bool parsed = std::sscanf(val.c_str(), "%d,%d,%d", &m, &ic, &rw) == 3;
+                const bool march = ic > 0 && ic <= 4096 && (rw == R0 || rw == 2 * R0);
+                const bool box = ic == 0 && rw < 0 && rw >= -kBoxShapes;
+                if (parsed && m >= NUMA_FUSE_NONE && m <= NUMA_FUSE_PHASE && (march || box)) {
                     L.fuse = m;
                     L.ichunk = ic;
                     L.rows = rw;
-                    continue;
+                    if (m == NUMA_FUSE_NONE || fused_fits<T>(fused_group(L), rw)) continue;
+                    L.fuse = NUMA_FUSE_AUTO;  // an entry this level cannot run: time again
                 }
             }
             const int64_t N = L.n * L.n * L.n;
@@ -1037,6 +1080,12 @@ private:
                     cands.push_back({md, fill, rw});
                     for (int ic : {16, 32, 64})
                         if (ic != fill) cands.push_back({md, ic, rw});
+                }
+            for (int md : {(int)NUMA_FUSE_SWEEP, (int)NUMA_FUSE_PHASE})
+                for (int sh = 1; sh <= kBoxShapes; ++sh) {
+                    if (md == NUMA_FUSE_PHASE && cfg_.pre <= 1) continue;
+                    L.fuse = md;
+                    if (fused_fits<T>(fused_group(L), -sh)) cands.push_back({md, 0, -sh});
                 }
             hipEvent_t e0, e1;
             NUMA_HIP_CHECK(hipEventCreate(&e0));

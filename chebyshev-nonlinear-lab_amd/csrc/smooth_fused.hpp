@@ -24,6 +24,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "stencil.hpp"
@@ -49,8 +50,64 @@ struct FusedGeom {
     static_assert(kOutW > 0 && kOutH > 0, "halo larger than the region");
 };
 
+// Planes of the b register ring and of the unrolled march (see k_poly_fused):
+// STEPS + 1 planes where lcm(6, STEPS + 1) <= 12, else 6 or 12.
+constexpr int fused_b_ring(int steps) {
+    return steps <= 3 ? steps + 1 : steps <= 5 ? 6 : 12;
+}
+constexpr int fused_unroll(int steps) { return steps == 3 || steps == 6 ? 12 : 6; }
+
+// Raw buffer access (MUBUF) to one plane: a uniform descriptor (base, size in
+// bytes) and a 32-bit per-thread byte offset.  Loads past the size return
+// zero and stores past it are dropped, so out-of-domain rows and planes need
+// no branches: their offset (kNoOffset) or size (0) is out of range.
+constexpr uint32_t kNoOffset = 0x80000000u;
+constexpr int kBufWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufWord3);
+}
+
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+template <class T>
+__device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, v), r, (int)off, 0, 0);
+}
+
+// Calls f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, U-1>{}).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int U, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, U>{});
+}
+
+// Waves per SIMD the register allocation must allow: 8 (two 16-wave
+// workgroups per CU, at most 64 VGPRs) for the variants whose rings fit that
+// without spilling, else no bound.
 template <class T, int STEPS, int R>
-__global__ __launch_bounds__(kFusedThreads) void k_poly_fused(int n, int ichunk, int tiles_k,
+constexpr int fused_waves_per_eu() {
+    constexpr int words = (int)(sizeof(T) / 4) * R;  // 32-bit registers per ring value
+    return (words <= 2 && STEPS <= 2) || (words <= 4 && STEPS == 1) ? 8 : 1;
+}
+
+template <class T, int STEPS, int R>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kFusedThreads),
+                          amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R>()))) void
+k_poly_fused(int n, int ichunk, int tiles_k,
                                                               int tiles_j, int chunks,
                                                               double h2inv_d, PolySteps cs,
                                                               const T* __restrict__ x,
@@ -60,7 +117,16 @@ __global__ __launch_bounds__(kFusedThreads) void k_poly_fused(int n, int ichunk,
     using A = SmoothArith<T>;
     using G = FusedGeom<STEPS, R>;
     constexpr int W = G::kW, H = G::kH;
-    // LDS ring: level s (s < STEPS), 2 planes of (H+2) x (W+2) with a zero border
+    // Register rings are indexed by the plane's position t = q - q_first in
+    // the march: x of each level by t mod 3, d by t mod 2, b by t mod BR
+    // (BR >= STEPS + 1 planes).  The march is unrolled by U planes, a
+    // multiple of all three periods, so every ring slot is a compile-time
+    // register and no value moves between planes.
+    constexpr int BR = fused_b_ring(STEPS);
+    constexpr int U = fused_unroll(STEPS);
+    static_assert(U % 6 == 0 && U % BR == 0 && BR > STEPS, "ring periods");
+    // LDS ring: level s (s < STEPS), 2 planes (by t mod 2) of (H+2) x (W+2)
+    // with a zero border
     __shared__ T ring[STEPS][2][H + 2][W + 2];
 
     // XCD-aware block -> tile: the blocks an XCD receives (every 8th) walk
@@ -88,23 +154,27 @@ __global__ __launch_bounds__(kFusedThreads) void k_poly_fused(int n, int ichunk,
 
     const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int k = k0 + lane;
-    const int64_t N1 = n;
+    const int64_t nn = (int64_t)n * n;  // plane stride
     bool col_in[R];
     int jrow[R];
+    // byte offset of the row's column within a plane (n <= 2048: < 2^25), for
+    // the loads (kNoOffset outside the domain) and for the output store
+    // (kNoOffset unless an output column of this region)
+    const uint32_t plane_bytes = (uint32_t)nn * (uint32_t)sizeof(T);
+    const bool out_k = lane >= STEPS && lane < W - STEPS;
+    uint32_t off[R], off_out[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         jrow[r] = ty + (kFusedThreads / 64) * r;
         const int j = j0 + jrow[r];
         col_in[r] = (unsigned)j < (unsigned)n && (unsigned)k < (unsigned)n;
+        off[r] = col_in[r] ? (uint32_t)(j * n + k) * (uint32_t)sizeof(T) : kNoOffset;
+        off_out[r] = out_k && jrow[r] >= STEPS && jrow[r] < H - STEPS ? off[r] : kNoOffset;
     }
-    const bool out_k = lane >= STEPS && lane < W - STEPS;
 
-    // register rings (per row): xs[s][0..2] = x_s at planes (q-s-2, q-s-1, q-s),
-    // ds[s][0..1] = d produced by step s at planes (q-2-s, q-1-s),
-    // bb[0..STEPS] = b at planes (q, q-1, ..., q-STEPS).
     T xs[STEPS][R][3];
     T ds[STEPS][R][2];
-    T bb[STEPS + 1][R];
+    T bb[BR][R];
 #pragma unroll
     for (int s = 0; s < STEPS; ++s)
 #pragma unroll
@@ -113,75 +183,89 @@ __global__ __launch_bounds__(kFusedThreads) void k_poly_fused(int n, int ichunk,
             ds[s][r][0] = ds[s][r][1] = T(0);
         }
 #pragma unroll
-    for (int s = 0; s <= STEPS; ++s)
+    for (int s = 0; s < BR; ++s)
 #pragma unroll
         for (int r = 0; r < R; ++r) bb[s][r] = T(0);
 
+    // plane q of x (zero if x is null) and b; zeros outside the domain
     auto load_plane = [&](int q, T (&vx)[R], T (&vb)[R]) {
+        const bool q_in = (unsigned)q < (unsigned)n;
+        const int64_t base = (int64_t)(q_in ? q : 0) * nn;
+        const uint32_t bytes = q_in ? plane_bytes : 0u;
+        const __amdgpu_buffer_rsrc_t rb = plane_rsrc(b + base, bytes);
+        const __amdgpu_buffer_rsrc_t rx = plane_rsrc(x ? x + base : b, x ? bytes : 0u);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            vx[r] = T(0);
-            vb[r] = T(0);
-            if (col_in[r] && (unsigned)q < (unsigned)n) {
-                const int64_t id = ((int64_t)q * N1 + (j0 + jrow[r])) * N1 + k;
-                if (x) vx[r] = x[id];
-                vb[r] = b[id];
-            }
+            vx[r] = buf_load<T>(rx, off[r]);
+            vb[r] = buf_load<T>(rb, off[r]);
         }
     };
 
     const int q_first = ib - STEPS, q_last = ie + STEPS - 1;
+    const int nplanes = q_last - q_first + 1;
     T nx[R], nb[R];
     load_plane(q_first, nx, nb);
     __syncthreads();  // zero borders written
-    for (int q = q_first; q <= q_last; ++q) {
-        // shift the rings by one plane and take in the loaded plane q
+
+    // One plane of the march, t = q - q_first, PH = t mod U.
+    auto plane = [&](auto ph, int q) {
+        constexpr int PH = decltype(ph)::value;
+        // take in the loaded plane q
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-#pragma unroll
-            for (int s = STEPS; s > 0; --s) bb[s][r] = bb[s - 1][r];
-            bb[0][r] = nb[r];
-#pragma unroll
-            for (int s = 0; s < STEPS; ++s) {
-                xs[s][r][0] = xs[s][r][1];
-                xs[s][r][1] = xs[s][r][2];
-                ds[s][r][0] = ds[s][r][1];
-            }
-            xs[0][r][2] = nx[r];
-            ring[0][q & 1][jrow[r] + 1][lane + 1] = nx[r];
+            bb[PH % BR][r] = nb[r];
+            xs[0][r][PH % 3] = nx[r];
+            ring[0][PH & 1][jrow[r] + 1][lane + 1] = nx[r];
         }
         if (q < q_last) load_plane(q + 1, nx, nb);  // in flight during the steps
-#pragma unroll
-        for (int s = 0; s < STEPS; ++s) {
-            const int p = q - 1 - s;  // plane advanced by step s
+        static_for<STEPS>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            constexpr int tp = PH - 1 - s + 6 * U;  // t of the plane advanced (mod U), >= 0
+            const int p = q - 1 - s;
             const bool p_in = (unsigned)p < (unsigned)n;
-            const int slot = p & 1;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int jr = jrow[r];
-                const A c = static_cast<A>(xs[s][r][1]);
-                const A im = static_cast<A>(xs[s][r][0]), ip = static_cast<A>(xs[s][r][2]);
-                const A jm = static_cast<A>(ring[s][slot][jr][lane + 1]);
-                const A jp = static_cast<A>(ring[s][slot][jr + 2][lane + 1]);
-                const A km = static_cast<A>(ring[s][slot][jr + 1][lane]);
-                const A kp = static_cast<A>(ring[s][slot][jr + 1][lane + 2]);
-                const A res =
-                    static_cast<A>(bb[s + 1][r]) - h2inv * lap_num<A>(c, im, ip, jm, jp, km, kp);
-                const A dn = s == 0 ? c2[0] * res
-                                    : c1[s] * static_cast<A>(ds[s - 1][r][0]) + c2[s] * res;
+                const A c = static_cast<A>(xs[s][r][tp % 3]);
+                const A im = static_cast<A>(xs[s][r][(tp - 1) % 3]);
+                const A ip = static_cast<A>(xs[s][r][(tp + 1) % 3]);
+                const A jm = static_cast<A>(ring[s][tp & 1][jr][lane + 1]);
+                const A jp = static_cast<A>(ring[s][tp & 1][jr + 2][lane + 1]);
+                const A km = static_cast<A>(ring[s][tp & 1][jr + 1][lane]);
+                const A kp = static_cast<A>(ring[s][tp & 1][jr + 1][lane + 2]);
+                const A res = static_cast<A>(bb[tp % BR][r]) -
+                              h2inv * lap_num<A>(c, im, ip, jm, jp, km, kp);
+                A dn;
+                if constexpr (s == 0)
+                    dn = c2[0] * res;
+                else
+                    dn = c1[s] * static_cast<A>(ds[s - 1][r][tp & 1]) + c2[s] * res;
                 const bool live = p_in && col_in[r];
                 const T xn = live ? static_cast<T>(c + dn) : T(0);
-                if (s + 1 < STEPS) {
-                    ds[s][r][1] = live ? static_cast<T>(dn) : T(0);
-                    xs[s + 1][r][2] = xn;
-                    ring[s + 1][(q - 1 - s) & 1][jr + 1][lane + 1] = xn;
-                } else if (live && out_k && jr >= STEPS && jr < H - STEPS && p >= ib && p < ie) {
-                    xo[((int64_t)p * N1 + (j0 + jr)) * N1 + k] = xn;
+                if constexpr (s + 1 < STEPS) {
+                    ds[s][r][tp & 1] = live ? static_cast<T>(dn) : T(0);
+                    xs[s + 1][r][tp % 3] = xn;
+                    ring[s + 1][tp & 1][jr + 1][lane + 1] = xn;
+                } else {
+                    // output planes [ib, ie) of the chunk, output columns
+                    const bool p_out = p >= ib && p < ie;
+                    const __amdgpu_buffer_rsrc_t ro =
+                        plane_rsrc(xo + (int64_t)(p_out ? p : 0) * nn, p_out ? plane_bytes : 0u);
+                    buf_store<T>(xn, ro, off_out[r]);
                 }
             }
-        }
+        });
         __syncthreads();
-    }
+    };
+    // whole groups of U planes without tests (no value moves between
+    // registers), then the remaining planes
+    int t0 = 0;
+    for (; t0 + U <= nplanes; t0 += U)
+        static_for<U>([&](auto ph) { plane(ph, q_first + t0 + decltype(ph)::value); });
+    static_for<U>([&](auto ph) {
+        constexpr int PH = decltype(ph)::value;
+        if (t0 + PH < nplanes) plane(ph, q_first + t0 + PH);
+    });
 }
 
 // Launch geometry of k_poly_fused for an n^3 level.
@@ -192,6 +276,140 @@ inline dim3 fused_grid(int n, int ichunk, int& tiles_k, int& tiles_j, int& chunk
     tiles_j = (n + G::kOutH - 1) / G::kOutH;
     chunks = (n + ichunk - 1) / ichunk;
     const int64_t total = (int64_t)tiles_k * tiles_j * chunks;
+    return dim3((unsigned)((total + 7) / 8 * 8));
+}
+
+// ---------------------------------------------------------------------------
+// Box form of the same STEPS smoothing steps, for the small and middle levels
+// of the hierarchy, where the march above is latency-bound (few workgroups,
+// one barrier per plane along a chain of chunk + 2 STEPS planes).
+//
+// A workgroup of 512 threads owns a box of EI x EJ x EK output nodes.  It
+// loads the box grown by STEPS nodes on every side (x, b; zeros outside the
+// domain) at once, so all its loads are in flight together, keeps x in a
+// double-buffered LDS copy of the grown box and b, d, x of its own nodes in
+// registers (node e = threadIdx.x + 512 m, box order k fastest), and runs the
+// steps with one barrier each.  Step s advances the nodes at least s + 1 from
+// the box surface; their neighbours are at least s from it, which step s - 1
+// advanced (or the load, for s = 0), so every LDS read is unconditional and
+// in the box.  The last step advances exactly the output nodes; they go
+// through LDS to coalesced stores.  Arithmetic and roundings are those of
+// k_poly_fused.
+constexpr int kBoxThreads = 512;
+
+template <int STEPS, int EI, int EJ, int EK>
+struct BoxGeom {
+    static constexpr int BI = EI + 2 * STEPS, BJ = EJ + 2 * STEPS, BK = EK + 2 * STEPS;
+    static constexpr int kNodes = BI * BJ * BK;
+    static constexpr int kPer = (kNodes + kBoxThreads - 1) / kBoxThreads;
+};
+
+template <class T, int STEPS, int EI, int EJ, int EK>
+__global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, int tiles_j,
+                                                          int total, double h2inv_d, PolySteps cs,
+                                                          const T* __restrict__ x,
+                                                          const T* __restrict__ b,
+                                                          T* __restrict__ xo) {
+#pragma clang fp contract(off)
+    using A = SmoothArith<T>;
+    using G = BoxGeom<STEPS, EI, EJ, EK>;
+    constexpr int BI = G::BI, BJ = G::BJ, BK = G::BK, NB = G::kNodes, P = G::kPer;
+    constexpr int SI = BJ * BK;  // i stride of the box
+    __shared__ T buf[2][NB];
+
+    // XCD-aware block -> box (k fastest), as in k_poly_fused
+    const int G8 = gridDim.x >> 3;
+    const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
+    if (lin >= total) return;  // uniform per block, before any barrier
+    const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ti = lin / (tiles_k * tiles_j);
+    const int i0 = ti * EI - STEPS, j0 = tj * EJ - STEPS, k0 = tk * EK - STEPS;
+
+    const A h2inv = static_cast<A>(h2inv_d);
+    A c1[STEPS], c2[STEPS];
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+        c1[s] = static_cast<A>(cs.c1[s]);
+        c2[s] = static_cast<A>(cs.c2[s]);
+    }
+    // the box's planes in the domain, [ia, iz): one buffer descriptor each
+    // for x and b (a span of at most BI planes: < 2^32 bytes for n <= 2048)
+    const int64_t nn = (int64_t)n * n;
+    const uint32_t row_bytes = (uint32_t)n * (uint32_t)sizeof(T);
+    const uint32_t plane_bytes = (uint32_t)nn * (uint32_t)sizeof(T);
+    const int ia = max(i0, 0), iz = min(i0 + BI, n);
+    const uint32_t span = (uint32_t)(iz - ia) * plane_bytes;
+    const __amdgpu_buffer_rsrc_t rb = plane_rsrc(b + ia * nn, span);
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(x ? x + ia * nn : b, x ? span : 0u);
+
+    // per owned node: x, b, d and its depth (distance to the box surface;
+    // 0 for slots past the box, which no step advances) | 256 if in the domain
+    T xv[P], bv[P], dv[P];
+    int info[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = (int)threadIdx.x + m * kBoxThreads;
+        const int ii = e / SI, jj = (e / BK) % BJ, kk = e % BK;
+        const int gi = i0 + ii, gj = j0 + jj, gk = k0 + kk;
+        const bool node = e < NB;
+        const bool in = node && (unsigned)gi < (unsigned)n && (unsigned)gj < (unsigned)n &&
+                        (unsigned)gk < (unsigned)n;
+        const int dep = min(min(min(ii, BI - 1 - ii), min(jj, BJ - 1 - jj)), min(kk, BK - 1 - kk));
+        info[m] = node ? (dep | (in ? 256 : 0)) : 0;
+        const uint32_t off = in ? (uint32_t)(gi - ia) * plane_bytes + (uint32_t)gj * row_bytes +
+                                      (uint32_t)gk * (uint32_t)sizeof(T)
+                                : kNoOffset;
+        xv[m] = buf_load<T>(rx, off);
+        bv[m] = buf_load<T>(rb, off);
+        dv[m] = T(0);
+        if (node) buf[0][e] = xv[m];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+        const T* cur = buf[s & 1];
+        T* nxt = buf[(s + 1) & 1];
+#pragma unroll
+        for (int m = 0; m < P; ++m) {
+            if ((info[m] & 255) < s + 1) continue;
+            const int e = (int)threadIdx.x + m * kBoxThreads;
+            const A c = static_cast<A>(xv[m]);
+            const A im = static_cast<A>(cur[e - SI]), ip = static_cast<A>(cur[e + SI]);
+            const A jm = static_cast<A>(cur[e - BK]), jp = static_cast<A>(cur[e + BK]);
+            const A km = static_cast<A>(cur[e - 1]), kp = static_cast<A>(cur[e + 1]);
+            const A res = static_cast<A>(bv[m]) - h2inv * lap_num<A>(c, im, ip, jm, jp, km, kp);
+            const A dn = s == 0 ? c2[0] * res : c1[s] * static_cast<A>(dv[m]) + c2[s] * res;
+            const bool live = (info[m] & 256) != 0;
+            const T xn = live ? static_cast<T>(c + dn) : T(0);
+            dv[m] = live ? static_cast<T>(dn) : T(0);
+            xv[m] = xn;
+            nxt[e] = xn;
+        }
+        __syncthreads();
+    }
+    // output nodes (the box shrunk by STEPS), k fastest, from the last LDS copy
+    const T* fin = buf[STEPS & 1];
+    const int oi = ti * EI, oj = tj * EJ, ok = tk * EK;
+    const int oz = min(oi + EI, n);
+    const __amdgpu_buffer_rsrc_t ro = plane_rsrc(xo + oi * nn, (uint32_t)(oz - oi) * plane_bytes);
+    constexpr int NO = EI * EJ * EK;
+#pragma unroll
+    for (int o = (int)threadIdx.x; o < NO; o += kBoxThreads) {
+        const int ii = o / (EJ * EK), jj = (o / EK) % EJ, kk = o % EK;
+        const bool in = oj + jj < n && ok + kk < n;  // planes past n: out of the span
+        const uint32_t off = in ? (uint32_t)ii * plane_bytes + (uint32_t)(oj + jj) * row_bytes +
+                                      (uint32_t)(ok + kk) * (uint32_t)sizeof(T)
+                                : kNoOffset;
+        buf_store<T>(fin[((ii + STEPS) * BJ + jj + STEPS) * BK + kk + STEPS], ro, off);
+    }
+}
+
+// Launch geometry of k_poly_box for an n^3 level.
+template <int EI, int EJ, int EK>
+inline dim3 box_grid(int n, int& tiles_k, int& tiles_j, int& total) {
+    tiles_k = (n + EK - 1) / EK;
+    tiles_j = (n + EJ - 1) / EJ;
+    const int tiles_i = (n + EI - 1) / EI;
+    total = tiles_k * tiles_j * tiles_i;
     return dim3((unsigned)((total + 7) / 8 * 8));
 }
 

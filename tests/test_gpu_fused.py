@@ -1,7 +1,7 @@
 """Temporally blocked smoothing (one launch per sweep or per pre/post phase)
 against one launch per step: the V-cycle output is bit for bit the same for
 every fusion mode, plane chunk, precision, smoother and degree, on grid sizes
-that leave partial tiles and chunks; the autotuned choice is recorded in the
+that leave partial tiles, chunks and boxes; the autotuned choice is recorded in the
 JSON tune cache; a solve with autotuned fusion equals the unfused solve."""
 import json
 import os
@@ -67,6 +67,18 @@ def test_fused_smoothing_is_bitwise_unfused(ctx, spec, n, precision):
             assert fus[0] == (fuse, ichunk, rows)
             np.testing.assert_array_equal(out, ref,
                                           err_msg=f"fuse={fuse} ichunk={ichunk} rows={rows}")
+        # box form: one workgroup per box of output nodes, up to 3 steps
+        for box, (ei, ej, ek) in H.BOX_EDGES.items():
+            nbytes = 8 if precision == H.PRECISION_FP64 else 4
+            nodes = (ei + 2 * group) * (ej + 2 * group) * (ek + 2 * group)
+            fits = group <= 3 and 2 * nodes * nbytes <= 160 * 1024  # two LDS copies
+            if not fits:
+                with pytest.raises(H.NumaError):
+                    _vcycle(ctx, n, _cfg(spec, fuse, 0, box), precision, r)
+                continue
+            out, fus = _vcycle(ctx, n, _cfg(spec, fuse, 0, box), precision, r)
+            assert fus[0] == (fuse, 0, box)
+            np.testing.assert_array_equal(out, ref, err_msg=f"fuse={fuse} box={box}")
 
 
 def test_autotune_records_choice(ctx):
@@ -79,7 +91,7 @@ def test_autotune_records_choice(ctx):
     assert all(m in (H.FUSE_NONE, H.FUSE_SWEEP, H.FUSE_PHASE) for m, _, _ in fus), fus
     with open(path) as f:
         cache = json.load(f)
-    keys = [k for k in cache if "|mg-smooth-v2|f32|n=96|" in k]
+    keys = [k for k in cache if "|mg-smooth-v3|f32|n=96|" in k]
     assert keys, cache
     mode, ichunk, rows = (int(v) for v in cache[keys[0]].split(","))
     assert (mode, ichunk, rows) == fus[0]
