@@ -49,13 +49,20 @@ def make_params(H, forcing: str):
     return H.krylov_params(gmres_restart=30, gmres_max_iters=600)
 
 
+# Coarsest level (16^3 at 256^3 with 5 levels) solved exactly by sine
+# transforms in every leg.
+COARSE = -1
+
+
 def leg_configs(H, levels: int):
     """fp64: the plain method (one kernel per smoothing step); mixed: FGMRES in
     mixed precision with autotuned temporally blocked smoothing; fp64_fused: the
     FP64 method with the same smoothing kernels as mixed (reported, so that the
     share of the speed-up that comes from precision is visible)."""
-    plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE)
-    fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO)
+    plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE,
+                      coarse_iters=COARSE)
+    fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO,
+                      coarse_iters=COARSE)
     fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                            precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=plain)
     mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
@@ -248,7 +255,7 @@ def learned_leg(H, ctx, args, params) -> dict:
                      "loss_last_window": round(float(hist[-k:].mean()), 4)}}
     log(f"learned: trained on {st.pairs} pairs in {st.seconds:.2f} s, loss "
         f"{out['train']['loss_first_window']} -> {out['train']['loss_last_window']}")
-    mg = H.mg_cheb(levels=args.levels, degree=3, pre=2, post=2)
+    mg = H.mg_cheb(levels=args.levels, degree=3, pre=2, post=2, coarse_iters=COARSE)
     cfg_l = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_LEARNED,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
     cfg_g = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
@@ -338,7 +345,7 @@ def sweep_leg(H, ctx, args, params) -> dict:
     xs = [ctx.array(x) for x in x0]
     cfg = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                           precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
-                          mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO))
+                          mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=COARSE))
     out = {"workload": f"bratu3d_sweep_{SWEEP_COUNT}_instances_n{'-'.join(map(str, SWEEP_SIZES))}"
                        f"_lambda1.0-6.7_u0_0.1noise_mixed_newton_fgmres30_gmg",
            "seed": SWEEP_SEED, "sizes": {str(k): int((n == k).sum()) for k in SWEEP_SIZES}}
@@ -467,6 +474,14 @@ def main() -> int:
     traffic = None
     if f0[0] == H.FUSE_NONE:
         kname = "k_stencil_lds<EpPoly> (finest Chebyshev smoothing step)"
+    elif f0[2] < 0:  # box form
+        steps = 3
+        ei, ej, ek = H.BOX_EDGES[int(f0[2])]
+        kbase = (f"k_poly_box<{'float' if main_leg == 'mixed' else 'double'},{steps},"
+                 f"{ei},{ej},{ek}>")
+        kname = f"{kbase} (finest-level fused Chebyshev sweep, box form)"
+        boxes = -(-args.n // ei) * -(-args.n // ej) * -(-args.n // ek)
+        traffic = measured_traffic(kbase, args.n, 0, -(-boxes // 8) * 8 * 512)
     else:
         steps = 3 if f0[0] == H.FUSE_SWEEP else 6
         kbase = f"k_poly_fused<{'float' if main_leg == 'mixed' else 'double'},{steps},{f0[2]}>"
@@ -490,7 +505,9 @@ def main() -> int:
            "config": {"workload": f"bratu3d_{args.n}^3_lambda{args.lam}_newton_"
                       f"{'fgmres' if main_leg == 'mixed' else 'gmres'}30_gmg{args.levels}"
                       "_cheb3_v22_fd_jv", "n": args.n, "lambda": args.lam,
-                      "tol": 1e-8, "forcing": args.forcing, "levels": args.levels},
+                      "tol": 1e-8, "forcing": args.forcing, "levels": args.levels,
+                      "coarse_solve": "exact (sine transforms)" if COARSE < 0 else
+                      f"{COARSE} Chebyshev steps"},
            "newton_iters": int(r.iters), "gmres_iters": int(r.n_lin_iters),
            "n_F": int(r.n_F), "n_JVP": int(r.n_JVP), "n_prec": int(r.n_prec),
            "final_rel_residual": r.res / r.res0, "roofline": roofline,

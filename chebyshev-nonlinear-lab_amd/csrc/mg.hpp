@@ -622,56 +622,98 @@ inline int fused_fill_chunk(int n, int steps, int rows, int cu_count) {
 // the 7-point -Lap_h with zero Dirichlet values is diagonal in the basis
 // S_pk = sin(pi (p+1)(k+1)/(n+1)) along each axis, with eigenvalues
 // h2inv (c_p + c_q + c_r), c_p = 2 - 2 cos(pi (p+1)/(n+1)), and S S = (n+1)/2 I,
-// so x = (2/(n+1))^3 S3 diag(1 / (h2inv (c_p + c_q + c_r))) S3 b.  One
-// workgroup: three passes of n-point transforms along k, j, i (4 threads per
-// line, every 4th output each), the scaling, three passes back; FP64, the
-// iterate double-buffered in LDS.
+// so x = (2/(n+1))^3 S3 diag(1 / (h2inv (c_p + c_q + c_r))) S3 b.
+//
+// One workgroup of 16 waves on a 16^3 block zero-padded past n (S is zero
+// past n, so padded values stay zero).  A transform along one axis is sixteen
+// 16 x 16 x 16 FP64 products, one per wave, each four
+// v_mfma_f64_16x16x4f64 steps (lane l: A[l%16][4c + l/16], B[4c + l/16][l%16];
+// result D[l/16 + 4r][l%16], r = 0..3); three passes, the scaling, three
+// passes back, the block double-buffered in LDS.  `tab` holds S (16 x 16,
+// padded) and c (16), computed on the host.
+constexpr int kDstTab = 16 * 16 + 16;
+
 template <class T>
-__global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv, const T* b, T* xout) {
+__global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv,
+                                                     const double* __restrict__ tab,
+                                                     const T* __restrict__ b,
+                                                     T* __restrict__ xout) {
 #pragma clang fp contract(off)
-    __shared__ double u[2][4096];
+    __shared__ double u[2][4096];  // (i*16 + j)*16 + k
     __shared__ double S[16][16];
-    __shared__ double c1[16];
-    const int N1 = n, N2 = n * n, N = N2 * n, t = threadIdx.x;
-    const double th = M_PI / (double)(n + 1);
-    if (t < N2) S[t / n][t % n] = sin(th * (double)((t / n + 1) * (t % n + 1)));
-    if (t < n) c1[t] = 2.0 - 2.0 * cos(th * (double)(t + 1));
-    for (int id = t; id < N; id += 1024) u[0][id] = static_cast<double>(b[id]);
+    __shared__ double cp[16];
+    const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    if (t < 256) S[t >> 4][t & 15] = tab[t];
+    if (t < 16) cp[t] = tab[256 + t];
+    for (int id = t; id < 4096; id += 1024) {
+        const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
+        u[0][id] = i < n && j < n && k < n ? static_cast<double>(b[(i * n + j) * n + k]) : 0.0;
+    }
     __syncthreads();
+    const int r16 = lane & 15, q = lane >> 4;
+    using d4 = __attribute__((ext_vector_type(4))) double;
     int cur = 0;
-    // one pass along the axis of stride st; line l = (a, c) starts at a*A + c*C
-    auto pass = [&](int st, int A, int C) {
-        const int l = t >> 2, qd = t & 3;
-        if (l < N2) {
-            const int base = (l / n) * A + (l % n) * C;
-            const double* in = u[cur];
-            double* out = u[cur ^ 1];
-            double v[16];
+    // Pass with A(row, kk) / B(kk, col) read through `ld_a` / `ld_b` and
+    // D(row, col) written through `st`; wave g takes product g.
+    auto pass = [&](auto ld_a, auto ld_b, auto st) {
+        const double* in = u[cur];
+        double* out = u[cur ^ 1];
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = k < n ? in[base + k * st] : 0.0;
-            for (int p = qd; p < n; p += 4) {
-                double s = 0.0;
+        for (int c = 0; c < 4; ++c)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ld_a(in, r16, 4 * c + q),
+                                                       ld_b(in, 4 * c + q, r16), acc, 0, 0, 0);
 #pragma unroll
-                for (int k = 0; k < 16; ++k) s += S[p][k < n ? k : 0] * v[k];
-                out[base + p * st] = s;
-            }
-        }
+        for (int r = 0; r < 4; ++r) st(out, q + 4 * r, r16, acc[r]);
         __syncthreads();
         cur ^= 1;
     };
-    pass(1, N2, N1);
-    pass(N1, N2, 1);
-    pass(N2, N1, 1);
+    // along k: D[j][p] = sum_k U[g][j][k] S[p][k]
+    auto pass_k = [&]() {
+        pass([&](const double* in, int row, int kk) { return in[(g * 16 + row) * 16 + kk]; },
+             [&](const double*, int kk, int col) { return S[col][kk]; },
+             [&](double* out, int row, int col, double v) { out[(g * 16 + row) * 16 + col] = v; });
+    };
+    // along j: D[p][k] = sum_j S[p][j] U[g][j][k]
+    auto pass_j = [&]() {
+        pass([&](const double*, int row, int kk) { return S[row][kk]; },
+             [&](const double* in, int kk, int col) { return in[(g * 16 + kk) * 16 + col]; },
+             [&](double* out, int row, int col, double v) { out[(g * 16 + row) * 16 + col] = v; });
+    };
+    // along i: D[p][k] = sum_i S[p][i] U[i][g][k]
+    auto pass_i = [&]() {
+        pass([&](const double*, int row, int kk) { return S[row][kk]; },
+             [&](const double* in, int kk, int col) { return in[(kk * 16 + g) * 16 + col]; },
+             [&](double* out, int row, int col, double v) { out[(row * 16 + g) * 16 + col] = v; });
+    };
+    pass_k();
+    pass_j();
+    pass_i();
     const double f = 8.0 / ((double)(n + 1) * (double)(n + 1) * (double)(n + 1));
-    for (int id = t; id < N; id += 1024) {
-        const int i = id / N2, j = (id / N1) % N1, k = id % N1;
-        u[cur][id] = f * (u[cur][id] / (h2inv * (c1[i] + c1[j] + c1[k])));
+    for (int id = t; id < 4096; id += 1024) {
+        const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
+        if (i < n && j < n && k < n)
+            u[cur][id] = f * (u[cur][id] / (h2inv * (cp[i] + cp[j] + cp[k])));
     }
     __syncthreads();
-    pass(1, N2, N1);
-    pass(N1, N2, 1);
-    pass(N2, N1, 1);
-    for (int id = t; id < N; id += 1024) xout[id] = static_cast<T>(u[cur][id]);
+    pass_k();
+    pass_j();
+    pass_i();
+    for (int id = t; id < 4096; id += 1024) {
+        const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
+        if (i < n && j < n && k < n) xout[(i * n + j) * n + k] = static_cast<T>(u[cur][id]);
+    }
+}
+
+// The table of k_coarse_dst for an n^3 level: S (padded with zeros) and c.
+inline std::vector<double> dst_table(int64_t n) {
+    std::vector<double> tab(kDstTab, 0.0);
+    const double th = M_PI / (double)(n + 1);
+    for (int64_t p = 0; p < n; ++p) {
+        for (int64_t k = 0; k < n; ++k) tab[(size_t)(p * 16 + k)] = std::sin(th * (double)((p + 1) * (k + 1)));
+        tab[(size_t)(256 + p)] = 2.0 - 2.0 * std::cos(th * (double)(p + 1));
+    }
+    return tab;
 }
 
 inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
@@ -760,6 +802,15 @@ public:
         }
         require(cfg.coarse_iters >= 0 || levels_.back().n <= 16,
                 "the exact coarse solve (coarse_iters < 0) needs a coarsest level of at most 16^3");
+        if (cfg.coarse_iters < 0) {
+            const std::vector<double> tab = dst_table(levels_.back().n);
+            void* p = nullptr;
+            NUMA_HIP_CHECK(hipMalloc(&p, tab.size() * sizeof(double)));
+            allocs_.push_back(p);
+            dst_tab_ = static_cast<double*>(p);
+            NUMA_HIP_CHECK(hipMemcpy(dst_tab_, tab.data(), tab.size() * sizeof(double),
+                                     hipMemcpyHostToDevice));
+        }
         set_resid_restrict(-1);
     }
     void set_resid_restrict(int64_t ich) override {
@@ -888,6 +939,8 @@ private:
         int rows = 4;               // rows per thread of fused launches
         int rr_ich = 0;             // coarse planes per workgroup of k_resid_restrict (0: off)
     };
+
+    double* dst_tab_ = nullptr;  // table of the exact coarse solve (coarse_iters < 0)
 
     T* alloc(int64_t N) {
         void* p = nullptr;
@@ -1132,7 +1185,7 @@ private:
         const bool rb = cfg_.smoother == NUMA_SMOOTH_RBGS;
         if (cfg_.coarse_iters < 0) {  // exact (sine-transform) solve, n <= 16
             hipLaunchKernelGGL((k_coarse_dst<T>), dim3(1), dim3(1024), 0, c_.stream, (int)L.n,
-                               L.h2inv, (const T*)L.b, L.x);
+                               L.h2inv, (const double*)dst_tab_, (const T*)L.b, L.x);
             NUMA_HIP_LAUNCH_CHECK();
             return;
         }
