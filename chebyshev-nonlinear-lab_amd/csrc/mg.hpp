@@ -500,14 +500,21 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
 // One fused launch of `steps` polynomial smoothing steps (x == nullptr: the
 // iterate starts at zero) with R rows per thread (the region is 64 x 16R).
 // Row counts: {2, 4} for FP32, {1, 2} for FP64 storage.
+// With norm_part, every workgroup also writes the sum of squares of the x it
+// stores there; returns the number of workgroups.
 template <class T, int S, int R>
-inline void launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolySteps& cs,
-                            const T* x, const T* b, T* xo) {
+inline unsigned launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolySteps& cs,
+                                const T* x, const T* b, T* xo, double* norm_part) {
     int tk, tj, ch;
     const dim3 g = fused_grid<S, R>(n, ichunk, tk, tj, ch);
-    hipLaunchKernelGGL((k_poly_fused<T, S, R>), g, dim3(kFusedThreads), 0, c.stream, n, ichunk, tk,
-                       tj, ch, h2inv, cs, x, b, xo);
+    if (norm_part)
+        hipLaunchKernelGGL((k_poly_fused<T, S, R, true>), g, dim3(kFusedThreads), 0, c.stream, n,
+                           ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
+    else
+        hipLaunchKernelGGL((k_poly_fused<T, S, R, false>), g, dim3(kFusedThreads), 0, c.stream, n,
+                           ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
     NUMA_HIP_LAUNCH_CHECK();
+    return g.x;
 }
 
 template <class T>
@@ -536,53 +543,57 @@ constexpr bool fused_fits(int steps, int rows) {
 }
 
 template <class T, int S, int SH>
-inline void launch_box_ss(Ctx& c, int n, double h2inv, const PolySteps& cs, const T* x, const T* b,
-                          T* xo) {
+inline unsigned launch_box_ss(Ctx& c, int n, double h2inv, const PolySteps& cs, const T* x,
+                              const T* b, T* xo, double* norm_part) {
     constexpr int EI = box_edge_i(SH), EJ = box_edge_j(SH), EK = box_edge_k(SH);
     if constexpr (fused_fits<T>(S, -SH)) {
         int tk, tj, total;
         const dim3 g = box_grid<EI, EJ, EK>(n, tk, tj, total);
-        hipLaunchKernelGGL((k_poly_box<T, S, EI, EJ, EK>), g, dim3(kBoxThreads), 0, c.stream, n,
-                           tk, tj, total, h2inv, cs, x, b, xo);
+        if (norm_part)
+            hipLaunchKernelGGL((k_poly_box<T, S, EI, EJ, EK, true>), g, dim3(kBoxThreads), 0,
+                               c.stream, n, tk, tj, total, h2inv, cs, x, b, xo, norm_part);
+        else
+            hipLaunchKernelGGL((k_poly_box<T, S, EI, EJ, EK, false>), g, dim3(kBoxThreads), 0,
+                               c.stream, n, tk, tj, total, h2inv, cs, x, b, xo, norm_part);
         NUMA_HIP_LAUNCH_CHECK();
+        return g.x;
     } else {
         throw Error(NUMA_ERR_ARG, "box smoothing: steps per launch x box exceed the LDS");
     }
 }
 
 template <class T, int S>
-inline void launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv, const PolySteps& cs,
-                           const T* x, const T* b, T* xo) {
+inline unsigned launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv,
+                               const PolySteps& cs, const T* x, const T* b, T* xo,
+                               double* norm_part) {
     constexpr int R0 = fused_rows_small<T>();
     switch (rows) {
-        case -1: launch_box_ss<T, S, 1>(c, n, h2inv, cs, x, b, xo); return;
-        case -2: launch_box_ss<T, S, 2>(c, n, h2inv, cs, x, b, xo); return;
-        case -3: launch_box_ss<T, S, 3>(c, n, h2inv, cs, x, b, xo); return;
+        case -1: return launch_box_ss<T, S, 1>(c, n, h2inv, cs, x, b, xo, norm_part);
+        case -2: return launch_box_ss<T, S, 2>(c, n, h2inv, cs, x, b, xo, norm_part);
+        case -3: return launch_box_ss<T, S, 3>(c, n, h2inv, cs, x, b, xo, norm_part);
         default: break;
     }
-    if (rows == R0) {
-        launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo);
-        return;
-    }
+    if (rows == R0) return launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part);
     if constexpr (fused_fits<T>(S, 2 * R0)) {
-        if (rows == 2 * R0) {
-            launch_fused_sr<T, S, 2 * R0>(c, n, ichunk, h2inv, cs, x, b, xo);
-            return;
-        }
+        if (rows == 2 * R0)
+            return launch_fused_sr<T, S, 2 * R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part);
     }
     throw Error(NUMA_ERR_ARG, "unsupported steps x rows per thread for fused smoothing");
 }
 
+// One fused launch; with norm_part (kMaxPartials doubles) also the sums of
+// squares of the written x per workgroup.  Returns the number of workgroups.
 template <class T>
-inline void launch_fused(Ctx& c, int n, int steps, int rows, int ichunk, double h2inv,
-                         const PolySteps& cs, const T* x, const T* b, T* xo) {
+inline unsigned launch_fused(Ctx& c, int n, int steps, int rows, int ichunk, double h2inv,
+                             const PolySteps& cs, const T* x, const T* b, T* xo,
+                             double* norm_part = nullptr) {
     switch (steps) {
-        case 1: launch_fused_s<T, 1>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
-        case 2: launch_fused_s<T, 2>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
-        case 3: launch_fused_s<T, 3>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
-        case 4: launch_fused_s<T, 4>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
-        case 5: launch_fused_s<T, 5>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
-        case 6: launch_fused_s<T, 6>(c, n, rows, ichunk, h2inv, cs, x, b, xo); break;
+        case 1: return launch_fused_s<T, 1>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 2: return launch_fused_s<T, 2>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 3: return launch_fused_s<T, 3>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 4: return launch_fused_s<T, 4>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 5: return launch_fused_s<T, 5>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 6: return launch_fused_s<T, 6>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
         default: throw Error(NUMA_ERR_ARG, "fused smoothing supports 1..6 steps");
     }
 }
@@ -722,6 +733,10 @@ inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1
 struct MGBase {
     virtual ~MGBase() = default;
     virtual void apply(const void* r, void* z) = 0;
+    // apply(), and ||z||_2 (FP64 sum of the squares of the stored z) to the
+    // device slot norm: from the last smoothing launch when that is a fused
+    // one, else by a separate reduction.
+    virtual void apply_norm(const void* r, void* z, double* norm) = 0;
     virtual int64_t levels() const = 0;
     virtual int64_t level_n(int64_t l) const = 0;
     // One finest-level smoothing step on internal buffers (primitive timing);
@@ -803,13 +818,16 @@ public:
         require(cfg.coarse_iters >= 0 || levels_.back().n <= 16,
                 "the exact coarse solve (coarse_iters < 0) needs a coarsest level of at most 16^3");
         if (cfg.coarse_iters < 0) {
-            const std::vector<double> tab = dst_table(levels_.back().n);
+            // on the context's stream: a null-stream copy would break a graph
+            // capture that another thread runs on its own context
+            dst_host_ = dst_table(levels_.back().n);
             void* p = nullptr;
-            NUMA_HIP_CHECK(hipMalloc(&p, tab.size() * sizeof(double)));
+            NUMA_HIP_CHECK(hipMalloc(&p, dst_host_.size() * sizeof(double)));
             allocs_.push_back(p);
             dst_tab_ = static_cast<double*>(p);
-            NUMA_HIP_CHECK(hipMemcpy(dst_tab_, tab.data(), tab.size() * sizeof(double),
-                                     hipMemcpyHostToDevice));
+            NUMA_HIP_CHECK(hipMemcpyAsync(dst_tab_, dst_host_.data(),
+                                          dst_host_.size() * sizeof(double),
+                                          hipMemcpyHostToDevice, c.stream));
         }
         set_resid_restrict(-1);
     }
@@ -923,6 +941,22 @@ public:
         final_dst_ = nullptr;
         F.b = nullptr;
     }
+    void apply_norm(const void* rv, void* zv, double* norm) override {
+        norm_slot_ = norm;
+        norm_done_ = false;
+        try {
+            apply(rv, zv);
+        } catch (...) {
+            norm_slot_ = nullptr;
+            throw;
+        }
+        norm_slot_ = nullptr;
+        if (!norm_done_) {
+            const int64_t N = levels_[0].n * levels_[0].n * levels_[0].n;
+            launch_dot<T, T>(c_, N, static_cast<const T*>(zv), static_cast<const T*>(zv),
+                             Epilogue{EPI_SQRT, 1.0, nullptr, norm, nullptr});
+        }
+    }
 
 private:
     struct Level {
@@ -940,7 +974,8 @@ private:
         int rr_ich = 0;             // coarse planes per workgroup of k_resid_restrict (0: off)
     };
 
-    double* dst_tab_ = nullptr;  // table of the exact coarse solve (coarse_iters < 0)
+    double* dst_tab_ = nullptr;      // table of the exact coarse solve (coarse_iters < 0)
+    std::vector<double> dst_host_;  // its host copy (source of the asynchronous upload)
 
     T* alloc(int64_t N) {
         void* p = nullptr;
@@ -1075,9 +1110,16 @@ private:
                 cs.c2[q] = c2s[(size_t)(st + q)];
             }
             T* dst = st + cnt >= total ? last_dst : nullptr;
-            launch_fused<T>(c_, (int)L.n, cnt, L.rows, L.ichunk, L.h2inv, cs,
-                            (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x,
-                            (const T*)L.b, dst ? dst : L.x2);
+            // the V-cycle's output launch also gives ||z|| when asked
+            double* np = dst && norm_slot_ ? c_.red.partials : nullptr;
+            const unsigned nb = launch_fused<T>(
+                c_, (int)L.n, cnt, L.rows, L.ichunk, L.h2inv, cs,
+                (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x, (const T*)L.b,
+                dst ? dst : L.x2, np);
+            if (np) {
+                c_.finish(nb, Epilogue{EPI_SQRT, 1.0, nullptr, norm_slot_, nullptr});
+                norm_done_ = true;
+            }
             if (dst)
                 final_done_ = true;
             else
@@ -1161,7 +1203,9 @@ private:
                     NUMA_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
                     ms = std::min(ms, t);
                 }
-                if (ms < best) {
+                // a later candidate must win by more than 2% (timing noise
+                // would otherwise flip between equal choices from run to run)
+                if (ms < 0.98 * best) {
                     best = ms;
                     pick = cd;
                 }
@@ -1270,6 +1314,8 @@ private:
     bool tuned_ = false;
     T* final_dst_ = nullptr;  // output of the V-cycle being applied
     bool final_done_ = false;
+    double* norm_slot_ = nullptr;  // apply_norm: device slot of ||z||
+    bool norm_done_ = false;
 };
 
 inline MGBase* make_mg(Ctx& c, int64_t n, const numa_mg_config& cfg, int64_t precision) {

@@ -104,15 +104,14 @@ constexpr int fused_waves_per_eu() {
     return (words <= 2 && STEPS <= 2) || (words <= 4 && STEPS == 1) ? 8 : 1;
 }
 
-template <class T, int STEPS, int R>
+// NORM: every workgroup also writes the sum of squares (FP64) of the x it
+// stores to norm_part[blockIdx.x] (0 for workgroups without a tile).
+template <class T, int STEPS, int R, bool NORM = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kFusedThreads),
                           amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R>()))) void
-k_poly_fused(int n, int ichunk, int tiles_k,
-                                                              int tiles_j, int chunks,
-                                                              double h2inv_d, PolySteps cs,
-                                                              const T* __restrict__ x,
-                                                              const T* __restrict__ b,
-                                                              T* __restrict__ xo) {
+k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2inv_d, PolySteps cs,
+             const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ xo,
+             double* __restrict__ norm_part) {
 #pragma clang fp contract(off)
     using A = SmoothArith<T>;
     using G = FusedGeom<STEPS, R>;
@@ -134,7 +133,10 @@ k_poly_fused(int n, int ichunk, int tiles_k,
     const int G8 = gridDim.x >> 3;
     const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
     const int total = tiles_k * tiles_j * chunks;
-    if (lin >= total) return;  // uniform per block, before any barrier
+    if (lin >= total) {  // uniform per block, before any barrier
+        if (NORM && threadIdx.x == 0) norm_part[blockIdx.x] = 0.0;
+        return;
+    }
     {
         T* flat = &ring[0][0][0][0];
         for (int e = threadIdx.x; e < STEPS * 2 * (H + 2) * (W + 2); e += kFusedThreads)
@@ -203,6 +205,7 @@ k_poly_fused(int n, int ichunk, int tiles_k,
 
     const int q_first = ib - STEPS, q_last = ie + STEPS - 1;
     const int nplanes = q_last - q_first + 1;
+    double nsq = 0.0;  // NORM: squares of the stored x
     T nx[R], nb[R];
     load_plane(q_first, nx, nb);
     __syncthreads();  // zero borders written
@@ -252,6 +255,10 @@ k_poly_fused(int n, int ichunk, int tiles_k,
                     const __amdgpu_buffer_rsrc_t ro =
                         plane_rsrc(xo + (int64_t)(p_out ? p : 0) * nn, p_out ? plane_bytes : 0u);
                     buf_store<T>(xn, ro, off_out[r]);
+                    if constexpr (NORM) {
+                        const double v = p_out && off_out[r] != kNoOffset ? (double)xn : 0.0;
+                        nsq += v * v;
+                    }
                 }
             }
         });
@@ -266,6 +273,10 @@ k_poly_fused(int n, int ichunk, int tiles_k,
         constexpr int PH = decltype(ph)::value;
         if (t0 + PH < nplanes) plane(ph, q_first + t0 + PH);
     });
+    if constexpr (NORM) {
+        const double t = block_sum<kFusedThreads>(nsq);
+        if (threadIdx.x == 0) norm_part[blockIdx.x] = t;
+    }
 }
 
 // Launch geometry of k_poly_fused for an n^3 level.
@@ -304,12 +315,13 @@ struct BoxGeom {
     static constexpr int kPer = (kNodes + kBoxThreads - 1) / kBoxThreads;
 };
 
-template <class T, int STEPS, int EI, int EJ, int EK>
+template <class T, int STEPS, int EI, int EJ, int EK, bool NORM = false>
 __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, int tiles_j,
                                                           int total, double h2inv_d, PolySteps cs,
                                                           const T* __restrict__ x,
                                                           const T* __restrict__ b,
-                                                          T* __restrict__ xo) {
+                                                          T* __restrict__ xo,
+                                                          double* __restrict__ norm_part) {
 #pragma clang fp contract(off)
     using A = SmoothArith<T>;
     using G = BoxGeom<STEPS, EI, EJ, EK>;
@@ -320,7 +332,10 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
     // XCD-aware block -> box (k fastest), as in k_poly_fused
     const int G8 = gridDim.x >> 3;
     const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
-    if (lin >= total) return;  // uniform per block, before any barrier
+    if (lin >= total) {  // uniform per block, before any barrier
+        if (NORM && threadIdx.x == 0) norm_part[blockIdx.x] = 0.0;
+        return;
+    }
     const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ti = lin / (tiles_k * tiles_j);
     const int i0 = ti * EI - STEPS, j0 = tj * EJ - STEPS, k0 = tk * EK - STEPS;
 
@@ -392,6 +407,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
     const int oz = min(oi + EI, n);
     const __amdgpu_buffer_rsrc_t ro = plane_rsrc(xo + oi * nn, (uint32_t)(oz - oi) * plane_bytes);
     constexpr int NO = EI * EJ * EK;
+    double nsq = 0.0;  // NORM: squares of the stored x
 #pragma unroll
     for (int o = (int)threadIdx.x; o < NO; o += kBoxThreads) {
         const int ii = o / (EJ * EK), jj = (o / EK) % EJ, kk = o % EK;
@@ -399,7 +415,16 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
         const uint32_t off = in ? (uint32_t)ii * plane_bytes + (uint32_t)(oj + jj) * row_bytes +
                                       (uint32_t)(ok + kk) * (uint32_t)sizeof(T)
                                 : kNoOffset;
-        buf_store<T>(fin[((ii + STEPS) * BJ + jj + STEPS) * BK + kk + STEPS], ro, off);
+        const T v = fin[((ii + STEPS) * BJ + jj + STEPS) * BK + kk + STEPS];
+        buf_store<T>(v, ro, off);
+        if constexpr (NORM) {
+            const double d = in && oi + ii < n ? (double)v : 0.0;
+            nsq += d * d;
+        }
+    }
+    if constexpr (NORM) {
+        const double t = block_sum<kBoxThreads>(nsq);
+        if (threadIdx.x == 0) norm_part[blockIdx.x] = t;
     }
 }
 

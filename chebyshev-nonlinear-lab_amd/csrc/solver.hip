@@ -375,10 +375,12 @@ private:
     // Exactly one of dotv (reduce out*dotv) / b (out = b - J v, reduce out^2)
     // is set when the launch reduces.
     template <class TV>
+    // vnorm_given: ||v|| is already in the S_VNORM slot (from the
+    // preconditioner that produced v).
     void jv(const double* x, const double* F0, const TV* v, T* out, const double* scale,
-            const T* dotv, const double* b, const Epilogue& ep) {
+            const T* dotv, const double* b, const Epilogue& ep, bool vnorm_given = false) {
         if (cfg_.jv_mode == NUMA_DERIV_FD) {
-            norm(v, MISC(S_VNORM));
+            if (!vnorm_given) norm(v, MISC(S_VNORM));
             const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), MISC(S_XNORM),
                             MISC(S_VNORM)};
             const LdXpHV<TV> L{x, v, st, 0.0};
@@ -417,6 +419,20 @@ private:
 
     // Preconditioner M^-1 in -> out; returns the result pointer (in itself
     // for no preconditioner, IdentityPrecond of gmres.hpp:37-39).
+    // precond(), and with vnorm (FD J v next) ||out|| into the S_VNORM slot
+    // when the V-cycle can give it with its last launch; returns whether it did.
+    const T* precond_for_jv(const T* in, T* out, bool& vnorm) {
+        vnorm = false;
+        const bool gmg = cfg_.precond == NUMA_PRECOND_GMG ||
+                         (cfg_.precond == NUMA_PRECOND_LEARNED && !(net_ && use_net_));
+        if (cfg_.jv_mode != NUMA_DERIV_FD || !gmg || !mg_ || capture_ || in == out)
+            return precond(in, out);
+        mg_->apply_norm(in, out, MISC(S_VNORM));
+        if (cfg_.precond == NUMA_PRECOND_LEARNED) ++lstats_.gmg_applications;
+        vnorm = true;
+        return out;
+    }
+
     const T* precond(const T* in, T* out) {
         const T* res = in;
         switch (cfg_.precond) {
@@ -722,11 +738,12 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             const bool by_net = cfg_.precond == NUMA_PRECOND_LEARNED && net_ && use_net_;
             run_step(j, x0, F0, [&]() {
                 const T* zj = nullptr;
-                timed(T_PREC, [&]() { zj = precond(V_[j], zdst); });
+                bool vnorm = false;
+                timed(T_PREC, [&]() { zj = precond_for_jv(V_[j], zdst, vnorm); });
                 // w = s_j J z_j into V_{j+1}; first MGS dot -> H[0][j] = s_0 <w, V_0raw>
                 timed(T_JVP, [&]() {
                     jv<T>(x0, F0, zj, V_[j + 1], S(j), V_[0], nullptr,
-                          Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr});
+                          Epilogue{EPI_STORE, 1.0, S(0), H(0, j), nullptr}, vnorm);
                 });
                 for (size_t i = 0; i <= j; ++i) {
                     const bool last = i == j;
@@ -1074,6 +1091,17 @@ int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev) {
     numa_hip::require(mg && r_dev && z_dev, "null argument");
     NUMA_HIP_CHECK(hipSetDevice(mg->ctx->device));
     mg->impl->apply(r_dev, z_dev);
+    NUMA_API_END
+}
+
+int numa_hip_mg_apply_norm(numa_hip_mg* mg, const void* r_dev, void* z_dev, double* norm_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(mg && r_dev && z_dev && norm_host, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(mg->ctx->device));
+    numa_hip::Ctx& c = *mg->ctx;
+    double* slot = c.dscal + (numa_hip::Ctx::kScalars - 1);  // this entry's own slot
+    mg->impl->apply_norm(r_dev, z_dev, slot);
+    c.read_scalars(slot, norm_host, 1);
     NUMA_API_END
 }
 

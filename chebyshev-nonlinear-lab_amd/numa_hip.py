@@ -243,6 +243,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_mg_create": [_vp, _i64, C.POINTER(MGConfig), _i64, C.POINTER(_vp)],
         "numa_hip_mg_levels": [_vp, C.POINTER(_i64), C.POINTER(_i64)],
         "numa_hip_mg_apply": [_vp, _vp, _vp],
+        "numa_hip_mg_apply_norm": [_vp, _vp, _vp, C.POINTER(C.c_double)],
         "numa_hip_mg_destroy": [_vp],
         "numa_hip_bratu_solver_create": [_vp, _i64, C.c_double, C.POINTER(SolverConfig), _i64,
                                          C.POINTER(_vp)],
@@ -473,6 +474,12 @@ class PoissonMG:
 
     def __call__(self, r: DeviceArray, z: DeviceArray) -> None:
         check(self.ctx.lib.numa_hip_mg_apply(self.h, r.ptr, z.ptr))
+
+    def apply_norm(self, r: DeviceArray, z: DeviceArray) -> float:
+        """z = V-cycle(r) and ||z||_2 (from the last launch when it is fused)."""
+        out = C.c_double()
+        check(self.ctx.lib.numa_hip_mg_apply_norm(self.h, r.ptr, z.ptr, C.byref(out)))
+        return out.value
 
     def fusion(self) -> list[tuple[int, int, int]]:
         """(NUMA_FUSE_* mode, plane chunk, rows per thread) per level."""

@@ -225,6 +225,12 @@ int numa_hip_mg_levels(const numa_hip_mg* mg, int64_t* levels, int64_t* sizes_ho
  * PoissonMG::operator() (poisson_mg.hpp:69-80).  r and z are double* for
  * NUMA_PRECISION_FP64 and float* for NUMA_PRECISION_MIXED. */
 int numa_hip_mg_apply(numa_hip_mg* mg, const void* r_dev, void* z_dev);
+/* numa_hip_mg_apply and *norm_host = ||z||_2 (FP64 sum of squares of the
+ * stored z), taken by the V-cycle's last launch when that is a fused
+ * smoothing launch.  The finite-difference J v of operators/operator.hpp:
+ * 101-114 needs ||z|| of the preconditioned vector; the solvers use this
+ * form.  No counterpart in the reference. */
+int numa_hip_mg_apply_norm(numa_hip_mg* mg, const void* r_dev, void* z_dev, double* norm_host);
 int numa_hip_mg_destroy(numa_hip_mg* mg);
 /* Per level: the smoother fusion in use (NUMA_FUSE_*; NUMA_FUSE_AUTO until the
  * first V-cycle has tuned it), its plane chunk and rows per thread (the

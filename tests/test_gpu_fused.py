@@ -2,7 +2,9 @@
 against one launch per step: the V-cycle output is bit for bit the same for
 every fusion mode, plane chunk, precision, smoother and degree, on grid sizes
 that leave partial tiles, chunks and boxes; the autotuned choice is recorded in the
-JSON tune cache; a solve with autotuned fusion equals the unfused solve."""
+JSON tune cache; a solve with autotuned fusion takes the unfused solve's
+iterations to the same solution (its FD steps take ||z|| from the fused output
+launch)."""
 import json
 import os
 
@@ -118,5 +120,32 @@ def test_solve_with_autotuned_fusion_matches_unfused(ctx, precision):
         if fuse == H.FUSE_AUTO:
             assert all(m != H.FUSE_AUTO for m, _, _ in s.mg_fusion())
         s.close()
-    assert results[0] == results[1]
-    np.testing.assert_array_equal(sols[0], sols[1])
+    # same V-cycles bit for bit; ||z|| of the FD step comes from the fused
+    # output launch instead of a separate reduction (another summation order),
+    # so the iterates differ in the last bits: same counts, and the converged
+    # ||F|| (set by the last inexact linear solve) and solutions agree closely
+    assert results[0][:3] == results[1][:3]
+    assert abs(results[0][3] - results[1][3]) <= 1e-2 * results[0][3]
+    assert np.linalg.norm(sols[0] - sols[1]) <= 1e-10 * np.linalg.norm(sols[0])
+
+
+@pytest.mark.parametrize("precision", [H.PRECISION_MIXED, H.PRECISION_FP64])
+@pytest.mark.parametrize("fuse,ichunk,rows", [(H.FUSE_NONE, 0, 0), (H.FUSE_SWEEP, 0, 0),
+                                              (H.FUSE_PHASE, 16, 0), (H.FUSE_SWEEP, 0, -2)])
+def test_vcycle_norm_from_the_output_launch(ctx, precision, fuse, ichunk, rows):
+    # apply_norm: the same z as apply, and ||z|| of the stored values (taken
+    # by the fused output launch, or by a separate reduction without fusion)
+    n = 45
+    dt = np.float32 if precision == H.PRECISION_MIXED else np.float64
+    r = np.random.default_rng(3).normal(size=n ** 3).astype(dt)
+    cfg = H.mg_cheb(levels=4, fuse=fuse, ichunk=ichunk, rows=rows)
+    mg = H.PoissonMG(ctx, n, cfg, precision)
+    dr, z1, z2 = ctx.array(r), ctx.empty(n ** 3, dt), ctx.empty(n ** 3, dt)
+    mg(dr, z1)
+    for rep in range(2):
+        nz = mg.apply_norm(dr, z2)
+        z = z2.numpy()
+        np.testing.assert_array_equal(z, z1.numpy())
+        ref = np.sqrt(np.sum(z.astype(np.float64) ** 2))
+        assert abs(nz - ref) <= 1e-13 * ref, (rep, nz, ref)
+    mg.close()

@@ -13,10 +13,10 @@ import numa_hip as H
 pytestmark = pytest.mark.gpu
 
 
-def _cfg(precision, fuse=H.FUSE_AUTO):
+def _cfg(precision, fuse=H.FUSE_AUTO, coarse_iters=24):
     mixed = precision == H.PRECISION_MIXED
     return H.solver_config(precond=H.PRECOND_GMG, precision=precision, flexible=1 if mixed else 0,
-                           mg=H.mg_cheb(levels=4, fuse=fuse))
+                           mg=H.mg_cheb(levels=4, fuse=fuse, coarse_iters=coarse_iters))
 
 
 @pytest.mark.parametrize("precision", [H.PRECISION_MIXED, H.PRECISION_FP64])
@@ -52,11 +52,13 @@ def _instances(count, sizes, seed):
     return n, lam, x0
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_sweep_matches_solves_alone(ctx, graphs):
+# coarse_iters -1: the exact coarse solve, whose set-up (a table upload) runs
+# in one worker while others capture their Arnoldi steps
+@pytest.mark.parametrize("graphs,coarse", [(True, 24), (False, 24), (True, -1)])
+def test_sweep_matches_solves_alone(ctx, graphs, coarse):
     n, lam, x0 = _instances(12, (16, 24, 32), 7)
     p = H.krylov_params(gmres_restart=30)
-    cfg = _cfg(H.PRECISION_MIXED)
+    cfg = _cfg(H.PRECISION_MIXED, coarse_iters=coarse)
     sw = H.Sweep(ctx, cfg, workers=4, restart=30)
     xs = [ctx.array(x) for x in x0]
     for run in range(2):  # the second run reuses solvers and captured graphs
