@@ -199,9 +199,13 @@ __global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, i
     const int64_t F1 = nf, F2 = F1 * F1;
     const int tid = threadIdx.x;
 
-    // slots of the padded region: fine (j, k) offset (-1 outside the domain)
-    // and the r index of interior slots (-1 for the halo ring)
-    int64_t off[kRRSlots];
+    // slots of the padded region: byte offset of the fine (j, k) column in a
+    // plane (kNoOffset outside the domain), and the r index of interior slots
+    // (-1 for the halo ring).  r outside the domain is stored as an exact 0,
+    // so the restriction adds its 27 terms without tests: a +0 term leaves
+    // the sum (which starts at +0) unchanged.
+    const uint32_t plane_bytes = (uint32_t)F2 * (uint32_t)sizeof(T);
+    uint32_t off[kRRSlots];
     int ridx[kRRSlots];
 #pragma unroll
     for (int s = 0; s < kRRSlots; ++s) {
@@ -209,50 +213,49 @@ __global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, i
         const int row = e / kRRPW, col = e % kRRPW;
         const int fj = 2 * J0 - 1 + row, fk = 2 * K0 - 1 + col;
         const bool valid = e < kRRPW * kRRPH;
-        off[s] = valid && fj >= 0 && fj < nf && fk >= 0 && fk < nf ? (int64_t)fj * F1 + fk : -1;
+        off[s] = valid && fj >= 0 && fj < nf && fk >= 0 && fk < nf
+                     ? (uint32_t)(fj * nf + fk) * (uint32_t)sizeof(T)
+                     : kNoOffset;
         ridx[s] = valid && row >= 1 && row <= kRRFH && col >= 1 && col <= kRRFW
                       ? (row - 1) * kRRFW + (col - 1)
                       : -1;
     }
-    auto load_x = [&](int q, T (&v)[kRRSlots]) {
+    // plane q of a (zeros outside the domain)
+    auto load_plane = [&](const T* a, int q, T (&v)[kRRSlots]) {
+        const bool q_in = (unsigned)q < (unsigned)nf;
+        const __amdgpu_buffer_rsrc_t rs =
+            plane_rsrc(a + (int64_t)(q_in ? q : 0) * F2, q_in ? plane_bytes : 0u);
 #pragma unroll
-        for (int s = 0; s < kRRSlots; ++s)
-            v[s] = off[s] >= 0 && q >= 0 && q < nf ? x[(int64_t)q * F2 + off[s]] : T(0);
-    };
-    auto load_b = [&](int q, T (&v)[kRRSlots]) {
-#pragma unroll
-        for (int s = 0; s < kRRSlots; ++s)
-            v[s] = off[s] >= 0 && ridx[s] >= 0 && q >= 0 && q < nf ? b[(int64_t)q * F2 + off[s]]
-                                                                   : T(0);
+        for (int s = 0; s < kRRSlots; ++s) v[s] = buf_load<T>(rs, off[s]);
     };
 
     // coarse node of this thread
     const int cK = K0 + (tid % kRRK), cJ = J0 + tid / kRRK;
     const bool active = cK < nc && cJ < nc;
+    const uint32_t coff = active ? (uint32_t)(cJ * nc + cK) * (uint32_t)sizeof(T) : kNoOffset;
+    const uint32_t cplane_bytes = (uint32_t)nc * (uint32_t)nc * (uint32_t)sizeof(T);
     const int lr = 2 * (cJ - J0), lc = 2 * (cK - K0);  // its first fine row / column in rpl
     const double w[3] = {0.25, 0.5, 0.25};
     auto add_terms = [&](double& acc, int a) {
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb) {
-            if (2 * cJ + bb >= nf) continue;
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                if (2 * cK + cc >= nf) continue;
+            for (int cc = 0; cc < 3; ++cc)
                 acc += w[a] * w[bb] * w[cc] *
                        static_cast<double>(rpl[(lr + bb) * kRRFW + lc + cc]);
-            }
         }
     };
     auto store = [&](int I, double acc) {
-        if (active) bc[((int64_t)I * nc + cJ) * nc + cK] = static_cast<T>(acc);
+        buf_store<T>(static_cast<T>(acc), plane_rsrc(bc + (int64_t)I * nc * nc, cplane_bytes),
+                     coff);
     };
 
     const int fb = 2 * Ib, fe = min(2 * Ie, nf - 1);
     T xm[kRRSlots], xc[kRRSlots], xp[kRRSlots], nx[kRRSlots], bv[kRRSlots], nb[kRRSlots];
-    load_x(fb - 1, xm);
-    load_x(fb, xc);
-    load_x(fb + 1, xp);
-    load_b(fb, bv);
+    load_plane(x, fb - 1, xm);
+    load_plane(x, fb, xc);
+    load_plane(x, fb + 1, xp);
+    load_plane(b, fb, bv);
     double acc = 0.0;
     for (int fi = fb; fi <= fe; ++fi) {
 #pragma unroll
@@ -261,8 +264,8 @@ __global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, i
             if (e < kRRPW * kRRPH) xpl[e] = xc[s];
         }
         if (fi < fe) {  // next planes in flight during this one
-            load_x(fi + 2, nx);
-            load_b(fi + 1, nb);
+            load_plane(x, fi + 2, nx);
+            load_plane(b, fi + 1, nb);
         }
         __syncthreads();
 #pragma unroll
@@ -274,7 +277,8 @@ __global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, i
                                 static_cast<double>(xp[s]), static_cast<double>(xpl[e - kRRPW]),
                                 static_cast<double>(xpl[e + kRRPW]),
                                 static_cast<double>(xpl[e - 1]), static_cast<double>(xpl[e + 1]));
-            rpl[ridx[s]] = static_cast<T>(static_cast<double>(bv[s]) - h2inv * sum);
+            const T r = static_cast<T>(static_cast<double>(bv[s]) - h2inv * sum);
+            rpl[ridx[s]] = off[s] != kNoOffset ? r : T(0);
         }
         __syncthreads();
         if ((fi & 1) == 0) {
