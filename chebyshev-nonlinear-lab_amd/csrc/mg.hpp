@@ -15,6 +15,8 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <utility>
 
 #include "blas.hpp"
@@ -1189,6 +1191,8 @@ private:
             hipEvent_t e0, e1;
             NUMA_HIP_CHECK(hipEventCreate(&e0));
             NUMA_HIP_CHECK(hipEventCreate(&e1));
+            // $NUMA_HIP_TUNE_LOG set: every candidate's time per phase to stderr
+            const bool log = std::getenv("NUMA_HIP_TUNE_LOG") != nullptr;
             double best = 1e300;
             Cand pick = cands[0];
             for (const Cand& cd : cands) {
@@ -1207,6 +1211,9 @@ private:
                     NUMA_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
                     ms = std::min(ms, t);
                 }
+                if (log)
+                    std::fprintf(stderr, "numa_hip tune: n=%lld mode=%d ichunk=%d rows=%d %.2f us\n",
+                                 (long long)L.n, cd.mode, cd.ichunk, cd.rows, ms * 1e3 / 3.0);
                 // a later candidate must win by more than 2% (timing noise
                 // would otherwise flip between equal choices from run to run)
                 if (ms < 0.98 * best) {
