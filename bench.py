@@ -54,15 +54,19 @@ def make_params(H, forcing: str):
 COARSE = -1
 
 
-def leg_configs(H, levels: int):
+def leg_configs(H, levels: int, smooth_rows=None):
     """fp64: the plain method (one kernel per smoothing step); mixed: FGMRES in
     mixed precision with autotuned temporally blocked smoothing; fp64_fused: the
     FP64 method with the same smoothing kernels as mixed (reported, so that the
     share of the speed-up that comes from precision is visible)."""
     plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE,
                       coarse_iters=COARSE)
-    fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO,
-                      coarse_iters=COARSE)
+    if smooth_rows is None:
+        fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO,
+                          coarse_iters=COARSE)
+    else:
+        fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_SWEEP,
+                          rows=smooth_rows, coarse_iters=COARSE)
     fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                            precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=plain)
     mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
@@ -421,6 +425,10 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--smooth-rows", type=int, default=None,
+                    help="pin the fused legs' smoothing to one launch per sweep with this "
+                         "rows-per-thread value (2 or 4 for FP32 march, -1..-3 box) on every "
+                         "level, plane chunks from the wave-filling model, instead of autotuning")
     ap.add_argument("--only", choices=["chandra", "learned", "sweep"], default=None,
                     help="run only this leg and print its JSON (for profiling)")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
@@ -453,7 +461,7 @@ def main() -> int:
         print(json.dumps({args.only: part}))
         ctx.close()
         return 0
-    cfgs = leg_configs(H, args.levels)
+    cfgs = leg_configs(H, args.levels, args.smooth_rows)
     legs = {}
     for name in args.legs.split(","):
         log(f"leg {name}: n={args.n} lambda={args.lam}")

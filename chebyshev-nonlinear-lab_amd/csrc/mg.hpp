@@ -513,10 +513,14 @@ inline unsigned launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const P
                                 const T* x, const T* b, T* xo, double* norm_part) {
     int tk, tj, ch;
     const dim3 g = fused_grid<S, R>(n, ichunk, tk, tj, ch);
-    if (norm_part)
-        hipLaunchKernelGGL((k_poly_fused<T, S, R, true>), g, dim3(kFusedThreads), 0, c.stream, n,
-                           ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
-    else
+    if (norm_part) {
+        // the variants that hold the FP64 accumulator without spilling
+        if constexpr (S <= 3 && R == (sizeof(T) == 4 ? 2 : 1))
+            hipLaunchKernelGGL((k_poly_fused<T, S, R, true>), g, dim3(kFusedThreads), 0, c.stream,
+                               n, ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
+        else
+            throw Error(NUMA_ERR_ARG, "fused sum of squares: at most 3 steps, smallest rows");
+    } else
         hipLaunchKernelGGL((k_poly_fused<T, S, R, false>), g, dim3(kFusedThreads), 0, c.stream, n,
                            ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
     NUMA_HIP_LAUNCH_CHECK();
@@ -1116,10 +1120,20 @@ private:
                 cs.c2[q] = c2s[(size_t)(st + q)];
             }
             T* dst = st + cnt >= total ? last_dst : nullptr;
-            // the V-cycle's output launch also gives ||z|| when asked
-            double* np = dst && norm_slot_ ? c_.red.partials : nullptr;
+            // The V-cycle's output launch also gives ||z|| when asked: the box
+            // form, or the march with the smallest rows per thread (any rows
+            // give the same x; the accumulator spills with more), for sweeps
+            // of up to 3 steps; else a separate reduction follows.
+            const bool with_norm = dst && norm_slot_ && cnt <= 3;
+            double* np = with_norm ? c_.red.partials : nullptr;
+            int rows = L.rows, ichunk = L.ichunk;
+            constexpr int R0 = fused_rows_small<T>();
+            if (with_norm && rows > R0) {
+                rows = R0;
+                ichunk = fused_fill_chunk<T>((int)L.n, cnt, R0, c_.cu_count);
+            }
             const unsigned nb = launch_fused<T>(
-                c_, (int)L.n, cnt, L.rows, L.ichunk, L.h2inv, cs,
+                c_, (int)L.n, cnt, rows, ichunk, L.h2inv, cs,
                 (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x, (const T*)L.b,
                 dst ? dst : L.x2, np);
             if (np) {

@@ -98,17 +98,20 @@ __device__ __forceinline__ void static_for(F&& f) {
 // Waves per SIMD the register allocation must allow: 8 (two 16-wave
 // workgroups per CU, at most 64 VGPRs) for the variants whose rings fit that
 // without spilling, else no bound.
-template <class T, int STEPS, int R>
+template <class T, int STEPS, int R, bool NORM>
 constexpr int fused_waves_per_eu() {
     constexpr int words = (int)(sizeof(T) / 4) * R;  // 32-bit registers per ring value
+    if (NORM) return 1;  // the FP64 accumulator: no bound
     return (words <= 2 && STEPS <= 2) || (words <= 4 && STEPS == 1) ? 8 : 1;
 }
 
 // NORM: every workgroup also writes the sum of squares (FP64) of the x it
-// stores to norm_part[blockIdx.x] (0 for workgroups without a tile).
+// stores to norm_part[blockIdx.x] (0 for workgroups without a tile).  Used
+// with the smallest rows per thread and at most 3 steps (the variants that
+// hold the accumulator without spilling).
 template <class T, int STEPS, int R, bool NORM = false>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, kFusedThreads),
-                          amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R>()))) void
+                          amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R, NORM>()))) void
 k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2inv_d, PolySteps cs,
              const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ xo,
              double* __restrict__ norm_part) {
