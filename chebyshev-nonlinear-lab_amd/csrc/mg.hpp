@@ -1123,11 +1123,15 @@ private:
             // The V-cycle's output launch also gives ||z|| when asked: the box
             // form, or the march with the smallest rows per thread (any rows
             // give the same x; the accumulator spills with more), for sweeps
-            // of up to 3 steps; else a separate reduction follows.
-            const bool with_norm = dst && norm_slot_ && cnt <= 3;
+            // of up to 3 steps; else a separate reduction follows.  FP32
+            // switches from 4 to 2 rows for it (equally fast); FP64 only uses
+            // it where the level runs 1 row (its 1-row march is slower than a
+            // separate reduction).
+            constexpr int R0 = fused_rows_small<T>();
+            const bool with_norm = dst && norm_slot_ && cnt <= 3 &&
+                                   (L.rows <= R0 || sizeof(T) == 4);
             double* np = with_norm ? c_.red.partials : nullptr;
             int rows = L.rows, ichunk = L.ichunk;
-            constexpr int R0 = fused_rows_small<T>();
             if (with_norm && rows > R0) {
                 rows = R0;
                 ichunk = fused_fill_chunk<T>((int)L.n, cnt, R0, c_.cu_count);
