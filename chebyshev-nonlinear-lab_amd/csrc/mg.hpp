@@ -50,12 +50,12 @@ struct EpPoly {
                                  P1 km, P1 kp) const {
 #pragma clang fp contract(off)
         const A cc = static_cast<A>(c.a);
-        const A res = static_cast<A>(e.b) -
-                      static_cast<A>(h2inv) *
-                          lap_num<A>(cc, static_cast<A>(im.a), static_cast<A>(ip.a),
-                                     static_cast<A>(jm.a), static_cast<A>(jp.a),
-                                     static_cast<A>(km.a), static_cast<A>(kp.a));
-        const A dn = USE_D ? static_cast<A>(c1) * static_cast<A>(e.d) + static_cast<A>(c2) * res
+        const A res = poly_res<A>(static_cast<A>(e.b), static_cast<A>(h2inv), cc,
+                                  static_cast<A>(im.a), static_cast<A>(ip.a),
+                                  static_cast<A>(jm.a), static_cast<A>(jp.a),
+                                  static_cast<A>(km.a), static_cast<A>(kp.a));
+        const A dn = USE_D ? poly_dir<A>(static_cast<A>(c1), static_cast<A>(e.d),
+                                         static_cast<A>(c2), res)
                            : static_cast<A>(c2) * res;
         if constexpr (WRITE_D) d[id] = static_cast<T>(dn);
         xn[id] = static_cast<T>(cc + dn);

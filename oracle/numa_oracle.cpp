@@ -277,9 +277,26 @@ struct MG {
         }
     }
 
+    // b - A x at a node with the smoothers' fused multiply-adds:
+    // s = fma(6, x, -x_im) - x_ip - x_jm - x_jp - x_km - x_kp (missing
+    // neighbours count as 0), res = fma(-h2inv, s, b).
+    static inline double res_fma_at(const double* x, const double* b, long n, double h2inv,
+                                     long i, long j, long k, long id) {
+        double s = std::fma(6.0, x[id], i > 0 ? -x[id - n * n] : -0.0);
+        if (i + 1 < n) s -= x[id + n * n];
+        if (j > 0) s -= x[id - n];
+        if (j + 1 < n) s -= x[id + n];
+        if (k > 0) s -= x[id - 1];
+        if (k + 1 < n) s -= x[id + 1];
+        return std::fma(-h2inv, s, b[id]);
+    }
+
     // One Chebyshev / Jacobi-type step:  d <- c1 d + c2 (b - A x);  x <- x + d.
-    // With c1 = 0, c2 = omega/(6 h2inv) it is one weighted-Jacobi sweep.
-    static void poly_step(Level& L, double c1, double c2, bool use_d) {
+    // With c1 = 0, c2 = omega/(6 h2inv) it is one weighted-Jacobi sweep.  The
+    // smoother (fused) rounds as the device smoothing kernels do: res by
+    // res_fma_at, d = fma(c1, d, c2 res); the coarsest-level iteration keeps
+    // separate products (as the device's one-workgroup coarse solve).
+    static void poly_step(Level& L, double c1, double c2, bool use_d, bool fused = false) {
         const long n = L.n;
         const double* x = L.x.data();
         const double* b = L.b.data();
@@ -290,8 +307,14 @@ struct MG {
             for (long j = 0; j < n; ++j)
                 for (long k = 0; k < n; ++k) {
                     const long id = idx(n, i, j, k);
-                    const double res = b[id] - apply_at(x, n, L.h2inv, i, j, k, id);
-                    const double dn = use_d ? c1 * d[id] + c2 * res : c2 * res;
+                    double res, dn;
+                    if (fused) {
+                        res = res_fma_at(x, b, n, L.h2inv, i, j, k, id);
+                        dn = use_d ? std::fma(c1, d[id], c2 * res) : c2 * res;
+                    } else {
+                        res = b[id] - apply_at(x, n, L.h2inv, i, j, k, id);
+                        dn = use_d ? c1 * d[id] + c2 * res : c2 * res;
+                    }
                     d[id] = dn;
                     xn[id] = x[id] + dn;
                 }
@@ -299,15 +322,15 @@ struct MG {
     }
 
     // Chebyshev iteration on D^-1 A over [alpha, beta], `deg` steps.
-    static void chebyshev(Level& L, int deg, double alpha, double beta) {
+    static void chebyshev(Level& L, int deg, double alpha, double beta, bool fused = false) {
         const double theta = 0.5 * (beta + alpha), delta = 0.5 * (beta - alpha);
         const double sigma = theta / delta;
         const double dinv = 1.0 / (6.0 * L.h2inv);
         double rho = 1.0 / sigma;
-        poly_step(L, 0.0, dinv / theta, false);
+        poly_step(L, 0.0, dinv / theta, false, fused);
         for (int k = 1; k < deg; ++k) {
             const double rho_new = 1.0 / (2.0 * sigma - rho);
-            poly_step(L, rho_new * rho, 2.0 * rho_new / delta * dinv, true);
+            poly_step(L, rho_new * rho, 2.0 * rho_new / delta * dinv, true, fused);
             rho = rho_new;
         }
     }
@@ -318,10 +341,12 @@ struct MG {
     void smooth(Level& L, int order) const {
         switch (cfg.smoother) {
             case NUMA_SMOOTH_RBGS: smooth_rbgs(L, order); break;
-            case NUMA_SMOOTH_WJACOBI: poly_step(L, 0.0, cfg.omega / (6.0 * L.h2inv), false); break;
+            case NUMA_SMOOTH_WJACOBI:
+                poly_step(L, 0.0, cfg.omega / (6.0 * L.h2inv), false, true);
+                break;
             default: {
                 const double b = lmax(L.n);
-                chebyshev(L, (int)cfg.degree, b / cfg.cheb_ratio, b);
+                chebyshev(L, (int)cfg.degree, b / cfg.cheb_ratio, b, true);
             }
         }
     }
