@@ -192,6 +192,7 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
     }
 
     const int lane = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    A h_row[R];  // h2inv in the domain, 0 for columns outside it
     const int k = k0 + lane;
     const int64_t nn = (int64_t)n * n;  // plane stride
     bool col_in[R];
@@ -207,6 +208,7 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
         jrow[r] = ty + (kFusedThreads / 64) * r;
         const int j = j0 + jrow[r];
         col_in[r] = (unsigned)j < (unsigned)n && (unsigned)k < (unsigned)n;
+        h_row[r] = col_in[r] ? h2inv : A(0);
         off[r] = col_in[r] ? (uint32_t)(j * n + k) * (uint32_t)sizeof(T) : kNoOffset;
         off_out[r] = out_k && jrow[r] >= STEPS && jrow[r] < H - STEPS ? off[r] : kNoOffset;
     }
@@ -262,7 +264,7 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
             constexpr int s = decltype(sc)::value;
             constexpr int tp = PH - 1 - s + 6 * U;  // t of the plane advanced (mod U), >= 0
             const int p = q - 1 - s;
-            const bool p_in = (unsigned)p < (unsigned)n;
+            const bool p_in = (unsigned)p < (unsigned)n;  // uniform
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int jr = jrow[r];
@@ -273,17 +275,19 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
                 const A jp = static_cast<A>(ring[s][tp & 1][jr + 2][lane + 1]);
                 const A km = static_cast<A>(ring[s][tp & 1][jr + 1][lane]);
                 const A kp = static_cast<A>(ring[s][tp & 1][jr + 1][lane + 2]);
+                // h = 0 outside the domain (columns: h_row; planes: p_in), where
+                // b = 0 and x = 0: then res = 0, d = 0 and x stays exactly 0
+                const A h = p_in ? h_row[r] : A(0);
                 const A res =
-                    poly_res<A>(static_cast<A>(bb[tp % BR][r]), h2inv, c, im, ip, jm, jp, km, kp);
+                    poly_res<A>(static_cast<A>(bb[tp % BR][r]), h, c, im, ip, jm, jp, km, kp);
                 A dn;
                 if constexpr (s == 0)
                     dn = c2[0] * res;
                 else
                     dn = poly_dir<A>(c1[s], static_cast<A>(ds[s - 1][r][tp & 1]), c2[s], res);
-                const bool live = p_in && col_in[r];
-                const T xn = live ? static_cast<T>(c + dn) : T(0);
+                const T xn = static_cast<T>(c + dn);
                 if constexpr (s + 1 < STEPS) {
-                    ds[s][r][tp & 1] = live ? static_cast<T>(dn) : T(0);
+                    ds[s][r][tp & 1] = static_cast<T>(dn);
                     xs[s + 1][r][tp % 3] = xn;
                     ring[s + 1][tp & 1][jr + 1][lane + 1] = xn;
                 } else {
