@@ -9,6 +9,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -75,6 +77,16 @@ struct Epilogue {
     double* out;
     double* out2;
 };
+
+// Calls f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, U-1>{}).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int U, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, U>{});
+}
 
 __device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -119,16 +119,6 @@ __device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_
             __builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, v), r, (int)off, 0, 0);
 }
 
-// Calls f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, U-1>{}).
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-    (f(std::integral_constant<int, I>{}), ...);
-}
-template <int U, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-    static_for_impl(f, std::make_integer_sequence<int, U>{});
-}
-
 // Waves per SIMD the register allocation must allow: 8 (two 16-wave
 // workgroups per CU, at most 64 VGPRs) for the variants whose rings fit that
 // without spilling, else no bound.

@@ -161,13 +161,15 @@ __global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, 
     const int k0 = bx * TK, j0 = by * TJ, k = k0 + tk;
     const int i0 = bz * ichunk, i1 = min(n, i0 + ichunk);
     const int64_t N1 = n, N2 = N1 * N1;
+    // node (q, j, k) is id = q N2 + col with col = j n + k < 2^22 (n <= 2048):
+    // the plane part is uniform, col a 32-bit per-thread offset
     bool own[RW];
-    int64_t col[RW];
+    uint32_t col[RW];  // 0 for columns outside the domain (a valid address)
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int j = j0 + tj + 4 * r;
         own[r] = j < n && k < n;
-        col[r] = own[r] ? (int64_t)j * N1 + k : 0;
+        col[r] = own[r] ? (uint32_t)(j * n + k) : 0u;
     }
     // halo node of this thread: threads 0..127 the rows j0-1 and j0+TJ,
     // threads 128..128+2TJ-1 the columns k0-1 and k0+TK
@@ -185,66 +187,76 @@ __global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, 
         hk = side ? k0 + TK : k0 - 1;
     }
     const bool hown = hr >= 0 && hj >= 0 && hj < n && hk >= 0 && hk < n;
-    const int64_t hcol = hown ? (int64_t)hj * N1 + hk : 0;
-    // loads at invalid nodes go to node 0 and their values are replaced by zero
-    auto own_ok = [&](int r, int q) { return own[r] && q >= 0 && q < n; };
-    auto own_id = [&](int r, int q) -> int64_t {
-        return own_ok(r, q) ? (int64_t)q * N2 + col[r] : 0;
-    };
-    auto halo_id = [&](int q) -> int64_t {
-        return hown && q >= 0 && q < n ? (int64_t)q * N2 + hcol : 0;
-    };
+    const uint32_t hcol = hown ? (uint32_t)(hj * n + hk) : 0u;
+    // loads of planes outside the domain go to plane 0, of columns outside it
+    // to offset 0; their values are replaced by zero
+    auto plane_of = [&](int q) -> int64_t { return (unsigned)q < (unsigned)n ? q * N2 : 0; };
+    auto own_ok = [&](int r, int q) { return own[r] && (unsigned)q < (unsigned)n; };
 
-    P pm[RW], pc[RW];
-    R q1[RW], q2[RW], q3[RW];  // own raw values of planes i+1, i+2, i+3
-    X e0[RW], e1[RW], e2[RW];  // epilogue loads of planes i, i+1, i+2
-    R h0 = L.load(halo_id(i0)), h1 = L.load(halo_id(i0 + 1)), h2 = h1;
+    // Register queues indexed by a plane's position t = p - i0 in the chunk
+    // modulo 3 (P: planes i-1, i, i+1; Q: raw own values of planes i+1..i+3;
+    // E: epilogue loads of planes i..i+2; H: halo raw values of planes
+    // i..i+2), the LDS plane by t mod 2.  The march is unrolled by 6 planes
+    // so that every slot is a compile-time register (no values move).
+    P Pv[3][RW];
+    R Q[3][RW];
+    X Ev[3][RW];
+    R H[3];
+    H[0] = L.load(plane_of(i0) + hcol);
+    H[1] = L.load(plane_of(i0 + 1) + hcol);
+    H[2] = H[1];
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-        const R a = L.load(own_id(r, i0 - 1)), b = L.load(own_id(r, i0));
-        q1[r] = L.load(own_id(r, i0 + 1));
-        q2[r] = L.load(own_id(r, i0 + 2));
-        q3[r] = q2[r];
-        e0[r] = E.preload(own_id(r, i0));
-        e1[r] = E.preload(own_id(r, i0 + 1));
-        e2[r] = e1[r];
-        pm[r] = own_ok(r, i0 - 1) ? L.make(a) : P::zero();
-        pc[r] = own_ok(r, i0) ? L.make(b) : P::zero();
+        const R a = L.load(plane_of(i0 - 1) + col[r]), b = L.load(plane_of(i0) + col[r]);
+        Q[1][r] = L.load(plane_of(i0 + 1) + col[r]);
+        Q[2][r] = L.load(plane_of(i0 + 2) + col[r]);
+        Q[0][r] = Q[2][r];
+        Ev[0][r] = E.preload(plane_of(i0) + col[r]);
+        Ev[1][r] = E.preload(plane_of(i0 + 1) + col[r]);
+        Ev[2][r] = Ev[1][r];
+        Pv[2][r] = own_ok(r, i0 - 1) ? L.make(a) : P::zero();
+        Pv[0][r] = own_ok(r, i0) ? L.make(b) : P::zero();
+        Pv[1][r] = Pv[0][r];
     }
     double acc = 0.0;
-    for (int i = i0; i < i1; ++i) {
-        P(&B)[TJ + 2][TK + 2] = buf[i & 1];
+    // plane i = i0 + t, PH = t mod 6
+    auto plane = [&](auto ph, int i) {
+        constexpr int PH = decltype(ph)::value;
+        constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;  // t, t+1, t+2 (= t-1)
+        P(&B)[TJ + 2][TK + 2] = buf[PH & 1];
 #pragma unroll
-        for (int r = 0; r < RW; ++r) B[tj + 4 * r + 1][tk + 1] = pc[r];
-        if (hr >= 0) B[hr][hc] = hown ? L.make(h0) : P::zero();
-        if (i + 3 <= i1) {
+        for (int r = 0; r < RW; ++r) B[tj + 4 * r + 1][tk + 1] = Pv[S0][r];
+        if (hr >= 0) B[hr][hc] = hown ? L.make(H[S0]) : P::zero();
+        if (i + 3 <= i1) {  // plane i+3 into the slot of plane i (consumed)
 #pragma unroll
-            for (int r = 0; r < RW; ++r) q3[r] = L.load(own_id(r, i + 3));
+            for (int r = 0; r < RW; ++r) Q[S0][r] = L.load(plane_of(i + 3) + col[r]);
         }
-        if (i + 2 < i1) {
+        if (i + 2 < i1) {  // plane i+2 into the slots of plane i-1
+            const int64_t pb = plane_of(i + 2);
 #pragma unroll
-            for (int r = 0; r < RW; ++r) e2[r] = E.preload(own_id(r, i + 2));
-            h2 = L.load(halo_id(i + 2));
+            for (int r = 0; r < RW; ++r) Ev[S2][r] = E.preload(pb + col[r]);
+            H[S2] = L.load(pb + hcol);
         }
         __syncthreads();
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-            const P pp = own_ok(r, i + 1) ? L.make(q1[r]) : P::zero();
+            const P pp = own_ok(r, i + 1) ? L.make(Q[S1][r]) : P::zero();
             if (own[r]) {
                 const int row = tj + 4 * r + 1;
-                acc += E((int64_t)i * N2 + col[r], e0[r], pc[r], pm[r], pp, B[row - 1][tk + 1],
-                         B[row + 1][tk + 1], B[row][tk], B[row][tk + 2]);
+                acc += E((int64_t)i * N2 + col[r], Ev[S0][r], Pv[S0][r], Pv[S2][r], pp,
+                         B[row - 1][tk + 1], B[row + 1][tk + 1], B[row][tk], B[row][tk + 2]);
             }
-            pm[r] = pc[r];
-            pc[r] = pp;
-            q1[r] = q2[r];
-            q2[r] = q3[r];
-            e0[r] = e1[r];
-            e1[r] = e2[r];
+            Pv[S1][r] = pp;
         }
-        h0 = h1;
-        h1 = h2;
-    }
+    };
+    int t0 = 0;
+    const int count = i1 - i0;
+    for (; t0 + 6 <= count; t0 += 6)
+        static_for<6>([&](auto ph) { plane(ph, i0 + t0 + decltype(ph)::value); });
+    static_for<6>([&](auto ph) {
+        constexpr int PH = decltype(ph)::value;
+        if (t0 + PH < count) plane(ph, i0 + t0 + PH);
+    });
     if constexpr (REDUCE) block_partial<256>(acc, red, lin);
 }
 
