@@ -20,6 +20,7 @@
 // finished instance stops counting and stops changing its iterate.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <limits>
 #include <memory>
@@ -35,6 +36,18 @@ namespace numa_hip {
 // batched kernels over the padded [Bp][Np] layout; grid (x chunks, B)
 // ---------------------------------------------------------------------------
 constexpr int kChBX = 16;  // blocks per instance of the batched streaming kernels
+// K is stored (FP64, plus bf16 hi / lo for split-BF16) when it takes at most
+// this many bytes; NUMA_HIP_CHANDRA_STORE_K=0 in the environment at creation
+// keeps the generating kernels instead.
+constexpr size_t kChKStoreMax = size_t(16) << 30;
+constexpr int kKdRT = 1, kKdCT = 4;  // k_gemm_f64_kd row, column tiles per wave
+// k_gemm_bf16x3_k: waves, row tiles per wave; 4 column tiles per wave (k chunk
+// 64) where the padded batch is a multiple of 128, else 2 (k chunk 128)
+constexpr int kKbWR = 4, kKbRT = 1;
+static_assert(kChPadN % (64 * kKdRT) == 0 && kChPadN % (kKbWR * 32 * kKbRT) == 0 &&
+                  kChPadN % (4 * kGbK) == 0 && kChPadB % (16 * kKdCT) == 0 &&
+                  kChPadB % 64 == 0,
+              "padded N and batch must hold whole contraction tiles and split-K chunks");
 
 struct BEpi {   // per-instance epilogue of a batched reduction
     int kind;   // EPI_STORE / EPI_SQRT / EPI_NORM_SCALE
@@ -278,9 +291,29 @@ public:
         if (prec_ == NUMA_PRECISION_MIXED) {
             Xh_ = static_cast<__bf16*>(zalloc(vec * sizeof(__bf16)));
             Xl_ = static_cast<__bf16*>(zalloc(vec * sizeof(__bf16)));
-            ksplit_ = (Np_ / kGbRows) * (Bp_ / kGbCols) >= 512 ? 1 : 2;
-            if ((Np_ / kGbRows) * (Bp_ / kGbCols) < 128) ksplit_ = 4;
+            // split K until the stored-K grid has 512 workgroups (at most 4 slices);
+            // the generating kernel uses the same slices, so both return the same bits
+            kb_ct_ = Bp_ % 128 == 0 ? 4 : 2;
+            const int64_t tiles = (Np_ / (kKbWR * 32 * kKbRT)) * (Bp_ / (32 * kb_ct_));
+            ksplit_ = 1;
+            while (ksplit_ < 4 && tiles * ksplit_ < 512) ksplit_ *= 2;
             P_ = static_cast<float*>(alloc((size_t)ksplit_ * vec * sizeof(float)));
+        }
+        const char* sk = std::getenv("NUMA_HIP_CHANDRA_STORE_K");
+        const size_t kbytes = (size_t)Np_ * (ldd() * 8 + (prec_ == NUMA_PRECISION_MIXED ? ldb() * 4 : 0));
+        if (!(sk && sk[0] == '0' && sk[1] == 0) && kbytes <= kChKStoreMax) {
+            Kd_ = static_cast<double*>(alloc((size_t)Np_ * ldd() * sizeof(double)));
+            if (prec_ == NUMA_PRECISION_MIXED) {
+                Kh_ = static_cast<__bf16*>(alloc((size_t)Np_ * ldb() * sizeof(__bf16)));
+                Kl_ = static_cast<__bf16*>(alloc((size_t)Np_ * ldb() * sizeof(__bf16)));
+            }
+            hipLaunchKernelGGL(k_fill_k, dim3((unsigned)((Np_ + 255) / 256), (unsigned)Np_),
+                               dim3(256), 0, c_.stream, (int)Np_, (int)ldd(), (int)ldb(),
+                               (const double*)mu_,
+                               (const double*)w_, (const float*)muf_, (const float*)wf_, Kd_,
+                               Kh_, Kl_);
+            NUMA_HIP_LAUNCH_CHECK();
+            c_.sync();
         }
         // per-instance scalars: H ((m+1) x m x B), s ((m+1) x B), misc
         nsc_ = ((m_ + 1) * m_ + (m_ + 1) + 8) * Bp_;
@@ -403,6 +436,14 @@ private:
     }
 
     void gemm_f64(const double* Xin, double* Yout) {
+        if (Kd_) {
+            hipLaunchKernelGGL((k_gemm_f64_kd<kKdRT, kKdCT>),
+                               dim3((unsigned)(Np_ / (64 * kKdRT)), (unsigned)(Bp_ / (16 * kKdCT))),
+                               dim3(256), 0, c_.stream, (int)Np_, (int)ldd(), (const double*)Kd_,
+                               Xin, Yout);
+            NUMA_HIP_LAUNCH_CHECK();
+            return;
+        }
         hipLaunchKernelGGL(k_gemm_f64, dim3((unsigned)(Np_ / kG64Rows), (unsigned)(Bp_ / kG64Cols)),
                            dim3(256), 0, c_.stream, (int)Np_, (const double*)mu_, (const double*)w_,
                            Xin, Yout);
@@ -412,10 +453,26 @@ private:
         const int64_t total = (int64_t)Bp_ * Np_;
         hipLaunchKernelGGL(k_split_x, dim3(2048), dim3(256), 0, c_.stream, total, Xin, Xh_, Xl_);
         NUMA_HIP_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_gemm_bf16x3,
-                           dim3((unsigned)(Np_ / kGbRows), (unsigned)(Bp_ / kGbCols), (unsigned)ksplit_),
-                           dim3(256), 0, c_.stream, (int)Np_, ksplit_, (const float*)muf_,
-                           (const float*)wf_, (const __bf16*)Xh_, (const __bf16*)Xl_, P_);
+        if (Kh_) {
+            const dim3 grid((unsigned)(Np_ / (kKbWR * 32 * kKbRT)), (unsigned)(Bp_ / (32 * kb_ct_)),
+                            (unsigned)ksplit_);
+            const int64_t klen = Np_ / ksplit_;
+            auto kern = kb_ct_ == 4 ? (klen % 64 == 0 ? k_gemm_bf16x3_k<kKbWR, kKbRT, 4, 64>
+                                                      : k_gemm_bf16x3_k<kKbWR, kKbRT, 4, 32>)
+                                    : (klen % 128 == 0  ? k_gemm_bf16x3_k<kKbWR, kKbRT, 2, 128>
+                                       : klen % 64 == 0 ? k_gemm_bf16x3_k<kKbWR, kKbRT, 2, 64>
+                                                        : k_gemm_bf16x3_k<kKbWR, kKbRT, 2, 32>);
+            hipLaunchKernelGGL(kern, grid, dim3(kKbWR * 64), 0, c_.stream, (int)Np_, (int)ldb(),
+                               ksplit_,
+                               (const __bf16*)Kh_, (const __bf16*)Kl_, (const __bf16*)Xh_,
+                               (const __bf16*)Xl_, P_);
+        } else {
+            hipLaunchKernelGGL(k_gemm_bf16x3,
+                               dim3((unsigned)(Np_ / kGbRows), (unsigned)(Bp_ / kGbCols),
+                                    (unsigned)ksplit_),
+                               dim3(256), 0, c_.stream, (int)Np_, ksplit_, (const float*)muf_,
+                               (const float*)wf_, (const __bf16*)Xh_, (const __bf16*)Xl_, P_);
+        }
         NUMA_HIP_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_splitk_reduce, dim3(2048), dim3(256), 0, c_.stream, total, ksplit_,
                            (const float*)P_, Yout);
@@ -469,6 +526,12 @@ private:
     __bf16 *Xh_ = nullptr, *Xl_ = nullptr;
     float* P_ = nullptr;
     int ksplit_ = 1;
+    int kb_ct_ = 2;                             // k_gemm_bf16x3_k column tiles per wave
+    // stored K row strides (elements); a line of padding measured no faster
+    int64_t ldd() const { return Np_; }
+    int64_t ldb() const { return Np_; }
+    double* Kd_ = nullptr;                      // stored K (null: generated per use)
+    __bf16 *Kh_ = nullptr, *Kl_ = nullptr;      // its bf16 parts (split-BF16 handles)
     int64_t nsc_ = 0;
     double* sd_ = nullptr;
     double* hs_ = nullptr;
