@@ -368,7 +368,10 @@ int numa_hip_prolong3d(numa_hip_ctx* ctx, const double* coarse_dev, int64_t nc,
  * mu_i = (i-1/2)/N, w = 1/N; form 0: F = 1 + x(Ax) - x (reference), form 1:
  * F = x - 1/(1 - Ax).  Vectors are instance-major: x[b*n + i].  The contraction
  * runs on MFMA: FP64 for F; for J v FP64 (precision FP64) or split-BF16 with
- * FP32 accumulation (precision MIXED). */
+ * FP32 accumulation (precision MIXED).  The handle stores the kernel matrix
+ * (padded N^2 doubles, and for MIXED N^2 bf16 pairs) when that takes at most
+ * 16 GiB; NUMA_HIP_CHANDRA_STORE_K=0 in the environment at creation keeps the
+ * kernels that generate it per use (the results are bit-identical). */
 typedef struct numa_hip_chandra numa_hip_chandra;
 int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
                             int64_t rule, int64_t form, int64_t precision,
