@@ -353,6 +353,9 @@ __global__ __launch_bounds__(WR * 64) void k_gemm_bf16x3_k(int Np, int ldk, int 
     auto step = [&](auto pc, int c) {
         constexpr int p = decltype(pc)::value;
         load(c + 1 < nch ? c + 1 : c, ah[p ^ 1], al[p ^ 1]);
+        // keep the next chunk's loads ahead of this chunk's MFMAs (the
+        // scheduler otherwise sinks them to the end, under register pressure)
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
 #pragma unroll
