@@ -5,8 +5,11 @@
 //                                                 without the factor c)
 //   Y[b][i] = sum_j K_ij X[b][j]                 (the factor c_b is applied by the caller)
 //
-// K (N x N) is never stored: each wave generates the K values of its MFMA A
-// operand in registers from mu and w.  X is instance-major ([b][j], j
+// Two forms of each contraction: the generating kernels (k_gemm_f64,
+// k_gemm_bf16x3), where each wave computes the K values of its MFMA A operand
+// in registers from mu and w, and the stored-K kernels (k_gemm_f64_kd,
+// k_gemm_bf16x3_k), which stream K written once by k_fill_k with the same
+// expressions and return the same bits.  X is instance-major ([b][j], j
 // contiguous) and is staged through LDS per k-chunk, shared by the waves of a
 // workgroup.  Sizes are padded: N to a multiple of 128, B to a multiple of 64;
 // padded nodes have w = 0 (so they add nothing) and mu = 1.
