@@ -299,22 +299,6 @@ public:
             while (ksplit_ < 4 && tiles * ksplit_ < 512) ksplit_ *= 2;
             P_ = static_cast<float*>(alloc((size_t)ksplit_ * vec * sizeof(float)));
         }
-        const char* sk = std::getenv("NUMA_HIP_CHANDRA_STORE_K");
-        const size_t kbytes = (size_t)Np_ * (ldd() * 8 + (prec_ == NUMA_PRECISION_MIXED ? ldb() * 4 : 0));
-        if (!(sk && sk[0] == '0' && sk[1] == 0) && kbytes <= kChKStoreMax) {
-            Kd_ = static_cast<double*>(alloc((size_t)Np_ * ldd() * sizeof(double)));
-            if (prec_ == NUMA_PRECISION_MIXED) {
-                Kh_ = static_cast<__bf16*>(alloc((size_t)Np_ * ldb() * sizeof(__bf16)));
-                Kl_ = static_cast<__bf16*>(alloc((size_t)Np_ * ldb() * sizeof(__bf16)));
-            }
-            hipLaunchKernelGGL(k_fill_k, dim3((unsigned)((Np_ + 255) / 256), (unsigned)Np_),
-                               dim3(256), 0, c_.stream, (int)Np_, (int)ldd(), (int)ldb(),
-                               (const double*)mu_,
-                               (const double*)w_, (const float*)muf_, (const float*)wf_, Kd_,
-                               Kh_, Kl_);
-            NUMA_HIP_LAUNCH_CHECK();
-            c_.sync();
-        }
         // per-instance scalars: H ((m+1) x m x B), s ((m+1) x B), misc
         nsc_ = ((m_ + 1) * m_ + (m_ + 1) + 8) * Bp_;
         sd_ = static_cast<double*>(zalloc((size_t)nsc_ * sizeof(double)));
@@ -324,6 +308,7 @@ public:
         vptr_ = static_cast<const double**>(alloc((size_t)(m_ + 1) * sizeof(double*)));
         coef_ = static_cast<double*>(alloc((size_t)(m_ + 1) * Bp_ * sizeof(double)));
         NUMA_HIP_CHECK(hipMemcpy(vptr_, V_.data(), (m_ + 1) * sizeof(double*), hipMemcpyHostToDevice));
+        store_k();
     }
     ~ChandraBatch() {
         for (void* p : allocs_) (void)hipFree(p);
@@ -385,6 +370,40 @@ private:
     double* MISC(int k) { return sd_ + ((m_ + 1) * m_ + (m_ + 1) + k) * Bp_; }
     double* H(int64_t i, int64_t j) { return sd_ + (j * (m_ + 1) + i) * Bp_; }
     double* S(int64_t i) { return sd_ + ((m_ + 1) * m_ + i) * Bp_; }
+
+    // Stores K after every buffer the handle needs, and only if it fits in
+    // kChKStoreMax and the device has the memory: when an allocation fails the
+    // handle keeps the generating kernels (same results).
+    void store_k() {
+        const char* sk = std::getenv("NUMA_HIP_CHANDRA_STORE_K");
+        const size_t kbytes =
+            (size_t)Np_ * (ldd() * 8 + (prec_ == NUMA_PRECISION_MIXED ? ldb() * 4 : 0));
+        if ((sk && sk[0] == '0' && sk[1] == 0) || kbytes > kChKStoreMax) return;
+        void* kd = nullptr;
+        void* kh = nullptr;
+        void* kl = nullptr;
+        bool ok = hipMalloc(&kd, (size_t)Np_ * ldd() * sizeof(double)) == hipSuccess;
+        if (ok && prec_ == NUMA_PRECISION_MIXED)
+            ok = hipMalloc(&kh, (size_t)Np_ * ldb() * sizeof(__bf16)) == hipSuccess &&
+                 hipMalloc(&kl, (size_t)Np_ * ldb() * sizeof(__bf16)) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();  // the out-of-memory error is handled here
+            for (void* q : {kd, kh, kl})
+                if (q) (void)hipFree(q);
+            return;
+        }
+        for (void* q : {kd, kh, kl})
+            if (q) allocs_.push_back(q);
+        Kd_ = static_cast<double*>(kd);
+        Kh_ = static_cast<__bf16*>(kh);
+        Kl_ = static_cast<__bf16*>(kl);
+        hipLaunchKernelGGL(k_fill_k, dim3((unsigned)((Np_ + 255) / 256), (unsigned)Np_), dim3(256),
+                           0, c_.stream, (int)Np_, (int)ldd(), (int)ldb(), (const double*)mu_,
+                           (const double*)w_, (const float*)muf_, (const float*)wf_, Kd_, Kh_,
+                           Kl_);
+        NUMA_HIP_LAUNCH_CHECK();
+        c_.sync();
+    }
 
     void* alloc(size_t bytes) {
         void* p = nullptr;

@@ -370,7 +370,7 @@ int numa_hip_prolong3d(numa_hip_ctx* ctx, const double* coarse_dev, int64_t nc,
  * runs on MFMA: FP64 for F; for J v FP64 (precision FP64) or split-BF16 with
  * FP32 accumulation (precision MIXED).  The handle stores the kernel matrix
  * (padded N^2 doubles, and for MIXED N^2 bf16 pairs) when that takes at most
- * 16 GiB; NUMA_HIP_CHANDRA_STORE_K=0 in the environment at creation keeps the
+ * 16 GiB and the device can allocate it; NUMA_HIP_CHANDRA_STORE_K=0 in the environment at creation keeps the
  * kernels that generate it per use (the results are bit-identical). */
 typedef struct numa_hip_chandra numa_hip_chandra;
 int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
