@@ -480,6 +480,7 @@ def main() -> int:
     sm = L["prims"]["SMOOTH"]
     f0 = L["fusion"][0] if L["fusion"] else (H.FUSE_NONE, 0, 0)
     traffic = None
+    kbase = None
     if f0[0] == H.FUSE_NONE:
         kname = "k_stencil_lds<EpPoly> (finest Chebyshev smoothing step)"
     elif f0[2] < 0:  # box form
@@ -504,6 +505,11 @@ def main() -> int:
                                    "WRITE_SIZE per launch, copy-kernel calibrated)")
                 if traffic is not None else None,
                 "algorithmic_bytes_per_launch": sm["bytes"], "avg_launch_ms": sm["ms"]}
+    if kbase == "k_poly_fused<float,3,2>":
+        # counted in the AMD assembly of this kernel's main loop
+        roofline["limiter"] = ("VALU issue: 61 VALU instructions per plane per thread for 6 "
+                               "node updates (~55% of the launch; packed FP32 is not used), "
+                               "halo columns/rows recomputed (1.56x the useful updates)")
 
     out = {"metric": METRIC, "value": round(t_main, 6), "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_main * 1e3, 3),
