@@ -509,7 +509,7 @@ def main() -> int:
         # counted in the AMD assembly of this kernel's main loop
         roofline["limiter"] = ("VALU issue: 61 VALU instructions per plane per thread for 6 "
                                "node updates (~55% of the launch; packed FP32 is not used), "
-                               "halo columns/rows recomputed (1.56x the useful updates)")
+                               "halo columns, rows and planes recomputed (~1.8x the useful updates)")
 
     out = {"metric": METRIC, "value": round(t_main, 6), "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_main * 1e3, 3),
