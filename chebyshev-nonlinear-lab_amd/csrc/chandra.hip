@@ -123,9 +123,13 @@ __global__ __launch_bounds__(256) void k_ch_F(int N, int Np, int form, const dou
 
 // out = scale_b * J(X) V from Y = K V (Av = c_b Y); optionally out = r - J V
 // (restart residual, then the partial is ||out||^2).
+// Y is the FP64 contraction, or (P non-null) the ksplit FP32 split-K slices of
+// the split-BF16 contraction, slice stride `slice`, summed here in the order
+// k_splitk_reduce uses (so both give the same Y).
 __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const double* c,
                                                 const double* X, const double* Ax,
                                                 const double* V, const double* Y,
+                                                const float* P, int ksplit, int64_t slice,
                                                 const double* scale, const double* rhs,
                                                 double* out, Reduce red) {
 #pragma clang fp contract(off)
@@ -134,7 +138,14 @@ __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const do
     const double sc = scale ? scale[b] : 1.0;
     double acc = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
-        const double v = V[o + i], av = c[b] * Y[o + i], ax = Ax[o + i];
+        double y;
+        if (P) {
+            y = 0.0;
+            for (int k = 0; k < ksplit; ++k) y += (double)P[(size_t)k * slice + o + i];
+        } else {
+            y = Y[o + i];
+        }
+        const double v = V[o + i], av = c[b] * y, ax = Ax[o + i];
         double jv;
         if (form == 0) {
             jv = v * ax + X[o + i] * av - v;
@@ -468,6 +479,8 @@ private:
                            Xin, Yout);
         NUMA_HIP_LAUNCH_CHECK();
     }
+    // Split X, contract into the split-K slices P_, and (Yout non-null) sum
+    // them into Yout; the J v path passes null and sums them in k_ch_jv.
     void gemm_bf16(const double* Xin, double* Yout) {
         const int64_t total = (int64_t)Bp_ * Np_;
         hipLaunchKernelGGL(k_split_x, dim3(2048), dim3(256), 0, c_.stream, total, Xin, Xh_, Xl_);
@@ -493,6 +506,7 @@ private:
                                (const float*)wf_, (const __bf16*)Xh_, (const __bf16*)Xl_, P_);
         }
         NUMA_HIP_LAUNCH_CHECK();
+        if (!Yout) return;
         hipLaunchKernelGGL(k_splitk_reduce, dim3(2048), dim3(256), 0, c_.stream, total, ksplit_,
                            (const float*)P_, Yout);
         NUMA_HIP_LAUNCH_CHECK();
@@ -509,13 +523,15 @@ private:
 
     // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
     void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep) {
-        if (prec_ == NUMA_PRECISION_MIXED)
-            gemm_bf16(V, Y_);
+        const bool split = prec_ == NUMA_PRECISION_MIXED;
+        if (split)
+            gemm_bf16(V, nullptr);
         else
             gemm_f64(V, Y_);
         hipLaunchKernelGGL(k_ch_jv, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, (const double*)X_, (const double*)Ax_, V,
-                           (const double*)Y_, scale, rhs, out, c_.red);
+                           (const double*)Y_, split ? (const float*)P_ : nullptr, ksplit_,
+                           (int64_t)Bp_ * Np_, scale, rhs, out, c_.red);
         NUMA_HIP_LAUNCH_CHECK();
         if (ep) bfinish(*ep);
     }

@@ -25,7 +25,7 @@
 //   Operand layout (checked on the device): A lane l = K[row l%32][k 8(l/32)+e],
 //   B lane l = X[col l%32][k 8(l/32)+e], D lane l reg r =
 //   (row 8(r/4) + 4(l/32) + r%4, col l%32).  Split-K partial sums are reduced in
-//   a fixed order by k_splitk_reduce.
+//   a fixed order by k_splitk_reduce, or by the kernel that consumes them.
 #pragma once
 
 #include "common.hpp"
