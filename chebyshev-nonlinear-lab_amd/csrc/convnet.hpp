@@ -98,17 +98,37 @@ __device__ inline bool inside(int i, int j, int k, int n) {
     return (unsigned)i < (unsigned)n && (unsigned)j < (unsigned)n && (unsigned)k < (unsigned)n;
 }
 
-// Stage the (TI+2) x 10 x 34 halo of a tile (channel-last chunks), zeros outside.
-template <int TI = kTI>
+// Stage the (TI+2) x 10 x 34 halo of a tile (channel-last chunks), zeros
+// outside, with NT threads.  Each thread issues its loads in batches of up to
+// BATCH before the batch's LDS writes (chunks outside the domain load a valid
+// address and are replaced by zero), so a tile waits for one memory latency
+// per batch, not one per chunk.
+template <int TI, int NT, int BATCH>
 __device__ inline void stage_halo(const bf16x8* __restrict__ in, int n, int i0, int j0, int k0,
                                   bf16x8* lds) {
-    for (int e = threadIdx.x; e < (TI + 2) * kHJ * kHK * 3; e += blockDim.x) {
-        const int pt = e / 3, c = e - 3 * (e / 3);
-        const int hk = pt % kHK, hj = (pt / kHK) % kHJ, hi = pt / (kHK * kHJ);
-        const int gi = i0 + hi - 1, gj = j0 + hj - 1, gk = k0 + hk - 1;
-        bf16x8 v{};
-        if (inside(gi, gj, gk, n)) v = in[(((int64_t)gi * n + gj) * n + gk) * 3 + c];
-        lds[e] = v;
+    constexpr int NE = (TI + 2) * kHJ * kHK * 3;
+    constexpr int PER = (NE + NT - 1) / NT;
+    // a rolled loop over batches: unrolled, the tile-invariant index arithmetic
+    // of every chunk is hoisted out of the caller's tile loop and held in
+    // registers through its compute phase
+#pragma unroll 1
+    for (int u0 = 0; u0 < PER; u0 += BATCH) {
+        bf16x8 v[BATCH];
+        bool ok[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int e = (int)threadIdx.x + (u0 + u) * NT;
+            const int pt = e / 3, c = e - 3 * (e / 3);
+            const int hk = pt % kHK, hj = (pt / kHK) % kHJ, hi = pt / (kHK * kHJ);
+            const int gi = i0 + hi - 1, gj = j0 + hj - 1, gk = k0 + hk - 1;
+            ok[u] = u0 + u < PER && e < NE && inside(gi, gj, gk, n);
+            v[u] = in[ok[u] ? (((int64_t)gi * n + gj) * n + gk) * 3 + c : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int e = (int)threadIdx.x + (u0 + u) * NT;
+            if (u0 + u < PER && e < NE) lds[e] = ok[u] ? v[u] : bf16x8{};
+        }
     }
 }
 
@@ -158,7 +178,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv24(int n, int nsamples,
         int b, i0, j0, k0;
         T.decode(t, b, i0, j0, k0);
         __syncthreads();
-        stage_halo<kCTI>(in + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
+        stage_halo<kCTI, kConvThreads, 6>(in + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
         __syncthreads();
         floatx16 acc[4] = {};
         int cb[4];
@@ -285,31 +305,49 @@ __global__ __launch_bounds__(256) void k_conv_first(int n, const TIN* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// Last layer (24 -> 1), VALU FP32 on the LDS halo tile:
+// Last layer (24 -> 1) on the VALU over the LDS halo tile:
 // out[b][p] = (b5 + sum_tap sum_ci W[ci][tap] a[p + d(tap)][ci]) * os[b].
-// Thread -> 2 points of a line.
+// The weights are split once per workgroup into bf16 pairs W = Wh + Wl (about
+// 16 bits of each weight) and every activation pair enters two packed bf16
+// dot products (v_dot2c_f32_bf16: a.x w.x + a.y w.y + acc), so no activation
+// is converted to FP32.  Thread -> points k and k + 16 of a line: consecutive
+// lanes read halo points 48 bytes apart (distinct LDS banks) and store to
+// consecutive addresses.
 // ---------------------------------------------------------------------------
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
 template <class TOUT>
 __global__ __launch_bounds__(256) void k_conv_last(int n, int nsamples, const bf16x8* __restrict__ act,
                                                    const float* __restrict__ W,
                                                    const float* __restrict__ bias,
                                                    const float* __restrict__ os,
                                                    TOUT* __restrict__ out) {
+    constexpr int kPairs = kC / 2;  // channel pairs per tap
     __shared__ bf16x8 lds[kHalo * 3];
-    __shared__ float w[kTaps * kC];  // [tap][ci]
-    for (int e = threadIdx.x; e < kTaps * kC; e += 256) {
-        const int tap = e / kC, ci = e % kC;
-        w[e] = W[ci * kTaps + tap];
+    __shared__ __attribute__((aligned(16))) bf16x2 wh[kTaps * kPairs];  // [tap][pair]
+    __shared__ __attribute__((aligned(16))) bf16x2 wl[kTaps * kPairs];
+    for (int e = threadIdx.x; e < kTaps * kPairs; e += 256) {
+        const int tap = e / kPairs, pr = e % kPairs;
+        bf16x2 h, l;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float w = W[(2 * pr + q) * kTaps + tap];
+            const __bf16 hv = static_cast<__bf16>(w);
+            h[q] = hv;
+            l[q] = static_cast<__bf16>(w - static_cast<float>(hv));
+        }
+        wh[e] = h;
+        wl[e] = l;
     }
     const float b0 = bias[0];
     const Tiles<> T(n, nsamples);
-    const int L = threadIdx.x >> 4, kk = (threadIdx.x & 15) * 2;
+    const int L = threadIdx.x >> 4, kk = threadIdx.x & 15;
     const int64_t N3 = (int64_t)n * n * n;
     for (int64_t t = worker_id(); t < T.total; t += gridDim.x) {
         int b, i0, j0, k0;
         T.decode(t, b, i0, j0, k0);
         __syncthreads();
-        stage_halo(act + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
+        stage_halo<kTI, 256, 16>(act + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
         __syncthreads();
         const int base = ((L / kTJ + 1) * kHJ + (L % kTJ + 1)) * kHK + kk + 1;
         float acc0 = b0, acc1 = b0;
@@ -318,12 +356,16 @@ __global__ __launch_bounds__(256) void k_conv_last(int n, int nsamples, const bf
             const int p0 = (base + tap_off(tap)) * 3;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const bf16x8 v0 = lds[p0 + c], v1 = lds[p0 + 3 + c];
+                const bf16x8 v0 = lds[p0 + c], v1 = lds[p0 + 48 + c];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float wv = w[tap * kC + 8 * c + e];
-                    acc0 = fmaf(wv, static_cast<float>(v0[e]), acc0);
-                    acc1 = fmaf(wv, static_cast<float>(v1[e]), acc1);
+                for (int q = 0; q < 4; ++q) {
+                    const int e = tap * kPairs + 4 * c + q;
+                    const bf16x2 a0 = {v0[2 * q], v0[2 * q + 1]};
+                    const bf16x2 a1 = {v1[2 * q], v1[2 * q + 1]};
+                    acc0 = __builtin_amdgcn_fdot2_f32_bf16(a0, wh[e], acc0, false);
+                    acc0 = __builtin_amdgcn_fdot2_f32_bf16(a0, wl[e], acc0, false);
+                    acc1 = __builtin_amdgcn_fdot2_f32_bf16(a1, wh[e], acc1, false);
+                    acc1 = __builtin_amdgcn_fdot2_f32_bf16(a1, wl[e], acc1, false);
                 }
             }
         }
@@ -332,7 +374,7 @@ __global__ __launch_bounds__(256) void k_conv_last(int n, int nsamples, const bf
             const float sc = os[b];
             const int64_t idx = (int64_t)b * N3 + ((int64_t)gi * n + gj) * n + gk;
             if (gk < n) out[idx] = static_cast<TOUT>(acc0 * sc);
-            if (gk + 1 < n) out[idx + 1] = static_cast<TOUT>(acc1 * sc);
+            if (gk + 16 < n) out[idx + 16] = static_cast<TOUT>(acc1 * sc);
         }
     }
 }
@@ -466,7 +508,7 @@ __global__ __launch_bounds__(256) void k_wgrad_last(int n, int nsamples, const b
         int b, i0, j0, k0;
         T.decode(tile, b, i0, j0, k0);
         __syncthreads();
-        stage_halo(a5 + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
+        stage_halo<kTI, 256, 16>(a5 + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
         for (int e = t; e < kTilePts; e += 256) {
             const int L = e / kTK, kk = e % kTK;
             const int gi = i0 + L / kTJ, gj = j0 + L % kTJ, gk = k0 + kk;
