@@ -277,14 +277,20 @@ __global__ __launch_bounds__(256) void k_conv_first(int n, const TIN* __restrict
     const int k = (int)(p % n), j = (int)((p / n) % n), i = (int)(p / ((int64_t)n * n));
     const float s = xs[b];
     const TIN* xb = x + (size_t)b * N3;
-    float xn[kTaps];
+    // all 27 loads issued before the first use (an index outside the domain
+    // loads element 0 and is replaced by zero)
+    TIN xr[kTaps];
+    bool xok[kTaps];
 #pragma unroll
     for (int tap = 0; tap < kTaps; ++tap) {
         const int ii = i + tap / 9 - 1, jj = j + (tap / 3) % 3 - 1, kk = k + tap % 3 - 1;
-        xn[tap] = inside(ii, jj, kk, n)
-                      ? static_cast<float>(xb[((int64_t)ii * n + jj) * n + kk]) * s
-                      : 0.0f;
+        xok[tap] = inside(ii, jj, kk, n);
+        xr[tap] = xb[xok[tap] ? ((int64_t)ii * n + jj) * n + kk : 0];
     }
+    float xn[kTaps];
+#pragma unroll
+    for (int tap = 0; tap < kTaps; ++tap)
+        xn[tap] = xok[tap] ? static_cast<float>(xr[tap]) * s : 0.0f;
     bf16x8* ab = act + ((size_t)b * N3 + p) * 3;
     bf16x8* pb = pre + ((size_t)b * N3 + p) * 3;
 #pragma unroll

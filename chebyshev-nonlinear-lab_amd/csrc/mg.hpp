@@ -340,8 +340,12 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
     const int ncell = (nf + 1) / 2;
     if (K >= ncell || J >= ncell) return;
     const int64_t C1 = nc, C2 = C1 * C1, F1 = nf, F2 = F1 * F1;
+    // Every load of the thread (8 coarse, 8 fine values) is issued before the
+    // first use; an index out of range loads element 0 and its value is
+    // replaced (a branch per load made each one wait for the previous).
     // coarse block c[a][b][e] at (I-1+a, J-1+b, K-1+e), zero where invalid
-    double c[2][2][2];
+    T craw[2][2][2];
+    bool cok[2][2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -349,9 +353,31 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int ci = I - 1 + a, cj = J - 1 + b, ck = K - 1 + e;
-                const bool ok = ci >= 0 && ci < nc && cj >= 0 && cj < nc && ck >= 0 && ck < nc;
-                c[a][b][e] = ok ? dv(xc, ci * C2 + cj * C1 + ck) : 0.0;
+                cok[a][b][e] = ci >= 0 && ci < nc && cj >= 0 && cj < nc && ck >= 0 && ck < nc;
+                craw[a][b][e] = xc[cok[a][b][e] ? ci * C2 + cj * C1 + ck : 0];
             }
+    T fraw[2][2][2];
+    bool fok[2][2][2];
+    int64_t fid[2][2][2];
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+        for (int pj = 0; pj < 2; ++pj)
+#pragma unroll
+            for (int pk = 0; pk < 2; ++pk) {
+                const int fi = 2 * I + pi, fj = 2 * J + pj, fk = 2 * K + pk;
+                fok[pi][pj][pk] = fi < nf && fj < nf && fk < nf;
+                fid[pi][pj][pk] = fok[pi][pj][pk] ? (int64_t)fi * F2 + (int64_t)fj * F1 + fk : 0;
+                fraw[pi][pj][pk] = xf[fid[pi][pj][pk]];
+            }
+    double c[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+                c[a][b][e] = cok[a][b][e] ? static_cast<double>(craw[a][b][e]) : 0.0;
     // weights: [parity][slot] for slot 0 = coarse I-1, 1 = coarse I
     auto wts = [nc](int idx, double w[2]) {
         // even fine index 2*idx
@@ -365,18 +391,13 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
     const double wo[2] = {0.0, 1.0};
 #pragma unroll
     for (int pi = 0; pi < 2; ++pi) {
-        const int fi = 2 * I + pi;
-        if (fi >= nf) continue;
         const double* wi = pi ? wo : we_i;
 #pragma unroll
         for (int pj = 0; pj < 2; ++pj) {
-            const int fj = 2 * J + pj;
-            if (fj >= nf) continue;
             const double* wj = pj ? wo : we_j;
 #pragma unroll
             for (int pk = 0; pk < 2; ++pk) {
-                const int fk = 2 * K + pk;
-                if (fk >= nf) continue;
+                if (!fok[pi][pj][pk]) continue;
                 const double* wk = pk ? wo : we_k;
                 double sum = 0.0;
 #pragma unroll
@@ -387,8 +408,8 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
                         for (int e = 0; e < 2; ++e)
                             if (wi[a] != 0.0 && wj[b] != 0.0 && wk[e] != 0.0)
                                 sum += wi[a] * wj[b] * wk[e] * c[a][b][e];
-                const int64_t id = (int64_t)fi * F2 + (int64_t)fj * F1 + fk;
-                xf[id] = static_cast<T>(dv(xf, id) + sum);
+                xf[fid[pi][pj][pk]] =
+                    static_cast<T>(static_cast<double>(fraw[pi][pj][pk]) + sum);
             }
         }
     }
