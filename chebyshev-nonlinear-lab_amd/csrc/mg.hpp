@@ -126,13 +126,21 @@ __global__ __launch_bounds__(kBlock) void k_rbgs(int n, double inv, int color, c
         const int j = (int)((id / N1) % N1);
         const int i = (int)(id / N2);
         if (((i + j + k) & 1) != color) continue;
-        double s = dv(b, id) * inv;
-        if (i > 0) s += dv(x, id - N2);
-        if (i + 1 < n) s += dv(x, id + N2);
-        if (j > 0) s += dv(x, id - N1);
-        if (j + 1 < n) s += dv(x, id + N1);
-        if (k > 0) s += dv(x, id - 1);
-        if (k + 1 < n) s += dv(x, id + 1);
+        // the seven loads are issued before the first use (a neighbour outside
+        // the domain loads the node itself and is not added)
+        const bool im = i > 0, ip = i + 1 < n, jm = j > 0, jp = j + 1 < n, km = k > 0,
+                   kp = k + 1 < n;
+        const T vb = b[id];
+        const T vim = x[im ? id - N2 : id], vip = x[ip ? id + N2 : id];
+        const T vjm = x[jm ? id - N1 : id], vjp = x[jp ? id + N1 : id];
+        const T vkm = x[km ? id - 1 : id], vkp = x[kp ? id + 1 : id];
+        double s = static_cast<double>(vb) * inv;
+        if (im) s += static_cast<double>(vim);
+        if (ip) s += static_cast<double>(vip);
+        if (jm) s += static_cast<double>(vjm);
+        if (jp) s += static_cast<double>(vjp);
+        if (km) s += static_cast<double>(vkm);
+        if (kp) s += static_cast<double>(vkp);
         x[id] = static_cast<T>(s / 6.0);
     }
 }
