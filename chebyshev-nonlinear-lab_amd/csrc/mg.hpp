@@ -693,12 +693,23 @@ __global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv,
     __shared__ double S[16][16];
     __shared__ double cp[16];
     const int t = threadIdx.x, lane = t & 63, g = t >> 6;
-    if (t < 256) S[t >> 4][t & 15] = tab[t];
-    if (t < 16) cp[t] = tab[256 + t];
-    for (int id = t; id < 4096; id += 1024) {
+    // all loads (table and right-hand side) issued before the first LDS write;
+    // indices outside the level load element 0 and are replaced
+    const double ts = tab[t < 256 ? t : 0];
+    const double tc = tab[256 + (t < 16 ? t : 0)];
+    T bv[4];
+    bool bok[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int id = t + 1024 * m;
         const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
-        u[0][id] = i < n && j < n && k < n ? static_cast<double>(b[(i * n + j) * n + k]) : 0.0;
+        bok[m] = i < n && j < n && k < n;
+        bv[m] = b[bok[m] ? (i * n + j) * n + k : 0];
     }
+    if (t < 256) S[t >> 4][t & 15] = ts;
+    if (t < 16) cp[t] = tc;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) u[0][t + 1024 * m] = bok[m] ? static_cast<double>(bv[m]) : 0.0;
     __syncthreads();
     const int r16 = lane & 15, q = lane >> 4;
     using d4 = __attribute__((ext_vector_type(4))) double;
