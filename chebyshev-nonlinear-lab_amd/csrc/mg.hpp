@@ -454,11 +454,18 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
     bool in[kCoarseMaxPts];
     // neighbour masks shared by the column: j-1, j+1, k-1, k+1 inside
     const bool hjm = j > 0, hjp = j + 1 < n, hkm = k > 0, hkp = k + 1 < n;
+    // the right-hand side loads are issued before the first use (a point
+    // outside the level loads element 0 and is replaced by zero)
+    T braw[kCoarseMaxPts];
 #pragma unroll
     for (int p = 0; p < kCoarseMaxPts; ++p) {
         const int i = i0 + p;
         in[p] = col && i < n;
-        bl[p] = in[p] ? dv(b, i * N2 + jk) : 0.0;
+        braw[p] = b[in[p] ? i * N2 + jk : 0];
+    }
+#pragma unroll
+    for (int p = 0; p < kCoarseMaxPts; ++p) {
+        bl[p] = in[p] ? static_cast<double>(braw[p]) : 0.0;
         dl[p] = xr[p] = xn[p] = 0.0;
     }
     for (int id = threadIdx.x; id < N; id += kCoarseThreads) xs[0][id] = 0.0;
