@@ -407,7 +407,13 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
         xv[m] = buf_load<T>(rx, off);
         bv[m] = buf_load<T>(rb, off);
         dv[m] = T(0);
-        if (node) buf[0][e] = xv[m];
+    }
+    // LDS writes after every load is in flight (a write next to its load made
+    // each slot wait for its own load)
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+        const int e = (int)threadIdx.x + m * kBoxThreads;
+        if (e < NB) buf[0][e] = xv[m];
     }
     __syncthreads();
 #pragma unroll
