@@ -7,11 +7,18 @@
 
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <sys/file.h>
+#include <unistd.h>
+
+#include <atomic>
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <map>
 #include <mutex>
+#include <random>
+#include <set>
 #include <sstream>
 #include <string>
 
@@ -33,6 +40,7 @@ public:
     void store(const std::string& key, const std::string& val) {
         std::lock_guard<std::mutex> g(mu_);
         m_[key] = val;
+        mine_.insert(key);
         save();
     }
     std::string path() const { return path_; }
@@ -55,14 +63,15 @@ private:
         }
         load();
     }
-    void load() {
-        if (path_.empty()) return;
-        std::ifstream f(path_);
+    void load() { parse(path_, m_); }
+    // strings between quotes, alternating key / value
+    static void parse(const std::string& path, std::map<std::string, std::string>& out) {
+        if (path.empty()) return;
+        std::ifstream f(path);
         if (!f) return;
         std::stringstream ss;
         ss << f.rdbuf();
         const std::string s = ss.str();
-        // strings between quotes, alternating key / value
         std::string cur, key;
         bool in = false, have_key = false;
         for (char ch : s) {
@@ -72,7 +81,7 @@ private:
                         key = cur;
                         have_key = true;
                     } else {
-                        m_[key] = cur;
+                        out[key] = cur;
                         have_key = false;
                     }
                     cur.clear();
@@ -83,25 +92,50 @@ private:
             }
         }
     }
+    // Processes that tune at once (a benchmark beside a test run) share the
+    // file: under an exclusive flock on "<path>.lock" the file is re-read, the
+    // entries of other processes merged in (the keys this process tuned itself keep its values),
+    // and the result written to a temporary name unique to this process and
+    // renamed over the file, so no reader ever sees a partial file.
     void save() {
         if (path_.empty()) return;
         std::error_code ec;
         const std::filesystem::path p(path_);
         if (p.has_parent_path()) std::filesystem::create_directories(p.parent_path(), ec);
-        const std::string tmp = path_ + ".tmp";
+        const int fd = ::open((path_ + ".lock").c_str(), O_CREAT | O_RDWR | O_CLOEXEC, 0644);
+        if (fd >= 0) (void)::flock(fd, LOCK_EX);
+        std::map<std::string, std::string> disk;
+        parse(path_, disk);
+        for (const auto& [k, v] : disk)
+            if (!mine_.count(k)) m_[k] = v;
+        static std::atomic<unsigned long> seq{0};
+        const std::string tmp = path_ + ".tmp." + std::to_string((long)::getpid()) + "." +
+                                std::to_string(seq.fetch_add(1)) + "." +
+                                std::to_string(std::random_device{}());
+        bool ok = false;
         {
             std::ofstream f(tmp, std::ios::trunc);
-            if (!f) return;
-            f << "{\n";
-            size_t i = 0;
-            for (const auto& [k, v] : m_) f << "  \"" << k << "\": \"" << v << "\"" << (++i < m_.size() ? ",\n" : "\n");
-            f << "}\n";
+            if (f) {
+                f << "{\n";
+                size_t i = 0;
+                for (const auto& [k, v] : m_)
+                    f << "  \"" << k << "\": \"" << v << "\"" << (++i < m_.size() ? ",\n" : "\n");
+                f << "}\n";
+                f.flush();
+                ok = f.good();
+            }
         }
-        std::filesystem::rename(tmp, path_, ec);
+        if (ok) std::filesystem::rename(tmp, path_, ec);
+        if (!ok || ec) std::filesystem::remove(tmp, ec);
+        if (fd >= 0) {
+            (void)::flock(fd, LOCK_UN);
+            (void)::close(fd);
+        }
     }
 
     std::mutex mu_;
     std::map<std::string, std::string> m_;
+    std::set<std::string> mine_;  // keys this process has tuned itself
     std::string path_;
 };
 

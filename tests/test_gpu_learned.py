@@ -168,3 +168,12 @@ def test_trained_network_gains(ctx, trained, rhs):
     print("rel_res after", steps, "FGMRES steps,", rhs, "rhs:", out)
     assert out["trained"] < out["untrained"], out
     assert out["trained"] < out["none"], out
+    # Against the method without the network (FGMRES + V-cycle): the network
+    # alone does not reach the V-cycle's reduction (measured at 64^3: 3.1e-3
+    # vs 2.8e-8 with a random right-hand side).  The ratio is recorded; this
+    # comparison is an expected failure, not a pass.
+    ratio = out["trained"] / out["gmg"]
+    print("trained / gmg residual ratio:", ratio)
+    if not out["trained"] <= out["gmg"]:
+        pytest.xfail(f"network alone loses to the V-cycle: {out['trained']:.3g} vs "
+                     f"{out['gmg']:.3g} after {steps} FGMRES steps (ratio {ratio:.3g})")

@@ -332,8 +332,44 @@ def test_chandra_golden_and_moment_identity(oracle):
             N = 64
             x, r = oracle.chandra_solve(N, c, cfg, O.krylov_params(rtol_nl=1e-13), rule=rule,
                                         form=form)
+            assert r.converged, (rule, form, c)
             w = oracle.gauss_legendre_unit(N)[1] if rule == 0 else np.full(N, 1.0 / N)
             assert abs(np.dot(w, x) - 2 / c * (1 - np.sqrt(1 - c))) < 1e-11
+
+
+def _fd_jvp(F, x, v):
+    """Forward difference of operators/operator.hpp:101-114, from F alone."""
+    h = np.sqrt(np.finfo(float).eps) * (1.0 + np.linalg.norm(x)) / np.linalg.norm(v)
+    return (F(x + h * v) - F(x)) / h
+
+
+def _fd_d2(F, x, v):
+    """Central second difference of operators/operator.hpp:118-134, from F alone."""
+    h = np.finfo(float).eps ** 0.25 * (1.0 + np.linalg.norm(x)) / np.linalg.norm(v)
+    return (F(x + h * v) - 2.0 * F(x) + F(x - h * v)) / (h * h)
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("form", [0, 1])
+def test_chandra_derivatives_consistent_with_eval(oracle, rule, form):
+    """Analytic J v and F''[v,v] of both forms (reference F = 1 + x Ax - x and
+    the Kelley form F = x - 1/(1 - Ax)) against differences of F itself, under
+    both quadrature rules: test_chandrasekhar_op.cpp:75-99 (N = 24, c = 0.8,
+    x = 1 + 0.2 sin(0.37 i + 0.5), v = 0.5 cos(0.23 i + 0.1); 2e-6 / 2e-4).
+    Only chandra_eval enters the differences, so a wrong sign, power or
+    operand in either derivative fails here."""
+    N, c = 24, 0.8
+    i = np.arange(N)
+    x = 1.0 + 0.2 * np.sin(0.37 * i + 0.5)
+    v = 0.5 * np.cos(0.23 * i + 0.1)
+    F = lambda y: oracle.chandra_eval(N, c, y, rule=rule, form=form)  # noqa: E731
+    jv = oracle.chandra_jvp(N, c, x, v, rule=rule, form=form)
+    d2 = oracle.chandra_d2(N, c, x, v, rule=rule, form=form)
+    assert rel(_fd_jvp(F, x, v), jv) < 2e-6
+    assert rel(_fd_d2(F, x, v), d2) < 2e-4
+    # the second derivative is not the first: a d2 that returned J v would pass
+    # neither the bound above nor this one
+    assert rel(jv, d2) > 1e-2
 
 
 # ---- tensor-free Chebyshev 2018 ------------------------------------------------
