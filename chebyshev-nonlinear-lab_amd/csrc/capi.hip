@@ -3,6 +3,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 
 #include "blas.hpp"
 #include "stencil.hpp"
@@ -57,7 +58,7 @@ __global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partial
 }
 
 void Ctx::finish(unsigned count, const Epilogue& ep) {
-    require(count >= 1 && count <= (unsigned)kMaxPartials, "bad reduction size");
+    require(count >= 1 && count <= (unsigned)max_partials, "bad reduction size");
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishThreads), 0, stream,
                        (const double*)red.partials, count, ep);
     NUMA_HIP_LAUNCH_CHECK();
@@ -83,6 +84,7 @@ Ctx::Ctx(int dev) : device(dev) {
 }
 
 Ctx::~Ctx() {
+    if (!owns) return;
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     if (red.partials) (void)hipFree(red.partials);
@@ -142,6 +144,44 @@ void jvp_fd_impl(Ctx& c, int64_t n, double lambda, const double* x, const double
                                            0.0, 1.0},
         Epilogue{});
 }
+
+// ---- scratch of the no-allocation entry point ----------------------------
+// The two norms of numa_hip_bratu_jvp_fd_noalloc and their reduction partials
+// live in this file-scope device array: one copy per device, part of the
+// library image (no allocation at run time), held until the process ends.
+// stream_blocks() launches at most 4096 blocks, so a dot never needs more
+// partials than kNoAllocPartials, whatever the grid size.
+constexpr int kNoAllocPartials = 4096;
+constexpr int kNoAllocSlots = 8;
+static __device__ __align__(256) double scratch[kNoAllocPartials + kNoAllocSlots];
+
+struct NoAllocDevice {
+    double* ptr = nullptr;  // device address of `scratch` on this device
+    int cus = 0;
+};
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+// The device's copy of `scratch`, resolved once per device with that device
+// current (host code never passes the array's own name to a kernel: on the
+// host it is only a stand-in for the symbol).
+NoAllocDevice noalloc_device(int dev) {
+    require(dev >= 0 && dev < kNoAllocMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        NUMA_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        require(p != nullptr, "no device address for the scratch array", NUMA_ERR_HIP);
+        int cus = 0;
+        NUMA_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        d.cus = cus > 0 ? cus : 1;
+        d.ptr = static_cast<double*>(p);
+    }
+    return d;
+}
+
 }  // namespace
 
 extern "C" {
@@ -281,6 +321,26 @@ int numa_hip_bratu_jvp_fd_mixed(numa_hip_ctx* ctx, int64_t n, double lambda, con
     check_grid(n);
     require(x_dev && F0_dev && v_dev && out_dev, "null device pointer");
     jvp_fd_impl<float, float>(c, n, lambda, x_dev, F0_dev, v_dev, out_dev);
+    NUMA_API_END
+}
+
+int numa_hip_bratu_jvp_fd_noalloc(int device, void* stream, int64_t n, double lambda,
+                                  const double* x_dev, const double* F0_dev, const double* v_dev,
+                                  double* out_dev) {
+    NUMA_API_BEGIN
+    check_grid(n);
+    require(x_dev && F0_dev && v_dev && out_dev, "null device pointer");
+    int count = 0;
+    NUMA_HIP_CHECK(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, "no such HIP device: " + std::to_string(device));
+    NUMA_HIP_CHECK(hipSetDevice(device));
+    const NoAllocDevice d = noalloc_device(device);
+    const int64_t N = n * n * n;
+    require(numa_hip::stream_blocks(N) <= (unsigned)kNoAllocPartials,
+            "reduction exceeds the no-allocation scratch", NUMA_ERR_OVERFLOW);
+    Ctx c(device, static_cast<hipStream_t>(stream), d.ptr, kNoAllocPartials,
+          d.ptr + kNoAllocPartials, d.cus);
+    jvp_fd_impl<double, double>(c, n, lambda, x_dev, F0_dev, v_dev, out_dev);
     NUMA_API_END
 }
 

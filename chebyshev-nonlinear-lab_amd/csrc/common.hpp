@@ -163,7 +163,20 @@ struct Ctx {
     int cu_count = 256;
     int live_handles = 0;  // multigrid / solver handles created on this context
 
+    bool owns = true;      // false: stream and scratch belong to the caller
+    int max_partials = kMaxPartials;
+
     explicit Ctx(int dev);
+    // A context over a stream and scratch memory that the caller provides
+    // (`partials` holds max_partials doubles): it creates, allocates and
+    // frees nothing.
+    Ctx(int dev, hipStream_t s, double* partials, int max_partials_, double* scalars, int cus)
+        : device(dev), stream(s), dscal(scalars), cu_count(cus), owns(false),
+          max_partials(max_partials_) {
+        red.partials = partials;
+    }
+    Ctx(const Ctx&) = delete;
+    Ctx& operator=(const Ctx&) = delete;
     ~Ctx();
     // One-block sum of the first `count` partials and the epilogue.
     void finish(unsigned count, const Epilogue& ep);

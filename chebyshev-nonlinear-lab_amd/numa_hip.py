@@ -237,6 +237,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_bratu_jvp_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_bratu_jvp_fd_mixed": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_bratu_d2_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
+        "numa_hip_bratu_jvp_fd_noalloc": [C.c_int, _vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_dot": [_vp, _i64, _vp, _vp, _dp],
         "numa_hip_dot_f32": [_vp, _i64, _vp, _vp, _dp],
         "numa_hip_axpy": [_vp, _i64, C.c_double, _vp, _vp],

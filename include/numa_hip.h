@@ -199,6 +199,19 @@ int numa_hip_bratu_jvp_fd(numa_hip_ctx* ctx, int64_t n, double lambda,
 int numa_hip_bratu_jvp_fd_mixed(numa_hip_ctx* ctx, int64_t n, double lambda,
                                 const double* x_dev, const double* F0_dev,
                                 const float* v_dev, float* out_dev);
+/* No counterpart in the reference: numa_hip_bratu_jvp_fd without a context,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * the same result bit for bit.  It allocates nothing: ||x||, ||v|| and their
+ * reduction partials live in a file-scope device array of the library
+ * (4104 doubles, 32.8 KB per device, held until the process ends).  The
+ * kernels run on `stream` (a hipStream_t of `device`, NULL = the null
+ * stream), which becomes the current device.  n <= 2048, as above.  Calls
+ * that overlap on one device share that array and race: serialise them (one
+ * stream, or events). */
+int numa_hip_bratu_jvp_fd_noalloc(int device, void* stream, int64_t n, double lambda,
+                                  const double* x_dev, const double* F0_dev,
+                                  const double* v_dev, double* out_dev);
 /* Central-difference F''(x)[v,v] = (F(x+hv) - 2F0 + F(x-hv))/h^2,
  * h = eps^(1/4)(1+||x||)/||v||.  Replaces d2_fd (operator.hpp:118-134). */
 int numa_hip_bratu_d2_fd(numa_hip_ctx* ctx, int64_t n, double lambda,
