@@ -766,7 +766,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
         bool any = false;
         for (int64_t b = 0; b < B; ++b) {
             if (!active[b]) continue;
-            if (rnorm[b] <= p.rtol_nl * res[b].res0) {
+            if (nl_converged(p, rnorm[b], res[b].res0, res[b].iters, rprev[b])) {
                 res[b].converged = 1;
                 active[b] = 0;
                 continue;
@@ -784,7 +784,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
                 e = std::max(e, 0.5 * p.rtol_nl * res[b].res0 / rnorm[b]);
                 e = std::min(e, p.eta_max);
             }
-            eta[b] = e;
+            eta[b] = tight_cap(p, e, rnorm[b], res[b].res0);
         }
         if (!any) break;
         gmres_batch(active, eta, p.gmres_restart, p.gmres_max_iters, g);

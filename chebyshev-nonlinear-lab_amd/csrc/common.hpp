@@ -185,6 +185,31 @@ struct Ctx {
     void sync() { NUMA_HIP_CHECK(hipStreamSynchronize(stream)); }
 };
 
+// ---------------------------------------------------------------------------
+// Final-solve safeguard of the outer Newton / Chebyshev loops (an addition to
+// methods/krylov_newton.hpp:54-114; off when tight_eta = 0, the reference's
+// behaviour).  Below T = max(tight_res, rtol_nl) ||F(x0)|| the forcing term
+// is at most tight_eta, and the solve counts as converged only once its last
+// accepted step started below T (else one more step is taken).  The final
+// iterate then carries a linear error of at most tight_eta * T and a
+// nonlinear one of order T^2, whatever path the iterations took.
+// ---------------------------------------------------------------------------
+inline double tight_threshold(const numa_krylov_params& p, double res0) {
+    return std::max(p.tight_res, p.rtol_nl) * res0;
+}
+inline double tight_cap(const numa_krylov_params& p, double eta, double rnorm, double res0) {
+    if (p.tight_eta > 0.0 && rnorm <= tight_threshold(p, res0)) return std::min(eta, p.tight_eta);
+    return eta;
+}
+// last_start: ||F|| where the last accepted step started (ignored when
+// iters = 0).
+inline bool nl_converged(const numa_krylov_params& p, double rnorm, double res0, int64_t iters,
+                         double last_start) {
+    if (!(rnorm <= p.rtol_nl * res0)) return false;
+    if (!(p.tight_eta > 0.0) || iters == 0) return true;
+    return last_start <= tight_threshold(p, res0);
+}
+
 // Launch shape of grid-stride streaming kernels over N elements: each thread
 // moves 2 x 2 consecutive elements per iteration (16-byte FP64 / 8-byte FP32
 // accesses, contiguous across the wave); up to 4096 blocks (16 per CU).

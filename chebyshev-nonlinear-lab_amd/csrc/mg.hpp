@@ -1212,6 +1212,7 @@ private:
             key << dev << "|mg-smooth-v3|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
                 << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
             std::string val;
+            const auto held = TuneCache::get().tuning(key.str());
             if (TuneCache::get().lookup(key.str(), val)) {
                 int m = 0, ic = 0, rw = 0;
                 constexpr int R0 = fused_rows_small<T>();

@@ -35,8 +35,8 @@ namespace numa_hip {
 
 // Eisenstat-Walker choice 2 with Kelley's safeguards (krylov_newton.hpp:54-65),
 // or a constant forcing term.
-inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
-                      double eta_prev, double res0) {
+inline double forcing_ew(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
+                         double eta_prev, double res0) {
     if (p.forcing == NUMA_FORCING_CONSTANT) return p.eta_init;
     if (k == 0) return p.eta_init;
     double eta = p.gamma * std::pow(rnorm / rprev, p.alpha);
@@ -45,6 +45,10 @@ inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, doub
     eta = std::min(eta, p.eta_max);
     eta = std::max(eta, 0.5 * p.rtol_nl * res0 / rnorm);
     return std::min(eta, p.eta_max);
+}
+inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
+                      double eta_prev, double res0) {
+    return tight_cap(p, forcing_ew(p, k, rnorm, rprev, eta_prev, res0), rnorm, res0);
 }
 
 template <class T>
@@ -969,7 +973,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         if (!std::isfinite(rnorm)) break;
     }
     for (int64_t k = 0; !tf && k < p.max_nl; ++k) {
-        if (rnorm <= p.rtol_nl * out.res0) {
+        if (nl_converged(p, rnorm, out.res0, out.iters, rprev)) {
             out.converged = 1;
             break;
         }

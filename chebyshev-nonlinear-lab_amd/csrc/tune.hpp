@@ -16,6 +16,7 @@
 #include <filesystem>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <random>
 #include <set>
@@ -44,6 +45,21 @@ public:
         save();
     }
     std::string path() const { return path_; }
+
+    // Held while one key is tuned: solvers of several threads that need the
+    // same key wait for the first one's choice instead of timing their own
+    // (under each other's load), so every solver of the process runs the
+    // same choice for a key.
+    std::unique_lock<std::mutex> tuning(const std::string& key) {
+        std::mutex* m = nullptr;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto& slot = key_mu_[key];
+            if (!slot) slot = std::make_unique<std::mutex>();
+            m = slot.get();
+        }
+        return std::unique_lock<std::mutex>(*m);
+    }
 
     // "<arch>|<CUs>cu" of a device
     static std::string device_key(int dev) {
@@ -136,6 +152,7 @@ private:
     std::mutex mu_;
     std::map<std::string, std::string> m_;
     std::set<std::string> mine_;  // keys this process has tuned itself
+    std::map<std::string, std::unique_ptr<std::mutex>> key_mu_;
     std::string path_;
 };
 

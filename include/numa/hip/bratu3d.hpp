@@ -105,12 +105,16 @@ struct KrylovParams {
     double min_lambda = 1e-4;
     bool recycle = false;
     int forcing = NUMA_FORCING_EW;  // or NUMA_FORCING_CONSTANT (eta = eta_init)
+    // final-solve safeguard (0 = off): eta <= tight_eta once ||F|| <= tight_res ||F(x0)||
+    double tight_res = 0.0;
+    double tight_eta = 0.0;
 
     numa_krylov_params c() const {
         return numa_krylov_params{rtol_nl,       (int64_t)max_nl,         eta_init,   eta_max,
                                   gamma,         alpha,                   eta2_factor, eta2_max,
                                   (int64_t)gmres_restart, (int64_t)gmres_max_iters, armijo,
-                                  min_lambda,    recycle ? 1 : 0,         forcing};
+                                  min_lambda,    recycle ? 1 : 0,         forcing,
+                                  tight_res,     tight_eta};
     }
 };
 

@@ -76,6 +76,12 @@ typedef struct numa_krylov_params {
     int64_t recycle;
     /* additions (no reference counterpart) */
     int64_t forcing;         /* NUMA_FORCING_EW (reference) or _CONSTANT (eta = eta_init) */
+    /* Final-solve safeguard (0 = off, the reference): once ||F|| <= tight_res
+     * ||F(x0)||, the forcing term is capped at tight_eta, so the last Newton
+     * step overshoots the tolerance and the final iterate no longer depends
+     * on the path by which it got there. */
+    double tight_res;
+    double tight_eta;
 } numa_krylov_params;
 
 /* Mirrors numa::op::KrylovResult (krylov_newton.hpp:40-49) without the

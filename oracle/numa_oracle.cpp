@@ -718,8 +718,33 @@ numa_gmres_result gmres_recycled(const ApplyFn& apply, const ApplyFn& precond, c
 // ---------------------------------------------------------------------------
 // Nonlinear outer loops.  methods/krylov_newton.hpp:54-189.
 // ---------------------------------------------------------------------------
+double forcing_ew(const numa_krylov_params& p, size_t k, double rnorm, double rprev,
+                  double eta_prev, double res0);
+
+// The build's final-solve safeguard (no reference counterpart; off when
+// tight_eta = 0, the reference's behaviour).  Below T = max(tight_res,
+// rtol_nl) ||F(x0)|| the forcing term is at most tight_eta, and the solve is
+// converged only once its last accepted step started below T.
+double tight_threshold(const numa_krylov_params& p, double res0) {
+    return std::max(p.tight_res, p.rtol_nl) * res0;
+}
+
+bool nl_converged(const numa_krylov_params& p, double rnorm, double res0, int64_t iters,
+                  double last_start) {
+    if (!(rnorm <= p.rtol_nl * res0)) return false;
+    if (!(p.tight_eta > 0.0) || iters == 0) return true;
+    return last_start <= tight_threshold(p, res0);
+}
+
 double forcing(const numa_krylov_params& p, size_t k, double rnorm, double rprev,
                double eta_prev, double res0) {
+    const double eta = forcing_ew(p, k, rnorm, rprev, eta_prev, res0);
+    if (p.tight_eta > 0.0 && rnorm <= tight_threshold(p, res0)) return std::min(eta, p.tight_eta);
+    return eta;
+}
+
+double forcing_ew(const numa_krylov_params& p, size_t k, double rnorm, double rprev,
+                  double eta_prev, double res0) {
     if (p.forcing == NUMA_FORCING_CONSTANT) return p.eta_init;
     // krylov_newton.hpp:54-65
     if (k == 0) return p.eta_init;
@@ -754,7 +779,7 @@ void outer_solve(const Problem& P, DVec& x, const numa_krylov_params& p, bool ch
     out.res0 = rnorm;
     double rprev = rnorm, eta_prev = p.eta_init;
     for (size_t k = 0; k < (size_t)p.max_nl; ++k) {
-        if (rnorm <= p.rtol_nl * out.res0) {
+        if (nl_converged(p, rnorm, out.res0, out.iters, rprev)) {
             out.converged = 1;
             break;
         }

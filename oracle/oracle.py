@@ -28,7 +28,8 @@ class KrylovParams(C.Structure):
                 ("eta_max", C.c_double), ("gamma", C.c_double), ("alpha", C.c_double),
                 ("eta2_factor", C.c_double), ("eta2_max", C.c_double),
                 ("gmres_restart", _i64), ("gmres_max_iters", _i64), ("armijo", C.c_double),
-                ("min_lambda", C.c_double), ("recycle", _i64), ("forcing", _i64)]
+                ("min_lambda", C.c_double), ("recycle", _i64), ("forcing", _i64),
+                ("tight_res", C.c_double), ("tight_eta", C.c_double)]
 
 
 class KrylovResult(C.Structure):
@@ -111,7 +112,8 @@ def krylov_params(**kw) -> KrylovParams:
     """Defaults of numa::op::KrylovParams (methods/krylov_newton.hpp:24-38)."""
     p = KrylovParams(rtol_nl=1e-8, max_nl=50, eta_init=0.5, eta_max=0.9, gamma=0.9, alpha=2.0,
                      eta2_factor=10.0, eta2_max=0.5, gmres_restart=50, gmres_max_iters=500,
-                     armijo=1e-4, min_lambda=1e-4, recycle=0, forcing=FORCING_EW)
+                     armijo=1e-4, min_lambda=1e-4, recycle=0, forcing=FORCING_EW,
+                     tight_res=0.0, tight_eta=0.0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

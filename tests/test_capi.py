@@ -24,12 +24,12 @@ def test_library_exports_declared_symbols():
 
 
 def test_pod_layouts_are_8_byte_fields():
-    assert ctypes.sizeof(H.KrylovParams) == 14 * 8
+    assert ctypes.sizeof(H.KrylovParams) == 16 * 8
     assert ctypes.sizeof(H.KrylovResult) == 11 * 8
     assert ctypes.sizeof(H.GmresResult) == 6 * 8
     assert ctypes.sizeof(H.MGConfig) == 13 * 8
     assert ctypes.sizeof(H.SolverConfig) == 19 * 8
-    assert ctypes.sizeof(H.NetTrainConfig) == 40 * 8
+    assert ctypes.sizeof(H.NetTrainConfig) == 42 * 8
     assert ctypes.sizeof(H.NetTrainStats) == 6 * 8
 
 

@@ -1,0 +1,220 @@
+"""The benchmark workloads at their full sizes on the GPU, against the CPU
+oracle on the same inputs.
+
+* Bratu3D 32^3, lambda = 6.0, u0 = 0: FP64 Newton with the FD J v,
+  GMRES(30) + V(2,2) weighted-Jacobi multigrid, constant forcing 1e-3 (the
+  plain CPU method): every counter identical to the oracle, solution to 1e-10.
+* Bratu3D 256^3, lambda = 6.5, u0 = 0: Newton and inexact Chebyshev (FD
+  F''[v,v]), GMRES(30) / FGMRES(30) + 5-level V(2,2) with the degree-3
+  Chebyshev smoother, FP64 and mixed precision.  FP64 counters identical to
+  the oracle's; mixed Newton count within +-1; every solution within 1e-10
+  (relative L2) of the oracle's FP64 solution.
+* Chandrasekhar H-equation, midpoint rule, Kelley form, N = 8192, a batch of
+  256 values of c in [0.5, 0.9999], Newton-GMRES(20), tol 1e-10: all 256
+  instances satisfy the moment identity sum_i x_i / N = (2/c)(1 - sqrt(1-c));
+  three of them (smallest c, largest c, one between) against the oracle's
+  solve for that c (FP64: counters identical, 1e-11; mixed: 1e-9, Newton +-1).
+* The batched sweep: 96 instances, n from {32, 48, 64, 96}, lambda in
+  [1, 6.7], u0 = 0.1 U[0,1): every instance of the multi-worker graph-replay
+  sweep equals the same instance solved alone, bit for bit; a seeded sample
+  of instances against the oracle (1e-10, Newton +-1).
+
+Solves that are compared with the oracle across different Krylov paths (mixed
+vs FP64, FGMRES vs GMRES) use the final-solve safeguard (forcing term capped
+at 1e-6 once ||F|| <= 1e-6 ||F0||, and converged only once the last step
+started there, identically in the oracle), so that the
+final iterate does not depend on the path; the FP64 comparisons hold with or
+without it.  The threshold is relative to ||F(x0)||: the sweep's noisy starts
+inflate ||F(x0)|| about 200-fold over lambda sqrt(N), which is why it is 1e-6
+rather than the 1e-5 that suffices from u0 = 0.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+import numa_hip as H
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("converged", "iters", "n_F", "n_JVP", "n_D2", "n_lin_iters", "n_lin_failures", "n_prec")
+SAFEGUARD = dict(tight_res=1e-6, tight_eta=1e-6)
+
+
+def _ocfg(cfg):
+    import oracle as O
+    m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
+    return O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=cfg.precision,
+                          flexible=cfg.flexible, jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
+
+
+def _oparams(p):
+    import oracle as O
+    return O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
+
+
+def _counters(r):
+    return {k: getattr(r, k) for k in KEYS}
+
+
+# ---- config 0: 32^3 weighted-Jacobi GMG, constant forcing ----------------------
+def test_bratu32_wjacobi_constant_forcing(ctx, oracle):
+    n, lam = 32, 6.0
+    cfg = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                          precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
+                          mg=H.mg_wjacobi(coarse_iters=-1))
+    p = H.krylov_params(gmres_restart=30, gmres_max_iters=600, eta_init=1e-3,
+                        forcing=H.FORCING_CONSTANT)
+    print(f"config 0: n={n} lambda={lam} wjacobi V(2,2) forcing 1e-3", flush=True)
+    s = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+    x = np.zeros(n ** 3)
+    r = s.solve_host(p, x)
+    s.close()
+    xo, ro = oracle.bratu_solve(n, lam, _ocfg(cfg), _oparams(p))
+    assert r.converged and r.res <= 1e-8 * r.res0
+    assert _counters(r) == _counters(ro)
+    assert rel(x, xo) < 1e-10, rel(x, xo)
+
+
+# ---- config 1: 256^3, Newton and Chebyshev, FP64 and mixed ---------------------
+BIG_N, BIG_LAM = 256, 6.5
+
+
+def _big_params():
+    return H.krylov_params(gmres_restart=30, gmres_max_iters=600, **SAFEGUARD)
+
+
+def _big_cfg(method, precision):
+    mixed = precision == H.PRECISION_MIXED
+    mg = H.mg_cheb(levels=5, degree=3, pre=2, post=2, coarse_iters=-1,
+                   fuse=H.FUSE_AUTO if mixed else H.FUSE_NONE)
+    return H.solver_config(method=method, precond=H.PRECOND_GMG, precision=precision,
+                           flexible=1 if mixed else 0, jv_mode=H.DERIV_FD, d2_mode=H.DERIV_FD,
+                           mg=mg)
+
+
+@pytest.fixture(scope="module")
+def oracle_256(oracle):
+    cache = {}
+
+    def get(method):
+        if method not in cache:
+            cfg = _big_cfg(method, H.PRECISION_FP64)
+            print(f"oracle: n={BIG_N} lambda={BIG_LAM} method={method} "
+                  f"({oracle.threads()} threads)", flush=True)
+            cache[method] = oracle.bratu_solve(BIG_N, BIG_LAM, _ocfg(cfg), _oparams(_big_params()))
+        return cache[method]
+
+    return get
+
+
+@pytest.mark.parametrize("precision", [H.PRECISION_FP64, H.PRECISION_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("method", [H.METHOD_NEWTON, H.METHOD_CHEBYSHEV], ids=["newton", "cheb"])
+def test_bratu256(ctx, oracle_256, method, precision):
+    xo, ro = oracle_256(method)
+    assert ro.converged
+    print(f"gpu: n={BIG_N} method={method} precision={precision}", flush=True)
+    s = H.BratuSolver(ctx, BIG_N, BIG_LAM, _big_cfg(method, precision), restart=30)
+    x = ctx.empty(BIG_N ** 3)
+    x.upload(np.zeros(BIG_N ** 3))
+    r = s.solve(_big_params(), x)
+    xg = x.numpy()
+    s.close()
+    x.free()
+    print(f"  gpu {_counters(r)}, oracle {_counters(ro)}, rel-L2 {rel(xg, xo):.3e}")
+    assert r.converged and r.res <= 1e-8 * r.res0
+    if method == H.METHOD_CHEBYSHEV:
+        assert r.n_D2 >= 1
+    if precision == H.PRECISION_FP64:
+        assert _counters(r) == _counters(ro)
+    else:
+        assert abs(r.iters - ro.iters) <= 1
+    assert rel(xg, xo) <= 1e-10, rel(xg, xo)
+
+
+# ---- config 2: Chandrasekhar N = 8192 x 256 values of c -----------------------
+CH_N, CH_B = 8192, 256
+
+
+def _chandra_cs():
+    return np.random.default_rng(20261203).uniform(0.5, 0.9999, CH_B)
+
+
+@pytest.mark.parametrize("precision", [H.PRECISION_FP64, H.PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_chandrasekhar_8192x256(ctx, oracle, precision):
+    import oracle as O
+    cs = _chandra_cs()
+    p = H.krylov_params(rtol_nl=1e-10, gmres_restart=20, gmres_max_iters=400)
+    print(f"chandrasekhar N={CH_N} B={CH_B} precision={precision}", flush=True)
+    ch = H.ChandrasekharBatch(ctx, CH_N, cs, rule=1, form=1, precision=precision, restart=20)
+    x = ctx.array(np.ones(CH_B * CH_N))
+    res = ch.solve_device(p, x)
+    X = x.numpy().reshape(CH_B, CH_N)
+    ch.close()
+    x.free()
+    assert all(r.converged for r in res)
+    want = 2.0 / cs * (1.0 - np.sqrt(1.0 - cs))
+    mom = X.mean(axis=1)
+    assert np.max(np.abs(mom - want) / want) < 1e-9, np.max(np.abs(mom - want) / want)
+    ocfg = O.solver_config(method=0, precond=O.PRECOND_NONE, jv_mode=O.DERIV_ANALYTIC)
+    for b in (int(np.argmin(cs)), int(np.argmax(cs)), 100):
+        xo, ro = oracle.chandra_solve(CH_N, float(cs[b]), ocfg, _oparams(p), rule=1, form=1)
+        e = rel(X[b], xo)
+        print(f"  c={cs[b]:.6f}: gpu iters {res[b].iters} oracle {ro.iters}, rel {e:.2e}")
+        if precision == H.PRECISION_FP64:
+            assert _counters(res[b]) == _counters(ro)
+            assert e < 1e-11, (b, e)
+        else:
+            assert abs(res[b].iters - ro.iters) <= 1
+            assert e < 1e-9, (b, e)
+
+
+# ---- config 3: the 96-instance batched sweep ----------------------------------
+SW_SEED, SW_COUNT, SW_SIZES = 20261203, 96, (32, 48, 64, 96)
+
+
+def _sweep_cfg():
+    return H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                           precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
+                           mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=-1))
+
+
+def test_sweep_96_instances(ctx, oracle):
+    n, lam = H.sweep_instances(SW_SEED, SW_COUNT, sizes=SW_SIZES)
+    x0 = [H.uniform_host(SW_SEED * 7919 + 13 + i, int(n[i]) ** 3, 0.1) for i in range(SW_COUNT)]
+    assert sorted(set(int(v) for v in n)) == list(SW_SIZES)
+    p = H.krylov_params(gmres_restart=30, gmres_max_iters=600, **SAFEGUARD)
+    cfg = _sweep_cfg()
+    print(f"sweep: {SW_COUNT} instances, 8 workers, graphs", flush=True)
+    xs = [ctx.array(v) for v in x0]
+    sw = H.Sweep(ctx, cfg, workers=8, restart=30)
+    res, sec = sw.run(n, lam, xs, p, graphs=True)
+    sw.close()
+    got = [x.numpy() for x in xs]
+    for x in xs:
+        x.free()
+    assert all(r.converged for r in res)
+    print(f"  {SW_COUNT / sec:.0f} solves/s; solving each instance alone", flush=True)
+    for i in range(SW_COUNT):
+        ni = int(n[i])
+        s = H.BratuSolver(ctx, ni, float(lam[i]), cfg, restart=30)
+        xi = x0[i].copy()
+        r = s.solve_host(p, xi)
+        s.close()
+        np.testing.assert_array_equal(got[i], xi, err_msg=f"instance {i} (n={ni})")
+        assert (r.iters, r.n_lin_iters, r.res) == (res[i].iters, res[i].n_lin_iters, res[i].res), i
+    # a seeded sample against the FP64 oracle (GMRES, same multigrid)
+    rng = np.random.default_rng(5)
+    sample = [int(np.flatnonzero(n == k)[0]) for k in SW_SIZES[:3]]
+    sample += [int(i) for i in rng.choice(np.flatnonzero(n <= 64), 2, replace=False)]
+    o64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                          precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
+                          mg=H.mg_cheb(levels=5, coarse_iters=-1))
+    for i in sample:
+        xo, ro = oracle.bratu_solve(int(n[i]), float(lam[i]), _ocfg(o64), _oparams(p),
+                                    x0=x0[i])
+        e = rel(got[i], xo)
+        print(f"  instance {i}: n={int(n[i])} lambda={lam[i]:.4f} newton {res[i].iters} "
+              f"(oracle {ro.iters}), rel-L2 {e:.2e}")
+        assert ro.converged
+        assert abs(res[i].iters - ro.iters) <= 1
+        assert e <= 1e-10, (i, e)

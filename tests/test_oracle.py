@@ -520,3 +520,30 @@ def test_exact_coarse_solve_is_the_inverse(oracle, n):
     if n <= 5:
         A = np.column_stack([oracle.poisson_apply(n, e) for e in np.eye(n ** 3)])
         assert rel(z, np.linalg.solve(A, r)) < 1e-12
+
+
+# ---- final-solve safeguard -------------------------------------------------------
+def test_final_solve_safeguard_makes_the_solution_path_independent(oracle):
+    """Three Krylov paths (V-cycle, Jacobi, no preconditioner) at n = 24,
+    lambda = 6: with the reference's forcing they stop at different points of
+    ||F|| <= 1e-8 ||F0|| (solutions ~5e-9 from the solution); with the safeguard
+    (tight_res 1e-5, tight_eta 1e-6) every path ends within 1e-11 of the
+    solution converged to 1e-13, and the safeguard off is the reference's
+    loop (the calibration-row tests above pin that)."""
+    n, lam = 24, 6.0
+    mg = cheb_cfg(3)
+    cfgs = [O.solver_config(precond=O.PRECOND_GMG, mg=mg), O.solver_config(precond=O.PRECOND_JACOBI),
+            O.solver_config(precond=O.PRECOND_NONE)]
+    ref, r = oracle.bratu_solve(n, lam, cfgs[0], O.krylov_params(
+        rtol_nl=1e-13, gmres_restart=30, gmres_max_iters=3000, eta_init=1e-10, forcing=O.FORCING_CONSTANT))
+    assert r.converged and r.res <= 1e-13 * r.res0
+    plain = [oracle.bratu_solve(n, lam, c, O.krylov_params(gmres_restart=30, gmres_max_iters=4000))
+             for c in cfgs]
+    safe = [oracle.bratu_solve(n, lam, c, O.krylov_params(gmres_restart=30, gmres_max_iters=4000,
+                                                          tight_res=1e-5, tight_eta=1e-6))
+            for c in cfgs]
+    assert all(r.converged for _, r in plain + safe)
+    assert max(rel(x, ref) for x, _ in plain) > 1e-9
+    assert max(rel(x, ref) for x, _ in safe) < 1e-11
+    for (_, a), (_, b) in zip(plain, safe):
+        assert b.iters >= a.iters
