@@ -102,6 +102,22 @@ void launch_dot(Ctx& c, int64_t N, const TA* a, const TB* b, const Epilogue& ep)
     c.finish(nb, ep);
 }
 
+// Fault injection: element 0 of v and the scalar slots s1, s2 (each may be
+// null) become NaN.  One thread.
+template <class T>
+__global__ void k_poison(T* v, double* s1, double* s2) {
+    const double nan = __builtin_nan("");
+    if (v) v[0] = static_cast<T>(nan);
+    if (s1) s1[0] = nan;
+    if (s2) s2[0] = nan;
+}
+
+template <class T>
+void launch_poison(Ctx& c, T* v, double* s1, double* s2) {
+    hipLaunchKernelGGL((k_poison<T>), dim3(1), dim3(1), 0, c.stream, v, s1, s2);
+    NUMA_HIP_LAUNCH_CHECK();
+}
+
 // y += a x  (operator.hpp:32-41)
 template <bool VEC, class TX, class TY>
 __global__ __launch_bounds__(kBlock) void k_axpy(int64_t N, double a, const TX* x, TY* y) {

@@ -218,6 +218,13 @@ public:
         ratio_ = ratio;
     }
     LearnedStats learned_stats() const override { return lstats_; }
+    void inject_nan(int64_t what, int64_t call) override {
+        require(what == NUMA_INJECT_F || what == NUMA_INJECT_JV, "unknown injection target");
+        require(call >= 0, "negative call index");
+        inj_what_ = call > 0 ? what : -1;
+        inj_at_ = call;
+        inj_count_ = 0;
+    }
     const MGBase* multigrid() const override { return mg_.get(); }
 
     numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
@@ -362,6 +369,15 @@ private:
         else
             launch_stencil<true>(c_, n_, LdStep<double, true, true>{x, z, a, u, b},
                                  EpF<true>{h2inv_, lambda_, F, wout, lam_dev()}, ep);
+        if (inject_fires(NUMA_INJECT_F)) launch_poison<double>(c_, F, MISC(S_FNORM), nullptr);
+    }
+
+    // fault injection (inject_nan): true when this call of `what` is the armed one
+    bool inject_fires(int64_t what) {
+        if (inj_what_ != what) return false;
+        if (++inj_count_ != inj_at_) return false;
+        inj_what_ = -1;
+        return true;
     }
 
     // Everything that depends only on the Newton point x: ||x|| for the FD
@@ -419,6 +435,7 @@ private:
                     EpJvAn<T, false, false>{h2inv_, lambda_, scale, x, out, nullptr, nullptr, 1.0, lam_dev()},
                     ep);
         }
+        if (inject_fires(NUMA_INJECT_JV)) launch_poison<T>(c_, out, ep.out, ep.out2);
     }
 
     // Preconditioner M^-1 in -> out; returns the result pointer (in itself
@@ -560,8 +577,10 @@ private:
     }
     bool in_solve_ = false;  // graphs serve solve() (fixed iterate buffers) only
     bool graph_ok() const {
-        return graphs_ && !timing_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_;
+        return graphs_ && !timing_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_ &&
+               inj_what_ < 0;
     }
+    int64_t inj_what_ = -1, inj_at_ = 0, inj_count_ = 0;  // inject_nan state
     // ---- timers (operators/timing.hpp:16-108): device time of each primitive
     // call, from a pair of HIP events on the solver stream, summed after the
     // solve's final synchronisation.
@@ -980,6 +999,9 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         const double eta1 = forcing(p, k, rnorm, rprev, eta_prev, out.res0);
         const double eta2 = std::min(p.eta2_max, p.eta2_factor * eta1);
         timed(T_SETUP, [&]() { point_setup(xc); });
+        // z = 0 before the solve (krylov_newton.hpp:88, 136); GMRES zeroes it
+        // itself unless it returns before iterating
+        if (p.gmres_restart == 0 || p.gmres_max_iters == 0) launch_fill<double>(c_, N_, 0.0, z_);
         const auto g1 = gmres(xc, Fc, Fc, z_, eta1, p.gmres_restart, p.gmres_max_iters, true,
                               nullptr, rec);
         out.n_JVP += g1.matvecs;
@@ -992,6 +1014,7 @@ void BratuSolver<T>::solve(const numa_krylov_params& p, double* xdev, numa_krylo
         if (cheb) {
             timed(T_D2, [&]() { d2(xc, Fc, z_, w_); });
             ++out.n_D2;
+            if (p.gmres_restart == 0 || p.gmres_max_iters == 0) launch_fill<double>(c_, N_, 0.0, u_);
             const auto g2 = rec ? gmres_recycled(xc, Fc, w_, u_, eta2, p.gmres_restart,
                                                  p.gmres_max_iters)
                                 : gmres(xc, Fc, w_, u_, eta2, p.gmres_restart, p.gmres_max_iters,
@@ -1252,6 +1275,13 @@ int numa_hip_solver_timers(numa_hip_solver* s, numa_timers* out, int64_t reset) 
     numa_hip::require(s && out, "null argument");
     NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
     *out = s->impl->timers(reset != 0);
+    NUMA_API_END
+}
+
+int numa_hip_solver_inject_nan(numa_hip_solver* s, int64_t what, int64_t call) {
+    NUMA_API_BEGIN
+    numa_hip::require(s != nullptr, "null solver");
+    s->impl->inject_nan(what, call);
     NUMA_API_END
 }
 

@@ -63,6 +63,9 @@ struct SolverBase {
     // vec (device, may be null) receives the last iterate.
     virtual numa_eigen_result eigen_smallest(const double* x, const numa_eigen_params& p,
                                              double* vec) = 0;
+    // NaN into the `call`-th F evaluation (what 0) or J v product (what 1)
+    // from now on; fires once; call = 0 disarms.
+    virtual void inject_nan(int64_t what, int64_t call) = 0;
 };
 
 SolverBase* make_bratu_solver(Ctx& c, int64_t n, double lambda, const numa_solver_config& cfg,

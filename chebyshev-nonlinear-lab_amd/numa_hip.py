@@ -25,6 +25,7 @@ _i64 = C.c_int64
 _vp = C.c_void_p
 _dp = C.POINTER(C.c_double)
 
+INJECT_F, INJECT_JV = 0, 1
 NUMA_OK = 0
 ERR_NAMES = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "overflow",
              5: "bad state"}
@@ -276,6 +277,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_net_cost": [_vp, _i64, _i64, _i64, _i64, _dp],
         "numa_hip_solver_set_net": [_vp, _vp, C.c_double],
         "numa_hip_solver_learned_stats": [_vp, C.POINTER(_i64)],
+        "numa_hip_solver_inject_nan": [_vp, _i64, _i64],
         "numa_hip_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
         "numa_hip_mg_set_resid_restrict": [_vp, _i64],
         "numa_hip_solver_set_graphs": [_vp, _i64],
@@ -584,6 +586,11 @@ class BratuSolver:
         m, ic, rw = (_i64 * k)(), (_i64 * k)(), (_i64 * k)()
         check(self.ctx.lib.numa_hip_solver_mg_fusion(self.h, C.byref(L), m, ic, rw))
         return [(int(m[i]), int(ic[i]), int(rw[i])) for i in range(L.value)]
+
+    def inject_nan(self, what: int, call: int) -> None:
+        """NaN into the call-th F evaluation (INJECT_F) or J v product
+        (INJECT_JV) of the following solves; fires once; call = 0 disarms."""
+        check(self.ctx.lib.numa_hip_solver_inject_nan(self.h, what, call))
 
     def learned_stats(self) -> dict:
         s = (_i64 * 3)()

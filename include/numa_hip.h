@@ -558,6 +558,21 @@ int numa_hip_sweep_instances(int64_t seed, int64_t count, const int64_t* sizes, 
 /* out[i] = scale * U[0,1) from std::mt19937_64(seed) (host only). */
 int numa_hip_uniform_host(int64_t seed, int64_t count, double scale, double* out);
 
+/* ---- fault injection (tests of the failure paths) ------------------------ */
+/* Forces a non-finite value into one primitive call of the solver, from the
+ * host, so that the numerical failure guards of the reference (the
+ * non-finite Givens denominator and back-substitution pivot, gmres.hpp:
+ * 165-169, 185-190; the line search, krylov_newton.hpp:96-104) are exercised
+ * on the device: the `call`-th F evaluation (what = NUMA_INJECT_F: element 0
+ * of F and ||F|| become NaN) or J v product (what = NUMA_INJECT_JV: element 0
+ * of the product and its fused reduction become NaN), counted from this call
+ * over the following solves; it fires once.  call = 0 disarms.  While armed
+ * the Arnoldi steps run eagerly.  No counterpart in the reference, whose
+ * tests force these paths by wrapping the operator (test_operators.cpp:
+ * 141-157). */
+enum { NUMA_INJECT_F = 0, NUMA_INJECT_JV = 1 };
+int numa_hip_solver_inject_nan(numa_hip_solver* s, int64_t what, int64_t call);
+
 /* ---- per-call primitive costs (instrumentation) -------------------------- */
 /* The reference freezes per-call costs c_F, c_JVP, c_D2, c_prec of its
  * primitives (experiments/c1/frozen/primitive_costs.csv, measured by

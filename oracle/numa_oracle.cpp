@@ -1014,6 +1014,18 @@ ApplyFn make_bratu_precond(const Bratu& f, const numa_solver_config& cfg,
     return [](const DVec& r, DVec& z) { z = r; };  // IdentityPrecond, gmres.hpp:37-39
 }
 
+// Fault injection of the failure-path tests (orc_set_inject): element 0 of
+// the `at`-th F evaluation (what 0) or J v product (what 1) becomes NaN,
+// once.  The same hook as the device solver's numa_hip_solver_inject_nan.
+thread_local int g_inj_what = -1;
+thread_local int64_t g_inj_at = 0, g_inj_count = 0;
+
+void inject(int what, DVec& o) {
+    if (g_inj_what != what || ++g_inj_count != g_inj_at) return;
+    g_inj_what = -1;
+    o[0] = std::numeric_limits<double>::quiet_NaN();
+}
+
 Problem bratu_problem(const Bratu& f, const numa_solver_config& cfg) {
     Problem P;
     P.n = (size_t)f.size();
@@ -1022,17 +1034,20 @@ Problem bratu_problem(const Bratu& f, const numa_solver_config& cfg) {
     P.eval = [&f](const DVec& x, DVec& o) {
         o.resize(x.size());
         f.eval(x.data(), o.data());
+        inject(NUMA_INJECT_F, o);
     };
     P.jvp_needs_F = cfg.jv_mode == NUMA_DERIV_FD;
     if (cfg.jv_mode == NUMA_DERIV_FD)
         P.jvp = [&f](const DVec& x, const DVec& Fx, double xn, const DVec& v, DVec& o) {
             o.resize(x.size());
             jvp_fd(f, x.data(), v.data(), o.data(), Fx.data(), xn);
+            inject(NUMA_INJECT_JV, o);
         };
     else
         P.jvp = [&f](const DVec& x, const DVec&, double, const DVec& v, DVec& o) {
             o.resize(x.size());
             f.jvp(x.data(), v.data(), o.data());
+            inject(NUMA_INJECT_JV, o);
         };
     if (cfg.d2_mode == NUMA_DERIV_FD)
         P.d2 = [&f](const DVec& x, const DVec& Fx, double xn, const DVec& v, DVec& o) {
@@ -1176,6 +1191,12 @@ numa_mg_config default_mg() {
 // C ABI
 // ===========================================================================
 extern "C" {
+
+void orc_set_inject(int64_t what, int64_t at) {
+    g_inj_what = at > 0 ? (int)what : -1;
+    g_inj_at = at;
+    g_inj_count = 0;
+}
 
 int orc_num_threads(void) {
 #ifdef _OPENMP

@@ -24,6 +24,10 @@ extern "C" {
 #endif
 
 int orc_num_threads(void);
+/* Fault injection: NaN into element 0 of the `at`-th Bratu F evaluation
+ * (what = NUMA_INJECT_F) or J v product (NUMA_INJECT_JV) of the following
+ * solves of this thread; fires once; at = 0 disarms. */
+void orc_set_inject(int64_t what, int64_t at);
 
 /* Bratu3D (problems/bratu3d.hpp) */
 void orc_bratu_eval(int64_t n, double lambda, const double* u, double* out);

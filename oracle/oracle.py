@@ -191,6 +191,13 @@ class Oracle:
         L.orc_chandra_solve.argtypes = [_i64, C.c_double, _i64, _i64, C.POINTER(SolverConfig),
                                         C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_inject.argtypes = [_i64, _i64]
+        L.orc_set_inject.restype = None
+
+    def set_inject(self, what: int, at: int) -> None:
+        """NaN into the at-th F evaluation (what 0) or J v product (what 1) of
+        the following Bratu solves; fires once; at = 0 disarms."""
+        self.lib.orc_set_inject(what, at)
 
     def threads(self) -> int:
         return int(self.lib.orc_num_threads())
@@ -366,6 +373,8 @@ class Reference:
         L.ref_bratu_gmres.argtypes = [_i64, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                       C.c_double, _i64, _i64, _dp]
         L.ref_bratu_solve.argtypes = [_i64, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.ref_bratu_solve_inject.argtypes = [_i64, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                             C.c_int, _i64]
         L.ref_tf2018_curvature.argtypes = [_i64, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
         L.ref_bratu_eigen_smallest.argtypes = [_i64, C.c_double, _dp, C.c_double, C.c_double,
                                                _i64, _i64, _i64, _dp, _dp]
@@ -409,6 +418,16 @@ class Reference:
         pv = params_vector(params)
         res = np.zeros(10)
         self.lib.ref_bratu_solve(n, lam, method, precond, fd, _ptr(pv), _ptr(x), _ptr(res))
+        return x, _result_dict(res)
+
+    def bratu_solve_inject(self, n, lam, method, precond, params: KrylovParams, what, at):
+        """Analytic-derivative solve from u0 = 0 with NaN in element 0 of the
+        at-th F evaluation (what 0) or J v product (what 1)."""
+        x = np.zeros(n ** 3)
+        pv = params_vector(params)
+        res = np.zeros(10)
+        self.lib.ref_bratu_solve_inject(n, lam, method, precond, _ptr(pv), _ptr(x), _ptr(res),
+                                        what, at)
         return x, _result_dict(res)
 
     def bratu_eigen_smallest(self, n, lam, x, eigen_rtol=1e-7, linear_rtol=1e-10, max_outer=30,

@@ -18,6 +18,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <limits>
 
 using namespace numa::op;
 
@@ -34,6 +35,29 @@ struct FDBratu {
     void eval(const DVec& x, DVec& out) const { f.eval(x, out); }
     void jvp(const DVec& x, const DVec& v, DVec& out) const { jvp_fd(f, x, v, out); }
     void d2(const DVec& x, const DVec& v, DVec& out) const { d2_fd(f, x, v, out); }
+};
+
+// Bratu3D (analytic derivatives) whose `at`-th eval (what 0) or jvp (what 1)
+// returns NaN in element 0, once: the failure paths of the reference's own
+// loops under a non-finite F or J v.
+struct NanBratu {
+    const Bratu3D& f;
+    int what;
+    std::size_t at;
+    mutable std::size_t calls = 0;
+    std::size_t size() const { return f.size(); }
+    void poison(int w, DVec& out) const {
+        if (w == what && ++calls == at) out[0] = std::numeric_limits<double>::quiet_NaN();
+    }
+    void eval(const DVec& x, DVec& out) const {
+        f.eval(x, out);
+        poison(0, out);
+    }
+    void jvp(const DVec& x, const DVec& v, DVec& out) const {
+        f.jvp(x, v, out);
+        poison(1, out);
+    }
+    void d2(const DVec& x, const DVec& v, DVec& out) const { f.d2(x, v, out); }
 };
 
 KrylovParams to_params(const double* p) {
@@ -216,6 +240,20 @@ void ref_bratu_solve(int64_t n, double lambda, int method, int precond, int fd,
     const DVec x0 = vec(x, n * n * n);
     KrylovResult r = fd ? run_prec(FDBratu{f}, f, x0, p, method, precond)
                         : run_prec(f, f, x0, p, method, precond);
+    put(r.x, x);
+    from_result(r, result);
+}
+
+// ref_bratu_solve with the analytic derivatives and a NaN injected into the
+// `at`-th F evaluation (what 0) or J v product (what 1).
+void ref_bratu_solve_inject(int64_t n, double lambda, int method, int precond,
+                            const double* params, double* x, double* result, int what,
+                            int64_t at) {
+    Bratu3D f((std::size_t)n, lambda);
+    const KrylovParams p = to_params(params);
+    const DVec x0 = vec(x, n * n * n);
+    const NanBratu g{f, what, (std::size_t)at};
+    KrylovResult r = run_prec(g, f, x0, p, method, precond);
     put(r.x, x);
     from_result(r, result);
 }

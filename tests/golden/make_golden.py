@@ -121,12 +121,48 @@ def spectral_gaps():
 TF_SOLVES = ((4, 1.0, 1, 0), (8, 3.0, 1, 0), (15, 2.0, 2, 0), (8, 1.0, 1, 1), (7, 6.0, 0, 0))
 
 
+# Failure paths (test_operators.cpp:141-157 and the non-finite guards of
+# gmres.hpp:165-169, 185-190, krylov_newton.hpp:94-104): Bratu3D n = 8,
+# lambda = 6, u0 = 0, analytic derivatives, BratuJacobi, GMRES(30).
+# (method, restart, max_iters, what, at): what -1 = no injection, 0 = NaN in
+# the at-th F evaluation, 1 = NaN in the at-th J v product.
+FAILURE_SOLVES = ((0, 0, 500, -1, 0), (0, 30, 0, -1, 0), (1, 0, 500, -1, 0),
+                  (0, 30, 500, 0, 1), (0, 30, 500, 0, 2), (0, 30, 500, 0, 3),
+                  (0, 30, 500, 1, 1), (0, 30, 500, 1, 7), (1, 30, 500, 0, 3),
+                  (1, 30, 500, 1, 7), (1, 30, 500, 1, 30))
+
+
+def failure_paths(ref):
+    out = {"n": 8, "lambda": 6.0, "precond": 1, "solves": [], "gmres": []}
+    for method, restart, max_iters, what, at in FAILURE_SOLVES:
+        p = O.krylov_params(gmres_restart=restart, gmres_max_iters=max_iters)
+        if what < 0:
+            _, r = ref.bratu_solve(8, 6.0, method, 1, 0, p)
+        else:
+            _, r = ref.bratu_solve_inject(8, 6.0, method, 1, p, what, at)
+        out["solves"].append({"method": method, "restart": restart, "max_iters": max_iters,
+                              "what": what, "at": at, "result": r})
+    # GMRES on J(x0) z = b: b = 0, restart = 0, max_iters = 0
+    N = 8 ** 3
+    for name, b, restart, max_iters in (("zero_rhs", np.zeros(N), 30, 100),
+                                        ("restart0", O.sin_point(N, 1.0), 0, 100),
+                                        ("maxiter0", O.sin_point(N, 1.0), 30, 0)):
+        _, g = ref.bratu_gmres(8, 6.0, 0, 1, O.sin_point(N, 0.1), b, 1e-10, restart, max_iters)
+        out["gmres"].append({"case": name, "restart": restart, "max_iters": max_iters,
+                             "result": g})
+    return out
+
+
 def main():
     with open(os.path.join(HERE, "reference_csv_rows.json"), "w") as f:
         json.dump(csv_rows(), f, indent=1)
     if "--csv-only" in sys.argv:
         return
     ref = O.Reference()
+    with open(os.path.join(HERE, "failure_paths.json"), "w") as f:
+        json.dump(failure_paths(ref), f, indent=1)
+    if "--failure-only" in sys.argv:
+        return
     data = {
         "calibration_n31": calibration_rows(),
         "start_norms": start_norms(),

@@ -547,3 +547,35 @@ def test_final_solve_safeguard_makes_the_solution_path_independent(oracle):
     assert max(rel(x, ref) for x, _ in safe) < 1e-11
     for (_, a), (_, b) in zip(plain, safe):
         assert b.iters >= a.iters
+
+
+# ---- failure paths --------------------------------------------------------------
+FAIL = json.load(open(os.path.join(GOLDEN, "failure_paths.json")))
+FAIL_KEYS = ("converged", "iters", "n_F", "n_JVP", "n_D2", "n_lin_iters", "n_lin_failures", "n_prec")
+
+
+def _same_residual(got, want, converged):
+    if np.isnan(want):
+        return np.isnan(got)
+    return abs(got - want) <= (5e-3 if converged else 1e-9) * abs(want)
+
+
+@pytest.mark.parametrize("case", FAIL["solves"],
+                         ids=lambda c: f"m{c['method']}-r{c['restart']}-i{c['max_iters']}-w{c['what']}-a{c['at']}")
+def test_failure_paths_match_reference(oracle, case):
+    """restart = 0, max_iters = 0 and a NaN in the k-th F evaluation or J v
+    product, against the reference's own loops (oracle/_ref, fixture
+    tests/golden/failure_paths.json): every counter and flag identical."""
+    cfg = O.solver_config(method=case["method"], precond=O.PRECOND_JACOBI,
+                          jv_mode=O.DERIV_ANALYTIC, d2_mode=O.DERIV_ANALYTIC)
+    p = O.krylov_params(gmres_restart=case["restart"], gmres_max_iters=case["max_iters"])
+    oracle.set_inject(max(case["what"], 0), case["at"] if case["what"] >= 0 else 0)
+    try:
+        _, r = oracle.bratu_solve(FAIL["n"], FAIL["lambda"], cfg, p)
+    finally:
+        oracle.set_inject(0, 0)
+    want = case["result"]
+    got = O.result_dict(r)
+    assert {k: got[k] for k in FAIL_KEYS} == {k: want[k] for k in FAIL_KEYS}
+    assert _same_residual(got["res0"], want["res0"], False)
+    assert _same_residual(got["res"], want["res"], want["converged"])
