@@ -394,6 +394,17 @@ int numa_hip_dot_f32(numa_hip_ctx* ctx, int64_t n, const float* a_dev, const flo
     NUMA_API_END
 }
 
+int numa_hip_scal(numa_hip_ctx* ctx, int64_t n, double a, const double* x_dev, double* out_dev) {
+    NUMA_API_BEGIN
+    Ctx& c = ctx_of(ctx);
+    require(n >= 0, "bad argument");
+    if (n > 0) {
+        require(x_dev && out_dev, "null device pointer");
+        numa_hip::launch_scale_to(c, n, a, x_dev, out_dev);
+    }
+    NUMA_API_END
+}
+
 int numa_hip_axpy(numa_hip_ctx* ctx, int64_t n, double a, const double* x_dev, double* y_dev) {
     NUMA_API_BEGIN
     Ctx& c = ctx_of(ctx);

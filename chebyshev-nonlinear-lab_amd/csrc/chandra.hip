@@ -161,6 +161,38 @@ __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const do
     block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
 }
 
+// out = F''(X)[V, V] from Y = K V (Av = c_b Y): 2 v Av (form 0,
+// chandrasekhar_op.hpp:81-93) or -2 (Av)^2 / q^3 with q = 1 - Ax (form 1).
+__global__ __launch_bounds__(256) void k_ch_d2(int N, int Np, int form, const double* c,
+                                                const double* Ax, const double* V,
+                                                const double* Y, double* out) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const double av = c[b] * Y[o + i];
+        double d;
+        if (form == 0) {
+            d = 2.0 * V[o + i] * av;
+        } else {
+            const double q = 1.0 - Ax[o + i];
+            d = -2.0 * (av * av) / (q * q * q);
+        }
+        out[o + i] = d;
+    }
+}
+
+// Xt_b = (X_b - Z_b) - 0.5 U_b: the Chebyshev step (krylov_newton.hpp:162)
+__global__ __launch_bounds__(256) void k_ch_cheb_trial(int N, int Np, const double* X,
+                                                        const double* Z, const double* U,
+                                                        double* Xt) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.y;
+    const size_t o = (size_t)b * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256)
+        Xt[o + i] = X[o + i] - Z[o + i] - 0.5 * U[o + i];
+}
+
 // Xt_b = X_b - lam_b Z_b
 __global__ __launch_bounds__(256) void k_ch_trial(int N, int Np, const double* X, const double* Z,
                                                    const double* lam, double* Xt) {
@@ -347,6 +379,28 @@ public:
         c_.sync();
     }
 
+    // out = F''(x)[v, v] for all instances (user layout), FP64.
+    void d2_user(const double* x, const double* v, double* out) {
+        pack(x, X_, true);
+        eval_F(X_, F_, Ax_, MISC(0));
+        pack(v, V_[0], true);
+        d2(V_[0], R_);
+        pack(R_, out, false);
+        c_.sync();
+    }
+
+    // NUMA_METHOD_NEWTON (newton_krylov) or NUMA_METHOD_CHEBYSHEV
+    // (chebyshev_inexact) for solve().
+    void set_method(int method) {
+        require(method == NUMA_METHOD_NEWTON || method == NUMA_METHOD_CHEBYSHEV,
+                "the Chandrasekhar solver runs Newton or inexact Chebyshev");
+        method_ = method;
+        if (method == NUMA_METHOD_CHEBYSHEV && !W_) {
+            W_ = static_cast<double*>(zalloc((size_t)Bp_ * Np_ * sizeof(double)));
+            U_ = static_cast<double*>(zalloc((size_t)Bp_ * Np_ * sizeof(double)));
+        }
+    }
+
     // Average ms of one batched contraction and its useful FLOP count 2 N^2 B.
     double gemm_cost(int precision, int64_t reps, double* flops) {
         require(reps > 0, "reps must be positive");
@@ -521,6 +575,14 @@ private:
         bfinish(BEpi{EPI_SQRT, nullptr, norms, nullptr});
     }
 
+    // out = F''(X_)[V, V], FP64 contraction (Ax_ holds A X_)
+    void d2(const double* V, double* out) {
+        gemm_f64(V, Y_);
+        hipLaunchKernelGGL(k_ch_d2, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
+                           (const double*)c_dev_, (const double*)Ax_, V, (const double*)Y_, out);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+
     // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
     void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep) {
         const bool split = prec_ == NUMA_PRECISION_MIXED;
@@ -547,7 +609,10 @@ private:
         numa_gmres_result res{};
     };
     void gmres_batch(const std::vector<char>& active, const std::vector<double>& rtol,
-                     int64_t restart, int64_t max_iters, std::vector<numa_gmres_result>& out);
+                     int64_t restart, int64_t max_iters, std::vector<numa_gmres_result>& out,
+                     const double* rhs, double* sol);
+    int method_ = NUMA_METHOD_NEWTON;
+    double *W_ = nullptr, *U_ = nullptr;  // F''[z,z] and the second solve (Chebyshev)
 
     Ctx& c_;
     int64_t N_, B_, Np_ = 0, Bp_ = 0;
@@ -575,19 +640,20 @@ private:
     std::vector<void*> allocs_;
 };
 
-// Lockstep GMRES(m) for all active instances on J(X_) z = F_ into Z_
+// Lockstep GMRES(m) for all active instances on J(X_) z = rhs into sol
 // (gmres.hpp:61-223 per instance; IdentityPrecond, whose applications are
 // counted as the reference counts them).
 void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vector<double>& rtol,
                                int64_t restart, int64_t max_iters,
-                               std::vector<numa_gmres_result>& out) {
+                               std::vector<numa_gmres_result>& out, const double* rhs,
+                               double* sol) {
     const int64_t B = B_;
     const size_t m = (size_t)restart;
     out.assign((size_t)B, numa_gmres_result{});
     // x = 0 for everybody (the Newton caller passes z = 0)
-    NUMA_HIP_CHECK(hipMemsetAsync(Z_, 0, (size_t)Bp_ * Np_ * sizeof(double), c_.stream));
+    NUMA_HIP_CHECK(hipMemsetAsync(sol, 0, (size_t)Bp_ * Np_ * sizeof(double), c_.stream));
     std::vector<double> bnorm((size_t)B), beta((size_t)B);
-    bdot(F_, F_, BEpi{EPI_SQRT, nullptr, MISC(1), nullptr});
+    bdot(rhs, rhs, BEpi{EPI_SQRT, nullptr, MISC(1), nullptr});
     read(MISC(1), B, bnorm.data());
     std::vector<char> done((size_t)B, 1);
     for (int64_t b = 0; b < B; ++b) {
@@ -619,14 +685,14 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
     std::vector<char> notdone((size_t)B);
     for (int64_t b = 0; b < B; ++b) notdone[b] = !done[b];
     while (any(notdone)) {
-        // cycle start: r = F (first) or F - J z
+        // cycle start: r = rhs (first) or rhs - J sol
         if (first) {
-            NUMA_HIP_CHECK(hipMemcpyAsync(V_[0], F_, (size_t)Bp_ * Np_ * sizeof(double),
+            NUMA_HIP_CHECK(hipMemcpyAsync(V_[0], rhs, (size_t)Bp_ * Np_ * sizeof(double),
                                           hipMemcpyDeviceToDevice, c_.stream));
             bdot(V_[0], V_[0], BEpi{EPI_NORM_SCALE, nullptr, MISC(2), S(0)});
         } else {
             const BEpi ep{EPI_NORM_SCALE, nullptr, MISC(2), S(0)};
-            jv(Z_, nullptr, F_, V_[0], &ep);
+            jv(sol, nullptr, rhs, V_[0], &ep);
             for (int64_t b = 0; b < B; ++b)
                 if (notdone[b]) ++out[b].matvecs;
         }
@@ -727,7 +793,7 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
                                           hipMemcpyHostToDevice, c_.stream));
             hipLaunchKernelGGL(k_ch_combine, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
                                (int)B, (int)kmax, (const double* const*)vptr_,
-                               (const double*)coef_, Z_);
+                               (const double*)coef_, sol);
             NUMA_HIP_LAUNCH_CHECK();
             c_.sync();
         }
@@ -743,7 +809,8 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
     }
 }
 
-// newton_krylov (krylov_newton.hpp:69-114) for every instance, in lockstep.
+// newton_krylov (krylov_newton.hpp:69-114) or chebyshev_inexact (:116-189)
+// for every instance, in lockstep.
 void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov_result* res) {
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t B = B_;
@@ -787,7 +854,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
             eta[b] = tight_cap(p, e, rnorm[b], res[b].res0);
         }
         if (!any) break;
-        gmres_batch(active, eta, p.gmres_restart, p.gmres_max_iters, g);
+        gmres_batch(active, eta, p.gmres_restart, p.gmres_max_iters, g, F_, Z_);
         std::vector<char> pending((size_t)B, 0);
         for (int64_t b = 0; b < B; ++b) {
             if (!active[b]) continue;
@@ -802,8 +869,63 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
             lam[b] = 1.0;
             pending[b] = 1;
         }
-        // Armijo backtracking (krylov_newton.hpp:96-104), all pending instances at once
         std::vector<char> accepted((size_t)B, 0);
+        auto any_of = [](const std::vector<char>& f) {
+            return std::any_of(f.begin(), f.end(), [](char v) { return v != 0; });
+        };
+        auto select_trial = [&](const std::vector<double>& take) {
+            write(MISC(4), take);
+            for (auto pr : {std::pair<double*, double*>{Xt_, X_}, {Ft_, F_}, {Axt_, Ax_}}) {
+                hipLaunchKernelGGL(k_ch_select, bgrid(), dim3(256), 0, c_.stream, (int)N_,
+                                   (int)Np_, (const double*)MISC(4), (const double*)pr.first,
+                                   pr.second);
+                NUMA_HIP_LAUNCH_CHECK();
+            }
+        };
+        if (method_ == NUMA_METHOD_CHEBYSHEV && any_of(pending)) {
+            // chebyshev_inexact (krylov_newton.hpp:147-169): J u = F''[z, z] with
+            // eta2 = min(eta2_max, eta2_factor eta1), then the full step
+            // x - z - u/2 if it lowers ||F||, else the Newton step with backtracking
+            d2(Z_, W_);
+            std::vector<double> eta2((size_t)B, 0.0);
+            for (int64_t b = 0; b < B; ++b) {
+                if (!pending[b]) continue;
+                ++res[b].n_D2;
+                eta2[b] = std::min(p.eta2_max, p.eta2_factor * eta[b]);
+            }
+            std::vector<numa_gmres_result> g2;
+            gmres_batch(pending, eta2, p.gmres_restart, p.gmres_max_iters, g2, W_, U_);
+            std::vector<char> full((size_t)B, 0);
+            for (int64_t b = 0; b < B; ++b) {
+                if (!pending[b]) continue;
+                res[b].n_JVP += g2[b].matvecs;
+                res[b].n_lin_iters += g2[b].iters;
+                res[b].n_prec += g2[b].precs;
+                if (!g2[b].converged) ++res[b].n_lin_failures;
+                full[b] = !g2[b].breakdown;
+            }
+            if (any_of(full)) {
+                hipLaunchKernelGGL(k_ch_cheb_trial, bgrid(), dim3(256), 0, c_.stream, (int)N_,
+                                   (int)Np_, (const double*)X_, (const double*)Z_,
+                                   (const double*)U_, Xt_);
+                NUMA_HIP_LAUNCH_CHECK();
+                eval_F(Xt_, Ft_, Axt_, MISC(0));
+                read(MISC(0), B, fn.data());
+                std::vector<double> take((size_t)Bp_, 0.0);
+                for (int64_t b = 0; b < B; ++b) {
+                    if (!full[b]) continue;
+                    ++res[b].n_F;
+                    if (fn[b] < rnorm[b]) {
+                        pending[b] = 0;
+                        accepted[b] = 1;
+                        take[b] = 1.0;
+                        acc[b] = fn[b];
+                    }
+                }
+                select_trial(take);
+            }
+        }
+        // Armijo backtracking (krylov_newton.hpp:96-104), all pending instances at once
         while (std::any_of(pending.begin(), pending.end(), [](char v) { return v != 0; })) {
             std::vector<double> l((size_t)Bp_, 0.0);
             for (int64_t b = 0; b < B; ++b) l[b] = pending[b] ? lam[b] : 0.0;
@@ -830,13 +952,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
                     }
                 }
             }
-            write(MISC(4), take);
-            for (auto pr : {std::pair<double*, double*>{Xt_, X_}, {Ft_, F_}, {Axt_, Ax_}}) {
-                hipLaunchKernelGGL(k_ch_select, bgrid(), dim3(256), 0, c_.stream, (int)N_,
-                                   (int)Np_, (const double*)MISC(4), (const double*)pr.first,
-                                   pr.second);
-                NUMA_HIP_LAUNCH_CHECK();
-            }
+            select_trial(take);
         }
         for (int64_t b = 0; b < B; ++b) {
             if (!accepted[b]) continue;
@@ -908,6 +1024,41 @@ int numa_hip_chandra_jvp(numa_hip_chandra* h, const double* x_dev, const double*
     numa_hip::require(h && x_dev && v_dev && out_dev, "null argument");
     NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
     h->impl->jvp_user(x_dev, v_dev, out_dev);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_d2(numa_hip_chandra* h, const double* x_dev, const double* v_dev,
+                        double* out_dev) {
+    NUMA_API_BEGIN
+    numa_hip::require(h && x_dev && v_dev && out_dev, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    h->impl->d2_user(x_dev, v_dev, out_dev);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_set_method(numa_hip_chandra* h, int64_t method) {
+    NUMA_API_BEGIN
+    numa_hip::require(h != nullptr, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(h->ctx->device));
+    h->impl->set_method((int)method);
+    NUMA_API_END
+}
+
+int numa_hip_chandra_quadrature(int64_t n, int64_t rule, double* nodes_host,
+                                double* weights_host) {
+    NUMA_API_BEGIN
+    numa_hip::require(n > 0 && nodes_host && weights_host, "bad argument");
+    numa_hip::require(rule == 0 || rule == 1, "rule must be 0 (Gauss-Legendre) or 1 (midpoint)");
+    std::vector<double> mu, w;
+    if (rule == 0) {
+        numa_hip::gauss_legendre_unit((size_t)n, mu, w);
+    } else {
+        mu.resize((size_t)n);
+        w.assign((size_t)n, 1.0 / double(n));
+        for (int64_t i = 0; i < n; ++i) mu[(size_t)i] = (double(i) + 0.5) / double(n);
+    }
+    std::copy(mu.begin(), mu.end(), nodes_host);
+    std::copy(w.begin(), w.end(), weights_host);
     NUMA_API_END
 }
 

@@ -228,10 +228,51 @@ public:
     const MGBase* multigrid() const override { return mg_.get(); }
 
     numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
-                                  int64_t restart, int64_t max_iters, bool zero_guess) override {
+                                  int64_t restart, int64_t max_iters, bool zero_guess,
+                                  const double* r_init, bool save) override {
         point_setup(x0);
         eval_F(x0, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
-        auto r = gmres(x0, Fx_, b, z, rtol, restart, max_iters, zero_guess, nullptr, false);
+        auto r = gmres(x0, Fx_, b, z, rtol, restart, max_iters, zero_guess, r_init, save);
+        c_.sync();
+        collect_timers();
+        return r;
+    }
+
+    int64_t saved_space(double* hbar, double* const* V) override {
+        const int64_t k = saved_k_;
+        if (hbar)
+            for (int64_t i = 0; i <= k && k > 0; ++i)
+                for (int64_t j = 0; j < k; ++j) hbar[i * k + j] = saved_H_[(size_t)i][(size_t)j];
+        if (V && k > 0) {
+            for (int64_t i = 0; i <= k; ++i) {  // V_i = s_i Vraw_i
+                require(V[i] != nullptr, "null basis vector");
+                launch_convert<T, double>(c_, N_, V_[(size_t)i], V[i]);
+                launch_scale_to(c_, N_, saved_s_[(size_t)i], V[i], V[i]);
+            }
+            c_.sync();
+        }
+        return k;
+    }
+
+    numa_gmres_result gmres_recycled_entry(const double* x0, const double* b, double* z,
+                                           double rtol, int64_t restart, int64_t max_iters,
+                                           int64_t k, const double* hbar,
+                                           const double* const* V, bool save) override {
+        if (V) {  // load the given relation as the saved one (unit scales)
+            require(k >= 0 && k <= m_, "saved space larger than the solver's basis");
+            require(k == 0 || hbar != nullptr, "null Hbar");
+            saved_k_ = k;
+            saved_H_.assign((size_t)k + 1, std::vector<double>((size_t)k, 0.0));
+            saved_s_.assign((size_t)k + 1, 1.0);
+            for (int64_t i = 0; i <= k && k > 0; ++i) {
+                require(V[i] != nullptr, "null basis vector");
+                for (int64_t j = 0; j < k; ++j) saved_H_[(size_t)i][(size_t)j] = hbar[i * k + j];
+                launch_convert<double, T>(c_, N_, V[i], V_[(size_t)i]);
+            }
+        }
+        point_setup(x0);
+        eval_F(x0, nullptr, 0.0, nullptr, 0.0, Fx_, nullptr);
+        auto r = gmres_recycled(x0, Fx_, b, z, rtol, restart, max_iters, save);
         c_.sync();
         collect_timers();
         return r;
@@ -559,7 +600,8 @@ private:
                             double rtol, int64_t restart, int64_t max_iters, bool zero_guess,
                             const double* r_init, bool save);
     numa_gmres_result gmres_recycled(const double* x0, const double* F0, const double* b,
-                                     double* xl, double rtol, int64_t restart, int64_t max_iters);
+                                     double* xl, double rtol, int64_t restart, int64_t max_iters,
+                                     bool save = false);
 
     Ctx& c_;
     int64_t n_, N_;
@@ -854,10 +896,10 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
 template <class T>
 numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double* F0,
                                                  const double* b, double* xl, double rtol,
-                                                 int64_t restart, int64_t max_iters) {
+                                                 int64_t restart, int64_t max_iters, bool save) {
     launch_fill<double>(c_, N_, 0.0, xl);
     const int64_t k = saved_k_;
-    if (k == 0) return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+    if (k == 0) return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, save);
     std::vector<double> rhs((size_t)k + 1);
     for (int64_t i = 0; i <= k; ++i)
         launch_dot<T, double>(c_, N_, V_[(size_t)i], b,
@@ -873,7 +915,7 @@ numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double*
         }
         const double denom = std::hypot(R[j][j], R[j + 1][j]);
         if (!(denom > 0.0) || !std::isfinite(denom))
-            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, save);
         cs[j] = R[j][j] / denom;
         sn[j] = R[j + 1][j] / denom;
         R[j][j] = denom;
@@ -887,7 +929,7 @@ numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double*
         const double pivot = R[ii][ii];
         if (!(std::abs(pivot) > 0.0) || !std::isfinite(pivot)) {
             launch_fill<double>(c_, N_, 0.0, xl);
-            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, false);
+            return gmres(x0, F0, b, xl, rtol, restart, max_iters, true, nullptr, save);
         }
         double acc = rhs[ii];
         for (int64_t jj = ii + 1; jj < k; ++jj) acc -= R[ii][jj] * y[jj];
@@ -915,7 +957,7 @@ numa_gmres_result BratuSolver<T>::gmres_recycled(const double* x0, const double*
     const T* px = nullptr;
     timed(T_PREC, [&]() { px = precond(comb_, zb_); });
     launch_convert<T, double>(c_, N_, px, xl);
-    auto result = gmres(x0, F0, b, xl, rtol, restart, max_iters, false, r_, false);
+    auto result = gmres(x0, F0, b, xl, rtol, restart, max_iters, false, r_, save);
     ++result.precs;
     return result;
 }
@@ -1175,6 +1217,40 @@ int numa_hip_gmres(numa_hip_solver* s, const double* x0_dev, const double* b_dev
     numa_hip::require(restart >= 0 && max_iters >= 0, "negative restart / max_iters");
     NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
     *out = s->impl->gmres_entry(x0_dev, b_dev, z_dev, rtol, restart, max_iters, zero_guess != 0);
+    NUMA_API_END
+}
+
+int numa_hip_gmres_ex(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
+                      double* z_dev, double rtol, int64_t restart, int64_t max_iters,
+                      int64_t zero_guess, const double* r_init_dev, int64_t save,
+                      numa_gmres_result* out) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && x0_dev && b_dev && z_dev && out, "null argument");
+    numa_hip::require(restart >= 0 && max_iters >= 0, "negative restart / max_iters");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *out = s->impl->gmres_entry(x0_dev, b_dev, z_dev, rtol, restart, max_iters, zero_guess != 0,
+                                r_init_dev, save != 0);
+    NUMA_API_END
+}
+
+int numa_hip_gmres_space(numa_hip_solver* s, int64_t* k, double* hbar_host, double* const* V_dev) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && k, "null argument");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *k = s->impl->saved_space(hbar_host, V_dev);
+    NUMA_API_END
+}
+
+int numa_hip_gmres_recycled(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
+                            double* z_dev, double rtol, int64_t restart, int64_t max_iters,
+                            int64_t k, const double* hbar_host, const double* const* V_dev,
+                            int64_t save, numa_gmres_result* out) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && x0_dev && b_dev && z_dev && out, "null argument");
+    numa_hip::require(restart >= 0 && max_iters >= 0, "negative restart / max_iters");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    *out = s->impl->gmres_recycled_entry(x0_dev, b_dev, z_dev, rtol, restart, max_iters, k,
+                                         hbar_host, V_dev, save != 0);
     NUMA_API_END
 }
 

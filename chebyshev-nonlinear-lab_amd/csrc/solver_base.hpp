@@ -32,7 +32,20 @@ struct LearnedStats {
 struct SolverBase {
     virtual ~SolverBase() = default;
     virtual numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
-                                          int64_t restart, int64_t max_iters, bool zero_guess) = 0;
+                                          int64_t restart, int64_t max_iters, bool zero_guess,
+                                          const double* r_init = nullptr, bool save = false) = 0;
+    // The Arnoldi relation saved by the last gmres_entry(save): returns k and
+    // writes Hbar ((k+1) x k, row major) and V_0..V_k (FP64 device vectors)
+    // where the pointers are not null.
+    virtual int64_t saved_space(double* hbar, double* const* V) = 0;
+    // gmres_recycled (gmres.hpp:229-323) at x0 on the given relation (k, Hbar,
+    // V_0..V_k device vectors), or on the saved one when V is null; save keeps
+    // the last cycle of its GMRES.
+    virtual numa_gmres_result gmres_recycled_entry(const double* x0, const double* b, double* z,
+                                                   double rtol, int64_t restart,
+                                                   int64_t max_iters, int64_t k,
+                                                   const double* hbar, const double* const* V,
+                                                   bool save) = 0;
     virtual void solve(const numa_krylov_params& p, double* x, numa_krylov_result& out) = 0;
     virtual int64_t size() const = 0;
     virtual int64_t grid_n() const = 0;

@@ -263,6 +263,9 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_chandra_jvp": [_vp, _vp, _vp, _vp],
         "numa_hip_chandra_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
         "numa_hip_chandra_gemm_cost": [_vp, _i64, _i64, _dp, _dp],
+        "numa_hip_chandra_d2": [_vp, _vp, _vp, _vp],
+        "numa_hip_chandra_set_method": [_vp, _i64],
+        "numa_hip_chandra_quadrature": [_i64, _i64, _dp, _dp],
         "numa_hip_net_create": [_vp, _i64, C.POINTER(_vp)],
         "numa_hip_net_destroy": [_vp],
         "numa_hip_net_param_count": [C.POINTER(_i64)],
@@ -715,14 +718,24 @@ def tf_chebyshev_2018(ctx, n, lam, x0: np.ndarray, params=None, cfg=None, restar
     return x, r
 
 
+def chandra_quadrature(n: int, rule: int = 0):
+    """Nodes and weights on [0, 1]: Gauss-Legendre (rule 0) or midpoint (1)."""
+    load()
+    a, w = np.empty(n), np.empty(n)
+    check(_lib.numa_hip_chandra_quadrature(n, rule, a.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    return a, w
+
+
 class ChandrasekharBatch:
     """problems/chandrasekhar_op.hpp:31-136 for a batch of parameters c on the
     device (rule 0 Gauss-Legendre / 1 midpoint; form 0 reference / 1 Kelley),
-    solved by lockstep newton_krylov (krylov_newton.hpp:69-114).
+    solved by lockstep newton_krylov (krylov_newton.hpp:69-114) or, with
+    method = METHOD_CHEBYSHEV, chebyshev_inexact (:116-189).
     Vectors are instance-major numpy arrays of shape (batch, n)."""
 
     def __init__(self, ctx: Context, n: int, cvals, rule: int = 0, form: int = 0,
-                 precision: int = PRECISION_FP64, restart: int = 50):
+                 precision: int = PRECISION_FP64, restart: int = 50,
+                 method: int = METHOD_NEWTON):
         self.ctx, self.n = ctx, int(n)
         self.c = np.ascontiguousarray(cvals, np.float64).ravel()
         self.batch = self.c.size
@@ -732,6 +745,8 @@ class ChandrasekharBatch:
                                               int(restart), C.byref(h)))
         self.h = h
         ctx._adopt(self)
+        if method != METHOD_NEWTON:
+            check(ctx.lib.numa_hip_chandra_set_method(self.h, method))
 
     def _arr(self, x):
         x = np.ascontiguousarray(x, np.float64).reshape(self.batch, self.n)
@@ -748,6 +763,12 @@ class ChandrasekharBatch:
         dx, dv = self._arr(x), self._arr(v)
         do = self.ctx.empty(self.batch * self.n)
         check(self.ctx.lib.numa_hip_chandra_jvp(self.h, dx.ptr, dv.ptr, do.ptr))
+        return do.numpy().reshape(self.batch, self.n)
+
+    def d2(self, x, v):
+        dx, dv = self._arr(x), self._arr(v)
+        do = self.ctx.empty(self.batch * self.n)
+        check(self.ctx.lib.numa_hip_chandra_d2(self.h, dx.ptr, dv.ptr, do.ptr))
         return do.numpy().reshape(self.batch, self.n)
 
     def solve(self, params: KrylovParams, x0=None):

@@ -25,143 +25,10 @@
 // Misuse throws std::invalid_argument / std::overflow_error as the reference
 // does; numerical failure is reported in the result structs.
 
-#include <cstddef>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <utility>
-#include <vector>
-
-#include "../../numa_hip.h"
-#include "../operators/operator.hpp"
+#include "core.hpp"
+#include "gmres.hpp"
 
 namespace numa::hip {
-
-using numa::op::DVec;
-
-inline void check(int rc) {
-    if (rc == NUMA_OK) return;
-    const std::string msg = numa_hip_last_error();
-    if (rc == NUMA_ERR_ARG) throw std::invalid_argument(msg);
-    if (rc == NUMA_ERR_OVERFLOW) throw std::overflow_error(msg);
-    throw std::runtime_error(msg);
-}
-
-// One device + stream.  Must outlive every object created on it.
-class Context {
-public:
-    explicit Context(int device = 0) { check(numa_hip_ctx_create(device, &h_)); }
-    ~Context() {
-        if (h_) numa_hip_ctx_destroy(h_);
-    }
-    Context(const Context&) = delete;
-    Context& operator=(const Context&) = delete;
-    numa_hip_ctx* get() const { return h_; }
-
-private:
-    numa_hip_ctx* h_ = nullptr;
-};
-
-// Device buffer of n doubles.
-class DeviceVec {
-public:
-    DeviceVec(const Context& c, std::size_t n) : c_(&c), n_(n) {
-        check(numa_hip_malloc(c.get(), n * sizeof(double), &p_));
-    }
-    ~DeviceVec() {
-        if (p_) numa_hip_free(c_->get(), p_);
-    }
-    DeviceVec(const DeviceVec&) = delete;
-    DeviceVec& operator=(const DeviceVec&) = delete;
-    double* data() const { return static_cast<double*>(p_); }
-    void upload(const DVec& v) const {
-        if (v.size() != n_) throw std::invalid_argument("DeviceVec: size mismatch");
-        check(numa_hip_memcpy_h2d(c_->get(), p_, v.data(), n_ * sizeof(double)));
-    }
-    void download(DVec& v) const {
-        v.resize(n_);
-        check(numa_hip_memcpy_d2h(c_->get(), v.data(), p_, n_ * sizeof(double)));
-    }
-
-private:
-    const Context* c_;
-    std::size_t n_;
-    void* p_ = nullptr;
-};
-
-// methods/krylov_newton.hpp:24-38, plus the forcing-mode addition.
-struct KrylovParams {
-    double rtol_nl = 1e-8;
-    std::size_t max_nl = 50;
-    double eta_init = 0.5;
-    double eta_max = 0.9;
-    double gamma = 0.9;
-    double alpha = 2.0;
-    double eta2_factor = 10.0;
-    double eta2_max = 0.5;
-    std::size_t gmres_restart = 50;
-    std::size_t gmres_max_iters = 500;
-    double armijo = 1e-4;
-    double min_lambda = 1e-4;
-    bool recycle = false;
-    int forcing = NUMA_FORCING_EW;  // or NUMA_FORCING_CONSTANT (eta = eta_init)
-    // final-solve safeguard (0 = off): eta <= tight_eta once ||F|| <= tight_res ||F(x0)||
-    double tight_res = 0.0;
-    double tight_eta = 0.0;
-
-    numa_krylov_params c() const {
-        return numa_krylov_params{rtol_nl,       (int64_t)max_nl,         eta_init,   eta_max,
-                                  gamma,         alpha,                   eta2_factor, eta2_max,
-                                  (int64_t)gmres_restart, (int64_t)gmres_max_iters, armijo,
-                                  min_lambda,    recycle ? 1 : 0,         forcing,
-                                  tight_res,     tight_eta};
-    }
-};
-
-// methods/krylov_newton.hpp:40-49
-struct KrylovResult {
-    DVec x;
-    bool converged = false;
-    std::size_t iters = 0;
-    std::size_t n_F = 0, n_JVP = 0, n_D2 = 0;
-    std::size_t n_lin_iters = 0;
-    std::size_t n_lin_failures = 0;
-    std::size_t n_prec = 0;
-    double res0 = 0.0, res = 0.0;
-    double t_total = 0.0;  // seconds
-};
-
-// linalg/gmres.hpp:14-21
-struct GmresResult {
-    std::size_t iters = 0;
-    std::size_t matvecs = 0;
-    std::size_t precs = 0;
-    double rel_res = 0.0;
-    bool converged = false;
-    bool breakdown = false;
-};
-
-// operators/timing.hpp:16-24: seconds per primitive.  On the device each is
-// the GPU time of the primitive's kernels (HIP events on the solver stream).
-struct Timers {
-    double t_F = 0.0;
-    double t_JVP = 0.0;
-    double t_D2 = 0.0;
-    double t_prec = 0.0;
-    double t_setup_dynamic = 0.0;
-    void reset() { *this = Timers{}; }
-};
-
-// Derivative and precision options of the device solver.
-struct SolverOptions {
-    int precision = NUMA_PRECISION_FP64;
-    bool flexible = false;              // FGMRES
-    int jv_mode = NUMA_DERIV_ANALYTIC;  // Bratu3D::jvp as in the reference
-    int d2_mode = NUMA_DERIV_ANALYTIC;  // Bratu3D::d2 as in the reference
-    // If set, the solve adds its per-primitive times to *timers, as the
-    // reference's Timed<Op> / TimedPrecond wrappers do (timing.hpp:37-108).
-    Timers* timers = nullptr;
-};
 
 // problems/bratu3d.hpp:12-100 on the device.  eval / jvp / d2 stage host
 // vectors; the solvers below never do.
@@ -170,6 +37,8 @@ public:
     Bratu3D(const Context& c, std::size_t n, double lambda = 1.0) : c_(&c), n_(n), lambda_(lambda) {
         if (n == 0) throw std::invalid_argument("Bratu3D needs n > 0");
     }
+    // the reference's signature, on default_context()
+    explicit Bratu3D(std::size_t n, double lambda = 1.0) : Bratu3D(default_context(), n, lambda) {}
     std::size_t size() const { return n_ * n_ * n_; }
     std::size_t grid() const { return n_; }
     double lambda() const { return lambda_; }
@@ -208,9 +77,54 @@ private:
 
 static_assert(numa::op::SecondOrderOperator<Bratu3D>);
 
-// Preconditioner providers: configuration only; the device solver binds them
-// to the Newton point once per outer iteration as Provider::make does.
+// PoissonMG (linalg/poisson_mg.hpp:24-299): the V-cycle for -Lap_h on n^3
+// nodes as a preconditioner object, operator()(r, z) (poisson_mg.hpp:69-80).
+// The default configuration is the reference's (RB-GS V(2,2), coarsening
+// while n is odd); a numa_mg_config selects the even-n hierarchy and the
+// weighted-Jacobi / Chebyshev smoothers.  Not thread safe, as the reference.
+class PoissonMG {
+public:
+    static numa_mg_config reference_config() {
+        return numa_mg_config{NUMA_SMOOTH_RBGS, 2, 2, 3, 0, 1, 30, 0, 6.0 / 7.0, 4.0, 0, 0, 0};
+    }
+    explicit PoissonMG(std::size_t n) : PoissonMG(default_context(), n, reference_config()) {}
+    PoissonMG(const Context& c, std::size_t n, const numa_mg_config& cfg = reference_config())
+        : c_(&c), n_(n) {
+        if (n == 0) throw std::invalid_argument("PoissonMG needs n > 0");
+        check(numa_hip_mg_create(c.get(), (int64_t)n, &cfg, NUMA_PRECISION_FP64, &h_));
+    }
+    ~PoissonMG() {
+        if (h_) numa_hip_mg_destroy(h_);
+    }
+    PoissonMG(const PoissonMG&) = delete;
+    PoissonMG& operator=(const PoissonMG&) = delete;
+    std::size_t levels() const {
+        int64_t l = 0;
+        check(numa_hip_mg_levels(h_, &l, nullptr));
+        return (std::size_t)l;
+    }
+    std::size_t size() const { return n_ * n_ * n_; }
+    void operator()(const DVec& r, DVec& z) const {
+        if (r.size() != size()) throw std::invalid_argument("PoissonMG: dimension mismatch");
+        DeviceVec dr(*c_, size()), dz(*c_, size());
+        dr.upload(r);
+        check(numa_hip_mg_apply(h_, dr.data(), dz.data()));
+        dz.download(z);
+    }
+
+private:
+    const Context* c_;
+    std::size_t n_;
+    numa_hip_mg* h_ = nullptr;
+};
+
+// Preconditioner providers (krylov_newton.hpp:20-22).  make(x) binds the
+// preconditioner at a point and returns it as a callable (r, z) -> z = M^-1 r
+// for code that drives GMRES itself (bratu3d.hpp:104-120,
+// poisson_mg.hpp:303-320); apply(config) is what the device solvers below
+// use, which bind it on the device once per outer iteration instead.
 struct NoPrecond {
+    IdentityPrecond make(const DVec&) const { return {}; }
     numa_solver_config apply(numa_solver_config c) const {
         c.precond = NUMA_PRECOND_NONE;
         return c;
@@ -219,6 +133,19 @@ struct NoPrecond {
 
 struct BratuJacobi {
     const Bratu3D& f;
+    // z = r / (6 (n+1)^2 - lambda e^x), the diagonal computed on the device
+    struct Bound {
+        DVec d;
+        void operator()(const DVec& r, DVec& z) const {
+            z.resize(r.size());
+            for (std::size_t i = 0; i < r.size(); ++i) z[i] = r[i] / d[i];
+        }
+    };
+    Bound make(const DVec& x) const {
+        Bound b;
+        f.jacobi_diag(x, b.d);
+        return b;
+    }
     numa_solver_config apply(numa_solver_config c) const {
         c.precond = NUMA_PRECOND_JACOBI;
         return c;
@@ -227,15 +154,34 @@ struct BratuJacobi {
 
 struct PoissonMGProvider {
     numa_mg_config mg;
+    std::size_t n = 0;
+    const Context* ctx = nullptr;
     // PoissonMG(n): RB-GS V(2,2), coarsening while n is odd (poisson_mg.hpp:26-64)
-    explicit PoissonMGProvider(std::size_t = 0)
-        : mg{NUMA_SMOOTH_RBGS, 2, 2, 3, 0, 1, 30, 0, 6.0 / 7.0, 4.0} {}
-    explicit PoissonMGProvider(const numa_mg_config& cfg) : mg(cfg) {}
+    explicit PoissonMGProvider(std::size_t n_ = 0)
+        : mg(PoissonMG::reference_config()), n(n_) {}
+    explicit PoissonMGProvider(const numa_mg_config& cfg, std::size_t n_ = 0) : mg(cfg), n(n_) {}
+    PoissonMGProvider(const Context& c, std::size_t n_, const numa_mg_config& cfg)
+        : mg(cfg), n(n_), ctx(&c) {}
+    // The hierarchy is built once per provider and shared by every bound
+    // copy (poisson_mg.hpp:304-319: make binds without set-up).
+    struct Bound {
+        std::shared_ptr<PoissonMG> m;
+        void operator()(const DVec& r, DVec& z) const { (*m)(r, z); }
+    };
+    Bound make(const DVec&) const {
+        if (n == 0) throw std::invalid_argument("PoissonMGProvider: grid size not given");
+        if (!shared_)
+            shared_ = std::make_shared<PoissonMG>(ctx ? *ctx : default_context(), n, mg);
+        return Bound{shared_};
+    }
     numa_solver_config apply(numa_solver_config c) const {
         c.precond = NUMA_PRECOND_GMG;
         c.mg = mg;
         return c;
     }
+
+private:
+    mutable std::shared_ptr<PoissonMG> shared_;
 };
 
 namespace detail {
@@ -303,6 +249,27 @@ KrylovResult chebyshev_inexact(const Bratu3D& f, DVec x, const KrylovParams& p =
                                const PrecondProvider& pc = {}, const SolverOptions& o = {}) {
     return detail::solve(f, std::move(x), p,
                          pc.apply(detail::base_config(NUMA_METHOD_CHEBYSHEV, o)), o.timers);
+}
+
+// The outer loops on a Counted<Bratu3D> (operators/operator.hpp:68-98): the
+// solve runs on the wrapped operator's device kernels and the wrapper counts
+// the F, J v and F'' calls the solver reports, as the reference's wrapper
+// counts the calls made through it.
+template <class PrecondProvider = NoPrecond>
+KrylovResult newton_krylov(const numa::op::Counted<Bratu3D>& f, DVec x,
+                           const KrylovParams& p = {}, const PrecondProvider& pc = {},
+                           const SolverOptions& o = {}) {
+    auto r = newton_krylov(f.base(), std::move(x), p, pc, o);
+    f.add(numa::op::OpCounters{r.n_F, r.n_JVP, r.n_D2});
+    return r;
+}
+template <class PrecondProvider = NoPrecond>
+KrylovResult chebyshev_inexact(const numa::op::Counted<Bratu3D>& f, DVec x,
+                               const KrylovParams& p = {}, const PrecondProvider& pc = {},
+                               const SolverOptions& o = {}) {
+    auto r = chebyshev_inexact(f.base(), std::move(x), p, pc, o);
+    f.add(numa::op::OpCounters{r.n_F, r.n_JVP, r.n_D2});
+    return r;
 }
 
 // methods/tf_chebyshev_2018.hpp:53-57
@@ -382,30 +349,122 @@ EigenEstimate inverse_smallest_eigenpair(const Bratu3D& f, const DVec& x,
     return e;
 }
 
-// linalg/gmres.hpp:61-65 with apply(v, out) = J(x0) v and the provider's
-// preconditioner bound at x0.
-template <class PrecondProvider = NoPrecond>
-GmresResult gmres(const Bratu3D& f, const DVec& x0, const PrecondProvider& pc, const DVec& b,
-                  DVec& x, double rtol, std::size_t restart, std::size_t max_iters,
-                  bool zero_guess = true, const SolverOptions& o = {}) {
-    const numa_solver_config cfg = pc.apply(detail::base_config(NUMA_METHOD_NEWTON, o));
+namespace detail {
+
+inline GmresResult from_c(const numa_gmres_result& r) {
+    return GmresResult{(std::size_t)r.iters, (std::size_t)r.matvecs, (std::size_t)r.precs,
+                       r.rel_res, r.converged != 0, r.breakdown != 0};
+}
+
+// A device solver of J(x0) for the GMRES overloads below.
+template <class PrecondProvider>
+std::unique_ptr<numa_hip_solver, int (*)(numa_hip_solver*)> jacobian_solver(
+    const Bratu3D& f, const PrecondProvider& pc, std::size_t restart, const SolverOptions& o) {
+    const numa_solver_config cfg = pc.apply(base_config(NUMA_METHOD_NEWTON, o));
     numa_hip_solver* s = nullptr;
     check(numa_hip_bratu_solver_create(f.context().get(), (int64_t)f.grid(), f.lambda(), &cfg,
                                        (int64_t)std::max<std::size_t>(1, restart), &s));
-    std::unique_ptr<numa_hip_solver, int (*)(numa_hip_solver*)> guard(s, numa_hip_solver_destroy);
+    return {s, numa_hip_solver_destroy};
+}
+
+// Downloads the relation a device GMRES saved (V_i = s_i Vraw_i).
+inline void export_space(numa_hip_solver* s, const Context& c, std::size_t N, KrylovSpace& out) {
+    out.clear();
+    int64_t k = 0;
+    check(numa_hip_gmres_space(s, &k, nullptr, nullptr));
+    if (k <= 0) return;
+    std::vector<double> hb((size_t)((k + 1) * k));
+    std::vector<std::unique_ptr<DeviceVec>> dv;
+    std::vector<double*> ptrs;
+    for (int64_t i = 0; i <= k; ++i) {
+        dv.push_back(std::make_unique<DeviceVec>(c, N));
+        ptrs.push_back(dv.back()->data());
+    }
+    check(numa_hip_gmres_space(s, &k, hb.data(), ptrs.data()));
+    out.k = (std::size_t)k;
+    out.V.resize((size_t)k + 1);
+    for (int64_t i = 0; i <= k; ++i) dv[(size_t)i]->download(out.V[(size_t)i]);
+    out.Hbar.assign((size_t)k + 1, std::vector<double>((size_t)k, 0.0));
+    for (int64_t i = 0; i <= k; ++i)
+        for (int64_t j = 0; j < k; ++j) out.Hbar[(size_t)i][(size_t)j] = hb[(size_t)(i * k + j)];
+}
+
+}  // namespace detail
+
+// linalg/gmres.hpp:61-65 with apply(v, out) = J(x0) v and the provider's
+// preconditioner bound at x0, all on the device: save receives the last
+// cycle's relation (KrylovSpace, gmres.hpp:25-35), r_init replaces the
+// initial residual b - J x.
+template <class PrecondProvider = NoPrecond>
+GmresResult gmres(const Bratu3D& f, const DVec& x0, const PrecondProvider& pc, const DVec& b,
+                  DVec& x, double rtol, std::size_t restart, std::size_t max_iters,
+                  bool zero_guess = true, KrylovSpace* save = nullptr,
+                  const DVec* r_init = nullptr, const SolverOptions& o = {}) {
     const std::size_t N = f.size();
     if (x0.size() != N || b.size() != N) throw std::invalid_argument("vector size mismatch");
+    if (r_init && r_init->size() != N) {  // gmres.hpp:86-90
+        GmresResult r;
+        r.rel_res = 1.0;
+        r.breakdown = true;
+        return r;
+    }
+    auto s = detail::jacobian_solver(f, pc, restart, o);
     if (x.size() != N) x.assign(N, 0.0);
     DeviceVec dx0(f.context(), N), db(f.context(), N), dz(f.context(), N);
     dx0.upload(x0);
     db.upload(b);
     dz.upload(x);
+    std::unique_ptr<DeviceVec> dr;
+    if (r_init) {
+        dr = std::make_unique<DeviceVec>(f.context(), N);
+        dr->upload(*r_init);
+    }
     numa_gmres_result r{};
-    check(numa_hip_gmres(s, dx0.data(), db.data(), dz.data(), rtol, (int64_t)restart,
-                         (int64_t)max_iters, zero_guess ? 1 : 0, &r));
+    check(numa_hip_gmres_ex(s.get(), dx0.data(), db.data(), dz.data(), rtol, (int64_t)restart,
+                            (int64_t)max_iters, zero_guess ? 1 : 0, dr ? dr->data() : nullptr,
+                            save ? 1 : 0, &r));
     dz.download(x);
-    return GmresResult{(std::size_t)r.iters, (std::size_t)r.matvecs, (std::size_t)r.precs,
-                       r.rel_res, r.converged != 0, r.breakdown != 0};
+    if (save) detail::export_space(s.get(), f.context(), N, *save);
+    return detail::from_c(r);
+}
+
+// linalg/gmres.hpp:229-323 on J(x0), on the device: b projected on `space`
+// without J v products, then GMRES from the projected residual.
+template <class PrecondProvider = NoPrecond>
+GmresResult gmres_recycled(const Bratu3D& f, const DVec& x0, const PrecondProvider& pc,
+                           const DVec& b, DVec& x, double rtol, std::size_t restart,
+                           std::size_t max_iters, const KrylovSpace& space,
+                           KrylovSpace* save = nullptr, const SolverOptions& o = {}) {
+    const std::size_t N = f.size();
+    if (x0.size() != N || b.size() != N) throw std::invalid_argument("vector size mismatch");
+    const bool usable = detail::usable_space(space, N) && space.k <= restart;
+    auto s = detail::jacobian_solver(f, pc, std::max(restart, usable ? space.k : 0), o);
+    DeviceVec dx0(f.context(), N), db(f.context(), N), dz(f.context(), N);
+    dx0.upload(x0);
+    db.upload(b);
+    std::vector<std::unique_ptr<DeviceVec>> dv;
+    std::vector<const double*> ptrs;
+    std::vector<double> hb;
+    const int64_t k = usable ? (int64_t)space.k : 0;
+    for (int64_t i = 0; i <= k && k > 0; ++i) {
+        dv.push_back(std::make_unique<DeviceVec>(f.context(), N));
+        dv.back()->upload(space.V[(size_t)i]);
+        ptrs.push_back(dv.back()->data());
+        for (int64_t j = 0; j < k; ++j) hb.push_back(space.Hbar[(size_t)i][(size_t)j]);
+    }
+    if (k == 0) {  // no usable relation: the loader sees an empty one
+        dv.push_back(std::make_unique<DeviceVec>(f.context(), N));
+        ptrs.push_back(dv.back()->data());
+    }
+    numa_gmres_result r{};
+    check(numa_hip_gmres_recycled(s.get(), dx0.data(), db.data(), dz.data(), rtol,
+                                  (int64_t)restart, (int64_t)max_iters, k,
+                                  hb.empty() ? nullptr : hb.data(), ptrs.data(), save ? 1 : 0,
+                                  &r));
+    x.resize(N);
+    dz.download(x);
+    if (save) detail::export_space(s.get(), f.context(), N, *save);
+    return detail::from_c(r);
 }
 
 }  // namespace numa::hip

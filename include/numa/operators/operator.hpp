@@ -60,6 +60,14 @@ public:
     }
     const OpCounters& counters() const { return c_; }
     void reset() const { c_ = OpCounters{}; }
+    // Additions for device solvers that call the wrapped operator's kernels
+    // directly: the wrapped operator, and the calls the solver reports.
+    const Op& base() const { return op_; }
+    void add(const OpCounters& c) const {
+        c_.n_F += c.n_F;
+        c_.n_JVP += c.n_JVP;
+        c_.n_D2 += c.n_D2;
+    }
 
 private:
     const Op& op_;

@@ -232,6 +232,9 @@ int numa_hip_dot_f32(numa_hip_ctx* ctx, int64_t n, const float* a_dev,
                      const float* b_dev, double* result_host);
 int numa_hip_axpy(numa_hip_ctx* ctx, int64_t n, double a, const double* x_dev,
                   double* y_dev);
+/* out = a x (out may be x; a = 1 copies).  operator.hpp:19-41 has no scal; it
+ * is the copy / scale the Krylov layer of the C++ drop-in needs. */
+int numa_hip_scal(numa_hip_ctx* ctx, int64_t n, double a, const double* x_dev, double* out_dev);
 
 /* ---- geometric multigrid (linalg/poisson_mg.hpp) ------------------------ */
 /* Builds the level hierarchy of -Lap_h on n^3 interior nodes.  Replaces
@@ -275,6 +278,26 @@ int numa_hip_solver_destroy(numa_hip_solver* s);
 int numa_hip_gmres(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
                    double* z_dev, double rtol, int64_t restart, int64_t max_iters,
                    int64_t zero_guess, numa_gmres_result* out);
+/* numa_hip_gmres with the rest of gmres.hpp:61-65: r_init_dev (may be NULL)
+ * is the initial residual b - J x (no J v spent on it), save != 0 keeps the
+ * last Arnoldi cycle (KrylovSpace, gmres.hpp:25-35) in the solver for
+ * numa_hip_gmres_space / numa_hip_gmres_recycled. */
+int numa_hip_gmres_ex(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
+                      double* z_dev, double rtol, int64_t restart, int64_t max_iters,
+                      int64_t zero_guess, const double* r_init_dev, int64_t save,
+                      numa_gmres_result* out);
+/* The saved relation A M^-1 V_k = V_k+1 Hbar: *k, Hbar ((k+1) x k, row major,
+ * may be NULL) and, if V_dev is not NULL, V_0..V_k into the k+1 FP64 device
+ * vectors it points to. */
+int numa_hip_gmres_space(numa_hip_solver* s, int64_t* k, double* hbar_host, double* const* V_dev);
+/* gmres_recycled (gmres.hpp:229-323) on J(x0): b projected on the relation
+ * (k, Hbar, V_dev[0..k]), x0 = M^-1 V_k y, then GMRES from that residual; with
+ * V_dev NULL the relation saved by the last numa_hip_gmres_ex(save = 1).
+ * save != 0 keeps the last cycle of that GMRES in turn. */
+int numa_hip_gmres_recycled(numa_hip_solver* s, const double* x0_dev, const double* b_dev,
+                            double* z_dev, double rtol, int64_t restart, int64_t max_iters,
+                            int64_t k, const double* hbar_host, const double* const* V_dev,
+                            int64_t save, numa_gmres_result* out);
 /* Newton-Krylov or inexact Chebyshev from x (device, FP64, overwritten by
  * the final iterate).  Replaces numa::op::newton_krylov /
  * chebyshev_inexact (methods/krylov_newton.hpp:69-189). */
@@ -403,9 +426,21 @@ int numa_hip_chandra_eval(numa_hip_chandra* h, const double* x_dev, double* F_de
 /* J v for every instance (ChandrasekharOp::jvp, chandrasekhar_op.hpp:68-79). */
 int numa_hip_chandra_jvp(numa_hip_chandra* h, const double* x_dev, const double* v_dev,
                          double* out_dev);
-/* newton_krylov (krylov_newton.hpp:69-114, no preconditioner) for every
- * instance; x_dev holds the starting points and receives the iterates;
- * results[batch] receives each instance's KrylovResult. */
+/* F''(x)[v,v] for every instance (ChandrasekharOp::d2, chandrasekhar_op.hpp:
+ * 81-93: 2 v Av; Kelley form: -2 (Av)^2 / (1 - Ax)^3), FP64. */
+int numa_hip_chandra_d2(numa_hip_chandra* h, const double* x_dev, const double* v_dev,
+                        double* out_dev);
+/* The outer method of numa_hip_chandra_solve: NUMA_METHOD_NEWTON (the
+ * default, newton_krylov) or NUMA_METHOD_CHEBYSHEV (chebyshev_inexact,
+ * krylov_newton.hpp:116-189, with F''[z,z] from numa_hip_chandra_d2). */
+int numa_hip_chandra_set_method(numa_hip_chandra* h, int64_t method);
+/* Nodes and weights of the rule on [0, 1] (rule 0: gauss_legendre_unit,
+ * functions/gauss_legendre_dynamic.hpp:50-114; rule 1: midpoint), host. */
+int numa_hip_chandra_quadrature(int64_t n, int64_t rule, double* nodes_host,
+                                double* weights_host);
+/* newton_krylov (krylov_newton.hpp:69-114, no preconditioner), or the method
+ * set above, for every instance; x_dev holds the starting points and receives
+ * the iterates; results[batch] receives each instance's KrylovResult. */
 int numa_hip_chandra_solve(numa_hip_chandra* h, const numa_krylov_params* p, double* x_dev,
                            numa_krylov_result* results);
 /* Average ms of one batched contraction (FP64 or split-BF16) and its useful

@@ -153,6 +153,18 @@ def failure_paths(ref):
     return out
 
 
+def chandra_solves(ref):
+    """newton_krylov / chebyshev_inexact on ChandrasekharOp (Gauss-Legendre,
+    x0 = 1, default KrylovParams but rtol 1e-10): counters and solutions."""
+    out = []
+    for n in (8, 24, 64):
+        for c in (0.5, 0.9, 0.9999):
+            for method in (0, 1):
+                x, r = ref.chandra_solve(n, c, method, O.krylov_params(rtol_nl=1e-10))
+                out.append({"n": n, "c": c, "method": method, "result": r, "x": list(x)})
+    return out
+
+
 def main():
     with open(os.path.join(HERE, "reference_csv_rows.json"), "w") as f:
         json.dump(csv_rows(), f, indent=1)
@@ -161,6 +173,8 @@ def main():
     ref = O.Reference()
     with open(os.path.join(HERE, "failure_paths.json"), "w") as f:
         json.dump(failure_paths(ref), f, indent=1)
+    with open(os.path.join(HERE, "chandra_solves.json"), "w") as f:
+        json.dump(chandra_solves(ref), f, indent=1)
     if "--failure-only" in sys.argv:
         return
     data = {

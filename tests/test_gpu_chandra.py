@@ -138,3 +138,57 @@ def test_stored_k_matches_generated(ctx, monkeypatch, precision, n, batch):
         else:
             assert np.array_equal(a, b)
     assert all(c for _, _, c in got[0][4])
+
+
+@pytest.mark.parametrize("n,rule,form", [(8, 0, 0), (64, 1, 1), (200, 1, 0)])
+def test_eval_d2_fp64(ctx, oracle, n, rule, form):
+    import numa_hip as H
+    cs = np.array([0.5, 0.8, 0.99, 0.9999])
+    ch = H.ChandrasekharBatch(ctx, n, cs, rule=rule, form=form)
+    x = _x(cs.size, n)
+    v = _x(cs.size, n, seed=1) - 1.0
+    D = ch.d2(x, v)
+    for b, c in enumerate(cs):
+        assert rel(D[b], oracle.chandra_d2(n, c, x[b], v[b], rule=rule, form=form)) < 1e-13
+
+
+@pytest.mark.parametrize("n,rule,form,restart", [(8, 0, 0, 50), (64, 1, 1, 20), (200, 1, 0, 20)])
+def test_batched_chebyshev_fp64_matches_oracle(ctx, oracle, n, rule, form, restart):
+    """chebyshev_inexact (krylov_newton.hpp:116-189) in lockstep over the batch:
+    every counter identical to the oracle's solve for that c (whose form 0 /
+    Gauss-Legendre solves equal the reference build's; tests/test_oracle.py)."""
+    import numa_hip as H
+    import oracle as O
+    cs = np.array([0.5, 0.75, 0.9, 0.99, 0.9999, 0.2])
+    p = H.krylov_params(rtol_nl=1e-10, gmres_restart=restart)
+    ch = H.ChandrasekharBatch(ctx, n, cs, rule=rule, form=form, restart=restart,
+                              method=H.METHOD_CHEBYSHEV)
+    x, res = ch.solve(p)
+    cfg = O.solver_config(method=1, precond=O.PRECOND_NONE, jv_mode=O.DERIV_ANALYTIC,
+                          d2_mode=O.DERIV_ANALYTIC)
+    for b, c in enumerate(cs):
+        xo, ro = oracle.chandra_solve(n, c, cfg, _oparams(p), rule=rule, form=form)
+        got = {k: getattr(res[b], k) for k in KEYS}
+        want = {k: getattr(ro, k) for k in KEYS}
+        assert got == want, (c, got, want)
+        assert res[b].n_D2 >= 1
+        assert rel(x[b], xo) < 1e-11
+
+
+def test_batched_solves_match_reference_build(ctx):
+    """The reference's own newton_krylov / chebyshev_inexact on ChandrasekharOp
+    (tests/golden/chandra_solves.json, from the reference build), one batch per
+    (N, method) over c in {0.5, 0.9, 0.9999}: every counter identical."""
+    import numa_hip as H
+    cases = json.load(open(os.path.join(GOLDEN, "chandra_solves.json")))
+    for n in sorted({c["n"] for c in cases}):
+        for method in (0, 1):
+            sel = [c for c in cases if c["n"] == n and c["method"] == method]
+            cs = np.array([c["c"] for c in sel])
+            ch = H.ChandrasekharBatch(ctx, n, cs, rule=0, form=0, restart=50, method=method)
+            x, res = ch.solve(H.krylov_params(rtol_nl=1e-10))
+            ch.close()
+            for b, c in enumerate(sel):
+                got = {k: int(getattr(res[b], k)) for k in KEYS}
+                assert got == {k: int(c["result"][k]) for k in KEYS}, (n, method, c["c"])
+                assert rel(x[b], c["x"]) < 1e-11

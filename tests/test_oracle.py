@@ -579,3 +579,20 @@ def test_failure_paths_match_reference(oracle, case):
     assert {k: got[k] for k in FAIL_KEYS} == {k: want[k] for k in FAIL_KEYS}
     assert _same_residual(got["res0"], want["res0"], False)
     assert _same_residual(got["res"], want["res"], want["converged"])
+
+
+CHS = json.load(open(os.path.join(GOLDEN, "chandra_solves.json")))
+
+
+@pytest.mark.parametrize("case", CHS, ids=lambda c: f"n{c['n']}-c{c['c']}-m{c['method']}")
+def test_chandra_solves_match_reference(oracle, case):
+    """newton_krylov / chebyshev_inexact on the reference's ChandrasekharOp
+    (fixture from the reference build): every counter identical, solution to
+    1e-12."""
+    cfg = O.solver_config(method=case["method"], precond=O.PRECOND_NONE,
+                          jv_mode=O.DERIV_ANALYTIC, d2_mode=O.DERIV_ANALYTIC)
+    x, r = oracle.chandra_solve(case["n"], case["c"], cfg, O.krylov_params(rtol_nl=1e-10))
+    got = O.result_dict(r)
+    want = case["result"]
+    assert {k: got[k] for k in FAIL_KEYS} == {k: want[k] for k in FAIL_KEYS}
+    assert rel(x, case["x"]) < 1e-12
