@@ -42,11 +42,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_params(H, forcing: str):
+def make_params(H, forcing: str, safeguard: bool = False):
+    """The reference's KrylovParams with GMRES(30); safeguard: the final-solve
+    forcing cap (eta <= 1e-6 once ||F|| <= 1e-6 ||F0||, converged only once the
+    last step started there), which makes the final iterate independent of
+    the Krylov path."""
+    extra = dict(tight_res=SAFE_RES, tight_eta=SAFE_ETA) if safeguard else {}
     if forcing == "constant":
         return H.krylov_params(gmres_restart=30, gmres_max_iters=600, eta_init=1e-3,
-                               forcing=H.FORCING_CONSTANT)
-    return H.krylov_params(gmres_restart=30, gmres_max_iters=600)
+                               forcing=H.FORCING_CONSTANT, **extra)
+    return H.krylov_params(gmres_restart=30, gmres_max_iters=600, **extra)
+
+
+SAFE_RES, SAFE_ETA = 1e-6, 1e-6
 
 
 # Coarsest level (16^3 at 256^3 with 5 levels) solved exactly by sine
@@ -132,7 +140,7 @@ def measured_traffic(kernel: str, n: int, ichunk: int, grid: int):
 def primitive_table(H, s, reps=20):
     import ctypes as C
     names = {"F": H.PRIM_F, "JVP": H.PRIM_JVP, "PREC": H.PRIM_PREC, "SMOOTH": H.PRIM_SMOOTH,
-             "MGS": H.PRIM_MGS}
+             "MGS": H.PRIM_MGS, "COPY": H.PRIM_COPY}
     out = {}
     for name, prim in names.items():
         ms, b = C.c_double(), C.c_double()
@@ -151,7 +159,7 @@ def cpu_leg(args) -> None:
     m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
     ocfg = O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=0, flexible=0,
                           jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
-    p = make_params(H, args.forcing)
+    p = make_params(H, args.forcing, safeguard=args.cpu_safeguard)
     op = O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
     o = O.Oracle()
     t0 = time.perf_counter()
@@ -161,21 +169,108 @@ def cpu_leg(args) -> None:
     print(json.dumps({"seconds": t, "threads": o.threads(), "result": O.result_dict(r)}))
 
 
-def run_cpu_baseline(args):
+def run_cpu_baseline(args, safeguard: bool = False):
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     import tempfile
-    out = os.path.join(tempfile.gettempdir(), f"numa_cpu_solution_{os.getpid()}.npy")
+    out = os.path.join(tempfile.gettempdir(), f"numa_cpu_solution_{os.getpid()}_{int(safeguard)}.npy")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--n", str(args.n), "--lam",
            str(args.lam), "--levels", str(args.levels), "--forcing", args.forcing,
-           "--cpu-out", out]
-    log(f"cpu baseline: FP64 oracle solve at n={args.n} with {threads} threads")
+           "--cpu-out", out] + (["--cpu-safeguard"] if safeguard else [])
+    log(f"cpu baseline: FP64 oracle solve at n={args.n} with {threads} threads"
+        f"{' (final-solve safeguard)' if safeguard else ''}")
     p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=args.cpu_timeout)
     if p.returncode != 0:
         raise RuntimeError("cpu leg failed: " + p.stderr[-2000:])
     info = json.loads(p.stdout.strip().splitlines()[-1])
     return info, out
+
+
+CFG0_N, CFG0_LAM = 32, 6.0
+
+
+def config0_cfg(H):
+    """32^3, lambda = 6: FP64 Newton, FD J v, GMRES(30) + V(2,2) weighted-Jacobi
+    multigrid (full even-n hierarchy, exact coarsest solve), constant forcing 1e-3."""
+    return H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
+                           mg=H.mg_wjacobi(coarse_iters=-1))
+
+
+def config0_cpu_leg(args) -> None:
+    """Child process: the oracle's FP64 solve of the 32^3 plain-method
+    workload, one warm-up then `reps` timed solves."""
+    import numpy as np
+    import numa_hip as H
+    import oracle as O
+    cfg = config0_cfg(H)
+    m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
+    ocfg = O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=0, flexible=0,
+                          jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
+    p = make_params(H, "constant")
+    op = O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
+    o = O.Oracle()
+    o.bratu_solve(CFG0_N, CFG0_LAM, ocfg, op)  # warm-up
+    times = []
+    for _ in range(args.cfg0_reps):
+        t0 = time.perf_counter()
+        x, r = o.bratu_solve(CFG0_N, CFG0_LAM, ocfg, op)
+        times.append(time.perf_counter() - t0)
+    np.save(args.cpu_out, x)
+    print(json.dumps({"times": times, "threads": o.threads(), "result": O.result_dict(r)}))
+
+
+def config0_leg(H, ctx, args) -> dict:
+    """The reference's plain method at 32^3 on the GPU (median of 5 solves after
+    a warm-up) and the same FP64 solve by the CPU oracle (median of 5 after a
+    warm-up, OpenMP), with the counters and the solution compared."""
+    import numpy as np
+    cfg = config0_cfg(H)
+    p = make_params(H, "constant")
+    s = H.BratuSolver(ctx, CFG0_N, CFG0_LAM, cfg, restart=30)
+    x = ctx.empty(CFG0_N ** 3)
+    zeros = np.zeros(CFG0_N ** 3)
+    x.upload(zeros)
+    s.solve(p, x)
+    times = []
+    for _ in range(5):
+        x.upload(zeros)
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = s.solve(p, x)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+    xg = x.numpy()
+    s.close()
+    x.free()
+    keys = ("iters", "n_F", "n_JVP", "n_lin_iters", "n_prec")
+    out = {"workload": "bratu3d_32^3_lambda6.0_newton_gmres30_gmg_wjacobi_v22_fd_jv_forcing1e-3",
+           "gpu": {"median_s": round(float(np.median(times)), 6), "solves": 5,
+                   **{k: int(getattr(r, k)) for k in keys}}}
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    import tempfile
+    path = os.path.join(tempfile.gettempdir(), f"numa_cfg0_{os.getpid()}.npy")
+    pr = subprocess.run([sys.executable, os.path.abspath(__file__), "--config0-cpu-leg",
+                         "--cpu-out", path, "--cfg0-reps", "5"], env=env, capture_output=True,
+                        text=True, timeout=args.cpu_timeout)
+    if pr.returncode == 0:
+        info = json.loads(pr.stdout.strip().splitlines()[-1])
+        xc = np.load(path)
+        os.remove(path)
+        out["cpu_baseline"] = {"median_s": round(float(np.median(info["times"])), 6),
+                               "solves": len(info["times"]), "warmup": 1,
+                               "cores": info["threads"], "kind": "port",
+                               "cpu_model": cpu_model(),
+                               **{k: int(info["result"][k]) for k in keys}}
+        out["counters_identical"] = all(out["gpu"][k] == out["cpu_baseline"][k] for k in keys)
+        out["rel_l2_gpu_vs_cpu"] = float(np.linalg.norm(xg - xc) / np.linalg.norm(xc))
+        out["speedup_gpu_vs_cpu"] = round(out["cpu_baseline"]["median_s"] /
+                                          out["gpu"]["median_s"], 2)
+    log(f"config 0: {out}")
+    return out
 
 
 CH_N, CH_B, CH_SEED = 8192, 256, 20261203
@@ -242,23 +337,30 @@ CONV_FLOP_PER_POINT = 2 * 27 * (1 * 24 + 4 * 24 * 24 + 24 * 1)  # 127,008
 def learned_leg(H, ctx, args, params) -> dict:
     """Learned preconditioner.  The conv net is trained on the GPU on
     (residual, V(2,2) correction) pairs generated by mixed Newton-FGMRES solves
-    (16^3-32^3, lambda ~ U[1, 6.7], residual-norm loss), then Newton-FGMRES(30)
-    at n^3 for lambda in {5.0, 6.0, 6.5, 6.7} runs with the network and the
-    V-cycle fallback (a step reducing the residual by less than 0.5 switches the
-    rest of that linear solve to the V-cycle) next to the V-cycle alone."""
+    (lambda ~ U[1, 6.7], residual-norm loss; --learned-sizes, default 16^3-32^3
+    with --learned-pairs pairs, or 64^3-128^3 with 200k pairs for the full
+    training budget), then Newton-FGMRES(30) at n^3 for lambda in {5.0, 6.0,
+    6.5, 6.7} runs with the network and the V-cycle fallback (a step reducing
+    the residual by less than 0.5 switches the rest of that linear solve to the
+    V-cycle) next to the V-cycle alone, both with the final-solve safeguard so
+    that their solutions can be compared at 1e-10."""
     import numpy as np
     n = args.n
     net = H.ConvNet(ctx, seed=21)
-    tcfg = H.net_train_config(pairs=args.learned_pairs, batch=4, sizes=(16, 24, 32), lr=2e-3,
+    sizes = tuple(int(v) for v in args.learned_sizes.split(","))
+    tcfg = H.net_train_config(pairs=args.learned_pairs, batch=4, sizes=sizes, lr=2e-3,
                               seed=5, mg=H.mg_cheb(levels=3), loss=H.LOSS_RESIDUAL)
-    st, hist = net.train_solver(tcfg, hist_len=100000)
+    st, hist = net.train_solver(tcfg, hist_len=1000000)
     k = max(1, len(hist) // 10)
     out = {"train": {"pairs": int(st.pairs), "steps": int(st.steps), "solves": int(st.solves),
-                     "seconds": round(st.seconds, 3), "sizes": [16, 24, 32],
+                     "seconds": round(st.seconds, 3), "sizes": list(sizes),
                      "loss": "residual", "loss_first_window": round(float(hist[:k].mean()), 4),
                      "loss_last_window": round(float(hist[-k:].mean()), 4)}}
     log(f"learned: trained on {st.pairs} pairs in {st.seconds:.2f} s, loss "
         f"{out['train']['loss_first_window']} -> {out['train']['loss_last_window']}")
+    if args.learned_save:
+        np.save(args.learned_save, net.get_params())
+    params = make_params(H, args.forcing, safeguard=True)
     mg = H.mg_cheb(levels=args.levels, degree=3, pre=2, post=2, coarse_iters=COARSE)
     cfg_l = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_LEARNED,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
@@ -293,6 +395,10 @@ def learned_leg(H, ctx, args, params) -> dict:
             x.free()
         row["rel_l2_learned_vs_gmg"] = float(np.linalg.norm(sols["learned"] - sols["gmg"]) /
                                              np.linalg.norm(sols["gmg"]))
+        ref = getattr(args, "_safe_solution", None)
+        if ref is not None and lam == args.lam and ref.size == n ** 3:
+            row["rel_l2_learned_vs_cpu_fp64"] = float(np.linalg.norm(sols["learned"] - ref) /
+                                                      np.linalg.norm(ref))
         log(f"  lambda {lam}: {row}")
         runs.append(row)
     out["solves"] = runs
@@ -424,6 +530,9 @@ def main() -> int:
     ap.add_argument("--no-chandra", action="store_true", help="skip the Chandrasekhar leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
+    ap.add_argument("--learned-sizes", default="16,24,32",
+                    help="training grid sizes (the full budget: 64,96,128 with 200000 pairs)")
+    ap.add_argument("--learned-save", default="", help="write the trained parameters (.npy)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--smooth-rows", type=int, default=None,
                     help="pin the fused legs' smoothing to one launch per sweep with this "
@@ -431,7 +540,13 @@ def main() -> int:
                          "level, plane chunks from the wave-filling model, instead of autotuning")
     ap.add_argument("--only", choices=["chandra", "learned", "sweep"], default=None,
                     help="run only this leg and print its JSON (for profiling)")
+    ap.add_argument("--no-config0", action="store_true", help="skip the 32^3 plain-method leg")
+    ap.add_argument("--no-safeguard", action="store_true",
+                    help="skip the legs run with the final-solve safeguard")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-safeguard", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--config0-cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cfg0-reps", type=int, default=5, help=argparse.SUPPRESS)
     ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-out", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -441,6 +556,9 @@ def main() -> int:
         return 0
     if args.chandra_cpu_leg:
         chandra_cpu_sample(args)
+        return 0
+    if args.config0_cpu_leg:
+        config0_cpu_leg(args)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build "
@@ -498,9 +616,14 @@ def main() -> int:
                  f"{'sweep' if f0[0] == H.FUSE_SWEEP else 'phase'}, ichunk {f0[1]})")
         traffic = measured_traffic(kbase, args.n, int(f0[1]),
                                    fused_grid_threads(args.n, steps, int(f0[2]), int(f0[1])))
+    copy = L["prims"]["COPY"]
     roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(sm["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(sm["GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "measured_copy_GBs": round(copy["GBs"], 1),
+                "frac_of_measured_copy": round(sm["GBs"] / copy["GBs"], 4) if copy["GBs"] else None,
+                "measured_copy_kernel": "k_convert<true,double,double>, 2^24 doubles (read + "
+                                        "write 268 MB), HIP events, 20 launches",
                 "traffic_source": ("profiles/" + os.path.basename(PROFILE_MD) + " (FETCH_SIZE + "
                                    "WRITE_SIZE per launch, copy-kernel calibrated)")
                 if traffic is not None else None,
@@ -540,6 +663,44 @@ def main() -> int:
         out["fp64_fused_smoothing_s"] = round(tf, 6)
         out["speedup_vs_fp64_fused_smoothing"] = round(tf / t_main, 3)
 
+    if not args.no_safeguard:
+        # the same legs with the final-solve safeguard: their solutions no longer
+        # depend on the Krylov path, so each is compared with a CPU FP64 solve
+        # that takes its own path
+        ps = make_params(H, args.forcing, safeguard=True)
+        safe = {"params": {"tight_res": SAFE_RES, "tight_eta": SAFE_ETA}}
+        for name in ("mixed", "fp64"):
+            if name not in legs:
+                continue
+            s2, x2, r2, t2 = time_leg(H, ctx, args.n, args.lam, cfgs[name], ps, args.steps,
+                                      args.warmup, graphs=args.graphs and name != "fp64")
+            safe[name] = {"s": round(float(np.mean(t2)), 6), "newton_iters": int(r2.iters),
+                          "gmres_iters": int(r2.n_lin_iters),
+                          "final_rel_residual": r2.res / r2.res0, "x": x2.numpy()}
+            s2.close()
+            x2.free()
+        if "mixed" in safe and "fp64" in safe:
+            safe["speedup_vs_fp64"] = round(safe["fp64"]["s"] / safe["mixed"]["s"], 3)
+        if not args.no_cpu:
+            info_s, path_s = run_cpu_baseline(args, safeguard=True)
+            xs = np.load(path_s)
+            os.remove(path_s)
+            safe["cpu_fp64"] = {"s": round(info_s["seconds"], 3),
+                                "newton_iters": info_s["result"]["iters"],
+                                "gmres_iters": info_s["result"]["n_lin_iters"],
+                                "final_rel_residual": info_s["result"]["res"] /
+                                info_s["result"]["res0"]}
+            for name in ("mixed", "fp64"):
+                if name in safe:
+                    safe[name]["rel_l2_vs_cpu_fp64"] = float(
+                        np.linalg.norm(safe[name]["x"] - xs) / np.linalg.norm(xs))
+            args._safe_solution = xs
+        for name in ("mixed", "fp64"):
+            if name in safe:
+                safe[name].pop("x")
+        out["safeguarded"] = safe
+        log(f"safeguarded legs: {safe}")
+
     if not args.no_cpu:
         info, path = run_cpu_baseline(args)
         xc = np.load(path)
@@ -576,6 +737,8 @@ def main() -> int:
                                                 f"of the {CH_B} c values ({info['seconds']:.2f} s), "
                                                 f"scaled by {CH_B}/{info['instances']}"}
         out["chandrasekhar"] = ch
+    if not args.no_config0:
+        out["config0"] = config0_leg(H, ctx, args)
     if not args.no_sweep:
         out["sweep"] = sweep_leg(H, ctx, args, params)
     if not args.no_learned:

@@ -327,6 +327,9 @@ public:
                 case NUMA_PRIM_SMOOTH:
                     require(mg_ != nullptr, "no multigrid in this solver");
                     return mg_->smooth_step_for_timing();
+                case NUMA_PRIM_COPY:  // the streaming copy that calibrates the HBM peak
+                    launch_convert<double, double>(c_, N_, x2_, z_);
+                    return 16.0 * N;
                 case NUMA_PRIM_MGS:
                     launch_mgs<T>(c_, N_, V_[2], V_[0], H(0, 0), S(0), V_[1],
                                   Epilogue{EPI_STORE, 1.0, nullptr, MISC(S_TMP), nullptr});

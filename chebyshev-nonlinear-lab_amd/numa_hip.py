@@ -41,7 +41,7 @@ FUSE_NONE, FUSE_SWEEP, FUSE_PHASE, FUSE_AUTO = 0, 1, 2, 3
 # output box i x j x k per workgroup
 BOX_8x8x16, BOX_4x8x16, BOX_4x4x16 = -1, -2, -3
 BOX_EDGES = {BOX_8x8x16: (8, 8, 16), BOX_4x8x16: (4, 8, 16), BOX_4x4x16: (4, 4, 16)}
-PRIM_F, PRIM_JVP, PRIM_D2, PRIM_PREC, PRIM_SMOOTH, PRIM_MGS = 0, 1, 2, 3, 4, 5
+PRIM_F, PRIM_JVP, PRIM_D2, PRIM_PREC, PRIM_SMOOTH, PRIM_MGS, PRIM_COPY = 0, 1, 2, 3, 4, 5, 6
 LOSS_CORRECTION, LOSS_RESIDUAL = 0, 1
 
 

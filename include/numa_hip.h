@@ -622,7 +622,8 @@ enum {
     NUMA_PRIM_D2 = 2,     /* F''[v,v]                                        */
     NUMA_PRIM_PREC = 3,   /* one preconditioner application (V-cycle)        */
     NUMA_PRIM_SMOOTH = 4, /* one finest-level smoothing step                 */
-    NUMA_PRIM_MGS = 5     /* one fused MGS pass (w -= h V_i; dot(w, V_i+1))  */
+    NUMA_PRIM_MGS = 5,    /* one fused MGS pass (w -= h V_i; dot(w, V_i+1))  */
+    NUMA_PRIM_COPY = 6    /* FP64 vector copy (the measured HBM peak)        */
 };
 int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
                             double* ms_per_call, double* bytes_per_call);
