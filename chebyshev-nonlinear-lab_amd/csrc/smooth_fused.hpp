@@ -119,6 +119,27 @@ __device__ __forceinline__ void buf_store(T v, __amdgpu_buffer_rsrc_t r, uint32_
             __builtin_bit_cast(__attribute__((ext_vector_type(2))) uint32_t, v), r, (int)off, 0, 0);
 }
 
+// The value of the lane one to the left (wave_shr:1) or right (wave_shl:1)
+// in the wave, 0 for lane 0 / lane 63: a DPP move, no LDS access.  The wave
+// spans the region's 64 k columns, so these are the k - 1 / k + 1 neighbours,
+// and the region edge reads 0 as the LDS ring's zero border does.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL, class A>
+__device__ __forceinline__ A lane_shift(A v) {
+    if constexpr (sizeof(A) == 4) {
+        return __builtin_bit_cast(A, dpp_u32<CTRL>(__builtin_bit_cast(uint32_t, v)));
+    } else {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = dpp_u32<CTRL>((uint32_t)u), hi = dpp_u32<CTRL>((uint32_t)(u >> 32));
+        return __builtin_bit_cast(A, lo | (hi << 32));
+    }
+}
+constexpr int kDppWaveShr1 = 0x138;  // lane l reads lane l - 1
+constexpr int kDppWaveShl1 = 0x130;  // lane l reads lane l + 1
+
 // Waves per SIMD the register allocation must allow: 8 (two 16-wave
 // workgroups per CU, at most 64 VGPRs) for the variants whose rings fit that
 // without spilling, else no bound.
@@ -263,8 +284,8 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
                 const A ip = static_cast<A>(xs[s][r][(tp + 1) % 3]);
                 const A jm = static_cast<A>(ring[s][tp & 1][jr][lane + 1]);
                 const A jp = static_cast<A>(ring[s][tp & 1][jr + 2][lane + 1]);
-                const A km = static_cast<A>(ring[s][tp & 1][jr + 1][lane]);
-                const A kp = static_cast<A>(ring[s][tp & 1][jr + 1][lane + 2]);
+                const A km = lane_shift<kDppWaveShr1>(c);
+                const A kp = lane_shift<kDppWaveShl1>(c);
                 // h = 0 outside the domain (columns: h_row; planes: p_in), where
                 // b = 0 and x = 0: then res = 0, d = 0 and x stays exactly 0
                 const A h = p_in ? h_row[r] : A(0);
