@@ -51,12 +51,33 @@ __host__ __device__ constexpr int64_t mid_b(int l) { return mid_w(l) + kMidW; }
 constexpr int64_t kW5 = kMid0 + 4 * kMidStride, kB5 = kW5 + kC * kTaps;
 constexpr int64_t kParams = kB5 + 1;                                // 63625
 
-__device__ inline float gelu_f(float x) {
-    return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+// GELU(x) = x Phi(x) with the normal distribution function
+//   Phi(x) = 1 - erfc(z)/2 (x >= 0),  erfc(z)/2 (x < 0),   z = |x| / sqrt 2,
+// and erfc(z) = (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-z^2),
+// t = 1 / (1 + p z): Abramowitz & Stegun, Handbook of Mathematical Functions,
+// formula 7.1.26 (absolute error of erf at most 1.5e-7, far below the bf16
+// rounding of the activations).  A dozen VALU operations and two
+// transcendentals (reciprocal, 2^x) per value: the epilogues apply it to 24
+// channels of every point, where a library erf dominated the layer's time.
+// Also returns e = exp(-x^2 / 2), the unnormalised density (for GELU').
+__device__ inline float norm_cdf(float x, float& e) {
+    constexpr float p = 0.3275911f, a1 = 0.254829592f, a2 = -0.284496736f,
+                    a3 = 1.421413741f, a4 = -1.453152027f, a5 = 1.061405429f;
+    const float z = fabsf(x) * 0.70710678118654752f;
+    const float t = __builtin_amdgcn_rcpf(fmaf(p, z, 1.0f));
+    const float poly = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, a5, a4), a3), a2), a1);
+    e = __builtin_amdgcn_exp2f(-(z * z) * 1.4426950408889634f);  // exp(-z^2)
+    const float half_erfc = 0.5f * poly * e;
+    return x >= 0.0f ? 1.0f - half_erfc : half_erfc;
 }
-__device__ inline float gelu_grad_f(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) +
-           x * expf(-0.5f * x * x) * 0.39894228040143268f;
+__device__ inline float gelu_f(float x) {
+    float e;
+    return x * norm_cdf(x, e);
+}
+__device__ inline float gelu_grad_f(float x) {  // Phi(x) + x phi(x)
+    float e;
+    const float c = norm_cdf(x, e);
+    return fmaf(x * 0.39894228040143268f, e, c);
 }
 
 // Halo-point offset of a tap in a (kHI, kHJ, kHK) tile, and the 16-byte chunk
@@ -68,6 +89,26 @@ __host__ __device__ constexpr int tap_off(int tap) {
 __host__ __device__ constexpr int group_off(int g) {
     return g >= kGroups ? tap_off(kTaps - 1) * 3 + 2 : tap_off(g / 3) * 3 + g % 3;
 }
+
+// Smallest group offset, and the step between the two K groups 2s, 2s + 1
+// that the two half-waves read at MFMA step s: one 16-byte chunk within a
+// tap, or across a k-neighbour tap; 94 / 808 chunks where the pair straddles
+// the end of a k row / j row of taps; 0 for the padded 82nd group.
+constexpr int group_off_min() {
+    int m = 0;
+    for (int g = 0; g <= kGroups; ++g) m = group_off(g) < m ? group_off(g) : m;
+    return m;
+}
+constexpr int kGroupOffMin = group_off_min();
+constexpr int pair_step(int s) { return group_off(2 * s + 1) - group_off(2 * s); }
+constexpr bool pair_steps_known() {
+    for (int s = 0; s < kSteps; ++s) {
+        const int d = pair_step(s);
+        if (d != 0 && d != 1 && d != 94 && d != 808) return false;
+    }
+    return true;
+}
+static_assert(pair_steps_known(), "unexpected K group layout");
 
 template <int TI = kTI>
 struct Tiles {
@@ -132,6 +173,43 @@ __device__ inline void stage_halo(const bf16x8* __restrict__ in, int n, int i0, 
     }
 }
 
+// The halo of a tile in two halves, so that its loads can be in flight while
+// the workgroup finishes the previous tile: halo_load issues every chunk load
+// of this thread (PER of them, into registers; chunks outside the domain load
+// a valid address and are flagged), halo_store writes them to LDS (zeros for
+// the flagged ones).
+template <int TI, int NT>
+struct HaloRegs {
+    static constexpr int NE = (TI + 2) * kHJ * kHK * 3;
+    static constexpr int PER = (NE + NT - 1) / NT;
+    bf16x8 v[PER];
+    uint32_t ok = 0;  // bit u: chunk u is inside the domain
+    __device__ void load(const bf16x8* __restrict__ in, int n, int i0, int j0, int k0) {
+        ok = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            // an empty asm makes the chunk index opaque, so that its (tile
+            // invariant) decomposition is recomputed here rather than hoisted
+            // out of the tile loop and held in registers through the MFMAs
+            int e = (int)threadIdx.x + u * NT;
+            asm volatile("" : "+v"(e));
+            const int pt = e / 3, c = e - 3 * (e / 3);
+            const int hk = pt % kHK, hj = (pt / kHK) % kHJ, hi = pt / (kHK * kHJ);
+            const int gi = i0 + hi - 1, gj = j0 + hj - 1, gk = k0 + hk - 1;
+            const bool in_d = e < NE && inside(gi, gj, gk, n);
+            ok |= in_d ? (1u << u) : 0u;
+            v[u] = in[in_d ? (((int64_t)gi * n + gj) * n + gk) * 3 + c : 0];
+        }
+    }
+    __device__ void store(bf16x8* lds) const {
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int e = (int)threadIdx.x + u * NT;
+            if (e < NE) lds[e] = (ok >> u) & 1u ? v[u] : bf16x8{};
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // 24 -> 24 layer on MFMA.  D[point][co] = sum_k A[point][k] B[k][co] with
 // k = (group g = 3*tap + channel group, channel within group), 41 k-steps.
@@ -174,27 +252,43 @@ __global__ __launch_bounds__(kConvThreads) void k_conv24(int n, int nsamples,
     const int co = row;
     const float bco = (EPI != CE_BWD && co < kC) ? bias[co] : 0.0f;
     const int64_t N3 = (int64_t)n * n * n;
-    for (int64_t t = worker_id(); t < T.total; t += gridDim.x) {
+    // the halo of the next tile is loaded while this one finishes (its
+    // epilogue), and written to LDS at the top of the next iteration
+    HaloRegs<kCTI, kConvThreads> halo;
+    int64_t t = worker_id();
+    if (t < T.total) {
         int b, i0, j0, k0;
         T.decode(t, b, i0, j0, k0);
-        __syncthreads();
-        stage_halo<kCTI, kConvThreads, 6>(in + (size_t)b * N3 * 3, n, i0, j0, k0, lds);
+        halo.load(in + (size_t)b * N3 * 3, n, i0, j0, k0);
+    }
+    for (; t < T.total; t += gridDim.x) {
+        int b, i0, j0, k0;
+        T.decode(t, b, i0, j0, k0);
+        __syncthreads();  // the previous tile's staging reads are done
+        halo.store(lds);
         __syncthreads();
         floatx16 acc[4] = {};
-        int cb[4];
+        // A operand of step s: chunk cb[q] + group_off(2 s + half) = base[q] +
+        // hstep + a compile-time offset (folded into the LDS instruction), so
+        // that no per-step address is held in a register
+        int base[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int L = wave * 4 + q;
-            cb[q] = (((L / kTJ + 1) * kHJ + (L % kTJ + 1)) * kHK + row + 1) * 3;
+            base[q] = (((L / kTJ + 1) * kHJ + (L % kTJ + 1)) * kHK + row + 1) * 3 + kGroupOffMin;
         }
-#pragma unroll 2
-        for (int s = 0; s < kSteps; ++s) {
-            const int off = group_off(2 * s + half);
+        const int h1 = half, h94 = half * 94, h808 = half * 808;
+        static_for<kSteps>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            constexpr int o = group_off(2 * s) - kGroupOffMin;
+            constexpr int d = pair_step(s);
+            const int hs = d == 0 ? 0 : d == 1 ? h1 : d == 94 ? h94 : h808;
             const bf16x8 wv = wl[s * 64 + lane];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds[cb[q] + off], wv, acc[q], 0, 0, 0);
-        }
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds[base[q] + hs + o], wv, acc[q],
+                                                                 0, 0, 0);
+        });
         __syncthreads();  // halo reads done; the space now holds the staging
         if (co < kC) {
 #pragma unroll
@@ -206,6 +300,13 @@ __global__ __launch_bounds__(kConvThreads) void k_conv24(int n, int nsamples,
                 }
         }
         __syncthreads();
+        // the accumulators are dead: issue the next tile's halo loads now,
+        // they complete during this tile's epilogue
+        if (t + gridDim.x < T.total) {  // uniform
+            int b2, i2, j2, k2;
+            T.decode(t + gridDim.x, b2, i2, j2, k2);
+            halo.load(in + (size_t)b2 * N3 * 3, n, i2, j2, k2);
+        }
         for (int e = lane; e < 4 * kTK * 3; e += 64) {
             const int q = e / (kTK * 3), pt = (e / 3) % kTK, cg = e % 3;
             const int L = wave * 4 + q;
