@@ -331,6 +331,7 @@ def chandra_leg(H, ctx, args) -> dict:
 
 
 LEARN_LAMBDAS = (5.0, 6.0, 6.5, 6.7)
+DEFAULT_WEIGHTS = os.path.join(PKG, "weights", "convnet_200k_n64-128")
 CONV_FLOP_PER_POINT = 2 * 27 * (1 * 24 + 4 * 24 * 24 + 24 * 1)  # 127,008
 
 
@@ -347,17 +348,29 @@ def learned_leg(H, ctx, args, params) -> dict:
     import numpy as np
     n = args.n
     net = H.ConvNet(ctx, seed=21)
-    sizes = tuple(int(v) for v in args.learned_sizes.split(","))
-    tcfg = H.net_train_config(pairs=args.learned_pairs, batch=4, sizes=sizes, lr=2e-3,
-                              seed=5, mg=H.mg_cheb(levels=3), loss=H.LOSS_RESIDUAL)
-    st, hist = net.train_solver(tcfg, hist_len=1000000)
-    k = max(1, len(hist) // 10)
-    out = {"train": {"pairs": int(st.pairs), "steps": int(st.steps), "solves": int(st.solves),
-                     "seconds": round(st.seconds, 3), "sizes": list(sizes),
-                     "loss": "residual", "loss_first_window": round(float(hist[:k].mean()), 4),
-                     "loss_last_window": round(float(hist[-k:].mean()), 4)}}
-    log(f"learned: trained on {st.pairs} pairs in {st.seconds:.2f} s, loss "
-        f"{out['train']['loss_first_window']} -> {out['train']['loss_last_window']}")
+    wpath = args.learned_weights
+    if wpath and os.path.exists(wpath + ".npy") and not args.learned_retrain:
+        # parameters trained on the GPU by train_convnet.py at the full budget
+        net.set_params(np.load(wpath + ".npy"))
+        rec = json.load(open(wpath + ".json")) if os.path.exists(wpath + ".json") else {}
+        out = {"train": {k: rec.get(k) for k in ("pairs", "steps", "solves", "seconds", "sizes",
+                                                 "batch", "lr", "loss", "loss_first_window",
+                                                 "loss_last_window")}}
+        out["train"]["source"] = os.path.relpath(wpath, ROOT) + ".npy (train_convnet.py)"
+        log(f"learned: parameters from {wpath}.npy ({rec.get('pairs')} pairs)")
+    else:
+        sizes = tuple(int(v) for v in args.learned_sizes.split(","))
+        tcfg = H.net_train_config(pairs=args.learned_pairs, batch=4, sizes=sizes, lr=2e-3,
+                                  seed=5, mg=H.mg_cheb(levels=3), loss=H.LOSS_RESIDUAL)
+        st, hist = net.train_solver(tcfg, hist_len=1000000)
+        k = max(1, len(hist) // 10)
+        out = {"train": {"pairs": int(st.pairs), "steps": int(st.steps), "solves": int(st.solves),
+                         "seconds": round(st.seconds, 3), "sizes": list(sizes),
+                         "loss": "residual", "loss_first_window": round(float(hist[:k].mean()), 4),
+                         "loss_last_window": round(float(hist[-k:].mean()), 4),
+                         "source": "trained in this run"}}
+        log(f"learned: trained on {st.pairs} pairs in {st.seconds:.2f} s, loss "
+            f"{out['train']['loss_first_window']} -> {out['train']['loss_last_window']}")
     if args.learned_save:
         np.save(args.learned_save, net.get_params())
     params = make_params(H, args.forcing, safeguard=True)
@@ -533,6 +546,10 @@ def main() -> int:
     ap.add_argument("--learned-sizes", default="16,24,32",
                     help="training grid sizes (the full budget: 64,96,128 with 200000 pairs)")
     ap.add_argument("--learned-save", default="", help="write the trained parameters (.npy)")
+    ap.add_argument("--learned-weights", default=DEFAULT_WEIGHTS,
+                    help="parameters (path without .npy) to use instead of training")
+    ap.add_argument("--learned-retrain", action="store_true",
+                    help="train with --learned-pairs / --learned-sizes even if weights exist")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--smooth-rows", type=int, default=None,
                     help="pin the fused legs' smoothing to one launch per sweep with this "

@@ -379,8 +379,9 @@ void ConvNet::train_from_solver(const numa_net_train_config& cfg, double* hist, 
         const int64_t N3 = ncur * ncur * ncur;
         double* slot = histd.as<double>() + std::min<int64_t>(st.steps, hslots - 1);
         loss_grad(ncur, (int)fill, rb.as<float>(), zb.as<float>(), true, cfg.loss, slot);
-        // step size decays linearly to lr/10 over the pairs
-        adam(cfg.lr * (1.0 - 0.9 * (double)(st.pairs - fill) / (double)cfg.pairs));
+        // step size decays linearly to lr_end (lr/10 if 0) over the pairs
+        const double lr_end = cfg.lr_end > 0.0 ? cfg.lr_end : 0.1 * cfg.lr;
+        adam(cfg.lr + (lr_end - cfg.lr) * (double)(st.pairs - fill) / (double)cfg.pairs);
         (void)N3;
         ++st.steps;
         fill = 0;

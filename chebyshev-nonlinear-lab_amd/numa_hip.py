@@ -150,7 +150,7 @@ class NetTrainConfig(C.Structure):
     _fields_ = [("pairs", _i64), ("batch", _i64), ("sizes", _i64 * 4), ("lambda_lo", C.c_double),
                 ("lambda_hi", C.c_double), ("noise", C.c_double), ("lr", C.c_double),
                 ("seed", _i64), ("gmres_restart", _i64), ("mg", MGConfig),
-                ("solve", KrylovParams), ("loss", _i64)]
+                ("solve", KrylovParams), ("loss", _i64), ("lr_end", C.c_double)]
 
 
 class NetTrainStats(C.Structure):
@@ -804,14 +804,15 @@ class ChandrasekharBatch:
 def net_train_config(pairs: int = 2000, batch: int = 4, sizes=(32, 48, 64), lambda_lo: float = 1.0,
                      lambda_hi: float = 6.7, noise: float = 0.1, lr: float = 1e-3, seed: int = 7,
                      gmres_restart: int = 30, mg: MGConfig | None = None,
-                     solve: KrylovParams | None = None, loss: int = LOSS_RESIDUAL) -> NetTrainConfig:
+                     solve: KrylovParams | None = None, loss: int = LOSS_RESIDUAL,
+                     lr_end: float = 0.0) -> NetTrainConfig:
     sz = (_i64 * 4)(*([int(v) for v in sizes] + [0] * (4 - len(sizes))))
     return NetTrainConfig(pairs=pairs, batch=batch, sizes=sz, lambda_lo=lambda_lo,
                           lambda_hi=lambda_hi, noise=noise, lr=lr, seed=seed,
                           gmres_restart=gmres_restart,
                           mg=mg if mg is not None else mg_cheb(levels=5),
                           solve=solve if solve is not None else krylov_params(gmres_restart=30),
-                          loss=loss)
+                          loss=loss, lr_end=lr_end)
 
 
 class ConvNet:

@@ -478,8 +478,9 @@ typedef struct numa_net_train_config {
     int64_t gmres_restart;    /* FGMRES restart of the generating solves         */
     numa_mg_config mg;        /* V-cycle whose corrections are the targets       */
     numa_krylov_params solve; /* Newton parameters of the generating solves      */
-    int64_t loss;             /* NUMA_NET_LOSS_*; the step size decays linearly
-                                 from lr to lr/10 over the pairs                 */
+    int64_t loss;             /* NUMA_NET_LOSS_*                                  */
+    double lr_end;            /* the step size decays linearly from lr to lr_end
+                                 over the pairs (0: lr/10)                       */
 } numa_net_train_config;
 
 typedef struct numa_net_train_stats {

@@ -29,7 +29,7 @@ def test_pod_layouts_are_8_byte_fields():
     assert ctypes.sizeof(H.GmresResult) == 6 * 8
     assert ctypes.sizeof(H.MGConfig) == 13 * 8
     assert ctypes.sizeof(H.SolverConfig) == 19 * 8
-    assert ctypes.sizeof(H.NetTrainConfig) == 42 * 8
+    assert ctypes.sizeof(H.NetTrainConfig) == 43 * 8
     assert ctypes.sizeof(H.NetTrainStats) == 6 * 8
 
 
