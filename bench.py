@@ -331,16 +331,18 @@ def chandra_leg(H, ctx, args) -> dict:
 
 
 LEARN_LAMBDAS = (5.0, 6.0, 6.5, 6.7)
-DEFAULT_WEIGHTS = os.path.join(PKG, "weights", "convnet_200k_n64-128")
+DEFAULT_WEIGHTS = os.path.join(PKG, "weights", "convnet_200k_n16-32")
 CONV_FLOP_PER_POINT = 2 * 27 * (1 * 24 + 4 * 24 * 24 + 24 * 1)  # 127,008
 
 
 def learned_leg(H, ctx, args, params) -> dict:
     """Learned preconditioner.  The conv net is trained on the GPU on
     (residual, V(2,2) correction) pairs generated by mixed Newton-FGMRES solves
-    (lambda ~ U[1, 6.7], residual-norm loss; --learned-sizes, default 16^3-32^3
-    with --learned-pairs pairs, or 64^3-128^3 with 200k pairs for the full
-    training budget), then Newton-FGMRES(30) at n^3 for lambda in {5.0, 6.0,
+    (lambda ~ U[1, 6.7], residual-norm loss).  By default the parameters are
+    the ones train_convnet.py wrote for 200k pairs on 16^3-32^3 grids
+    (--learned-weights; the 64^3-128^3 grids at the same budget train a worse
+    network); --learned-retrain trains here with --learned-pairs pairs on
+    --learned-sizes grids instead.  Then Newton-FGMRES(30) at n^3 for lambda in {5.0, 6.0,
     6.5, 6.7} runs with the network and the V-cycle fallback (a step reducing
     the residual by less than 0.5 switches the rest of that linear solve to the
     V-cycle) next to the V-cycle alone, both with the final-solve safeguard so
@@ -544,7 +546,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-pairs", type=int, default=4000)
     ap.add_argument("--learned-sizes", default="16,24,32",
-                    help="training grid sizes (the full budget: 64,96,128 with 200000 pairs)")
+                    help="training grid sizes for --learned-retrain")
     ap.add_argument("--learned-save", default="", help="write the trained parameters (.npy)")
     ap.add_argument("--learned-weights", default=DEFAULT_WEIGHTS,
                     help="parameters (path without .npy) to use instead of training")

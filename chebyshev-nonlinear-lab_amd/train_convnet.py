@@ -12,7 +12,12 @@ between chunks; the step size decays linearly from --lr to --lr/10 over the
 whole budget.
 
     python3 chebyshev-nonlinear-lab_amd/train_convnet.py --pairs 200000 \\
-        --sizes 64,96,128 --out chebyshev-nonlinear-lab_amd/weights/convnet_200k_n64-128
+        --sizes 16,24,32 --out chebyshev-nonlinear-lab_amd/weights/convnet_200k_n16-32
+
+Both committed parameter sets were written this way: 200k pairs on 16^3-32^3
+grids (86 s on one MI355X; the default of the benchmark) and on 64^3-128^3
+grids (596 s; a worse preconditioner: its 13^3 receptive field sees a smaller
+part of the smooth error at those sizes).
 """
 from __future__ import annotations
 

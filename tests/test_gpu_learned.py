@@ -185,3 +185,33 @@ def test_trained_network_gains(ctx, trained, rhs):
     if not out["trained"] <= out["gmg"]:
         pytest.xfail(f"network alone loses to the V-cycle: {out['trained']:.3g} vs "
                      f"{out['gmg']:.3g} after {steps} FGMRES steps (ratio {ratio:.3g})")
+
+
+def test_committed_weights_gain(ctx):
+    """The committed parameters (200k pairs on 16^3-32^3 grids) load through
+    set_params and, as the preconditioner alone, leave a smaller FGMRES
+    residual after 8 steps on J(0) z = b (random b, 64^3) than no
+    preconditioner (measured 0.0068 vs 0.038) and than the 64^3-128^3 set
+    trained at the same budget (0.149)."""
+    import os
+    wdir = os.path.join(os.path.dirname(nh.__file__), "weights")
+    n, lam, steps = 64, 6.0, 8
+    rng = np.random.default_rng(31)
+    b = ctx.array(rng.normal(size=n ** 3))
+    x0 = ctx.array(np.zeros(n ** 3))
+    z = ctx.empty(n ** 3)
+    out = {}
+    for name in ("convnet_200k_n16-32", "convnet_200k_n64-128"):
+        net = nh.ConvNet(ctx, seed=0)
+        net.set_params(np.load(os.path.join(wdir, name + ".npy")))
+        s = _solver(ctx, n, lam, nh.PRECOND_LEARNED)
+        s.set_net(net, 1.0)
+        out[name] = s.gmres(x0, b, z, 1e-14, 30, steps).rel_res
+        s.close()
+        net.close()
+    s = _solver(ctx, n, lam, nh.PRECOND_NONE)
+    out["none"] = s.gmres(x0, b, z, 1e-14, 30, steps).rel_res
+    s.close()
+    print("rel_res after", steps, "FGMRES steps at 64^3:", out)
+    assert out["convnet_200k_n16-32"] < 0.5 * out["none"], out
+    assert out["convnet_200k_n16-32"] < out["convnet_200k_n64-128"], out

@@ -175,3 +175,24 @@ def test_device_gelu_formula():
     grad = phi + x * e / math.sqrt(2.0 * math.pi)
     assert np.max(np.abs(grad - cr.gelu_grad(x))) < 1.5e-7
     assert np.max(np.abs(x * phi - cr.gelu(x))) < 1.5e-6
+
+
+@pytest.mark.parametrize("name", ["convnet_200k_n16-32", "convnet_200k_n64-128"])
+def test_committed_weights_load(name):
+    """The committed parameter sets are plain FP32 arrays in the flat layout
+    (63,625 values, read without unpickling), their training record names the
+    200k-pair budget, and the oracle's forward pass over them is finite."""
+    import json
+    import os
+    import numa_hip as nh
+    path = os.path.join(os.path.dirname(nh.__file__), "weights", name)
+    w = np.load(path + ".npy")
+    assert w.dtype == np.float32 and w.shape == (63625,)
+    assert np.all(np.isfinite(w)) and np.abs(w).max() < 100.0
+    rec = json.load(open(path + ".json"))
+    assert rec["pairs"] == 200000 and rec["loss"] == "residual"
+    assert rec["loss_last_window"] < rec["loss_first_window"]
+    Ws, bs = cr.unflatten(w.astype(np.float64))
+    x = np.random.default_rng(3).normal(size=(8, 8, 8))
+    y = cr.forward(x, Ws, bs)[0]
+    assert y.shape == x.shape and np.all(np.isfinite(y)) and np.abs(y).max() > 0.0
