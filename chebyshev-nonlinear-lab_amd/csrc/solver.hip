@@ -10,10 +10,11 @@
 // (V_i = s_i * Vraw_i), which removes the normalisation pass.
 //
 // Precision: T = double (FP64) or float (mixed).  In mixed mode the Krylov
-// basis, Z vectors, J v output and the V-cycle are FP32; F(x), the difference
-// quotient's subtraction, the Newton / Chebyshev update, the linear iterate,
-// the Givens / least-squares solve and all dot products (FP64 accumulation)
-// stay FP64.
+// basis, Z vectors, J v output and the V-cycle are FP32; F(x), the Newton /
+// Chebyshev update, the linear iterate, the Givens / least-squares solve and
+// all dot products (FP64 accumulation) stay FP64.  The mixed FD J v evaluates
+// the difference quotient in the rearranged form h2inv A v - lambda e^x
+// (e^{h v} - 1) / h in FP64 arithmetic from v and a FP32 copy of lambda e^x.
 #include <algorithm>
 #include <chrono>
 #include <map>
@@ -51,6 +52,34 @@ inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, doub
     return tight_cap(p, forcing_ew(p, k, rnorm, rprev, eta_prev, res0), rnorm, res0);
 }
 
+// ||x||^2 (one partial per block) and E = lambda e^x in FP32, one pass over
+// x: the per-Newton-point data of the mixed-precision J v (EpJvFDe).
+// lambda from *lam_ptr when set.
+template <bool VEC>
+__global__ __launch_bounds__(kBlock) void k_xnorm_exp(int64_t N, const double* x, double lambda,
+                                                      const double* lam_ptr, float* E, Reduce red) {
+    const double lam = lam_ptr ? *lam_ptr : lambda;
+    double acc = 0.0;
+    for_each_pair(
+        N,
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<double> x0{}, x1{};
+            if (ok0) x0 = ldp<VEC>(x, q0);
+            if (ok1) x1 = ldp<VEC>(x, q1);
+            acc += x0.a * x0.a;
+            acc += x0.b * x0.b;
+            acc += x1.a * x1.a;
+            acc += x1.b * x1.b;
+            if (ok0) stp<true>(E, q0, Pair<float>{(float)(lam * exp(x0.a)), (float)(lam * exp(x0.b))});
+            if (ok1) stp<true>(E, q1, Pair<float>{(float)(lam * exp(x1.a)), (float)(lam * exp(x1.b))});
+        },
+        [&](int64_t i) {
+            acc += x[i] * x[i];
+            E[i] = (float)(lam * exp(x[i]));
+        });
+    block_partial<kBlock>(acc, red, blockIdx.x);
+}
+
 template <class T>
 class BratuSolver : public SolverBase {
 public:
@@ -76,6 +105,12 @@ public:
         w_ = allocd();
         r_ = allocd();
         if (cfg.precond == NUMA_PRECOND_JACOBI) diag_ = allocd();
+        if (sizeof(T) == 4 && cfg.jv_mode == NUMA_DERIV_FD) {  // lambda e^x of EpJvFDe
+            void* pe = nullptr;
+            NUMA_HIP_CHECK(hipMalloc(&pe, (size_t)N_ * sizeof(float)));
+            allocs_.push_back(pe);
+            E_ = static_cast<float*>(pe);
+        }
         if (cfg.method == NUMA_METHOD_TF2018) {
             tfa_ = allocd();
             tfb_ = allocd();
@@ -302,6 +337,15 @@ public:
                     eval_F(x2_, nullptr, 0.0, nullptr, 0.0, Fn_, nullptr);
                     return 16.0 * N;
                 case NUMA_PRIM_JVP: {  // stencil kernel only (||v|| precomputed)
+                    if (cfg_.jv_mode == NUMA_DERIV_FD && E_) {
+                        const FDStep stp{std::sqrt(std::numeric_limits<double>::epsilon()),
+                                         MISC(S_XNORM), MISC(S_VNORM)};
+                        launch_stencil<true>(c_, n_, LdVec<T>{zb_},
+                                             EpJvFDe<T, true, false>{h2inv_, stp, S(0), E_, V_[1],
+                                                                     V_[0], nullptr, 0.0, 1.0},
+                                             Epilogue{EPI_STORE, 1.0, S(0), H(0, 0), nullptr});
+                        return N * (4.0 + 3.0 * st);  // lambda e^x, v, out, V0
+                    }
                     if (cfg_.jv_mode == NUMA_DERIV_FD) {
                         const FDStep stp{std::sqrt(std::numeric_limits<double>::epsilon()),
                                          MISC(S_XNORM), MISC(S_VNORM)};
@@ -427,7 +471,19 @@ private:
     // Everything that depends only on the Newton point x: ||x|| for the FD
     // step, the Jacobi diagonal (BratuJacobi::make, bratu3d.hpp:251-255).
     void point_setup(const double* x) {
-        if (cfg_.jv_mode == NUMA_DERIV_FD || cfg_.d2_mode == NUMA_DERIV_FD) norm(x, MISC(S_XNORM));
+        if (E_) {  // ||x|| and lambda e^x in one pass
+            const unsigned nb = stream_blocks(N_);
+            if (aligned2(x, sizeof(double)))
+                hipLaunchKernelGGL(k_xnorm_exp<true>, dim3(nb), dim3(kBlock), 0, c_.stream, N_, x,
+                                   lambda_, lam_dev(), E_, c_.red);
+            else
+                hipLaunchKernelGGL(k_xnorm_exp<false>, dim3(nb), dim3(kBlock), 0, c_.stream, N_,
+                                   x, lambda_, lam_dev(), E_, c_.red);
+            NUMA_HIP_LAUNCH_CHECK();
+            c_.finish(nb, Epilogue{EPI_SQRT, 1.0, nullptr, MISC(S_XNORM), nullptr});
+        } else if (cfg_.jv_mode == NUMA_DERIV_FD || cfg_.d2_mode == NUMA_DERIV_FD) {
+            norm(x, MISC(S_XNORM));
+        }
         if (cfg_.precond == NUMA_PRECOND_JACOBI) {
             hipLaunchKernelGGL(k_jacobi_diag, dim3(stream_blocks(N_)), dim3(kBlock), 0, c_.stream,
                                N_, h2inv_, lambda_, lam_dev(), x, diag_);
@@ -443,7 +499,28 @@ private:
     // preconditioner that produced v).
     void jv(const double* x, const double* F0, const TV* v, T* out, const double* scale,
             const T* dotv, const double* b, const Epilogue& ep, bool vnorm_given = false) {
-        if (cfg_.jv_mode == NUMA_DERIV_FD) {
+        if (cfg_.jv_mode == NUMA_DERIV_FD && E_) {
+            // mixed precision: the rearranged quotient over v and lambda e^x
+            if (!vnorm_given) norm(v, MISC(S_VNORM));
+            const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), MISC(S_XNORM),
+                            MISC(S_VNORM)};
+            const LdVec<TV> L{v};
+            if (dotv)
+                launch_stencil<true>(c_, n_, L,
+                                     EpJvFDe<T, true, false>{h2inv_, st, scale, E_, out, dotv,
+                                                             nullptr, 0.0, 1.0},
+                                     ep);
+            else if (b)
+                launch_stencil<true>(c_, n_, L,
+                                     EpJvFDe<T, false, true>{h2inv_, st, scale, E_, out, nullptr,
+                                                             b, 0.0, 1.0},
+                                     ep);
+            else
+                launch_stencil<false>(c_, n_, L,
+                                      EpJvFDe<T, false, false>{h2inv_, st, scale, E_, out, nullptr,
+                                                               nullptr, 0.0, 1.0},
+                                      ep);
+        } else if (cfg_.jv_mode == NUMA_DERIV_FD) {
             if (!vnorm_given) norm(v, MISC(S_VNORM));
             const FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), MISC(S_XNORM),
                             MISC(S_VNORM)};
@@ -716,6 +793,7 @@ private:
     double *x2_ = nullptr, *Fx_ = nullptr, *Fn_ = nullptr, *z_ = nullptr, *u_ = nullptr,
            *w_ = nullptr, *r_ = nullptr, *diag_ = nullptr;
     double *tfa_ = nullptr, *tfb_ = nullptr;  // J v products of the TF-2018 curvature
+    float* E_ = nullptr;  // lambda e^x at the Newton point (mixed FD J v)
     std::vector<T*> V_, Z_;
     T* zb_ = nullptr;
     T* comb_ = nullptr;
