@@ -389,6 +389,45 @@ void launch_convert(Ctx& c, int64_t N, const TI* x, TO* y) {
     NUMA_HIP_LAUNCH_CHECK();
 }
 
+// y = x (converted) and the partials of sum x_i^2 in one pass; the partials
+// are summed in the order of k_dot(x, x), so the epilogue sees the same total.
+template <bool VEC, class TI, class TO>
+__global__ __launch_bounds__(kBlock) void k_convert_norm(int64_t N, const TI* x, TO* y,
+                                                         Reduce red) {
+    double acc = 0.0;
+    for_each_pair(
+        N,
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<TI> a0{}, a1{};
+            if (ok0) a0 = ldp<VEC>(x, q0);
+            if (ok1) a1 = ldp<VEC>(x, q1);
+            acc += (double)a0.a * (double)a0.a;
+            acc += (double)a0.b * (double)a0.b;
+            acc += (double)a1.a * (double)a1.a;
+            acc += (double)a1.b * (double)a1.b;
+            if (ok0) stp<VEC>(y, q0, Pair<TO>{static_cast<TO>(a0.a), static_cast<TO>(a0.b)});
+            if (ok1) stp<VEC>(y, q1, Pair<TO>{static_cast<TO>(a1.a), static_cast<TO>(a1.b)});
+        },
+        [&](int64_t i) {
+            acc += dv(x, i) * dv(x, i);
+            y[i] = static_cast<TO>(x[i]);
+        });
+    block_partial<kBlock>(acc, red, blockIdx.x);
+}
+
+template <class TI, class TO>
+void launch_convert_norm(Ctx& c, int64_t N, const TI* x, TO* y, const Epilogue& ep) {
+    const unsigned nb = stream_blocks(N);
+    if (aligned2(x, sizeof(TI)) && aligned2(y, sizeof(TO)))
+        hipLaunchKernelGGL((k_convert_norm<true, TI, TO>), dim3(nb), dim3(kBlock), 0, c.stream, N,
+                           x, y, c.red);
+    else
+        hipLaunchKernelGGL((k_convert_norm<false, TI, TO>), dim3(nb), dim3(kBlock), 0, c.stream,
+                           N, x, y, c.red);
+    NUMA_HIP_LAUNCH_CHECK();
+    c.finish(nb, ep);
+}
+
 inline void launch_axpy_to(Ctx& c, int64_t N, const double* x, double a, const double* d,
                            double* out) {
     hipLaunchKernelGGL(k_axpy_to, dim3(stream_blocks(N)), dim3(kBlock), 0, c.stream, N, x, a, d,

@@ -815,7 +815,15 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
                                         bool save) {
     numa_gmres_result res{};
     if (save) saved_k_ = 0;
-    norm(b, MISC(S_BNORM));
+    // From a zero guess the first residual is b itself: ||b||, s_0 = 1/||b||
+    // and V_0 = b come from one pass over b, and x_l is written (not
+    // accumulated) by the first cycle's update instead of being zeroed first.
+    const bool b_is_r0 = zero_guess && r_init == nullptr && restart > 0 && max_iters > 0;
+    if (b_is_r0)
+        launch_convert_norm<double, T>(c_, N_, b, V_[0],
+                                       Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BNORM), S(0)});
+    else
+        norm(b, MISC(S_BNORM));
     const double bnorm = read1(MISC(S_BNORM));
     if (bnorm == 0.0) {
         launch_fill<double>(c_, N_, 0.0, xl);
@@ -832,7 +840,11 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
         res.rel_res = 1.0;
         return res;
     }
-    if (zero_guess) launch_fill<double>(c_, N_, 0.0, xl);
+    bool xl_unset = zero_guess;  // x_l = 0 not yet stored (see above)
+    auto settle_xl = [&]() {
+        if (xl_unset) launch_fill<double>(c_, N_, 0.0, xl);
+        xl_unset = false;
+    };
     use_net_ = net_ != nullptr;
 
     const bool flex = cfg_.flexible != 0;
@@ -853,18 +865,25 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             norm(r_init, MISC(S_BETA), S(0));
             use_given = false;
             first = false;
+            beta = read1(MISC(S_BETA));
         } else if (first) {
-            launch_convert<double, T>(c_, N_, b, V_[0]);
-            norm(b, MISC(S_BETA), S(0));
+            if (b_is_r0) {
+                beta = bnorm;  // V_0 and s_0 are in place
+            } else {
+                launch_convert<double, T>(c_, N_, b, V_[0]);
+                norm(b, MISC(S_BETA), S(0));
+                beta = read1(MISC(S_BETA));
+            }
             first = false;
         } else {
+            settle_xl();
             timed(T_JVP, [&]() {
                 jv<double>(x0, F0, xl, V_[0], nullptr, nullptr, b,
                            Epilogue{EPI_NORM_SCALE, 1.0, nullptr, MISC(S_BETA), S(0)});
             });
             ++res.matvecs;
+            beta = read1(MISC(S_BETA));
         }
-        beta = read1(MISC(S_BETA));
         res.rel_res = beta / bnorm;
         if (res.rel_res <= rtol) {
             res.converged = 1;
@@ -914,6 +933,7 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             const double denom = std::hypot(Hh[j][j], Hh[j + 1][j]);
             if (!(denom > 0.0) || !std::isfinite(denom)) {
                 res.breakdown = 1;
+                settle_xl();
                 return res;
             }
             cs[j] = Hh[j][j] / denom;
@@ -937,6 +957,7 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
             const double pivot = Hh[ii][ii];
             if (!(std::abs(pivot) > 0.0) || !std::isfinite(pivot)) {
                 res.breakdown = 1;
+                settle_xl();
                 return res;
             }
             double acc = g[ii];
@@ -948,20 +969,25 @@ numa_gmres_result BratuSolver<T>::gmres(const double* x0, const double* F0, cons
         for (size_t i = 0; i < k; ++i) coef[i] = y[i] * s[i];
         if (flex) {
             for (size_t i = 0; i < k; ++i) vecs[i] = Z_.empty() ? V_[i] : Z_[i];
-            launch_combine<T, double>(c_, N_, vecs, coef, xl, true);
+            launch_combine<T, double>(c_, N_, vecs, coef, xl, !xl_unset);  // 0 + sum: same bits
         } else {
             for (size_t i = 0; i < k; ++i) vecs[i] = V_[i];
             launch_combine<T, T>(c_, N_, vecs, coef, comb_, false);
             const T* zq = nullptr;
             timed(T_PREC, [&]() { zq = precond(comb_, zb_); });
             ++res.precs;
-            launch_axpy<T, double>(c_, N_, 1.0, zq, xl);
+            if (xl_unset)
+                launch_convert<T, double>(c_, N_, zq, xl);  // 0 + 1 * zq
+            else
+                launch_axpy<T, double>(c_, N_, 1.0, zq, xl);
         }
+        xl_unset = false;
         if (res.rel_res <= rtol) {
             res.converged = 1;
             break;
         }
     }
+    settle_xl();
     if (save && k_last > 0) {
         saved_k_ = (int64_t)k_last;
         saved_H_.assign(k_last + 1, std::vector<double>(k_last, 0.0));
