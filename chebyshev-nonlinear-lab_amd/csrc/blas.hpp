@@ -47,16 +47,21 @@ __device__ inline void stp(T* p, int64_t q, const Pair<T>& v) {
 }
 
 // Visits the pairs of [0, N): f2(q0, q1, ok0, ok1) with q1 = q0 + 256, then
-// the odd last element (if any) with f1(N-1) on one thread.
+// the odd last element (if any) with f1(N-1) on one thread.  g is the block's
+// position in its (possibly batched) launch.
 template <class F2, class F1>
-__device__ inline void for_each_pair(int64_t N, F2&& f2, F1&& f1) {
+__device__ inline void for_each_pair(GridPos g, int64_t N, F2&& f2, F1&& f1) {
     const int64_t P = N >> 1;
-    const int64_t step = (int64_t)gridDim.x * (2 * kBlock);
-    for (int64_t base = (int64_t)blockIdx.x * (2 * kBlock); base < P; base += step) {
+    const int64_t step = (int64_t)g.nb * (2 * kBlock);
+    for (int64_t base = (int64_t)g.bid * (2 * kBlock); base < P; base += step) {
         const int64_t q0 = base + threadIdx.x, q1 = q0 + kBlock;
         f2(q0, q1, q0 < P, q1 < P);
     }
-    if ((N & 1) && blockIdx.x == 0 && threadIdx.x == 0) f1(N - 1);
+    if ((N & 1) && g.bid == 0 && threadIdx.x == 0) f1(N - 1);
+}
+template <class F2, class F1>
+__device__ inline void for_each_pair(int64_t N, F2&& f2, F1&& f1) {
+    for_each_pair(hw_grid(), N, f2, f1);
 }
 
 inline bool aligned2(const void* p, size_t elem) {
@@ -65,10 +70,10 @@ inline bool aligned2(const void* p, size_t elem) {
 
 // sum_i a_i b_i (b may equal a) -> one partial per block.
 template <bool VEC, class TA, class TB>
-__global__ __launch_bounds__(kBlock) void k_dot(int64_t N, const TA* a, const TB* b, Reduce red) {
+__device__ inline void dot_body(GridPos g, int64_t N, const TA* a, const TB* b, Reduce red) {
     double acc = 0.0;
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             Pair<TA> x0{}, x1{};
             Pair<TB> y0{}, y1{};
@@ -86,7 +91,11 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t N, const TA* a, const TB
             acc += (double)x1.b * (double)y1.b;
         },
         [&](int64_t i) { acc += dv(a, i) * dv(b, i); });
-    block_partial<kBlock>(acc, red, blockIdx.x);
+    block_partial<kBlock>(acc, red, g.bid);
+}
+template <bool VEC, class TA, class TB>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t N, const TA* a, const TB* b, Reduce red) {
+    dot_body<VEC, TA, TB>(hw_grid(), N, a, b, red);
 }
 
 template <class TA, class TB>
@@ -146,20 +155,24 @@ __global__ __launch_bounds__(kBlock) void k_axpy(int64_t N, double a, const TX* 
 }
 
 template <bool VEC, class T>
-__global__ __launch_bounds__(kBlock) void k_fill(int64_t N, T v, T* y) {
+__device__ inline void fill_body(GridPos g, int64_t N, T v, T* y) {
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             if (ok0) stp<VEC>(y, q0, Pair<T>{v, v});
             if (ok1) stp<VEC>(y, q1, Pair<T>{v, v});
         },
         [&](int64_t i) { y[i] = v; });
 }
+template <bool VEC, class T>
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t N, T v, T* y) {
+    fill_body<VEC, T>(hw_grid(), N, v, y);
+}
 
 template <bool VEC, class TI, class TO>
-__global__ __launch_bounds__(kBlock) void k_convert(int64_t N, const TI* x, TO* y) {
+__device__ inline void convert_body(GridPos g, int64_t N, const TI* x, TO* y) {
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             Pair<TI> a0{}, a1{};
             if (ok0) a0 = ldp<VEC>(x, q0);
@@ -169,6 +182,10 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t N, const TI* x, TO* 
         },
         [&](int64_t i) { y[i] = static_cast<TO>(x[i]); });
 }
+template <bool VEC, class TI, class TO>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t N, const TI* x, TO* y) {
+    convert_body<VEC, TI, TO>(hw_grid(), N, x, y);
+}
 
 // One fused modified-Gram-Schmidt pass (gmres.hpp:141-144):
 //   w -= h_i * (s_i V_i)            with h_i = *hi (already the true dot)
@@ -176,8 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t N, const TI* x, TO* 
 //   LAST:  partial of sum w * w         (-> H[j+1][j] = sqrt, s_{j+1} = 1/sqrt)
 // V_i are stored unnormalised with scales s_i.
 template <bool VEC, bool LAST, class T>
-__global__ __launch_bounds__(kBlock) void k_mgs(int64_t N, T* w, const T* Vi, const double* hi,
-                                                 const double* si, const T* Vnext, Reduce red) {
+__device__ inline void mgs_body(GridPos g, int64_t N, T* w, const T* Vi, const double* hi,
+                                const double* si, const T* Vnext, Reduce red) {
 #pragma clang fp contract(off)
     const double coef = (*hi) * (*si);
     double acc = 0.0;
@@ -189,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int64_t N, T* w, const T* Vi, co
         return (double)u.a * (double)v.a + (double)u.b * (double)v.b;
     };
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             Pair<T> w0{}, w1{}, v0{}, v1{}, n0{}, n1{};
             if (ok0) {
@@ -220,7 +237,12 @@ __global__ __launch_bounds__(kBlock) void k_mgs(int64_t N, T* w, const T* Vi, co
             const double wd = static_cast<double>(wn);
             acc += LAST ? wd * wd : wd * dv(Vnext, i);
         });
-    block_partial<kBlock>(acc, red, blockIdx.x);
+    block_partial<kBlock>(acc, red, g.bid);
+}
+template <bool VEC, bool LAST, class T>
+__global__ __launch_bounds__(kBlock) void k_mgs(int64_t N, T* w, const T* Vi, const double* hi,
+                                                 const double* si, const T* Vnext, Reduce red) {
+    mgs_body<VEC, LAST, T>(hw_grid(), N, w, Vi, hi, si, Vnext, red);
 }
 
 // y (=|+=) sum_k c_k X_k over up to kMaxComb vectors (x += V y, gmres.hpp:198-202).
@@ -233,11 +255,11 @@ struct CombArgs {
 };
 
 template <bool VEC, class T, class TY>
-__global__ __launch_bounds__(kBlock) void k_combine(int64_t N, CombArgs<T> a, TY* y,
-                                                     int accumulate) {
+__device__ inline void combine_body(GridPos g, int64_t N, const CombArgs<T>& a, TY* y,
+                                    int accumulate) {
 #pragma clang fp contract(off)
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             double s0a = 0.0, s0b = 0.0, s1a = 0.0, s1b = 0.0;
             if (accumulate) {
@@ -269,6 +291,11 @@ __global__ __launch_bounds__(kBlock) void k_combine(int64_t N, CombArgs<T> a, TY
             for (int k = 0; k < a.count; ++k) s += a.c[k] * dv(a.v[k], i);
             y[i] = static_cast<TY>(s);
         });
+}
+template <bool VEC, class T, class TY>
+__global__ __launch_bounds__(kBlock) void k_combine(int64_t N, CombArgs<T> a, TY* y,
+                                                     int accumulate) {
+    combine_body<VEC, T, TY>(hw_grid(), N, a, y, accumulate);
 }
 
 // z = r / d   (BratuJacobi::Bound, bratu3d.hpp:245-248)
@@ -392,11 +419,10 @@ void launch_convert(Ctx& c, int64_t N, const TI* x, TO* y) {
 // y = x (converted) and the partials of sum x_i^2 in one pass; the partials
 // are summed in the order of k_dot(x, x), so the epilogue sees the same total.
 template <bool VEC, class TI, class TO>
-__global__ __launch_bounds__(kBlock) void k_convert_norm(int64_t N, const TI* x, TO* y,
-                                                         Reduce red) {
+__device__ inline void convert_norm_body(GridPos g, int64_t N, const TI* x, TO* y, Reduce red) {
     double acc = 0.0;
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             Pair<TI> a0{}, a1{};
             if (ok0) a0 = ldp<VEC>(x, q0);
@@ -412,7 +438,12 @@ __global__ __launch_bounds__(kBlock) void k_convert_norm(int64_t N, const TI* x,
             acc += dv(x, i) * dv(x, i);
             y[i] = static_cast<TO>(x[i]);
         });
-    block_partial<kBlock>(acc, red, blockIdx.x);
+    block_partial<kBlock>(acc, red, g.bid);
+}
+template <bool VEC, class TI, class TO>
+__global__ __launch_bounds__(kBlock) void k_convert_norm(int64_t N, const TI* x, TO* y,
+                                                         Reduce red) {
+    convert_norm_body<VEC, TI, TO>(hw_grid(), N, x, y, red);
 }
 
 template <class TI, class TO>
@@ -533,6 +564,34 @@ void launch_combine(Ctx& c, int64_t N, const std::vector<const T*>& v,
         done += a.count;
         acc = true;
     }
+}
+
+// ||x||^2 (one partial per block) and E = lambda e^x in FP32, one pass over
+// x: the per-Newton-point data of the mixed-precision J v (EpJvFDe).
+// lambda from *lam_ptr when set.
+template <bool VEC>
+__device__ inline void xnorm_exp_body(GridPos g, int64_t N, const double* x, double lambda,
+                                      const double* lam_ptr, float* E, Reduce red) {
+    const double lam = lam_ptr ? *lam_ptr : lambda;
+    double acc = 0.0;
+    for_each_pair(
+        g, N,
+        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
+            Pair<double> x0{}, x1{};
+            if (ok0) x0 = ldp<VEC>(x, q0);
+            if (ok1) x1 = ldp<VEC>(x, q1);
+            acc += x0.a * x0.a;
+            acc += x0.b * x0.b;
+            acc += x1.a * x1.a;
+            acc += x1.b * x1.b;
+            if (ok0) stp<true>(E, q0, Pair<float>{(float)(lam * exp(x0.a)), (float)(lam * exp(x0.b))});
+            if (ok1) stp<true>(E, q1, Pair<float>{(float)(lam * exp(x1.a)), (float)(lam * exp(x1.b))});
+        },
+        [&](int64_t i) {
+            acc += x[i] * x[i];
+            E[i] = (float)(lam * exp(x[i]));
+        });
+    block_partial<kBlock>(acc, red, g.bid);
 }
 
 }  // namespace numa_hip

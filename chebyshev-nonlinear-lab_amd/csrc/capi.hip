@@ -18,43 +18,7 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 __global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partials, unsigned count,
                                                            Epilogue ep) {
-    // four loads in flight per round, added in index order (the same sum as a
-    // plain strided loop)
-    constexpr unsigned K = kFinishThreads;
-    double s = 0.0;
-    unsigned i = threadIdx.x;
-    for (; i + 3 * K < count; i += 4 * K) {
-        const double a0 = partials[i], a1 = partials[i + K], a2 = partials[i + 2 * K],
-                     a3 = partials[i + 3 * K];
-        s += a0;
-        s += a1;
-        s += a2;
-        s += a3;
-    }
-    if (i + 2 * K < count) {
-        const double a0 = partials[i], a1 = partials[i + K], a2 = partials[i + 2 * K];
-        s += a0;
-        s += a1;
-        s += a2;
-    } else if (i + K < count) {
-        const double a0 = partials[i], a1 = partials[i + K];
-        s += a0;
-        s += a1;
-    } else if (i < count) {
-        s += partials[i];
-    }
-    s = block_sum<kFinishThreads>(s);
-    if (threadIdx.x == 0) {
-        switch (ep.kind) {
-            case EPI_STORE: ep.out[0] = s * (ep.mul_ptr ? ep.mul_ptr[0] : ep.mul); break;
-            case EPI_SQRT: ep.out[0] = sqrt(s); break;
-            default: {
-                const double r = sqrt(s);
-                ep.out[0] = r;
-                if (ep.out2) ep.out2[0] = r > 0.0 ? 1.0 / r : 0.0;
-            }
-        }
-    }
+    finish_body(partials, count, ep);
 }
 
 void Ctx::finish(unsigned count, const Epilogue& ep) {

@@ -63,6 +63,13 @@ struct Reduce {
     double* partials;   // [kMaxPartials]
 };
 
+// Position of a block in its launch: the hardware's for a plain launch, the
+// instance-local one for a launch over an instance table (batch.hpp).
+struct GridPos {
+    unsigned bid, nb;
+};
+__device__ inline GridPos hw_grid() { return GridPos{blockIdx.x, gridDim.x}; }
+
 // Epilogues applied by the finish kernel to the reduced total s.
 enum EpiKind : int {
     EPI_STORE = 0,       // out[0] = s * mul
@@ -71,7 +78,7 @@ enum EpiKind : int {
 };
 
 struct Epilogue {
-    int kind;
+    int64_t kind;    // EpiKind (8 bytes: the struct has no padding bytes)
     double mul;      // scale for EPI_STORE (read from mul_ptr if not null)
     const double* mul_ptr;
     double* out;
@@ -113,6 +120,48 @@ template <int BLOCK>
 __device__ inline void block_partial(double acc, const Reduce& red, unsigned bid) {
     const double t = block_sum<BLOCK>(acc);
     if (threadIdx.x == 0) red.partials[bid] = t;
+}
+
+// Sum of partials[0, count) in a fixed order, then the epilogue (one block
+// of kFinishThreads threads).
+__device__ inline void finish_body(const double* partials, unsigned count, const Epilogue& ep) {
+    // four loads in flight per round, added in index order (the same sum as a
+    // plain strided loop)
+    constexpr unsigned K = kFinishThreads;
+    double s = 0.0;
+    unsigned i = threadIdx.x;
+    for (; i + 3 * K < count; i += 4 * K) {
+        const double a0 = partials[i], a1 = partials[i + K], a2 = partials[i + 2 * K],
+                     a3 = partials[i + 3 * K];
+        s += a0;
+        s += a1;
+        s += a2;
+        s += a3;
+    }
+    if (i + 2 * K < count) {
+        const double a0 = partials[i], a1 = partials[i + K], a2 = partials[i + 2 * K];
+        s += a0;
+        s += a1;
+        s += a2;
+    } else if (i + K < count) {
+        const double a0 = partials[i], a1 = partials[i + K];
+        s += a0;
+        s += a1;
+    } else if (i < count) {
+        s += partials[i];
+    }
+    s = block_sum<kFinishThreads>(s);
+    if (threadIdx.x == 0) {
+        switch (ep.kind) {
+            case EPI_STORE: ep.out[0] = s * (ep.mul_ptr ? ep.mul_ptr[0] : ep.mul); break;
+            case EPI_SQRT: ep.out[0] = sqrt(s); break;
+            default: {
+                const double r = sqrt(s);
+                ep.out[0] = r;
+                if (ep.out2) ep.out2[0] = r > 0.0 ? 1.0 / r : 0.0;
+            }
+        }
+    }
 }
 
 __global__ void __launch_bounds__(kFinishThreads) k_finish(const double* partials, unsigned count,
@@ -208,6 +257,24 @@ inline bool nl_converged(const numa_krylov_params& p, double rnorm, double res0,
     if (!(rnorm <= p.rtol_nl * res0)) return false;
     if (!(p.tight_eta > 0.0) || iters == 0) return true;
     return last_start <= tight_threshold(p, res0);
+}
+
+// Eisenstat-Walker choice 2 with Kelley's safeguards (krylov_newton.hpp:54-65),
+// or a constant forcing term.
+inline double forcing_ew(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
+                         double eta_prev, double res0) {
+    if (p.forcing == NUMA_FORCING_CONSTANT) return p.eta_init;
+    if (k == 0) return p.eta_init;
+    double eta = p.gamma * std::pow(rnorm / rprev, p.alpha);
+    const double safe = p.gamma * std::pow(eta_prev, p.alpha);
+    if (safe > 0.1) eta = std::max(eta, safe);
+    eta = std::min(eta, p.eta_max);
+    eta = std::max(eta, 0.5 * p.rtol_nl * res0 / rnorm);
+    return std::min(eta, p.eta_max);
+}
+inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
+                      double eta_prev, double res0) {
+    return tight_cap(p, forcing_ew(p, k, rnorm, rprev, eta_prev, res0), rnorm, res0);
 }
 
 // Launch shape of grid-stride streaming kernels over N elements: each thread

@@ -83,8 +83,7 @@ struct EpResid {
 
 // First step from x = 0:  d = c2 b;  xn = d  (arithmetic SmoothArith<T>).
 template <bool VEC, class T>
-__global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2d, const T* b, T* d,
-                                                        T* xn) {
+__device__ inline void poly_first_body(GridPos g, int64_t N, double c2d, const T* b, T* d, T* xn) {
 #pragma clang fp contract(off)
     using A = SmoothArith<T>;
     const A c2 = static_cast<A>(c2d);
@@ -92,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2d, co
         return Pair<T>{static_cast<T>(c2 * (A)v.a), static_cast<T>(c2 * (A)v.b)};
     };
     for_each_pair(
-        N,
+        g, N,
         [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
             Pair<T> b0{}, b1{};
             if (ok0) b0 = ldp<VEC>(b, q0);
@@ -112,6 +111,11 @@ __global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2d, co
             if (d) d[i] = static_cast<T>(dn);
             xn[i] = static_cast<T>(dn);
         });
+}
+template <bool VEC, class T>
+__global__ __launch_bounds__(kBlock) void k_poly_first(int64_t N, double c2d, const T* b, T* d,
+                                                        T* xn) {
+    poly_first_body<VEC, T>(hw_grid(), N, c2d, b, d, xn);
 }
 
 // Red-black Gauss-Seidel half sweep on one colour (poisson_mg.hpp:107-124):
@@ -190,18 +194,16 @@ constexpr int kRRPW = kRRFW + 2, kRRPH = kRRFH + 2;        // padded region of x
 constexpr int kRRSlots = (kRRPW * kRRPH + kRRThreads - 1) / kRRThreads;
 
 template <class T>
-__global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, int ich,
-                                                               int tiles_k, int tiles_j, int chunks,
-                                                               double h2inv,
-                                                               const T* __restrict__ x,
-                                                               const T* __restrict__ b,
-                                                               T* __restrict__ bc) {
+__device__ inline void resid_restrict_body(GridPos g, int nf, int nc, int ich, int tiles_k,
+                                           int tiles_j, int chunks, double h2inv,
+                                           const T* __restrict__ x, const T* __restrict__ b,
+                                           T* __restrict__ bc) {
 #pragma clang fp contract(off)
     __shared__ T xpl[kRRPH * kRRPW];
     __shared__ T rpl[kRRFH * kRRFW];
     // XCD-aware block -> tile (every 8th block walks consecutive tiles)
-    const int G8 = gridDim.x >> 3;
-    const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
+    const int G8 = (int)(g.nb >> 3);
+    const int lin = (int)(g.bid & 7) * G8 + (int)(g.bid >> 3);
     if (lin >= tiles_k * tiles_j * chunks) return;  // uniform per block
     const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ch = lin / (tiles_k * tiles_j);
     const int K0 = tk * kRRK, J0 = tj * kRRJ;
@@ -311,6 +313,15 @@ __global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, i
     }
     if (fe & 1) store((fe - 1) / 2, acc);  // nf even: the last node has no plane 2I+2
 }
+template <class T>
+__global__ __launch_bounds__(kRRThreads) void k_resid_restrict(int nf, int nc, int ich,
+                                                               int tiles_k, int tiles_j, int chunks,
+                                                               double h2inv,
+                                                               const T* __restrict__ x,
+                                                               const T* __restrict__ b,
+                                                               T* __restrict__ bc) {
+    resid_restrict_body<T>(hw_grid(), nf, nc, ich, tiles_k, tiles_j, chunks, h2inv, x, b, bc);
+}
 
 // 1D trilinear terms of a fine index (poisson_mg.hpp:191-220).
 __device__ inline int interp_terms(int f, int nc, int ci[2], double wi[2]) {
@@ -340,11 +351,12 @@ __device__ inline int interp_terms(int f, int nc, int ci[2], double wi[2]) {
 // weight 1, an even one takes I-1 (if I >= 1) and I (if I < nc) with weight
 // 1/2; terms are summed in the reference's order (zero-weight terms add 0).
 template <class T>
-__global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc, T* xf) {
+__device__ inline void prolong_add_body(int bx, int by, int bz, int nf, int nc, const T* xc,
+                                        T* xf) {
 #pragma clang fp contract(off)
-    const int K = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int I = blockIdx.z;
+    const int K = bx * 64 + (threadIdx.x & 63);
+    const int J = by * 4 + (threadIdx.x >> 6);
+    const int I = bz;
     const int ncell = (nf + 1) / 2;
     if (K >= ncell || J >= ncell) return;
     const int64_t C1 = nc, C2 = C1 * C1, F1 = nf, F2 = F1 * F1;
@@ -421,6 +433,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc
             }
         }
     }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add(int nf, int nc, const T* xc, T* xf) {
+    prolong_add_body<T>(blockIdx.x, blockIdx.y, blockIdx.z, nf, nc, xc, xf);
 }
 
 // Coarse solve by one workgroup with the level in LDS (x starts at zero).
@@ -691,10 +707,8 @@ inline int fused_fill_chunk(int n, int steps, int rows, int cu_count) {
 constexpr int kDstTab = 16 * 16 + 16;
 
 template <class T>
-__global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv,
-                                                     const double* __restrict__ tab,
-                                                     const T* __restrict__ b,
-                                                     T* __restrict__ xout) {
+__device__ inline void coarse_dst_body(int n, double h2inv, const double* __restrict__ tab,
+                                       const T* __restrict__ b, T* __restrict__ xout) {
 #pragma clang fp contract(off)
     __shared__ double u[2][4096];  // (i*16 + j)*16 + k
     __shared__ double S[16][16];
@@ -772,6 +786,24 @@ __global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv,
         if (i < n && j < n && k < n) xout[(i * n + j) * n + k] = static_cast<T>(u[cur][id]);
     }
 }
+template <class T>
+__global__ __launch_bounds__(1024) void k_coarse_dst(int n, double h2inv,
+                                                     const double* __restrict__ tab,
+                                                     const T* __restrict__ b,
+                                                     T* __restrict__ xout) {
+    coarse_dst_body<T>(n, h2inv, tab, b, xout);
+}
+
+// Coarse planes per workgroup of the fused residual / restriction onto an
+// nc^3 level: the chunk that fills the device in whole waves of resident
+// workgroups (a chunk of ich coarse planes marches 2 ich + 1 fine planes).
+// Any chunk gives the same values.
+template <class T>
+inline int resid_restrict_chunk(int64_t nc, int cu_count) {
+    static const int per_cu = blocks_per_cu(k_resid_restrict<T>, kRRThreads);
+    const int64_t tiles = ((nc + kRRK - 1) / kRRK) * ((nc + kRRJ - 1) / kRRJ);
+    return fill_chunk(nc, tiles, (int64_t)per_cu * cu_count, 1, 1, 64);
+}
 
 // The table of k_coarse_dst for an n^3 level: S (padded with zeros) and c.
 inline std::vector<double> dst_table(int64_t n) {
@@ -786,6 +818,25 @@ inline std::vector<double> dst_table(int64_t n) {
 
 inline double cheb_lmax(int64_t n) { return 1.0 + std::cos(M_PI / (double(n) + 1.0)); }
 inline double cheb_lmin(int64_t n) { return 1.0 - std::cos(M_PI / (double(n) + 1.0)); }
+
+// Chebyshev step coefficients (c1[k], c2[k]) of a degree-deg polynomial
+// smoother over [lo, hi] of D^-1 A on a level with D = 6 h2inv.
+inline void cheb_step_coefs(double h2inv, int deg, double lo, double hi, std::vector<double>& c1,
+                            std::vector<double>& c2) {
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    const double dinv = 1.0 / (6.0 * h2inv);
+    double rho = 1.0 / sigma;
+    c1.assign(deg, 0.0);
+    c2.assign(deg, 0.0);
+    c2[0] = dinv / theta;
+    for (int k = 1; k < deg; ++k) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        c1[k] = rho_new * rho;
+        c2[k] = 2.0 * rho_new / delta * dinv;
+        rho = rho_new;
+    }
+}
 
 struct MGBase {
     virtual ~MGBase() = default;
@@ -1043,21 +1094,8 @@ private:
 
     static void cheb_coefs(const Level& L, int deg, double lo, double hi, std::vector<double>& c1,
                            std::vector<double>& c2) {
-        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
-        const double sigma = theta / delta;
-        const double dinv = 1.0 / (6.0 * L.h2inv);
-        double rho = 1.0 / sigma;
-        c1.assign(deg, 0.0);
-        c2.assign(deg, 0.0);
-        c2[0] = dinv / theta;
-        for (int k = 1; k < deg; ++k) {
-            const double rho_new = 1.0 / (2.0 * sigma - rho);
-            c1[k] = rho_new * rho;
-            c2[k] = 2.0 * rho_new / delta * dinv;
-            rho = rho_new;
-        }
+        cheb_step_coefs(L.h2inv, deg, lo, hi, c1, c2);
     }
-
     // One step d <- c1 d + c2 (b - A x), x <- x + d (x double-buffered; with
     // dst the new x goes to dst and the level's buffers are left as they are).
     void poly_step(Level& L, double c1, double c2, bool use_d, bool write_d, bool from_zero,
@@ -1356,11 +1394,7 @@ private:
     // Coarse planes per workgroup of the fused residual / restriction: the
     // chunk that fills the device in whole waves of resident workgroups (a
     // chunk of ich coarse planes marches 2 ich + 1 fine planes).
-    int rr_chunk(int64_t nc) const {
-        static const int per_cu = blocks_per_cu(k_resid_restrict<T>, kRRThreads);
-        const int64_t tiles = ((nc + kRRK - 1) / kRRK) * ((nc + kRRJ - 1) / kRRJ);
-        return fill_chunk(nc, tiles, (int64_t)per_cu * c_.cu_count, 1, 1, 64);
-    }
+    int rr_chunk(int64_t nc) const { return resid_restrict_chunk<T>(nc, c_.cu_count); }
 
     // poisson_mg.hpp:262-294
     void cycle(size_t l) {

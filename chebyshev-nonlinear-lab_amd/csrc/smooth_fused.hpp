@@ -368,12 +368,10 @@ struct BoxGeom {
 };
 
 template <class T, int STEPS, int EI, int EJ, int EK, bool NORM = false>
-__global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, int tiles_j,
-                                                          int total, double h2inv_d, PolySteps cs,
-                                                          const T* __restrict__ x,
-                                                          const T* __restrict__ b,
-                                                          T* __restrict__ xo,
-                                                          double* __restrict__ norm_part) {
+__device__ __forceinline__ void poly_box_body(GridPos g, int n, int tiles_k, int tiles_j,
+                                              int total, double h2inv_d, const PolySteps& cs,
+                                              const T* __restrict__ x, const T* __restrict__ b,
+                                              T* __restrict__ xo, double* __restrict__ norm_part) {
 #pragma clang fp contract(off)
     using A = SmoothArith<T>;
     using G = BoxGeom<STEPS, EI, EJ, EK>;
@@ -382,10 +380,10 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
     __shared__ T buf[2][NB];
 
     // XCD-aware block -> box (k fastest), as in k_poly_fused
-    const int G8 = gridDim.x >> 3;
-    const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
+    const int G8 = (int)(g.nb >> 3);
+    const int lin = (int)(g.bid & 7) * G8 + (int)(g.bid >> 3);
     if (lin >= total) {  // uniform per block, before any barrier
-        if (NORM && threadIdx.x == 0) norm_part[blockIdx.x] = 0.0;
+        if (NORM && threadIdx.x == 0) norm_part[g.bid] = 0.0;
         return;
     }
     const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ti = lin / (tiles_k * tiles_j);
@@ -482,8 +480,18 @@ __global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, in
     }
     if constexpr (NORM) {
         const double t = block_sum<kBoxThreads>(nsq);
-        if (threadIdx.x == 0) norm_part[blockIdx.x] = t;
+        if (threadIdx.x == 0) norm_part[g.bid] = t;
     }
+}
+template <class T, int STEPS, int EI, int EJ, int EK, bool NORM = false>
+__global__ __launch_bounds__(kBoxThreads) void k_poly_box(int n, int tiles_k, int tiles_j,
+                                                          int total, double h2inv_d, PolySteps cs,
+                                                          const T* __restrict__ x,
+                                                          const T* __restrict__ b,
+                                                          T* __restrict__ xo,
+                                                          double* __restrict__ norm_part) {
+    poly_box_body<T, STEPS, EI, EJ, EK, NORM>(hw_grid(), n, tiles_k, tiles_j, total, h2inv_d, cs, x,
+                                              b, xo, norm_part);
 }
 
 // Launch geometry of k_poly_box for an n^3 level.

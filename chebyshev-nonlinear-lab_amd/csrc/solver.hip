@@ -34,50 +34,10 @@ struct numa_hip_mg {
 
 namespace numa_hip {
 
-// Eisenstat-Walker choice 2 with Kelley's safeguards (krylov_newton.hpp:54-65),
-// or a constant forcing term.
-inline double forcing_ew(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
-                         double eta_prev, double res0) {
-    if (p.forcing == NUMA_FORCING_CONSTANT) return p.eta_init;
-    if (k == 0) return p.eta_init;
-    double eta = p.gamma * std::pow(rnorm / rprev, p.alpha);
-    const double safe = p.gamma * std::pow(eta_prev, p.alpha);
-    if (safe > 0.1) eta = std::max(eta, safe);
-    eta = std::min(eta, p.eta_max);
-    eta = std::max(eta, 0.5 * p.rtol_nl * res0 / rnorm);
-    return std::min(eta, p.eta_max);
-}
-inline double forcing(const numa_krylov_params& p, int64_t k, double rnorm, double rprev,
-                      double eta_prev, double res0) {
-    return tight_cap(p, forcing_ew(p, k, rnorm, rprev, eta_prev, res0), rnorm, res0);
-}
-
-// ||x||^2 (one partial per block) and E = lambda e^x in FP32, one pass over
-// x: the per-Newton-point data of the mixed-precision J v (EpJvFDe).
-// lambda from *lam_ptr when set.
 template <bool VEC>
 __global__ __launch_bounds__(kBlock) void k_xnorm_exp(int64_t N, const double* x, double lambda,
                                                       const double* lam_ptr, float* E, Reduce red) {
-    const double lam = lam_ptr ? *lam_ptr : lambda;
-    double acc = 0.0;
-    for_each_pair(
-        N,
-        [&](int64_t q0, int64_t q1, bool ok0, bool ok1) {
-            Pair<double> x0{}, x1{};
-            if (ok0) x0 = ldp<VEC>(x, q0);
-            if (ok1) x1 = ldp<VEC>(x, q1);
-            acc += x0.a * x0.a;
-            acc += x0.b * x0.b;
-            acc += x1.a * x1.a;
-            acc += x1.b * x1.b;
-            if (ok0) stp<true>(E, q0, Pair<float>{(float)(lam * exp(x0.a)), (float)(lam * exp(x0.b))});
-            if (ok1) stp<true>(E, q1, Pair<float>{(float)(lam * exp(x1.a)), (float)(lam * exp(x1.b))});
-        },
-        [&](int64_t i) {
-            acc += x[i] * x[i];
-            E[i] = (float)(lam * exp(x[i]));
-        });
-    block_partial<kBlock>(acc, red, blockIdx.x);
+    xnorm_exp_body<VEC>(hw_grid(), N, x, lambda, lam_ptr, E, red);
 }
 
 template <class T>

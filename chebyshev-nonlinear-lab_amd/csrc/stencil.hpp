@@ -70,11 +70,11 @@ __device__ inline A lap_num(A c, A im, A ip, A jm, A jp, A km, A kp) {
 // fetched from HBM again).  A reduction's partials keep the natural tile
 // order, so sums are the same as with a 3-D grid.
 template <int TK, bool REDUCE, bool MARCH, class Load, class Epi>
-__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int gy, int gz,
-                                                  Load ldr, Epi epi, Reduce red) {
+__device__ inline void stencil_body(GridPos g, int n, int ichunk, int gx, int gy, int gz,
+                                    const Load& ldr, const Epi& epi, const Reduce& red) {
     constexpr int TJ = 256 / TK;
-    const unsigned G8 = gridDim.x >> 3;
-    const unsigned lin = (blockIdx.x & 7u) * G8 + (blockIdx.x >> 3);
+    const unsigned G8 = g.nb >> 3;
+    const unsigned lin = (g.bid & 7u) * G8 + (g.bid >> 3);
     if (lin >= (unsigned)(gx * gy * gz)) return;  // padding block, uniform
     const int bx = (int)(lin % (unsigned)gx), by = (int)((lin / (unsigned)gx) % (unsigned)gy);
     const int bz = (int)(lin / (unsigned)(gx * gy));
@@ -129,6 +129,11 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int 
         }
     }
     if constexpr (REDUCE) block_partial<256>(acc, red, lin);
+}
+template <int TK, bool REDUCE, bool MARCH, class Load, class Epi>
+__global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int gy, int gz,
+                                                  Load ldr, Epi epi, Reduce red) {
+    stencil_body<TK, REDUCE, MARCH, Load, Epi>(hw_grid(), n, ichunk, gx, gy, gz, ldr, epi, red);
 }
 
 // LDS-tiled form for large grids.  A block of 256 threads owns a 64 (k) x

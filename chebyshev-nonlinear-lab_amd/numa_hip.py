@@ -64,6 +64,16 @@ class KrylovResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(C.Structure):
+    """numa_batch_stats: counters of a batched solver."""
+    _fields_ = [("launches", _i64), ("graph_launches", _i64), ("arnoldi_steps", _i64),
+                ("host_syncs", _i64), ("bytes", C.c_double), ("tables", _i64),
+                ("table_bytes", _i64), ("graphs", _i64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class GmresResult(C.Structure):
     _fields_ = [("iters", _i64), ("matvecs", _i64), ("precs", _i64), ("rel_res", C.c_double),
                 ("converged", _i64), ("breakdown", _i64)]
@@ -307,6 +317,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_sweep_instances": [_i64, _i64, C.POINTER(_i64), _i64, C.c_double, C.c_double,
                                      C.POINTER(_i64), _dp],
         "numa_hip_uniform_host": [_i64, _i64, C.c_double, _dp],
+        "numa_hip_batch_create": [_vp, _i64, C.POINTER(_i64), _dp, C.POINTER(SolverConfig), _i64,
+                                  C.POINTER(_vp)],
+        "numa_hip_batch_destroy": [_vp],
+        "numa_hip_batch_set_graphs": [_vp, _i64],
+        "numa_hip_batch_set_smoothing": [_vp, _i64, C.POINTER(_i64)],
+        "numa_hip_batch_solve": [_vp, C.POINTER(KrylovParams), C.POINTER(_vp),
+                                 C.POINTER(KrylovResult), _dp],
+        "numa_hip_batch_stats": [_vp, _i64, C.POINTER(BatchStats)],
         "numa_hip_solver_mg_fusion": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                                       C.POINTER(_i64)],
     }
@@ -946,6 +964,67 @@ class Sweep:
     def close(self):
         if self.h and self.ctx.h:
             check(self.ctx.lib.numa_hip_sweep_destroy(self.h))
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched solver (numa_hip_batch): instances (n_i, lambda_i) solved
+    together, one launch per operation over the instance table and one
+    (graph) launch per lockstep Arnoldi step."""
+
+    def __init__(self, ctx: Context, ns, lambdas, cfg: SolverConfig, restart: int = 30):
+        self.ctx, self.cfg = ctx, cfg
+        self.n = np.ascontiguousarray(ns, np.int64)
+        lam = np.ascontiguousarray(lambdas, np.float64)
+        if self.n.shape != lam.shape:
+            raise ValueError("ns and lambdas differ in length")
+        h = _vp()
+        check(ctx.lib.numa_hip_batch_create(ctx.h, len(self.n),
+                                            self.n.ctypes.data_as(C.POINTER(_i64)),
+                                            lam.ctypes.data_as(_dp), C.byref(cfg), int(restart),
+                                            C.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    def set_graphs(self, on: bool = True):
+        check(self.ctx.lib.numa_hip_batch_set_graphs(self.h, 1 if on else 0))
+
+    def set_smoothing(self, forms):
+        """Pins the smoothing form of levels 0.. (0 per step, 1..3 box shape)."""
+        f = np.ascontiguousarray(forms, np.int64)
+        check(self.ctx.lib.numa_hip_batch_set_smoothing(self.h, len(f),
+                                                        f.ctypes.data_as(C.POINTER(_i64))))
+
+    def solve(self, xs: list, params: KrylovParams):
+        """Solves instance i from xs[i] (DeviceArray of n_i^3, overwritten);
+        returns (list of KrylovResult, wall seconds)."""
+        count = len(self.n)
+        if len(xs) != count:
+            raise ValueError(f"expected {count} iterates, got {len(xs)}")
+        for x, n in zip(xs, self.n):
+            if x.n != int(n) ** 3:
+                raise ValueError("iterate length does not match its grid")
+        ptrs = (_vp * count)(*[x.ptr for x in xs])
+        res = (KrylovResult * count)()
+        sec = C.c_double()
+        check(self.ctx.lib.numa_hip_batch_solve(self.h, C.byref(params), ptrs, res,
+                                                C.byref(sec)))
+        return list(res), sec.value
+
+    def stats(self, reset: bool = False) -> dict:
+        st = BatchStats()
+        check(self.ctx.lib.numa_hip_batch_stats(self.h, 1 if reset else 0, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h and self.ctx.h:
+            check(self.ctx.lib.numa_hip_batch_destroy(self.h))
         self.h = None
 
     def __del__(self):

@@ -594,6 +594,50 @@ int numa_hip_sweep_instances(int64_t seed, int64_t count, const int64_t* sizes, 
 /* out[i] = scale * U[0,1) from std::mt19937_64(seed) (host only). */
 int numa_hip_uniform_host(int64_t seed, int64_t count, double scale, double* out);
 
+/* Batched solver: up to 256 instances (grid n_i <= 127, lambda_i) solved
+ * together by mixed-precision Newton-FGMRES(m) with the V-cycle, every
+ * operation one kernel launch over the instance table and every Arnoldi step
+ * one launch (a hipGraph when graphs are on) for all instances still
+ * iterating.  Each instance follows its own Newton / GMRES decisions
+ * (newton_krylov, krylov_newton.hpp:69-114; gmres, gmres.hpp:61-223); its
+ * iterates are bit for bit those of a numa_hip_solver with the same
+ * configuration and unfused smoothing (cfg->mg.fuse = NUMA_FUSE_NONE).
+ * Requirements: method Newton, precision mixed, flexible GMRES, the V-cycle
+ * with Chebyshev or weighted-Jacobi smoothing and the exact coarse solve
+ * (coarse_iters < 0), FD J v, the same number of levels for every instance,
+ * 1 <= gmres_restart <= 32.  No reference counterpart (the reference solves
+ * one instance per call). */
+typedef struct numa_hip_batch numa_hip_batch;
+typedef struct numa_batch_stats {
+    int64_t launches;       /* kernel launches outside graphs                     */
+    int64_t graph_launches; /* Arnoldi steps replayed from graphs                 */
+    int64_t arnoldi_steps;  /* lockstep Arnoldi steps                             */
+    int64_t host_syncs;     /* stream synchronisations                            */
+    double bytes;           /* algorithmic HBM bytes of the active instances      */
+    int64_t tables;         /* argument tables on the device                      */
+    int64_t table_bytes;
+    int64_t graphs;         /* captured Arnoldi steps                             */
+} numa_batch_stats;
+int numa_hip_batch_create(numa_hip_ctx* ctx, int64_t count, const int64_t* n,
+                          const double* lambda, const numa_solver_config* cfg,
+                          int64_t gmres_restart, numa_hip_batch** out);
+int numa_hip_batch_destroy(numa_hip_batch* h);
+int numa_hip_batch_set_graphs(numa_hip_batch* h, int64_t enable);
+/* Smoothing form of multigrid levels 0 .. count-1: 0 one launch per smoothing
+ * step, s in 1..3 one launch per sweep in box shape s (8x8x16, 4x8x16,
+ * 4x4x16 output nodes), bit for bit the same values.  Without this call the
+ * batch times the forms per level at its first solve (cfg->mg.fuse !=
+ * NUMA_FUSE_NONE) and keeps the fastest (cached under the level's sizes). */
+int numa_hip_batch_set_smoothing(numa_hip_batch* h, int64_t count, const int64_t* form);
+/* Solves instance i from x_dev[i] (device, FP64, n_i^3, overwritten by the
+ * solution) on the context's stream; results[i] its KrylovResult; *seconds
+ * (may be NULL) the wall time. */
+int numa_hip_batch_solve(numa_hip_batch* h, const numa_krylov_params* p, double* const* x_dev,
+                         numa_krylov_result* results, double* seconds);
+/* Counters since creation or the last reset (reset != 0 clears them after
+ * reading). */
+int numa_hip_batch_stats(numa_hip_batch* h, int64_t reset, numa_batch_stats* out);
+
 /* ---- fault injection (tests of the failure paths) ------------------------ */
 /* Forces a non-finite value into one primitive call of the solver, from the
  * host, so that the numerical failure guards of the reference (the
