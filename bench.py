@@ -462,7 +462,11 @@ def sweep_leg(H, ctx, args, params) -> dict:
     leg's configuration, autotuned fused smoothing).  One eager worker (the
     instances one after another) next to W workers with captured Arnoldi
     steps; each configuration is run once untimed (allocation, tuning,
-    capture) and once timed."""
+    capture) and once timed.  Then the batched solver: all 96 instances in
+    one launch per operation over the instance table and one graph launch per
+    lockstep Arnoldi step (best of two timed solves after a tuning one), with
+    its algorithmic HBM bytes, and a sample checked bit for bit against the
+    single-instance solver with unfused smoothing."""
     import numpy as np
     n, lam = H.sweep_instances(SWEEP_SEED, SWEEP_COUNT, sizes=SWEEP_SIZES)
     x0 = [H.uniform_host(SWEEP_SEED * 7919 + 13 + i, int(n[i]) ** 3, 0.1)
@@ -494,6 +498,46 @@ def sweep_leg(H, ctx, args, params) -> dict:
             out[key]["max_abs_diff_vs_workers1"] = float(max(np.max(np.abs(a - b))
                                                              for a, b in zip(cur, sols)))
         log(f"sweep {key}: {out[key]}")
+    # the batched solver: one launch per operation over the instance table and
+    # one graph launch per lockstep Arnoldi step for all instances
+    b = H.Batch(ctx, n, lam, cfg, restart=30)
+    b.set_graphs(True)
+    times = []
+    for rep_ in range(3):  # tuning and capture, then two timed solves
+        for x, v in zip(xs, x0):
+            x.upload(v)
+        ctx.sync()
+        b.stats(reset=True)
+        res, sec = b.solve(xs, params)
+        st = b.stats()
+        times.append((sec, st))
+    sec, st = min(times[1:], key=lambda t: t[0])
+    b.close()
+    cur = [x.numpy() for x in xs]
+    out["batched_graphs"] = {
+        "seconds": round(sec, 4), "solves_per_s": round(SWEEP_COUNT / sec, 1),
+        "converged": int(sum(r.converged for r in res)),
+        "newton_total": int(sum(r.iters for r in res)),
+        "fgmres_total": int(sum(r.n_lin_iters for r in res)),
+        "lockstep_arnoldi_steps": st["arnoldi_steps"], "graph_launches": st["graph_launches"],
+        "other_launches": st["launches"], "host_syncs": st["host_syncs"],
+        "algorithmic_bytes": st["bytes"], "achieved_GBs": round(st["bytes"] / sec / 1e9, 1),
+        "frac_of_8000": round(st["bytes"] / sec / 1e9 / HBM_PEAK_GBS, 4),
+        "max_abs_diff_vs_workers1": float(max(np.max(np.abs(a - b_)) for a, b_ in zip(cur, sols)))}
+    # bit for bit the single-instance solver with unfused smoothing (sample)
+    alone = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
+                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=COARSE))
+    same = []
+    for k in SWEEP_SIZES:
+        i = int(np.flatnonzero(n == k)[0])
+        s = H.BratuSolver(ctx, int(n[i]), float(lam[i]), alone, restart=30)
+        xi = x0[i].copy()
+        s.solve_host(params, xi)
+        s.close()
+        same.append(bool(np.array_equal(xi, cur[i])))
+    out["batched_graphs"]["bitwise_vs_alone_unfused_sample"] = same
+    log(f"sweep batched_graphs: {out['batched_graphs']}")
     best = max((v["solves_per_s"], k) for k, v in out.items() if isinstance(v, dict) and
                "solves_per_s" in v)
     out["best"] = {"config": best[1], "solves_per_s": best[0]}

@@ -88,6 +88,8 @@ public:
                                         sizeof(double), sizeof(double), B_,
                                         hipMemcpyHostToDevice, c_.stream));
         c_.sync();
+        lvl_box_.assign(levels, 0);
+        lvl_rr_.assign(levels, 0);
         gst_.resize(B_);
         for (auto& s : gst_) {
             s.Hh.assign(m_ + 1, std::vector<double>(m_, 0.0));
@@ -119,9 +121,10 @@ public:
     void set_smoothing(int64_t count, const int64_t* form) {
         const size_t nl = inst_[0].lv.size();
         if (lvl_box_.size() != nl) lvl_box_.assign(nl, 0);
+        if (lvl_rr_.size() != nl) lvl_rr_.assign(nl, 0);
         require(count >= 0 && (count == 0 || form), "bad smoothing forms");
         for (int64_t l = 0; l < count && (size_t)l + 1 < nl; ++l) {
-            require(form[l] >= 0 && form[l] <= kBoxShapes, "smoothing form must be in [0, 3]");
+            require(form[l] >= 0 && form[l] <= kSweepForms, "smoothing form must be in [0, 7]");
             require(form[l] == 0 || box_ok(), "box smoothing needs the Chebyshev smoother of degree <= 3");
             lvl_box_[(size_t)l] = (int)form[l];
         }
@@ -150,6 +153,7 @@ public:
 
 private:
     static constexpr int64_t kBatchMaxN = 127;        // k_stencil without the plane march
+    static constexpr int kSweepForms = 7;             // smoothing forms 0 .. 7 (set_smoothing)
     static constexpr int64_t kBatchPartials = 16384;  // per-instance reduction partials
     enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_LAMBDA, S_LSA, kMisc = 8 };
 
@@ -521,6 +525,45 @@ private:
             },
             [&](const Inst& I) { return (from_zero ? 8.0 : 12.0) * (double)I.lv[l].N; });
     }
+    // One sweep per launch in the plane-marching form (R rows per thread,
+    // `ichunk` planes per workgroup).
+    template <int DEG, int R>
+    void march_sweep(size_t l, int ichunk, bool from_zero, bool dst) {
+        launch<OpPolyFused<float, DEG, R>>(
+            [&](Inst& I, typename OpPolyFused<float, DEG, R>::Args& a) {
+                const Level& L = I.lv[l];
+                int tk = 0, tj = 0, ch = 0;
+                const int ic = (int)std::min<int64_t>(ichunk, L.n);
+                const dim3 g = fused_grid<DEG, R>((int)L.n, ic, tk, tj, ch);
+                a.n = (int32_t)L.n;
+                a.ichunk = ic;
+                a.tiles_k = tk;
+                a.tiles_j = tj;
+                a.chunks = ch;
+                a.h2inv = L.h2inv;
+                for (int q = 0; q < DEG; ++q) {
+                    a.cs.c1[q] = q == 0 ? 0.0 : L.c1[(size_t)q];
+                    a.cs.c2[q] = L.c2[(size_t)q];
+                }
+                a.x = from_zero ? nullptr : L.x;
+                a.b = l == 0 ? I.vb : L.b;
+                a.xo = dst ? I.vout : L.x2;
+                return g.x;
+            },
+            [&](const Inst& I) { return (from_zero ? 8.0 : 12.0) * (double)I.lv[l].N; });
+    }
+    template <int DEG>
+    void sweep_form(size_t l, int form, bool from_zero, bool dst) {
+        switch (form) {
+            case 1: box_sweep<DEG, 1>(l, from_zero, dst); break;
+            case 2: box_sweep<DEG, 2>(l, from_zero, dst); break;
+            case 3: box_sweep<DEG, 3>(l, from_zero, dst); break;
+            case 4: march_sweep<DEG, 2>(l, 16, from_zero, dst); break;
+            case 5: march_sweep<DEG, 2>(l, 32, from_zero, dst); break;
+            case 6: march_sweep<DEG, 4>(l, 16, from_zero, dst); break;
+            default: march_sweep<DEG, 4>(l, 32, from_zero, dst); break;
+        }
+    }
     template <int DEG>
     void box_sweep_deg(size_t l, int shape, bool from_zero, bool dst) {
         switch (shape) {
@@ -543,9 +586,9 @@ private:
                 const bool dst = last_dst && s + 1 == sweeps;
                 const bool fz = from_zero && s == 0;
                 switch (cfg_.mg.degree) {
-                    case 1: box_sweep_deg<1>(l, lvl_box_[l], fz, dst); break;
-                    case 2: box_sweep_deg<2>(l, lvl_box_[l], fz, dst); break;
-                    default: box_sweep_deg<3>(l, lvl_box_[l], fz, dst); break;
+                    case 1: sweep_form<1>(l, lvl_box_[l], fz, dst); break;
+                    case 2: sweep_form<2>(l, lvl_box_[l], fz, dst); break;
+                    default: sweep_form<3>(l, lvl_box_[l], fz, dst); break;
                 }
                 if (dst) final_done_ = true;
                 else
@@ -581,16 +624,23 @@ private:
             return;
         }
         smooth(l, (int)cfg_.mg.pre, true, false);
+        resid_restrict(l);
+        cycle(l + 1);
+        prolong(l);
+        smooth(l, (int)cfg_.mg.post, false, l == 0);
+    }
+    // b_{l+1} = R (b_l - A x_l) in one pass (MG<T>::resid_restrict)
+    void resid_restrict(size_t l) {
         launch<OpResidRestrict<float>>(
             [&](Inst& I, OpResidRestrict<float>::Args& a) {
                 const Level& L = I.lv[l];
                 const Level& C = I.lv[l + 1];
                 a.nf = (int32_t)L.n;
                 a.nc = (int32_t)C.n;
-                a.ich = L.rr_ich;
+                a.ich = lvl_rr_[l] > 0 ? (int32_t)std::min<int64_t>(lvl_rr_[l], C.n) : L.rr_ich;
                 a.tiles_k = (int32_t)((C.n + kRRK - 1) / kRRK);
                 a.tiles_j = (int32_t)((C.n + kRRJ - 1) / kRRJ);
-                a.chunks = (int32_t)((C.n + L.rr_ich - 1) / L.rr_ich);
+                a.chunks = (int32_t)((C.n + a.ich - 1) / a.ich);
                 a.h2inv = L.h2inv;
                 a.x = L.x;
                 a.b = l == 0 ? I.vb : L.b;
@@ -599,7 +649,9 @@ private:
                 return (unsigned)((total + 7) / 8 * 8);
             },
             [&](const Inst& I) { return 8.5 * (double)I.lv[l].N; });
-        cycle(l + 1);
+    }
+    // x_l += P x_{l+1}
+    void prolong(size_t l) {
         launch<OpProlong<float>>(
             [&](Inst& I, OpProlong<float>::Args& a) {
                 const Level& L = I.lv[l];
@@ -614,7 +666,6 @@ private:
                 return (unsigned)((int64_t)a.gx * a.gy * ncell);
             },
             [&](const Inst& I) { return 8.5 * (double)I.lv[l].N; });
-        smooth(l, (int)cfg_.mg.post, false, l == 0);
     }
     // Smoothing form per level (all instances): time one pre-smoothing phase
     // from zero (b = 1) for one launch per step and each box shape, keep the
@@ -624,6 +675,7 @@ private:
         tuned_ = true;
         const size_t nl = inst_[0].lv.size();
         lvl_box_.assign(nl, 0);
+        lvl_rr_.assign(nl, 0);
         if (!box_ok() || cfg_.mg.fuse == NUMA_FUSE_NONE || cfg_.mg.pre == 0) return;
         const std::string dev = TuneCache::device_key(c_.device);
         const std::vector<char> all(B_, 1);
@@ -632,15 +684,17 @@ private:
             std::map<int64_t, int> sizes;
             for (auto& I : inst_) ++sizes[I.lv[l].n];
             std::ostringstream key;
-            key << dev << "|batch-smooth-v1|f32|deg=" << cfg_.mg.degree << "|pre=" << cfg_.mg.pre
+            key << dev << "|batch-smooth-v3|f32|deg=" << cfg_.mg.degree << "|pre=" << cfg_.mg.pre
                 << "|n=";
             for (auto& kv : sizes) key << kv.first << "x" << kv.second << ",";
             std::string val;
             const auto held = TuneCache::get().tuning(key.str());
             if (TuneCache::get().lookup(key.str(), val)) {
-                const int v = std::atoi(val.c_str());
-                if (v >= 0 && v <= kBoxShapes) {
+                int v = -1, r = -1;
+                if (std::sscanf(val.c_str(), "%d,%d", &v, &r) == 2 && v >= 0 && v <= kSweepForms &&
+                    r >= 0 && r <= 1024) {
                     lvl_box_[l] = v;
+                    lvl_rr_[l] = r;
                     continue;
                 }
             }
@@ -659,7 +713,7 @@ private:
             const bool log = std::getenv("NUMA_HIP_TUNE_LOG") != nullptr;
             double best = 1e300;
             int pick = 0;
-            for (int cand = 0; cand <= kBoxShapes; ++cand) {
+            for (int cand = 0; cand <= kSweepForms; ++cand) {
                 lvl_box_[l] = cand;
                 smooth(l, (int)cfg_.mg.pre, true, false);  // warm-up
                 float ms = 1e30f;
@@ -673,17 +727,44 @@ private:
                     ms = std::min(ms, t);
                 }
                 if (log)
-                    std::fprintf(stderr, "numa_hip batch tune: level %zu box %d %.2f us\n", l, cand,
-                                 ms * 1e3 / 3.0);
+                    std::fprintf(stderr, "numa_hip batch tune: level %zu form %d %.2f us\n", l,
+                                 cand, ms * 1e3 / 3.0);
                 if (ms < 0.98 * best) {
                     best = ms;
                     pick = cand;
                 }
             }
+            lvl_box_[l] = pick;
+            // coarse planes per workgroup of the residual / restriction
+            // (every chunk gives the same values): the lone instance's choice
+            // (0) or a fixed chunk
+            double best_rr = 1e300;
+            int pick_rr = 0;
+            for (int ich : {0, 2, 4, 8, 16, 32}) {
+                lvl_rr_[l] = ich;
+                resid_restrict(l);  // warm-up
+                float ms = 1e30f;
+                for (int trial = 0; trial < 5; ++trial) {
+                    NUMA_HIP_CHECK(hipEventRecord(e0, c_.stream));
+                    for (int rep = 0; rep < 3; ++rep) resid_restrict(l);
+                    NUMA_HIP_CHECK(hipEventRecord(e1, c_.stream));
+                    NUMA_HIP_CHECK(hipEventSynchronize(e1));
+                    float t = 0.0f;
+                    NUMA_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+                    ms = std::min(ms, t);
+                }
+                if (log)
+                    std::fprintf(stderr, "numa_hip batch tune: level %zu resid/restrict chunk %d %.2f us\n",
+                                 l, ich, ms * 1e3 / 3.0);
+                if (ms < 0.98 * best_rr) {
+                    best_rr = ms;
+                    pick_rr = ich;
+                }
+            }
+            lvl_rr_[l] = pick_rr;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            lvl_box_[l] = pick;
-            TuneCache::get().store(key.str(), std::to_string(pick));
+            TuneCache::get().store(key.str(), std::to_string(pick) + "," + std::to_string(pick_rr));
         }
         bytes_ = 0.0;
         launches_ = 0;
@@ -844,7 +925,8 @@ private:
     bool dry_ = false;
     bool in_step_ = false;
     bool tuned_ = false;
-    std::vector<int> lvl_box_;  // per level: 0 one launch per step, s > 0 box shape s
+    std::vector<int> lvl_box_;  // per level: smoothing form (0 one launch per step, set_smoothing)
+    std::vector<int> lvl_rr_;   // per level: residual / restriction chunk (0: per instance)
     bool graphs_ = false;
     struct Captured {
         hipGraphExec_t exec;

@@ -281,6 +281,25 @@ struct OpPolyBox {
     }
 };
 
+// One sweep of STEPS smoothing steps in the plane-marching form of
+// k_poly_fused (regions of 64 x 16R columns, `ichunk` planes per workgroup).
+template <class T, int STEPS, int R>
+struct OpPolyFused {
+    static constexpr int kThreads = kFusedThreads;
+    struct Args {
+        int32_t n, ichunk, tiles_k, tiles_j, chunks, pad;
+        double h2inv;
+        PolySteps cs;
+        const T* x;
+        const T* b;
+        T* xo;
+    };
+    __device__ static void run(const Args& a, GridPos g) {
+        poly_fused_body<T, STEPS, R, false>(g, a.n, a.ichunk, a.tiles_k, a.tiles_j, a.chunks,
+                                            a.h2inv, a.cs, a.x, a.b, a.xo, nullptr);
+    }
+};
+
 template <class T>
 struct OpResidRestrict {
     static constexpr int kThreads = kRRThreads;

@@ -155,11 +155,11 @@ constexpr int fused_waves_per_eu() {
 // with the smallest rows per thread and at most 3 steps (the variants that
 // hold the accumulator without spilling).
 template <class T, int STEPS, int R, bool NORM = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, kFusedThreads),
-                          amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R, NORM>()))) void
-k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2inv_d, PolySteps cs,
-             const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ xo,
-             double* __restrict__ norm_part) {
+__device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, int tiles_k,
+                                                int tiles_j, int chunks, double h2inv_d,
+                                                const PolySteps& cs, const T* __restrict__ x,
+                                                const T* __restrict__ b, T* __restrict__ xo,
+                                                double* __restrict__ norm_part) {
 #pragma clang fp contract(off)
     using A = SmoothArith<T>;
     using G = FusedGeom<STEPS, R>;
@@ -178,11 +178,11 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
 
     // XCD-aware block -> tile: the blocks an XCD receives (every 8th) walk
     // consecutive tiles, k fastest, so overlapping halos meet in one L2.
-    const int G8 = gridDim.x >> 3;
-    const int lin = (int)(blockIdx.x & 7) * G8 + (int)(blockIdx.x >> 3);
+    const int G8 = (int)(g.nb >> 3);
+    const int lin = (int)(g.bid & 7) * G8 + (int)(g.bid >> 3);
     const int total = tiles_k * tiles_j * chunks;
     if (lin >= total) {  // uniform per block, before any barrier
-        if (NORM && threadIdx.x == 0) norm_part[blockIdx.x] = 0.0;
+        if (NORM && threadIdx.x == 0) norm_part[g.bid] = 0.0;
         return;
     }
     {
@@ -327,8 +327,17 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
     });
     if constexpr (NORM) {
         const double t = block_sum<kFusedThreads>(nsq);
-        if (threadIdx.x == 0) norm_part[blockIdx.x] = t;
+        if (threadIdx.x == 0) norm_part[g.bid] = t;
     }
+}
+template <class T, int STEPS, int R, bool NORM = false>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kFusedThreads),
+                          amdgpu_waves_per_eu(fused_waves_per_eu<T, STEPS, R, NORM>()))) void
+k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2inv_d, PolySteps cs,
+             const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ xo,
+             double* __restrict__ norm_part) {
+    poly_fused_body<T, STEPS, R, NORM>(hw_grid(), n, ichunk, tiles_k, tiles_j, chunks, h2inv_d, cs,
+                                       x, b, xo, norm_part);
 }
 
 // Launch geometry of k_poly_fused for an n^3 level.

@@ -624,10 +624,13 @@ int numa_hip_batch_create(numa_hip_ctx* ctx, int64_t count, const int64_t* n,
 int numa_hip_batch_destroy(numa_hip_batch* h);
 int numa_hip_batch_set_graphs(numa_hip_batch* h, int64_t enable);
 /* Smoothing form of multigrid levels 0 .. count-1: 0 one launch per smoothing
- * step, s in 1..3 one launch per sweep in box shape s (8x8x16, 4x8x16,
- * 4x4x16 output nodes), bit for bit the same values.  Without this call the
- * batch times the forms per level at its first solve (cfg->mg.fuse !=
- * NUMA_FUSE_NONE) and keeps the fastest (cached under the level's sizes). */
+ * step; one launch per sweep in the box form, 1..3 = box shape 8x8x16,
+ * 4x8x16, 4x4x16 output nodes; or in the plane-marching form, 4 / 5 = 2 rows
+ * per thread and 16 / 32 planes per workgroup, 6 / 7 = 4 rows and 16 / 32
+ * planes.  Every form gives the same values bit for bit.  Without this call
+ * the batch times the forms (and the residual / restriction plane chunk) per
+ * level at its first solve (cfg->mg.fuse != NUMA_FUSE_NONE) and keeps the
+ * fastest (cached under the level's grid sizes). */
 int numa_hip_batch_set_smoothing(numa_hip_batch* h, int64_t count, const int64_t* form);
 /* Solves instance i from x_dev[i] (device, FP64, n_i^3, overwritten by the
  * solution) on the context's stream; results[i] its KrylovResult; *seconds

@@ -126,7 +126,7 @@ def test_batch_rejects_unsupported(ctx):
         H.Batch(ctx, [16], [6.0], bad)
 
 
-@pytest.mark.parametrize("shape", [1, 2, 3])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7])
 def test_batch_box_smoothing_is_bitwise(ctx, shape):
     """Every box shape of the batched smoothing on every level equals the
     single-instance solver's one launch per smoothing step."""

@@ -17,7 +17,11 @@ oracle on the same inputs.
 * The batched sweep: 96 instances, n from {32, 48, 64, 96}, lambda in
   [1, 6.7], u0 = 0.1 U[0,1): every instance of the multi-worker graph-replay
   sweep equals the same instance solved alone, bit for bit; a seeded sample
-  of instances against the oracle (1e-10, Newton +-1).
+  of instances against the oracle (1e-10, Newton +-1).  The same 96
+  instances in one batched solve (one launch per operation over the instance
+  table, one graph launch per lockstep Arnoldi step, autotuned smoothing
+  forms): every instance equals the single-instance solver with unfused
+  smoothing, bit for bit, and a seeded sample agrees with the oracle.
 
 Solves that are compared with the oracle across different Krylov paths (mixed
 vs FP64, FGMRES vs GMRES) use the final-solve safeguard (forcing term capped
@@ -209,6 +213,55 @@ def test_sweep_96_instances(ctx, oracle):
     o64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
                           mg=H.mg_cheb(levels=5, coarse_iters=-1))
+    for i in sample:
+        xo, ro = oracle.bratu_solve(int(n[i]), float(lam[i]), _ocfg(o64), _oparams(p),
+                                    x0=x0[i])
+        e = rel(got[i], xo)
+        print(f"  instance {i}: n={int(n[i])} lambda={lam[i]:.4f} newton {res[i].iters} "
+              f"(oracle {ro.iters}), rel-L2 {e:.2e}")
+        assert ro.converged
+        assert abs(res[i].iters - ro.iters) <= 1
+        assert e <= 1e-10, (i, e)
+
+
+def test_batched_sweep_96_instances(ctx, oracle):
+    n, lam = H.sweep_instances(SW_SEED, SW_COUNT, sizes=SW_SIZES)
+    x0 = [H.uniform_host(SW_SEED * 7919 + 13 + i, int(n[i]) ** 3, 0.1) for i in range(SW_COUNT)]
+    p = H.krylov_params(gmres_restart=30, gmres_max_iters=600, **SAFEGUARD)
+    cfg = _sweep_cfg()
+    print(f"batched sweep: {SW_COUNT} instances, graphs, autotuned smoothing", flush=True)
+    xs = [ctx.array(v) for v in x0]
+    b = H.Batch(ctx, n, lam, cfg, restart=30)
+    b.set_graphs(True)
+    for rep in range(2):  # the second solve replays the captured steps
+        for x, v in zip(xs, x0):
+            x.upload(v)
+        res, sec = b.solve(xs, p)
+    st = b.stats()
+    b.close()
+    got = [x.numpy() for x in xs]
+    for x in xs:
+        x.free()
+    assert all(r.converged for r in res)
+    assert st["graph_launches"] > 0
+    print(f"  {SW_COUNT / sec:.0f} solves/s, {st}; solving each instance alone", flush=True)
+    alone = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
+                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=-1))
+    for i in range(SW_COUNT):
+        s = H.BratuSolver(ctx, int(n[i]), float(lam[i]), alone, restart=30)
+        xi = x0[i].copy()
+        r = s.solve_host(p, xi)
+        s.close()
+        np.testing.assert_array_equal(got[i], xi, err_msg=f"instance {i} (n={int(n[i])})")
+        assert _counters(r) == _counters(res[i]), i
+    o64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
+                          precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
+                          mg=H.mg_cheb(levels=5, coarse_iters=-1))
+    rng = np.random.default_rng(11)
+    sample = [int(np.flatnonzero(n == k)[-1]) for k in SW_SIZES[:3]]
+    rest = np.setdiff1d(np.flatnonzero(n <= 64), sample)
+    sample += [int(i) for i in rng.choice(rest, 2, replace=False)]
     for i in sample:
         xo, ro = oracle.bratu_solve(int(n[i]), float(lam[i]), _ocfg(o64), _oparams(p),
                                     x0=x0[i])
