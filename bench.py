@@ -5,17 +5,21 @@ A step is one complete nonlinear solve of Bratu3D (n = 256 interior nodes per
 axis, lambda = 6.5, u0 = 0, ||F|| <= 1e-8 ||F(u0)||) with the device-resident
 Newton-Krylov solver: FD J v, GMRES(30) / FGMRES(30), 5-level GMG V(2,2) with
 the degree-3 Chebyshev smoother.  Three legs run on the same GPU:
-  * fp64:  plain FP64 GMG-GMRES, one kernel per smoothing step (the speed-up
-           denominator);
+  * fp64:  plain FP64 GMG-GMRES with the reference's forcing, one kernel per
+           smoothing step (the speed-up denominator);
   * mixed: FGMRES with FP32 Krylov basis / J v / V-cycle and FP64 F(x),
            difference quotient, Newton update and Givens solve, smoothing
-           sweeps temporally blocked (autotuned per level) (the value);
-  * fp64_fused: the FP64 method with the mixed leg's smoothing kernels
-           (reported next to the value).
-The timed region holds only solves, inputs resident in HBM.  Rank 0 prints
-one JSON line.  The CPU baseline is the FP64 CPU oracle (OpenMP) solving the
-same problem on the host, timed once; its solution is also the parity
-reference for the GPU solutions.
+           sweeps temporally blocked (autotuned per level), with the
+           final-solve safeguard so that its solution is within 1e-10 of the
+           FP64 reference's whatever Krylov path it takes (the value);
+  * fp64_fused: the FP64 method with the mixed leg's smoothing kernels and
+           the same safeguard (reported next to the value).
+`same_policy` times the mixed leg with the reference's forcing and the fp64
+leg with the safeguard, so both ratios are on record.  The timed region holds
+only solves, inputs resident in HBM.  Rank 0 prints one JSON line.  The CPU
+baseline is the FP64 CPU oracle (OpenMP) solving the same problem on the
+host; its solutions (reference forcing, and safeguard) are the parity
+references of the GPU solutions with the same policy.
 """
 from __future__ import annotations
 
@@ -44,9 +48,12 @@ def log(msg: str) -> None:
 
 def make_params(H, forcing: str, safeguard: bool = False):
     """The reference's KrylovParams with GMRES(30); safeguard: the final-solve
-    forcing cap (eta <= 1e-6 once ||F|| <= 1e-6 ||F0||, converged only once the
+    forcing cap (eta <= 1e-3 once ||F|| <= 1e-6 ||F0||, converged only once the
     last step started there), which makes the final iterate independent of
-    the Krylov path."""
+    the Krylov path: every path lands within ~6e-12 (relative L2) of the
+    converged solution at 256^3, so two solvers on different paths agree to
+    1e-10, at 4 more FGMRES iterations than the reference's forcing (whose
+    final iterate is only ~3e-10 from the converged solution)."""
     extra = dict(tight_res=SAFE_RES, tight_eta=SAFE_ETA) if safeguard else {}
     if forcing == "constant":
         return H.krylov_params(gmres_restart=30, gmres_max_iters=600, eta_init=1e-3,
@@ -54,7 +61,7 @@ def make_params(H, forcing: str, safeguard: bool = False):
     return H.krylov_params(gmres_restart=30, gmres_max_iters=600, **extra)
 
 
-SAFE_RES, SAFE_ETA = 1e-6, 1e-6
+SAFE_RES, SAFE_ETA = 1e-6, 1e-3
 
 
 # Coarsest level (16^3 at 256^3 with 5 levels) solved exactly by sine
@@ -605,7 +612,10 @@ def main() -> int:
                     help="run only this leg and print its JSON (for profiling)")
     ap.add_argument("--no-config0", action="store_true", help="skip the 32^3 plain-method leg")
     ap.add_argument("--no-safeguard", action="store_true",
-                    help="skip the legs run with the final-solve safeguard")
+                    help="skip the legs timed with the other forcing policy")
+    ap.add_argument("--reference-forcing", action="store_true",
+                    help="time every leg with the reference's forcing (no final-solve "
+                         "safeguard); parity then holds only along the CPU's Krylov path")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-safeguard", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--config0-cpu-leg", action="store_true", help=argparse.SUPPRESS)
@@ -643,11 +653,14 @@ def main() -> int:
         ctx.close()
         return 0
     cfgs = leg_configs(H, args.levels, args.smooth_rows)
+    psafe = make_params(H, args.forcing, safeguard=not args.reference_forcing)
+    leg_params = {"fp64": params, "mixed": psafe, "fp64_fused": psafe}
     legs = {}
     for name in args.legs.split(","):
         log(f"leg {name}: n={args.n} lambda={args.lam}")
-        s, x, res, times = time_leg(H, ctx, args.n, args.lam, cfgs[name], params, args.steps,
-                                    args.warmup, graphs=args.graphs and name != "fp64")
+        s, x, res, times = time_leg(H, ctx, args.n, args.lam, cfgs[name], leg_params[name],
+                                    args.steps, args.warmup,
+                                    graphs=args.graphs and name != "fp64")
         legs[name] = {"solver": s, "x": x, "res": res, "times": times,
                       "prims": primitive_table(H, s), "fusion": s.mg_fusion()}
         log(f"  primitives: " + ", ".join(f"{k} {v['ms']:.3f} ms {v['GBs']:.0f} GB/s"
@@ -706,6 +719,8 @@ def main() -> int:
                       f"{'fgmres' if main_leg == 'mixed' else 'gmres'}30_gmg{args.levels}"
                       "_cheb3_v22_fd_jv", "n": args.n, "lambda": args.lam,
                       "tol": 1e-8, "forcing": args.forcing, "levels": args.levels,
+                      "final_solve_safeguard": None if args.reference_forcing else
+                      {"tight_res": SAFE_RES, "tight_eta": SAFE_ETA},
                       "coarse_solve": "exact (sine transforms)" if COARSE < 0 else
                       f"{COARSE} Chebyshev steps"},
            "newton_iters": int(r.iters), "gmres_iters": int(r.n_lin_iters),
@@ -726,55 +741,70 @@ def main() -> int:
         out["fp64_fused_smoothing_s"] = round(tf, 6)
         out["speedup_vs_fp64_fused_smoothing"] = round(tf / t_main, 3)
 
-    if not args.no_safeguard:
-        # the same legs with the final-solve safeguard: their solutions no longer
-        # depend on the Krylov path, so each is compared with a CPU FP64 solve
-        # that takes its own path
-        ps = make_params(H, args.forcing, safeguard=True)
-        safe = {"params": {"tight_res": SAFE_RES, "tight_eta": SAFE_ETA}}
-        for name in ("mixed", "fp64"):
+    # The other policy for each leg: mixed with the reference's forcing (its
+    # solution is only as close to the FP64 reference's as the tolerance
+    # makes it: both stop ~3e-10 from the converged solution, on different
+    # Krylov paths) and fp64 with the safeguard (the like-for-like
+    # denominator of the value).
+    other = {}
+    if not args.no_safeguard and not args.reference_forcing:
+        for name, pp in (("mixed", params), ("fp64", psafe)):
             if name not in legs:
                 continue
-            s2, x2, r2, t2 = time_leg(H, ctx, args.n, args.lam, cfgs[name], ps, args.steps,
+            s2, x2, r2, t2 = time_leg(H, ctx, args.n, args.lam, cfgs[name], pp, args.steps,
                                       args.warmup, graphs=args.graphs and name != "fp64")
-            safe[name] = {"s": round(float(np.mean(t2)), 6), "newton_iters": int(r2.iters),
-                          "gmres_iters": int(r2.n_lin_iters),
-                          "final_rel_residual": r2.res / r2.res0, "x": x2.numpy()}
+            other[name] = {"s": round(float(np.mean(t2)), 6), "newton_iters": int(r2.iters),
+                           "gmres_iters": int(r2.n_lin_iters),
+                           "final_rel_residual": r2.res / r2.res0, "x": x2.numpy()}
             s2.close()
             x2.free()
-        if "mixed" in safe and "fp64" in safe:
-            safe["speedup_vs_fp64"] = round(safe["fp64"]["s"] / safe["mixed"]["s"], 3)
-        if not args.no_cpu:
-            info_s, path_s = run_cpu_baseline(args, safeguard=True)
-            xs = np.load(path_s)
-            os.remove(path_s)
-            safe["cpu_fp64"] = {"s": round(info_s["seconds"], 3),
-                                "newton_iters": info_s["result"]["iters"],
-                                "gmres_iters": info_s["result"]["n_lin_iters"],
-                                "final_rel_residual": info_s["result"]["res"] /
-                                info_s["result"]["res0"]}
-            for name in ("mixed", "fp64"):
-                if name in safe:
-                    safe[name]["rel_l2_vs_cpu_fp64"] = float(
-                        np.linalg.norm(safe[name]["x"] - xs) / np.linalg.norm(xs))
-            args._safe_solution = xs
-        for name in ("mixed", "fp64"):
-            if name in safe:
-                safe[name].pop("x")
-        out["safeguarded"] = safe
-        log(f"safeguarded legs: {safe}")
+        same = {"params": {"tight_res": SAFE_RES, "tight_eta": SAFE_ETA}}
+        if "mixed" in other:
+            same["mixed_reference_forcing"] = other["mixed"]
+            if "fp64" in legs:
+                same["mixed_reference_forcing"]["speedup_vs_fp64"] = round(
+                    float(np.mean(legs["fp64"]["times"])) / other["mixed"]["s"], 3)
+        if "fp64" in other:
+            same["fp64_safeguard"] = other["fp64"]
+            out["speedup_vs_fp64_same_policy"] = round(other["fp64"]["s"] / t_main, 3)
+        out["same_policy"] = same
 
     if not args.no_cpu:
         info, path = run_cpu_baseline(args)
         xc = np.load(path)
         os.remove(path)
-        parity = {"newton_iters_cpu_fp64": info["result"]["iters"]}
+        parity = {"newton_iters_cpu_fp64": info["result"]["iters"],
+                  "gmres_iters_cpu_fp64": info["result"]["n_lin_iters"]}
+        xs_cpu = None
+        if not args.reference_forcing:
+            info_s, path_s = run_cpu_baseline(args, safeguard=True)
+            xs_cpu = np.load(path_s)
+            os.remove(path_s)
+            parity["cpu_fp64_safeguard"] = {"s": round(info_s["seconds"], 3),
+                                            "newton_iters": info_s["result"]["iters"],
+                                            "gmres_iters": info_s["result"]["n_lin_iters"],
+                                            "final_rel_residual": info_s["result"]["res"] /
+                                            info_s["result"]["res0"]}
+            args._safe_solution = xs_cpu
+        def rl2(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        # every GPU solution against the CPU FP64 solve with the same policy
         for name, leg in legs.items():
             xg = leg["x"].numpy()
-            parity[f"{name}_rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(xg - xc) /
-                                                        np.linalg.norm(xc))
+            ref = xs_cpu if (leg_params[name] is psafe and xs_cpu is not None) else xc
+            parity[f"{name}_rel_l2_vs_cpu_fp64"] = rl2(xg, ref)
             parity[f"{name}_newton_iters"] = int(leg["res"].iters)
+            parity[f"{name}_policy"] = "safeguard" if ref is xs_cpu else "reference forcing"
+        for name, key, ref in (("mixed", "mixed_reference_forcing", xc),
+                               ("fp64", "fp64_safeguard", xs_cpu)):
+            if name in other and ref is not None:
+                out["same_policy"][key]["rel_l2_vs_cpu_fp64"] = rl2(other[name]["x"], ref)
+        if "mixed" in legs and xs_cpu is not None:
+            parity["mixed_rel_l2_vs_cpu_fp64_reference_forcing"] = rl2(legs["mixed"]["x"].numpy(), xc)
+            parity["cpu_safeguard_vs_cpu_reference_forcing"] = rl2(xs_cpu, xc)
         out["parity"] = parity
+    for v in other.values():
+        v.pop("x", None)
         out["cpu_baseline"] = {"value": round(info["seconds"], 3), "unit": "s",
                                "cores": info["threads"], "kind": "port",
                                "sample": f"one full FP64 solve of the same workload "
