@@ -67,6 +67,12 @@ SAFE_RES, SAFE_ETA = 1e-6, 1e-3
 # Coarsest level (16^3 at 256^3 with 5 levels) solved exactly by sine
 # transforms in every leg.
 COARSE = -1
+# Chebyshev smoothing interval [lmax / CHEB_RATIO, lmax] of D^-1 A in every
+# leg of the 256^3 workload: at 256^3 lambda = 6.5 (CPU oracle, FP64) ratio 4
+# takes 5 Newton / 18 GMRES iterations with the reference's forcing (22 with
+# the safeguard), ratio 30 takes 4 / 15 (19); 12: 21, 20: 20, 40: 21, 60: 23
+# with the safeguard.
+CHEB_RATIO = 30.0
 
 
 def leg_configs(H, levels: int, smooth_rows=None):
@@ -75,13 +81,13 @@ def leg_configs(H, levels: int, smooth_rows=None):
     FP64 method with the same smoothing kernels as mixed (reported, so that the
     share of the speed-up that comes from precision is visible)."""
     plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE,
-                      coarse_iters=COARSE)
+                      coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
     if smooth_rows is None:
         fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO,
-                          coarse_iters=COARSE)
+                          coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
     else:
         fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_SWEEP,
-                          rows=smooth_rows, coarse_iters=COARSE)
+                          rows=smooth_rows, coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
     fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                            precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=plain)
     mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
@@ -803,8 +809,6 @@ def main() -> int:
             parity["mixed_rel_l2_vs_cpu_fp64_reference_forcing"] = rl2(legs["mixed"]["x"].numpy(), xc)
             parity["cpu_safeguard_vs_cpu_reference_forcing"] = rl2(xs_cpu, xc)
         out["parity"] = parity
-    for v in other.values():
-        v.pop("x", None)
         out["cpu_baseline"] = {"value": round(info["seconds"], 3), "unit": "s",
                                "cores": info["threads"], "kind": "port",
                                "sample": f"one full FP64 solve of the same workload "
@@ -813,6 +817,8 @@ def main() -> int:
                                "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                                "newton_iters": info["result"]["iters"],
                                "gmres_iters": info["result"]["n_lin_iters"]}
+    for v in other.values():
+        v.pop("x", None)
     if not args.no_chandra:
         ch = chandra_leg(H, ctx, args)
         if not args.no_cpu:

@@ -593,9 +593,9 @@ constexpr int box_edge_k(int) { return 16; }
 // of the box form: 2 copies of the box grown by STEPS on every side.
 template <class T>
 constexpr bool fused_fits(int steps, int rows) {
-    if (rows < 0) {  // box form: up to 3 steps (one sweep of degree <= 3)
+    if (rows < 0) {  // box form: up to kFusedMaxSteps steps (a phase of sweeps)
         const int sh = -rows;
-        if (sh > kBoxShapes || steps > 3) return false;
+        if (sh > kBoxShapes || steps > kFusedMaxSteps) return false;
         return 2 * (int64_t)(box_edge_i(sh) + 2 * steps) * (box_edge_j(sh) + 2 * steps) *
                    (box_edge_k(sh) + 2 * steps) * (int64_t)sizeof(T) <=
                160 * 1024;
@@ -1247,7 +1247,7 @@ private:
             Level& L = levels_[l];
             if (L.fuse != NUMA_FUSE_AUTO) continue;
             std::ostringstream key;
-            key << dev << "|mg-smooth-v3|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
+            key << dev << "|mg-smooth-v4|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
                 << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
             std::string val;
             const auto held = TuneCache::get().tuning(key.str());

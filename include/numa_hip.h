@@ -130,7 +130,7 @@ typedef struct numa_mg_config {
     int64_t ichunk;          /* planes per workgroup of fused launches (0 = 32)    */
     int64_t rows;            /* rows per thread of fused launches (region 64 x 16
                                 rows): 0 = 2 (FP32) / 1 (FP64), or twice that;
-                                -1 / -2 / -3: box form (<= 3 steps per launch),
+                                -1 / -2 / -3: box form (<= 6 steps per launch),
                                 one workgroup per 8x8x16 / 4x8x16 / 4x4x16 (i x j
                                 x k) output nodes, ichunk unused               */
 } numa_mg_config;
