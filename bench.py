@@ -68,26 +68,31 @@ SAFE_RES, SAFE_ETA = 1e-6, 1e-3
 # transforms in every leg.
 COARSE = -1
 # Chebyshev smoothing interval [lmax / CHEB_RATIO, lmax] of D^-1 A in every
-# leg of the 256^3 workload: at 256^3 lambda = 6.5 (CPU oracle, FP64) ratio 4
-# takes 5 Newton / 18 GMRES iterations with the reference's forcing (22 with
-# the safeguard), ratio 30 takes 4 / 15 (19); 12: 21, 20: 20, 40: 21, 60: 23
-# with the safeguard.
-CHEB_RATIO = 30.0
+# leg of the 256^3 workload, the best for the plain FP64 leg and for the
+# value.  256^3 lambda = 6.5 on the MI355X (Newton / GMRES iterations):
+#   ratio  plain FP64   mixed + safeguard
+#     4      5 / 18      5 / 22   15.0 ms
+#    12      4 / 15      5 / 21   14.1 ms
+#    20      4 / 15      5 / 20   13.4 ms
+#    30      4 / 15      5 / 21   14.2 ms
+# (the CPU oracle, FP64: 40 -> 4 / 16 plain, 60 -> 5 / 23 with the safeguard).
+CHEB_RATIO = 20.0
 
 
-def leg_configs(H, levels: int, smooth_rows=None):
+def leg_configs(H, levels: int, smooth_rows=None, ratio: float = None):
     """fp64: the plain method (one kernel per smoothing step); mixed: FGMRES in
     mixed precision with autotuned temporally blocked smoothing; fp64_fused: the
     FP64 method with the same smoothing kernels as mixed (reported, so that the
     share of the speed-up that comes from precision is visible)."""
+    ratio = CHEB_RATIO if ratio is None else ratio
     plain = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_NONE,
-                      coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
+                      coarse_iters=COARSE, cheb_ratio=ratio)
     if smooth_rows is None:
         fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_AUTO,
-                          coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
+                          coarse_iters=COARSE, cheb_ratio=ratio)
     else:
         fused = H.mg_cheb(levels=levels, degree=3, pre=2, post=2, fuse=H.FUSE_SWEEP,
-                          rows=smooth_rows, coarse_iters=COARSE, cheb_ratio=CHEB_RATIO)
+                          rows=smooth_rows, coarse_iters=COARSE, cheb_ratio=ratio)
     fp64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                            precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD, mg=plain)
     mixed = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
@@ -168,7 +173,7 @@ def cpu_leg(args) -> None:
     import numpy as np
     import numa_hip as H
     import oracle as O
-    cfg = leg_configs(H, args.levels)["fp64"]
+    cfg = leg_configs(H, args.levels, ratio=args.cheb_ratio)["fp64"]
     m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
     ocfg = O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=0, flexible=0,
                           jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
@@ -190,7 +195,8 @@ def run_cpu_baseline(args, safeguard: bool = False):
     out = os.path.join(tempfile.gettempdir(), f"numa_cpu_solution_{os.getpid()}_{int(safeguard)}.npy")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--n", str(args.n), "--lam",
            str(args.lam), "--levels", str(args.levels), "--forcing", args.forcing,
-           "--cpu-out", out] + (["--cpu-safeguard"] if safeguard else [])
+           "--cpu-out", out, "--cheb-ratio", str(args.cheb_ratio)] + (
+               ["--cpu-safeguard"] if safeguard else [])
     log(f"cpu baseline: FP64 oracle solve at n={args.n} with {threads} threads"
         f"{' (final-solve safeguard)' if safeguard else ''}")
     p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=args.cpu_timeout)
@@ -619,6 +625,8 @@ def main() -> int:
     ap.add_argument("--no-config0", action="store_true", help="skip the 32^3 plain-method leg")
     ap.add_argument("--no-safeguard", action="store_true",
                     help="skip the legs timed with the other forcing policy")
+    ap.add_argument("--cheb-ratio", type=float, default=CHEB_RATIO,
+                    help="Chebyshev smoothing interval [lmax/ratio, lmax] of the 256^3 legs")
     ap.add_argument("--reference-forcing", action="store_true",
                     help="time every leg with the reference's forcing (no final-solve "
                          "safeguard); parity then holds only along the CPU's Krylov path")
@@ -658,7 +666,7 @@ def main() -> int:
         print(json.dumps({args.only: part}))
         ctx.close()
         return 0
-    cfgs = leg_configs(H, args.levels, args.smooth_rows)
+    cfgs = leg_configs(H, args.levels, args.smooth_rows, args.cheb_ratio)
     psafe = make_params(H, args.forcing, safeguard=not args.reference_forcing)
     leg_params = {"fp64": params, "mixed": psafe, "fp64_fused": psafe}
     legs = {}
