@@ -478,7 +478,8 @@ def sweep_leg(H, ctx, args, params) -> dict:
     """Batched sweep: 96 Bratu3D instances, n from {32,48,64,96} and lambda ~
     U[1.0, 6.7] drawn with std::mt19937_64(20261203), u0 = 0.1 U[0,1) from
     mt19937_64(seed*7919 + 13 + i); mixed Newton-FGMRES(30) + GMG (the main
-    leg's configuration, autotuned fused smoothing).  One eager worker (the
+    leg's configuration: smoothing interval ratio 20, autotuned fused
+    smoothing).  One eager worker (the
     instances one after another) next to W workers with captured Arnoldi
     steps; each configuration is run once untimed (allocation, tuning,
     capture) and once timed.  Then the batched solver: all 96 instances in
@@ -493,7 +494,8 @@ def sweep_leg(H, ctx, args, params) -> dict:
     xs = [ctx.array(x) for x in x0]
     cfg = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                           precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
-                          mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=COARSE))
+                          mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=COARSE,
+                                       cheb_ratio=CHEB_RATIO))
     out = {"workload": f"bratu3d_sweep_{SWEEP_COUNT}_instances_n{'-'.join(map(str, SWEEP_SIZES))}"
                        f"_lambda1.0-6.7_u0_0.1noise_mixed_newton_fgmres30_gmg",
            "seed": SWEEP_SEED, "sizes": {str(k): int((n == k).sum()) for k in SWEEP_SIZES}}
@@ -546,7 +548,8 @@ def sweep_leg(H, ctx, args, params) -> dict:
     # bit for bit the single-instance solver with unfused smoothing (sample)
     alone = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
-                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=COARSE))
+                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=COARSE,
+                                         cheb_ratio=CHEB_RATIO))
     same = []
     for k in SWEEP_SIZES:
         i = int(np.flatnonzero(n == k)[0])

@@ -42,6 +42,7 @@ pytestmark = pytest.mark.gpu
 
 KEYS = ("converged", "iters", "n_F", "n_JVP", "n_D2", "n_lin_iters", "n_lin_failures", "n_prec")
 SAFEGUARD = dict(tight_res=1e-6, tight_eta=1e-6)
+SW_RATIO = 20.0  # Chebyshev smoothing interval [lmax/20, lmax] (256^3 and sweep), as the benchmark
 
 
 def _ocfg(cfg):
@@ -89,7 +90,7 @@ def _big_params():
 
 def _big_cfg(method, precision):
     mixed = precision == H.PRECISION_MIXED
-    mg = H.mg_cheb(levels=5, degree=3, pre=2, post=2, coarse_iters=-1,
+    mg = H.mg_cheb(levels=5, degree=3, pre=2, post=2, coarse_iters=-1, cheb_ratio=SW_RATIO,
                    fuse=H.FUSE_AUTO if mixed else H.FUSE_NONE)
     return H.solver_config(method=method, precond=H.PRECOND_GMG, precision=precision,
                            flexible=1 if mixed else 0, jv_mode=H.DERIV_FD, d2_mode=H.DERIV_FD,
@@ -179,7 +180,8 @@ SW_SEED, SW_COUNT, SW_SIZES = 20261203, 96, (32, 48, 64, 96)
 def _sweep_cfg():
     return H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                            precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
-                           mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=-1))
+                           mg=H.mg_cheb(levels=5, fuse=H.FUSE_AUTO, coarse_iters=-1,
+                                        cheb_ratio=SW_RATIO))
 
 
 def test_sweep_96_instances(ctx, oracle):
@@ -212,7 +214,7 @@ def test_sweep_96_instances(ctx, oracle):
     sample += [int(i) for i in rng.choice(np.flatnonzero(n <= 64), 2, replace=False)]
     o64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
-                          mg=H.mg_cheb(levels=5, coarse_iters=-1))
+                          mg=H.mg_cheb(levels=5, coarse_iters=-1, cheb_ratio=SW_RATIO))
     for i in sample:
         xo, ro = oracle.bratu_solve(int(n[i]), float(lam[i]), _ocfg(o64), _oparams(p),
                                     x0=x0[i])
@@ -247,7 +249,8 @@ def test_batched_sweep_96_instances(ctx, oracle):
     print(f"  {SW_COUNT / sec:.0f} solves/s, {st}; solving each instance alone", flush=True)
     alone = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
-                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=-1))
+                            mg=H.mg_cheb(levels=5, fuse=H.FUSE_NONE, coarse_iters=-1,
+                                         cheb_ratio=SW_RATIO))
     for i in range(SW_COUNT):
         s = H.BratuSolver(ctx, int(n[i]), float(lam[i]), alone, restart=30)
         xi = x0[i].copy()
@@ -257,7 +260,7 @@ def test_batched_sweep_96_instances(ctx, oracle):
         assert _counters(r) == _counters(res[i]), i
     o64 = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                           precision=H.PRECISION_FP64, flexible=0, jv_mode=H.DERIV_FD,
-                          mg=H.mg_cheb(levels=5, coarse_iters=-1))
+                          mg=H.mg_cheb(levels=5, coarse_iters=-1, cheb_ratio=SW_RATIO))
     rng = np.random.default_rng(11)
     sample = [int(np.flatnonzero(n == k)[-1]) for k in SW_SIZES[:3]]
     rest = np.setdiff1d(np.flatnonzero(n <= 64), sample)
