@@ -474,7 +474,7 @@ def learned_leg(H, ctx, args, params) -> dict:
 SWEEP_SEED, SWEEP_COUNT, SWEEP_SIZES = 20261203, 96, (32, 48, 64, 96)
 
 
-def sweep_leg(H, ctx, args, params) -> dict:
+def sweep_leg(H, ctx, args, params, batch_only: bool = False) -> dict:
     """Batched sweep: 96 Bratu3D instances, n from {32,48,64,96} and lambda ~
     U[1.0, 6.7] drawn with std::mt19937_64(20261203), u0 = 0.1 U[0,1) from
     mt19937_64(seed*7919 + 13 + i); mixed Newton-FGMRES(30) + GMG (the main
@@ -500,7 +500,7 @@ def sweep_leg(H, ctx, args, params) -> dict:
                        f"_lambda1.0-6.7_u0_0.1noise_mixed_newton_fgmres30_gmg",
            "seed": SWEEP_SEED, "sizes": {str(k): int((n == k).sum()) for k in SWEEP_SIZES}}
     sols = None
-    for workers, graphs in ((1, False), (4, True), (8, True)):
+    for workers, graphs in (() if batch_only else ((1, False), (4, True), (8, True))):
         sw = H.Sweep(ctx, cfg, workers=workers, restart=30)
         for rep_ in range(2):
             for x, v in zip(xs, x0):
@@ -544,7 +544,8 @@ def sweep_leg(H, ctx, args, params) -> dict:
         "other_launches": st["launches"], "host_syncs": st["host_syncs"],
         "algorithmic_bytes": st["bytes"], "achieved_GBs": round(st["bytes"] / sec / 1e9, 1),
         "frac_of_8000": round(st["bytes"] / sec / 1e9 / HBM_PEAK_GBS, 4),
-        "max_abs_diff_vs_workers1": float(max(np.max(np.abs(a - b_)) for a, b_ in zip(cur, sols)))}
+        "max_abs_diff_vs_workers1": None if sols is None else
+        float(max(np.max(np.abs(a - b_)) for a, b_ in zip(cur, sols)))}
     # bit for bit the single-instance solver with unfused smoothing (sample)
     alone = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD,
@@ -623,8 +624,9 @@ def main() -> int:
                     help="pin the fused legs' smoothing to one launch per sweep with this "
                          "rows-per-thread value (2 or 4 for FP32 march, -1..-3 box) on every "
                          "level, plane chunks from the wave-filling model, instead of autotuning")
-    ap.add_argument("--only", choices=["chandra", "learned", "sweep"], default=None,
-                    help="run only this leg and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["chandra", "learned", "sweep", "batch"], default=None,
+                    help="run only this leg and print its JSON (for profiling); batch: the "
+                         "sweep's batched solver alone")
     ap.add_argument("--no-config0", action="store_true", help="skip the 32^3 plain-method leg")
     ap.add_argument("--no-safeguard", action="store_true",
                     help="skip the legs timed with the other forcing policy")
@@ -665,7 +667,8 @@ def main() -> int:
     if args.only:
         part = {"chandra": lambda: chandra_leg(H, ctx, args),
                 "learned": lambda: learned_leg(H, ctx, args, params),
-                "sweep": lambda: sweep_leg(H, ctx, args, params)}[args.only]()
+                "sweep": lambda: sweep_leg(H, ctx, args, params),
+                "batch": lambda: sweep_leg(H, ctx, args, params, batch_only=True)}[args.only]()
         print(json.dumps({args.only: part}))
         ctx.close()
         return 0
