@@ -19,6 +19,8 @@
 #include <cstdlib>
 #include <utility>
 
+#include <hip/hip_ext.h>
+
 #include "blas.hpp"
 #include "smooth_fused.hpp"
 #include "stencil.hpp"
@@ -560,11 +562,21 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_lds(int n, double h2i
 // Row counts: {2, 4} for FP32, {1, 2} for FP64 storage.
 // With norm_part, every workgroup also writes the sum of squares of the x it
 // stores there; returns the number of workgroups.
+// With ev0 / ev1 (no sum of squares) the launch is bracketed by events that
+// the dispatch itself stamps (hipExtLaunchKernelGGL: the kernel's own start
+// and end, without the queue's dispatch latency).
 template <class T, int S, int R>
 inline unsigned launch_fused_sr(Ctx& c, int n, int ichunk, double h2inv, const PolySteps& cs,
-                                const T* x, const T* b, T* xo, double* norm_part) {
+                                const T* x, const T* b, T* xo, double* norm_part,
+                                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
     int tk, tj, ch;
     const dim3 g = fused_grid<S, R>(n, ichunk, tk, tj, ch);
+    if (ev0 && !norm_part) {
+        hipExtLaunchKernelGGL((k_poly_fused<T, S, R, false>), g, dim3(kFusedThreads), 0, c.stream,
+                              ev0, ev1, 0, n, ichunk, tk, tj, ch, h2inv, cs, x, b, xo, norm_part);
+        NUMA_HIP_LAUNCH_CHECK();
+        return g.x;
+    }
     if (norm_part) {
         // the variants that hold the FP64 accumulator without spilling
         if constexpr (S <= 3 && R == (sizeof(T) == 4 ? 2 : 1))
@@ -627,7 +639,8 @@ inline unsigned launch_box_ss(Ctx& c, int n, double h2inv, const PolySteps& cs, 
 template <class T, int S>
 inline unsigned launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv,
                                const PolySteps& cs, const T* x, const T* b, T* xo,
-                               double* norm_part) {
+                               double* norm_part, hipEvent_t ev0 = nullptr,
+                               hipEvent_t ev1 = nullptr) {
     constexpr int R0 = fused_rows_small<T>();
     switch (rows) {
         case -1: return launch_box_ss<T, S, 1>(c, n, h2inv, cs, x, b, xo, norm_part);
@@ -635,10 +648,12 @@ inline unsigned launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv
         case -3: return launch_box_ss<T, S, 3>(c, n, h2inv, cs, x, b, xo, norm_part);
         default: break;
     }
-    if (rows == R0) return launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part);
+    if (rows == R0)
+        return launch_fused_sr<T, S, R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
     if constexpr (fused_fits<T>(S, 2 * R0)) {
         if (rows == 2 * R0)
-            return launch_fused_sr<T, S, 2 * R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part);
+            return launch_fused_sr<T, S, 2 * R0>(c, n, ichunk, h2inv, cs, x, b, xo, norm_part,
+                                                 ev0, ev1);
     }
     throw Error(NUMA_ERR_ARG, "unsupported steps x rows per thread for fused smoothing");
 }
@@ -648,14 +663,15 @@ inline unsigned launch_fused_s(Ctx& c, int n, int rows, int ichunk, double h2inv
 template <class T>
 inline unsigned launch_fused(Ctx& c, int n, int steps, int rows, int ichunk, double h2inv,
                              const PolySteps& cs, const T* x, const T* b, T* xo,
-                             double* norm_part = nullptr) {
+                             double* norm_part = nullptr, hipEvent_t ev0 = nullptr,
+                             hipEvent_t ev1 = nullptr) {
     switch (steps) {
-        case 1: return launch_fused_s<T, 1>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
-        case 2: return launch_fused_s<T, 2>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
-        case 3: return launch_fused_s<T, 3>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
-        case 4: return launch_fused_s<T, 4>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
-        case 5: return launch_fused_s<T, 5>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
-        case 6: return launch_fused_s<T, 6>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part);
+        case 1: return launch_fused_s<T, 1>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
+        case 2: return launch_fused_s<T, 2>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
+        case 3: return launch_fused_s<T, 3>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
+        case 4: return launch_fused_s<T, 4>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
+        case 5: return launch_fused_s<T, 5>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
+        case 6: return launch_fused_s<T, 6>(c, n, rows, ichunk, h2inv, cs, x, b, xo, norm_part, ev0, ev1);
         default: throw Error(NUMA_ERR_ARG, "fused smoothing supports 1..6 steps");
     }
 }
@@ -862,6 +878,16 @@ struct MGBase {
     // on every level: > 0 that chunk, 0 the separate residual and restriction
     // launches, < 0 the default choice.
     virtual void set_resid_restrict(int64_t ich) = 0;
+    // Instrumentation: with events on, every finest-level fused smoothing
+    // launch in the plane-marching form that reads the iterate (the
+    // steady-state sweep, not the first one from zero nor the one that also
+    // sums ||z||) gets a start / stop event pair stamped by its dispatch;
+    // sweep_time() returns their average duration and count (the stream is
+    // synchronised first) and clears them.  Graph replay is off while they
+    // are on.
+    virtual void set_sweep_events(bool on) = 0;
+    virtual bool sweep_events_on() const = 0;
+    virtual void sweep_time(double* avg_ms, int64_t* count) = 0;
 };
 
 template <class T>
@@ -945,7 +971,25 @@ public:
             levels_[l].rr_ich = ich < 0 ? rr_chunk(levels_[l + 1].n) : (int)ich;
     }
     ~MG() override {
+        for (auto& e : ev_pairs_) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         for (void* p : allocs_) (void)hipFree(p);
+    }
+    void set_sweep_events(bool on) override { sweep_events_ = on; }
+    bool sweep_events_on() const override { return sweep_events_; }
+    void sweep_time(double* avg_ms, int64_t* count) override {
+        NUMA_HIP_CHECK(hipStreamSynchronize(c_.stream));
+        double sum = 0.0;
+        for (size_t q = 0; q < ev_used_; ++q) {
+            float ms = 0.0f;
+            NUMA_HIP_CHECK(hipEventElapsedTime(&ms, ev_pairs_[q].first, ev_pairs_[q].second));
+            sum += ms;
+        }
+        *count = (int64_t)ev_used_;
+        *avg_ms = ev_used_ ? sum / (double)ev_used_ : 0.0;
+        ev_used_ = 0;
     }
 
     int64_t levels() const override { return (int64_t)levels_.size(); }
@@ -1221,10 +1265,26 @@ private:
                 rows = R0;
                 ichunk = fused_fill_chunk<T>((int)L.n, cnt, R0, c_.cu_count);
             }
+            // the plane-marching form only (the roofline kernel of the
+            // finest level)
+            const bool timed_sweep = sweep_events_ && &L == &levels_[0] &&
+                                     !(from_zero && st == 0) && !np && rows > 0;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timed_sweep) {
+                if (ev_used_ == ev_pairs_.size()) {
+                    hipEvent_t a = nullptr, b = nullptr;
+                    NUMA_HIP_CHECK(hipEventCreate(&a));
+                    NUMA_HIP_CHECK(hipEventCreate(&b));
+                    ev_pairs_.push_back({a, b});
+                }
+                e0 = ev_pairs_[ev_used_].first;
+                e1 = ev_pairs_[ev_used_].second;
+                ++ev_used_;
+            }
             const unsigned nb = launch_fused<T>(
                 c_, (int)L.n, cnt, rows, ichunk, L.h2inv, cs,
                 (from_zero && st == 0) ? (const T*)nullptr : (const T*)L.x, (const T*)L.b,
-                dst ? dst : L.x2, np);
+                dst ? dst : L.x2, np, e0, e1);
             if (np) {
                 c_.finish(nb, Epilogue{EPI_SQRT, 1.0, nullptr, norm_slot_, nullptr});
                 norm_done_ = true;
@@ -1427,6 +1487,9 @@ private:
     bool final_done_ = false;
     double* norm_slot_ = nullptr;  // apply_norm: device slot of ||z||
     bool norm_done_ = false;
+    bool sweep_events_ = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs_;
+    size_t ev_used_ = 0;
 };
 
 inline MGBase* make_mg(Ctx& c, int64_t n, const numa_mg_config& cfg, int64_t precision) {

@@ -221,6 +221,7 @@ public:
         inj_count_ = 0;
     }
     const MGBase* multigrid() const override { return mg_.get(); }
+    MGBase* multigrid() override { return mg_.get(); }
 
     numa_gmres_result gmres_entry(const double* x0, const double* b, double* z, double rtol,
                                   int64_t restart, int64_t max_iters, bool zero_guess,
@@ -660,7 +661,7 @@ private:
     bool in_solve_ = false;  // graphs serve solve() (fixed iterate buffers) only
     bool graph_ok() const {
         return graphs_ && !timing_ && cfg_.precond != NUMA_PRECOND_LEARNED && !capture_ &&
-               inj_what_ < 0;
+               inj_what_ < 0 && !(mg_ && mg_->sweep_events_on());
     }
     int64_t inj_what_ = -1, inj_at_ = 0, inj_count_ = 0;  // inject_nan state
     // ---- timers (operators/timing.hpp:16-108): device time of each primitive
@@ -1366,6 +1367,26 @@ int numa_hip_solver_mg_fusion(const numa_hip_solver* s, int64_t* levels, int64_t
     if (mg && mode_host && ichunk_host && rows_host)
         for (int64_t l = 0; l < mg->levels(); ++l)
             mg->fusion(l, mode_host + l, ichunk_host + l, rows_host + l);
+    NUMA_API_END
+}
+
+int numa_hip_solver_sweep_events(numa_hip_solver* s, int64_t enable) {
+    NUMA_API_BEGIN
+    numa_hip::require(s != nullptr, "null argument");
+    numa_hip::MGBase* mg = s->impl->multigrid();
+    numa_hip::require(mg != nullptr, "the solver has no multigrid preconditioner");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    mg->set_sweep_events(enable != 0);
+    NUMA_API_END
+}
+
+int numa_hip_solver_sweep_time(numa_hip_solver* s, double* avg_ms, int64_t* count) {
+    NUMA_API_BEGIN
+    numa_hip::require(s && avg_ms && count, "null argument");
+    numa_hip::MGBase* mg = s->impl->multigrid();
+    numa_hip::require(mg != nullptr, "the solver has no multigrid preconditioner");
+    NUMA_HIP_CHECK(hipSetDevice(s->ctx->device));
+    mg->sweep_time(avg_ms, count);
     NUMA_API_END
 }
 

@@ -60,6 +60,7 @@ struct SolverBase {
     virtual LearnedStats learned_stats() const = 0;
     // The multigrid preconditioner (null without one).
     virtual const MGBase* multigrid() const = 0;
+    virtual MGBase* multigrid() = 0;
     // Replay each Arnoldi step of solve() from a captured hipGraph (one graph
     // per step index and iterate buffers, captured at first use).  Solvers with
     // the learned preconditioner or a capture hook run eagerly.

@@ -321,6 +321,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                   C.POINTER(_vp)],
         "numa_hip_batch_destroy": [_vp],
         "numa_hip_batch_set_graphs": [_vp, _i64],
+        "numa_hip_solver_sweep_events": [_vp, _i64],
+        "numa_hip_solver_sweep_time": [_vp, _dp, C.POINTER(_i64)],
         "numa_hip_batch_set_smoothing": [_vp, _i64, C.POINTER(_i64)],
         "numa_hip_batch_solve": [_vp, C.POINTER(KrylovParams), C.POINTER(_vp),
                                  C.POINTER(KrylovResult), _dp],
@@ -607,6 +609,17 @@ class BratuSolver:
         m, ic, rw = (_i64 * k)(), (_i64 * k)(), (_i64 * k)()
         check(self.ctx.lib.numa_hip_solver_mg_fusion(self.h, C.byref(L), m, ic, rw))
         return [(int(m[i]), int(ic[i]), int(rw[i])) for i in range(L.value)]
+
+    def set_sweep_events(self, on: bool = True) -> None:
+        """Bracket every finest-level smoothing sweep that reads the iterate
+        with HIP events (the solve then runs eagerly)."""
+        check(self.ctx.lib.numa_hip_solver_sweep_events(self.h, 1 if on else 0))
+
+    def sweep_time(self) -> tuple[float, int]:
+        """(average ms, count) of the bracketed sweeps since the last call."""
+        ms, cnt = C.c_double(), _i64()
+        check(self.ctx.lib.numa_hip_solver_sweep_time(self.h, C.byref(ms), C.byref(cnt)))
+        return ms.value, int(cnt.value)
 
     def inject_nan(self, what: int, call: int) -> None:
         """NaN into the call-th F evaluation (INJECT_F) or J v product

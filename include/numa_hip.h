@@ -675,6 +675,17 @@ enum {
 };
 int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
                             double* ms_per_call, double* bytes_per_call);
+/* In-solve timing of the finest-level smoothing sweep (the roofline kernel of
+ * the benchmark): with enable != 0, every finest-level fused smoothing launch
+ * of the V-cycle in the plane-marching form that reads the iterate (not the
+ * first one from zero, nor the one that also sums ||z||) is launched with a
+ * start / stop event pair that the dispatch itself stamps
+ * (hipExtLaunchKernelGGL: the kernel's own execution), and the solver's
+ * Arnoldi steps run eagerly (no graphs).
+ * numa_hip_solver_sweep_time synchronises the stream and returns the average
+ * milliseconds per such launch since the last call and their count. */
+int numa_hip_solver_sweep_events(numa_hip_solver* s, int64_t enable);
+int numa_hip_solver_sweep_time(numa_hip_solver* s, double* avg_ms, int64_t* count);
 
 #ifdef __cplusplus
 }

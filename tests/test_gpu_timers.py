@@ -69,3 +69,32 @@ def test_campaign_row_reproduces_recorded_counters(ctx):
     line = campaign.csv_line(row)
     assert campaign.csv_line(campaign.parse_line(line)) == line
     assert line.startswith("calibration,bratu,B1,31,29791,")
+
+
+def test_sweep_events_time_the_finest_sweeps_without_changing_the_solve(ctx):
+    """Events around the finest-level smoothing sweeps that read the iterate
+    (the benchmark's roofline kernel, timed inside a solve): one per such
+    launch, positive durations, and the solve's iterates unchanged."""
+    n, lam = 64, 6.0
+    cfg = H.solver_config(precond=H.PRECOND_GMG, precision=H.PRECISION_MIXED, flexible=1,
+                          mg=H.mg_cheb(levels=4, fuse=H.FUSE_SWEEP, coarse_iters=-1))
+    p = H.krylov_params(gmres_restart=30)
+    x0 = H.uniform_host(3, n ** 3, 0.1)
+    a = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+    xa = x0.copy()
+    ra = a.solve_host(p, xa)
+    b = H.BratuSolver(ctx, n, lam, cfg, restart=30)
+    b.set_graphs(True)  # the events switch graph replay off
+    b.set_sweep_events(True)
+    xb = x0.copy()
+    rb = b.solve_host(p, xb)
+    ms, cnt = b.sweep_time()
+    np.testing.assert_array_equal(xa, xb)
+    assert (ra.iters, ra.n_lin_iters, ra.n_prec) == (rb.iters, rb.n_lin_iters, rb.n_prec)
+    # pre-smoothing: 2 sweeps, the first from zero; post: 2 sweeps, the last
+    # also sums ||z|| when the solve needs it (FD J v) -> 2 timed per V-cycle
+    assert cnt == 2 * rb.n_prec, (cnt, rb.n_prec)
+    assert 0.0 < ms < 1.0
+    assert b.sweep_time() == (0.0, 0)  # cleared by the read
+    a.close()
+    b.close()
