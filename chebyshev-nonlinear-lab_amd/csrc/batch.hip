@@ -152,7 +152,7 @@ public:
     }
 
 private:
-    static constexpr int64_t kBatchMaxN = 127;        // k_stencil without the plane march
+    static constexpr int64_t kBatchMaxN = 127;        // launch_stencil's forms below 128^3
     static constexpr int kSweepForms = 7;             // smoothing forms 0 .. 7 (set_smoothing)
     static constexpr int64_t kBatchPartials = 16384;  // per-instance reduction partials
     enum { S_FNORM = 0, S_XNORM, S_VNORM, S_BNORM, S_BETA, S_LAMBDA, S_LSA, kMisc = 8 };
@@ -346,19 +346,30 @@ private:
         (void)I;
         return (unsigned)(s.gx * s.gy * s.gz);
     }
-    // k_stencil over each instance's level-`lvl` grid; make(I) -> {loader, epilogue}
+    // resident workgroups of the LDS-tiled form on the device (its plane chunk)
+    template <bool REDUCE, class Load, class Epi>
+    int64_t lds_slots() const {
+        static const int per_cu =
+            blocks_per_cu(k_stencil_lds<kLdsRows, REDUCE, true, Load, Epi>, 256);
+        return (int64_t)per_cu * c_.cu_count;
+    }
+    // the stencil launch of each instance's level-`lvl` grid (launch_stencil's
+    // form for its size); make(I) -> {loader, epilogue}
     template <bool REDUCE, class Load, class Epi, class Make, class Bytes>
     void stencil(int lvl, Make&& make, Bytes&& bytes) {
         launch<OpStencil<REDUCE, Load, Epi>>(
             [&](Inst& I, typename OpStencil<REDUCE, Load, Epi>::Args& a) {
                 const int64_t n = lvl < 0 ? I.n : I.lv[(size_t)lvl].n;
-                const StencilShape s = stencil_shape(n, Epi::kMarch);
+                const bool lds = n >= kPlanesMinN;
+                const StencilShape s =
+                    lds ? stencil_shape_lds(n, lds_slots<REDUCE, Load, Epi>())
+                        : stencil_shape(n, Epi::kMarch);
                 a.n = (int32_t)n;
                 a.ichunk = s.ichunk;
                 a.gx = s.gx;
                 a.gy = s.gy;
                 a.gz = s.gz;
-                a.tk = s.tk;
+                a.tk = lds ? 0 : s.tk;
                 const auto le = make(I);
                 a.ldr = le.first;
                 a.epi = le.second;

@@ -77,8 +77,11 @@ struct LdStepDev {
 // Operations.  Args: the per-instance arguments (no padding bytes: tables are
 // cached by content); run(args, position) is the single-instance body.
 // ---------------------------------------------------------------------------
+// The stencil launch of a grid below 128^3, as launch_stencil makes it:
+// tk = 0 the LDS-tiled form with k_stencil's plane partials (n > 32), else
+// k_stencil with 64 (32) lanes along k and one node per thread.
 template <bool REDUCE, class Load, class Epi>
-struct OpStencil {  // k_stencil without the plane march (n < 128)
+struct OpStencil {
     static constexpr int kThreads = 256;
     struct Args {
         int32_t n, ichunk, gx, gy, gz, tk;
@@ -87,7 +90,10 @@ struct OpStencil {  // k_stencil without the plane march (n < 128)
         Reduce red;
     };
     __device__ static void run(const Args& a, GridPos g) {
-        if (a.tk == 64)
+        if (a.tk == 0)
+            stencil_lds_body<kLdsRows, REDUCE, true, Load, Epi>(g, a.n, a.ichunk, a.gx, a.gy,
+                                                                a.gz, a.ldr, a.epi, a.red);
+        else if (a.tk == 64)
             stencil_body<64, REDUCE, false, Load, Epi>(g, a.n, a.ichunk, a.gx, a.gy, a.gz, a.ldr,
                                                        a.epi, a.red);
         else

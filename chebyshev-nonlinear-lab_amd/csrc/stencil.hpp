@@ -145,13 +145,17 @@ __global__ __launch_bounds__(256) void k_stencil(int n, int ichunk, int gx, int 
 // plane's values go to a double-buffered LDS plane with the ring, where the
 // j / k neighbours are read; the i neighbours stay in registers.  One barrier
 // per plane.  Same Loader / Epilogue interface, arithmetic and outputs as
-// k_stencil (only a reduction's summation order differs).
-template <int RW, bool REDUCE, class Load, class Epi>
-__global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, int gy, int gz,
-                                                      Load ldr, Epi epi, Reduce red) {
+// k_stencil.  A reduction writes one partial per block, or with PLANES one
+// partial per plane and group of 4 rows: the partials of the one-node-per-
+// thread k_stencil (64 x 4 tiles, one plane each), in its tile order and with
+// its summation order (the same 64-lane wave sums, added over the 4 waves in
+// order), so the reduced total is bit for bit k_stencil's.
+template <int RW, bool REDUCE, bool PLANES, class Load, class Epi>
+__device__ inline void stencil_lds_body(GridPos g, int n, int ichunk, int gx, int gy, int gz,
+                                        const Load& ldr, const Epi& epi, const Reduce& red) {
     constexpr int TK = 64, TJ = 4 * RW;
-    const unsigned G8 = gridDim.x >> 3;
-    const unsigned lin = (blockIdx.x & 7u) * G8 + (blockIdx.x >> 3);
+    const unsigned G8 = g.nb >> 3;
+    const unsigned lin = (g.bid & 7u) * G8 + (g.bid >> 3);
     if (lin >= (unsigned)(gx * gy * gz)) return;  // padding block, uniform
     const int bx = (int)(lin % (unsigned)gx), by = (int)((lin / (unsigned)gx) % (unsigned)gy);
     const int bz = (int)(lin / (unsigned)(gx * gy));
@@ -224,6 +228,24 @@ __global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, 
         Pv[1][r] = Pv[0][r];
     }
     double acc = 0.0;
+    // PLANES: the wave sums of plane i, row group r, in slot i mod 2 (written
+    // after the plane's barrier, added up by thread 0 after the next one)
+    __shared__ double wsum[PLANES && REDUCE ? 2 : 1][PLANES && REDUCE ? RW : 1][4];
+    const int gy4 = (n + 3) / 4;  // row groups of k_stencil
+    auto flush = [&](int slot, int i) {
+        if constexpr (PLANES && REDUCE) {
+            if (threadIdx.x == 0) {
+#pragma unroll
+                for (int r = 0; r < RW; ++r) {
+                    const int g4 = by * RW + r;
+                    if (g4 >= gy4) continue;  // rows past n: no such k_stencil tile
+                    double t = 0.0;
+                    for (int w = 0; w < 4; ++w) t += wsum[slot][r][w];
+                    red.partials[(unsigned)bx + (unsigned)gx * ((unsigned)g4 + (unsigned)gy4 * (unsigned)i)] = t;
+                }
+            }
+        }
+    };
     // plane i = i0 + t, PH = t mod 6
     auto plane = [&](auto ph, int i) {
         constexpr int PH = decltype(ph)::value;
@@ -243,11 +265,21 @@ __global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, 
             H[S2] = L.load(pb + hcol);
         }
         __syncthreads();
+        if constexpr (PLANES && REDUCE) {
+            if (i > i0) flush((PH + 1) & 1, i - 1);
+        }
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             const P pp = own_ok(r, i + 1) ? L.make(Q[S1][r]) : P::zero();
-            if (own[r]) {
-                const int row = tj + 4 * r + 1;
+            const int row = tj + 4 * r + 1;
+            if constexpr (PLANES && REDUCE) {
+                double v = 0.0;  // as k_stencil's per-thread sum
+                if (own[r])
+                    v += E((int64_t)i * N2 + col[r], Ev[S0][r], Pv[S0][r], Pv[S2][r], pp,
+                           B[row - 1][tk + 1], B[row + 1][tk + 1], B[row][tk], B[row][tk + 2]);
+                const double ws = wave_sum(v);
+                if (tk == 0) wsum[PH & 1][r][tj] = ws;
+            } else if (own[r]) {
                 acc += E((int64_t)i * N2 + col[r], Ev[S0][r], Pv[S0][r], Pv[S2][r], pp,
                          B[row - 1][tk + 1], B[row + 1][tk + 1], B[row][tk], B[row][tk + 2]);
             }
@@ -262,7 +294,18 @@ __global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, 
         constexpr int PH = decltype(ph)::value;
         if (t0 + PH < count) plane(ph, i0 + t0 + PH);
     });
-    if constexpr (REDUCE) block_partial<256>(acc, red, lin);
+    if constexpr (REDUCE && PLANES) {
+        __syncthreads();
+        if (count > 0) flush((count - 1) & 1, i1 - 1);
+    } else if constexpr (REDUCE) {
+        block_partial<256>(acc, red, lin);
+    }
+}
+template <int RW, bool REDUCE, bool PLANES, class Load, class Epi>
+__global__ __launch_bounds__(256) void k_stencil_lds(int n, int ichunk, int gx, int gy, int gz,
+                                                      Load ldr, Epi epi, Reduce red) {
+    stencil_lds_body<RW, REDUCE, PLANES, Load, Epi>(hw_grid(), n, ichunk, gx, gy, gz, ldr, epi,
+                                                    red);
 }
 
 // Launch shape: 64x4 (k x j) tiles for n > 32, 32x8 below.  Epilogues with an
@@ -316,16 +359,41 @@ inline StencilShape stencil_shape_lds(int64_t n, int64_t slots) {
     return s;
 }
 
+// Grids from kPlanesMinN to kLdsMinN take the LDS-tiled form with the
+// one-node-per-thread form's reduction partials (the same values, sums and
+// totals as before, bit for bit).
+constexpr int64_t kPlanesMinN = 33;
+
+// Reduction partials of a launch over an n^3 grid: k_stencil's tiles (64 x 4
+// x 1 for n > 32), which the LDS-tiled form also writes below kLdsMinN.
+inline unsigned stencil_partials(int64_t n, bool heavy) {
+    if (n >= kLdsMinN) return 0;  // the LDS-tiled form's own count (its launch shape)
+    const StencilShape s = stencil_shape(n, heavy);
+    return (unsigned)(s.gx * s.gy * s.gz);
+}
+
 template <bool REDUCE, class Load, class Epi>
 void launch_stencil(Ctx& c, int64_t n, const Load& ldr, const Epi& epi, const Epilogue& ep) {
     if (n >= kLdsMinN) {
-        static const int per_cu = blocks_per_cu(k_stencil_lds<kLdsRows, REDUCE, Load, Epi>, 256);
+        static const int per_cu =
+            blocks_per_cu(k_stencil_lds<kLdsRows, REDUCE, false, Load, Epi>, 256);
         const StencilShape s = stencil_shape_lds(n, (int64_t)per_cu * c.cu_count);
-        hipLaunchKernelGGL((k_stencil_lds<kLdsRows, REDUCE, Load, Epi>), dim3(s.blocks),
+        hipLaunchKernelGGL((k_stencil_lds<kLdsRows, REDUCE, false, Load, Epi>), dim3(s.blocks),
                            dim3(256), 0, c.stream, (int)n, s.ichunk, s.gx, s.gy, s.gz, ldr, epi,
                            c.red);
         NUMA_HIP_LAUNCH_CHECK();
         if constexpr (REDUCE) c.finish((unsigned)(s.gx * s.gy * s.gz), ep);
+        return;
+    }
+    if (n >= kPlanesMinN) {
+        static const int per_cu =
+            blocks_per_cu(k_stencil_lds<kLdsRows, REDUCE, true, Load, Epi>, 256);
+        const StencilShape s = stencil_shape_lds(n, (int64_t)per_cu * c.cu_count);
+        hipLaunchKernelGGL((k_stencil_lds<kLdsRows, REDUCE, true, Load, Epi>), dim3(s.blocks),
+                           dim3(256), 0, c.stream, (int)n, s.ichunk, s.gx, s.gy, s.gz, ldr, epi,
+                           c.red);
+        NUMA_HIP_LAUNCH_CHECK();
+        if constexpr (REDUCE) c.finish(stencil_partials(n, Epi::kMarch), ep);
         return;
     }
     const StencilShape s = stencil_shape(n, Epi::kMarch);

@@ -136,3 +136,56 @@ def test_dot_deterministic(ctx):
     da = ctx.array(a)
     vals = {C_dot(ctx, da, da, a.size) for _ in range(5)}
     assert len(vals) == 1
+
+
+def _wave_tree(v):
+    """A 64-lane wave sum as the kernels do it (lane l adds lane l + off for
+    off = 32, 16, ..., 1; a lane past the wave adds its own value); lane 0."""
+    v = v.copy()
+    for off in (32, 16, 8, 4, 2, 1):
+        sh = np.concatenate([v[..., off:], v[..., 64 - off:]], axis=-1)
+        v = v + sh
+    return v[..., 0]
+
+
+def _finish(p):
+    """The one-block finish of a reduction: 1024 strided sequential sums, then
+    the wave sums of the 16 waves added in order."""
+    K = 1024
+    s = np.zeros(K)
+    for t in range(K):
+        acc = 0.0
+        for a in p[t::K]:
+            acc += a
+        s[t] = acc
+    ws = _wave_tree(s.reshape(16, 64))
+    tot = 0.0
+    for w in ws:
+        tot += w
+    return tot
+
+
+@pytest.mark.parametrize("n", [33, 48, 64, 96, 100, 127])
+def test_stencil_norm_sums_in_the_one_node_per_thread_order(ctx, n):
+    """Below 128^3 the LDS-tiled stencil writes the partials of the
+    one-node-per-thread form (64 x 4 tiles, one plane each, wave sums added
+    over the 4 waves in order), so ||F|| is that form's total bit for bit:
+    the reduction is redone on the host in exactly that order from the F the
+    kernel wrote."""
+    import numa_hip as H
+    rng = np.random.default_rng(n)
+    u = ctx.array(0.3 * rng.random(n ** 3))
+    F = ctx.empty(n ** 3)
+    nrm = H.Bratu3D(ctx, n, 6.0).eval(u, F)
+    c = F.numpy() ** 2
+    gx, gy4 = -(-n // 64), -(-n // 4)
+    pad = np.zeros((n, gy4 * 4, gx * 64))
+    pad[:, :n, :n] = c.reshape(n, n, n)
+    waves = _wave_tree(pad.reshape(n, gy4, 4, gx, 64).transpose(0, 1, 3, 2, 4))
+    part = np.zeros((n, gy4, gx))
+    for w in range(4):
+        part = part + waves[..., w]
+    want = np.sqrt(_finish(part.reshape(-1)))
+    assert nrm == want, (nrm, want)
+    u.free()
+    F.free()
