@@ -109,6 +109,34 @@ void jvp_fd_impl(Ctx& c, int64_t n, double lambda, const double* x, const double
         Epilogue{});
 }
 
+__global__ __launch_bounds__(numa_hip::kBlock) void k_capi_xnorm_exp(int64_t N, const double* x,
+                                                                   double lambda, float* E,
+                                                                   numa_hip::Reduce red) {
+    numa_hip::xnorm_exp_body<false>(numa_hip::hw_grid(), N, x, lambda, nullptr, E, red);
+}
+
+// The mixed solver's J v (EpJvFDe): ||x|| and E = lambda e^x in one pass, ||v||,
+// then the rearranged quotient.
+void jvp_fd_rearranged_impl(Ctx& c, int64_t n, double lambda, const double* x, const float* v,
+                            float* E, float* out) {
+    const int64_t N = n * n * n;
+    const unsigned nb = numa_hip::stream_blocks(N);
+    hipLaunchKernelGGL(k_capi_xnorm_exp, dim3(nb), dim3(numa_hip::kBlock), 0, c.stream, N, x,
+                       lambda, E, c.red);
+    NUMA_HIP_LAUNCH_CHECK();
+    c.finish(nb, Epilogue{numa_hip::EPI_SQRT, 1.0, nullptr, c.dscal + K_XNORM, nullptr});
+    numa_hip::launch_dot<float, float>(
+        c, N, v, v, Epilogue{numa_hip::EPI_SQRT, 1.0, nullptr, c.dscal + K_VNORM, nullptr});
+    const numa_hip::FDStep st{std::sqrt(std::numeric_limits<double>::epsilon()), c.dscal + K_XNORM,
+                              c.dscal + K_VNORM};
+    const double h2inv = (n + 1.0) * (n + 1.0);
+    numa_hip::launch_stencil<false>(
+        c, n, numa_hip::LdVec<float>{v},
+        numa_hip::EpJvFDe<float, false, false>{h2inv, st, nullptr, E, out, nullptr, nullptr, 0.0,
+                                               1.0},
+        Epilogue{});
+}
+
 // ---- scratch of the no-allocation entry point ----------------------------
 // The two norms of numa_hip_bratu_jvp_fd_noalloc and their reduction partials
 // live in this file-scope device array: one copy per device, part of the
@@ -285,6 +313,17 @@ int numa_hip_bratu_jvp_fd_mixed(numa_hip_ctx* ctx, int64_t n, double lambda, con
     check_grid(n);
     require(x_dev && F0_dev && v_dev && out_dev, "null device pointer");
     jvp_fd_impl<float, float>(c, n, lambda, x_dev, F0_dev, v_dev, out_dev);
+    NUMA_API_END
+}
+
+int numa_hip_bratu_jvp_fd_rearranged(numa_hip_ctx* ctx, int64_t n, double lambda,
+                                     const double* x_dev, const float* v_dev, float* E_dev,
+                                     float* out_dev) {
+    NUMA_API_BEGIN
+    Ctx& c = ctx_of(ctx);
+    check_grid(n);
+    require(x_dev && v_dev && E_dev && out_dev, "null device pointer");
+    jvp_fd_rearranged_impl(c, n, lambda, x_dev, v_dev, E_dev, out_dev);
     NUMA_API_END
 }
 

@@ -249,6 +249,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_bratu_jacobi_diag": [_vp, _i64, C.c_double, _vp, _vp],
         "numa_hip_bratu_jvp_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_bratu_jvp_fd_mixed": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
+        "numa_hip_bratu_jvp_fd_rearranged": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_bratu_d2_fd": [_vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_bratu_jvp_fd_noalloc": [C.c_int, _vp, _i64, C.c_double, _vp, _vp, _vp, _vp],
         "numa_hip_dot": [_vp, _i64, _vp, _vp, _dp],
@@ -471,6 +472,12 @@ class Bratu3D:
         f = (self.ctx.lib.numa_hip_bratu_jvp_fd_mixed if v.dtype == np.float32
              else self.ctx.lib.numa_hip_bratu_jvp_fd)
         check(f(self.ctx.h, self.n, self.lam, x.ptr, F0.ptr, v.ptr, out.ptr))
+
+    def jvp_fd_rearranged(self, x, v, E, out):
+        """The mixed solver's J v: x FP64; v, E (= lambda e^x, written) and
+        out FP32."""
+        check(self.ctx.lib.numa_hip_bratu_jvp_fd_rearranged(self.ctx.h, self.n, self.lam, x.ptr,
+                                                            v.ptr, E.ptr, out.ptr))
 
     def d2_fd(self, x, F0, v, out):
         check(self.ctx.lib.numa_hip_bratu_d2_fd(self.ctx.h, self.n, self.lam, x.ptr, F0.ptr,

@@ -205,6 +205,17 @@ int numa_hip_bratu_jvp_fd(numa_hip_ctx* ctx, int64_t n, double lambda,
 int numa_hip_bratu_jvp_fd_mixed(numa_hip_ctx* ctx, int64_t n, double lambda,
                                 const double* x_dev, const double* F0_dev,
                                 const float* v_dev, float* out_dev);
+/* The mixed solver's J v: the forward-difference quotient of
+ * numa_hip_bratu_jvp_fd (operator.hpp:101-114) in the rearranged form
+ *   out = h2inv (6v - sum nbrs) - E (e^{hv} - 1)/h,  E = lambda e^x,
+ * h = sqrt(eps)(1+||x||)/||v|| (out = 0 if ||v|| = 0): the Laplacian is
+ * linear, so only the exponential is differenced, without the rounding of
+ * x + hv and of F(x+hv) - F(x).  x in FP64; v, out and E (written here,
+ * n^3 floats, the per-Newton-point data the solver keeps) in FP32; the
+ * quotient in FP64. */
+int numa_hip_bratu_jvp_fd_rearranged(numa_hip_ctx* ctx, int64_t n, double lambda,
+                                     const double* x_dev, const float* v_dev, float* E_dev,
+                                     float* out_dev);
 /* No counterpart in the reference: numa_hip_bratu_jvp_fd without a context,
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc).  Same kernels and

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import rel
+import numa_hip as H
 
 pytestmark = pytest.mark.gpu
 
@@ -70,6 +71,55 @@ def test_jvp_fd(ctx, oracle, n):
     f.jvp_fd(dx, dF0, dv32, do32)
     ref32 = oracle.bratu_jvp_fd(n, lam, x, v32.astype(np.float64))
     assert rel(do32.numpy(), ref32) < 2e-6
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_jvp_fd_rearranged(ctx, oracle, n):
+    """The mixed solver's J v (v, E, out FP32) against the oracle's FP64
+    forward difference (F(x+hv) - F(x))/h at the same FP32 v; E = lambda e^x
+    to FP32 rounding."""
+    x, v = _inputs(n)
+    lam = 6.0
+    f = H.Bratu3D(ctx, n, lam)
+    v32 = v.astype(np.float32)
+    dE, do = ctx.empty(n ** 3, np.float32), ctx.empty(n ** 3, np.float32)
+    f.jvp_fd_rearranged(ctx.array(x), ctx.array(v32), dE, do)
+    assert rel(dE.numpy(), lam * np.exp(x)) < 1e-7
+    ref = oracle.bratu_jvp_fd(n, lam, x, v32.astype(np.float64))
+    assert rel(do.numpy(), ref) < 2e-6
+
+
+@pytest.mark.parametrize("hv", [5e-4, 9e-4, 2e-3, 3e-2])
+def test_jvp_fd_rearranged_step_sizes(ctx, oracle, hv):
+    """(e^{hv} - 1)/h on both sides of the Taylor form's bound |hv| <= 2^-10.
+    x = -C except 0 at the spike of v, with C such that h v = hv there (h =
+    sqrt(eps)(1 + ||x||)/||v||); lambda e^x vanishes off the spike, and
+    lambda = 1e4 makes the exponential term most of J v at the spike, so its
+    first-order factor 1 + hv/2 is far above the FP32 tolerance."""
+    n = 24
+    N = n ** 3
+    k = 12 * n * n + 12 * n + 12
+    v = 1e-3 * np.sin(np.arange(N, dtype=np.float64))
+    v[k] = 1.0
+    v32 = v.astype(np.float32)
+    vnorm = np.linalg.norm(v32.astype(np.float64))
+    xnorm = hv * vnorm / np.sqrt(np.finfo(float).eps) - 1.0
+    x = np.full(N, -xnorm / np.sqrt(N - 1))
+    x[k] = 0.0
+    lam = 1e4
+    f = H.Bratu3D(ctx, n, lam)
+    dE, do = ctx.empty(N, np.float32), ctx.empty(N, np.float32)
+    f.jvp_fd_rearranged(ctx.array(x), ctx.array(v32), dE, do)
+    ref = oracle.bratu_jvp_fd(n, lam, x, v32.astype(np.float64))
+    got = do.numpy()
+    assert rel(got, ref) < 2e-6
+    assert abs(got[k] - ref[k]) <= 2e-6 * abs(ref[k])
+    # the exponential term is most of J v at the spike: a quotient without
+    # its hv/2 term would miss by about hv/2 relative
+    h = np.sqrt(np.finfo(float).eps) * (1.0 + np.linalg.norm(x)) / vnorm
+    assert abs(h - hv) <= 1e-12 * hv
+    expo = lam * np.expm1(h) / h
+    assert expo > 0.5 * abs(ref[k])
 
 
 def test_jvp_fd_zero_direction(ctx, oracle):
