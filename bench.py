@@ -395,7 +395,8 @@ def learned_leg(H, ctx, args, params) -> dict:
     if args.learned_save:
         np.save(args.learned_save, net.get_params())
     params = make_params(H, args.forcing, safeguard=True)
-    mg = H.mg_cheb(levels=args.levels, degree=3, pre=2, post=2, coarse_iters=COARSE)
+    # the value leg's V-cycle (smoothing interval, autotuned fused smoothing)
+    mg = leg_configs(H, args.levels, args.smooth_rows, args.cheb_ratio)["mixed"].mg
     cfg_l = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_LEARNED,
                             precision=H.PRECISION_MIXED, flexible=1, jv_mode=H.DERIV_FD, mg=mg)
     cfg_g = H.solver_config(method=H.METHOD_NEWTON, precond=H.PRECOND_GMG,
