@@ -263,6 +263,20 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
     // One plane of the march, t = q - q_first, PH = t mod U.
     auto plane = [&](auto ph, int q) {
         constexpr int PH = decltype(ph)::value;
+        // The j neighbours of every step's plane were written in the previous
+        // iteration (before the barrier that ended it), and this iteration
+        // writes the other ring slots: all reads are issued first, so their
+        // latency overlaps the writes, the loads and the steps' VALU work.
+        T jmv[STEPS][R], jpv[STEPS][R];
+        static_for<STEPS>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            constexpr int tp = PH - 1 - s + 6 * U;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                jmv[s][r] = ring[s][tp & 1][jrow[r]][lane + 1];
+                jpv[s][r] = ring[s][tp & 1][jrow[r] + 2][lane + 1];
+            }
+        });
         // take in the loaded plane q
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -282,8 +296,8 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
                 const A c = static_cast<A>(xs[s][r][tp % 3]);
                 const A im = static_cast<A>(xs[s][r][(tp - 1) % 3]);
                 const A ip = static_cast<A>(xs[s][r][(tp + 1) % 3]);
-                const A jm = static_cast<A>(ring[s][tp & 1][jr][lane + 1]);
-                const A jp = static_cast<A>(ring[s][tp & 1][jr + 2][lane + 1]);
+                const A jm = static_cast<A>(jmv[s][r]);
+                const A jp = static_cast<A>(jpv[s][r]);
                 const A km = lane_shift<kDppWaveShr1>(c);
                 const A kp = lane_shift<kDppWaveShl1>(c);
                 // h = 0 outside the domain (columns: h_row; planes: p_in), where
