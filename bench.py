@@ -312,6 +312,10 @@ def chandra_leg(H, ctx, args) -> dict:
     p = H.krylov_params(rtol_nl=1e-10, gmres_restart=20, gmres_max_iters=400)
     out = {"workload": f"chandrasekhar_midpoint_kelley_N{CH_N}_batch{CH_B}_newton_gmres20_tol1e-10",
            "c_seed": CH_SEED}
+    # the device's measured dense MFMA throughput, beside the published peaks
+    peak64, peak16 = ctx.mfma_peak(1), ctx.mfma_peak(0)
+    out["measured_mfma_tflops"] = {"fp64_16x16x4": round(peak64, 1),
+                                   "bf16_32x32x16": round(peak16, 1)}
     sols = {}
     for name, prec in (("fp64", H.PRECISION_FP64), ("mixed", H.PRECISION_MIXED)):
         ch = H.ChandrasekharBatch(ctx, CH_N, cs, rule=1, form=1, precision=prec, restart=20)
@@ -330,14 +334,17 @@ def chandra_leg(H, ctx, args) -> dict:
                "total_jv": int(sum(r.n_JVP for r in res)),
                "gemm_fp64_ms": round(ms64, 4),
                "gemm_fp64_tflops": round(fl / (ms64 * 1e-3) / 1e12, 2),
-               "gemm_fp64_frac_of_78.6": round(fl / (ms64 * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4)}
+               "gemm_fp64_frac_of_78.6": round(fl / (ms64 * 1e-3) / 1e12 / FP64_MFMA_PEAK_TFLOPS, 4),
+               "gemm_fp64_frac_of_measured": round(fl / (ms64 * 1e-3) / 1e12 / peak64, 4)}
         if prec == H.PRECISION_MIXED:
             msb, _ = ch.gemm_cost(H.PRECISION_MIXED, 10)
             leg.update({"gemm_bf16x3_ms": round(msb, 4),
                         "gemm_bf16x3_useful_tflops": round(fl / (msb * 1e-3) / 1e12, 2),
                         "gemm_bf16x3_issued_tflops": round(3 * fl / (msb * 1e-3) / 1e12, 2),
                         "gemm_bf16x3_issued_frac_of_2500": round(
-                            3 * fl / (msb * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4)})
+                            3 * fl / (msb * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4),
+                        "gemm_bf16x3_issued_frac_of_measured": round(
+                            3 * fl / (msb * 1e-3) / 1e12 / peak16, 4)})
         out[name] = leg
         log(f"chandrasekhar {name}: {t*1e3:.1f} ms, converged {leg['converged']}/{CH_B}, "
             f"max newton {leg['max_newton']}, {leg}")
@@ -467,6 +474,9 @@ def learned_leg(H, ctx, args, params) -> dict:
     out["inference_useful_tflops"] = round(CONV_FLOP_PER_POINT * n ** 3 / (ms_inf * 1e-3) / 1e12, 1)
     out["inference_frac_of_bf16_2500"] = round(out["inference_useful_tflops"] /
                                                BF16_MFMA_PEAK_TFLOPS, 4)
+    peak16 = ctx.mfma_peak(0)
+    out["measured_bf16_mfma_tflops"] = round(peak16, 1)
+    out["inference_frac_of_measured_bf16"] = round(out["inference_useful_tflops"] / peak16, 4)
     out["train_step_64cubed_batch4_ms"] = round(ms_tr, 3)
     net.close()
     return out

@@ -1,5 +1,6 @@
 // capi.hip - C entry points: context, device memory, and the kernel-level
 // Bratu3D / BLAS-1 operations (include/numa_hip.h).
+#include <algorithm>
 #include <cstring>
 #include <limits>
 #include <memory>

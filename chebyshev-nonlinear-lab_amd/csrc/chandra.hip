@@ -974,6 +974,51 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
 
 }  // namespace numa_hip
 
+namespace {
+// ---- measured dense MFMA throughput ----------------------------------------
+// (in this file: built with the accumulators in VGPRs, as the contractions)
+// Each wave runs kPeakChains independent accumulator chains of one MFMA kind
+// for `iters` steps (no memory traffic; the sums go to `sink` only in a case
+// that does not occur, which keeps every product live).
+typedef __bf16 peak_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float peak_floatx16 __attribute__((ext_vector_type(16)));
+typedef double peak_doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kPeakThreads = 256;
+
+template <int KIND, int kPeakChains>
+__global__ __launch_bounds__(kPeakThreads) void k_mfma_peak(int iters, double* sink) {
+    const int t = (int)threadIdx.x;
+    double s = 0.0;
+    if constexpr (KIND == 0) {
+        peak_bf16x8 a, b;
+        for (int e = 0; e < 8; ++e) {
+            a[e] = static_cast<__bf16>(1e-3f * (float)(t + e));
+            b[e] = static_cast<__bf16>(2e-3f * (float)(t - e));
+        }
+        peak_floatx16 acc[kPeakChains] = {};
+#pragma unroll 2
+        for (int i = 0; i < iters; ++i)
+#pragma unroll
+            for (int q = 0; q < kPeakChains; ++q)
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+        for (int q = 0; q < kPeakChains; ++q)
+            for (int r = 0; r < 16; ++r) s += acc[q][r];
+    } else {
+        const double a = 1e-3 * t, b = 2e-3 * (t + 1);
+        peak_doublex4 acc[kPeakChains] = {};
+#pragma unroll 2
+        for (int i = 0; i < iters; ++i)
+#pragma unroll
+            for (int q = 0; q < kPeakChains; ++q)
+                acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+        for (int q = 0; q < kPeakChains; ++q)
+            for (int r = 0; r < 4; ++r) s += acc[q][r];
+    }
+    if (s == -1234.5) sink[blockIdx.x] = s;
+}
+
+}  // namespace
+
 struct numa_hip_chandra {
     numa_hip::Ctx* ctx;
     std::unique_ptr<numa_hip::ChandraBatch> impl;
@@ -983,6 +1028,52 @@ using numa_hip::Error;
 
 
 extern "C" {
+
+int numa_hip_mfma_peak(numa_hip_ctx* ctx, int64_t kind, double* tflops) {
+    NUMA_API_BEGIN
+    numa_hip::require(ctx && ctx->impl, "null context");
+    NUMA_HIP_CHECK(hipSetDevice(ctx->impl->device));
+    numa_hip::Ctx& c = *ctx->impl;
+    numa_hip::require(kind == 0 || kind == 1, "kind must be 0 (bf16) or 1 (fp64)");
+    numa_hip::require(tflops, "null result pointer");
+    const double flop_per_mfma = kind == 0 ? 2.0 * 32 * 32 * 16 : 2.0 * 16 * 16 * 4;
+    hipEvent_t e0, e1;
+    NUMA_HIP_CHECK(hipEventCreate(&e0));
+    NUMA_HIP_CHECK(hipEventCreate(&e1));
+    double best_tf = 0.0;
+    // 4 or 8 accumulator chains per wave, 4 or 8 workgroups of 4 waves per CU
+    for (int chains : {4, 8})
+        for (int per_cu : {4, 8}) {
+            const unsigned blocks = (unsigned)c.cu_count * per_cu;
+            const int iters = (kind == 0 ? 8192 : 4096) / chains * 4 / per_cu * 2;
+            const double flop =
+                flop_per_mfma * chains * (double)iters * (kPeakThreads / 64) * (double)blocks;
+            float best = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {  // the first is a warm-up
+                NUMA_HIP_CHECK(hipEventRecord(e0, c.stream));
+                auto go = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPeakThreads), 0, c.stream, iters,
+                                       c.red.partials);
+                };
+                if (kind == 0)
+                    chains == 4 ? go(k_mfma_peak<0, 4>) : go(k_mfma_peak<0, 8>);
+                else
+                    chains == 4 ? go(k_mfma_peak<1, 4>) : go(k_mfma_peak<1, 8>);
+                NUMA_HIP_LAUNCH_CHECK();
+                NUMA_HIP_CHECK(hipEventRecord(e1, c.stream));
+                NUMA_HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                NUMA_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) best = std::min(best, ms);
+            }
+            best_tf = std::max(best_tf, flop / (best * 1e-3) / 1e12);
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *tflops = best_tf;
+    NUMA_API_END
+}
+
 
 int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
                             int64_t rule, int64_t form, int64_t precision, int64_t gmres_restart,

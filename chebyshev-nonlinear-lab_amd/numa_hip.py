@@ -268,6 +268,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "numa_hip_solve": [_vp, C.POINTER(KrylovParams), _vp, C.POINTER(KrylovResult)],
         "numa_hip_solve_host": [_vp, C.POINTER(KrylovParams), _dp, C.POINTER(KrylovResult)],
         "numa_hip_primitive_cost": [_vp, _i64, _i64, _dp, _dp],
+        "numa_hip_mfma_peak": [_vp, _i64, _dp],
         "numa_hip_chandra_create": [_vp, _i64, _i64, _dp, _i64, _i64, _i64, _i64, C.POINTER(_vp)],
         "numa_hip_chandra_destroy": [_vp],
         "numa_hip_chandra_eval": [_vp, _vp, _vp, _dp],
@@ -371,6 +372,12 @@ class Context:
 
     def sync(self):
         check(self.lib.numa_hip_ctx_sync(self.h))
+
+    def mfma_peak(self, kind: int) -> float:
+        """Measured dense MFMA TFLOP/s: kind 0 bf16 (32x32x16), 1 fp64 (16x16x4)."""
+        out = C.c_double()
+        check(self.lib.numa_hip_mfma_peak(self.h, kind, C.byref(out)))
+        return out.value
 
     def close(self):
         if self.h:

@@ -686,6 +686,14 @@ enum {
 };
 int numa_hip_primitive_cost(numa_hip_solver* s, int64_t prim, int64_t reps,
                             double* ms_per_call, double* bytes_per_call);
+/* No counterpart in the reference (measurement): the device's dense MFMA
+ * throughput in TFLOP/s, measured on the context's stream with HIP events,
+ * kind 0 = v_mfma_f32_32x32x16_bf16, 1 = v_mfma_f64_16x16x4f64: 4 or 8
+ * workgroups of 4 waves per CU, every wave 4 or 8 independent accumulator
+ * chains, no memory traffic; the best of the four shapes (each the best of 2
+ * launches after a warm-up).  The measured peak beside the published one for
+ * the MFMA kernels' fractions. */
+int numa_hip_mfma_peak(numa_hip_ctx* ctx, int64_t kind, double* tflops);
 /* In-solve timing of the finest-level smoothing sweep (the roofline kernel of
  * the benchmark): with enable != 0, every finest-level fused smoothing launch
  * of the V-cycle in the plane-marching form that reads the iterate (not the

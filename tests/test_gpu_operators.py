@@ -239,3 +239,13 @@ def test_stencil_norm_sums_in_the_one_node_per_thread_order(ctx, n):
     assert nrm == want, (nrm, want)
     u.free()
     F.free()
+
+
+def test_mfma_peak_is_plausible(ctx):
+    """The measured dense MFMA throughput (a register-only loop of independent
+    accumulator chains) lies below the published MI355X peaks and within a
+    factor of three of them."""
+    bf16 = ctx.mfma_peak(0)
+    fp64 = ctx.mfma_peak(1)
+    assert 2500.0 / 3 < bf16 <= 2500.0 * 1.05, bf16
+    assert 78.6 / 3 < fp64 <= 78.6 * 1.05, fp64
