@@ -175,7 +175,8 @@ def cpu_leg(args) -> None:
     import oracle as O
     cfg = leg_configs(H, args.levels, ratio=args.cheb_ratio)["fp64"]
     m = O.MGConfig(**{k: getattr(cfg.mg, k) for k, _ in cfg.mg._fields_})
-    ocfg = O.SolverConfig(method=cfg.method, precond=cfg.precond, precision=0, flexible=0,
+    method = H.METHOD_CHEBYSHEV if args.cpu_method == "cheb" else cfg.method
+    ocfg = O.SolverConfig(method=method, precond=cfg.precond, precision=0, flexible=0,
                           jv_mode=cfg.jv_mode, d2_mode=cfg.d2_mode, mg=m)
     p = make_params(H, args.forcing, safeguard=args.cpu_safeguard)
     op = O.KrylovParams(**{k: getattr(p, k) for k, _ in p._fields_})
@@ -187,23 +188,66 @@ def cpu_leg(args) -> None:
     print(json.dumps({"seconds": t, "threads": o.threads(), "result": O.result_dict(r)}))
 
 
-def run_cpu_baseline(args, safeguard: bool = False):
+def run_cpu_baseline(args, safeguard: bool = False, method: str = "newton"):
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     import tempfile
-    out = os.path.join(tempfile.gettempdir(), f"numa_cpu_solution_{os.getpid()}_{int(safeguard)}.npy")
+    out = os.path.join(tempfile.gettempdir(),
+                       f"numa_cpu_solution_{os.getpid()}_{int(safeguard)}_{method}.npy")
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--n", str(args.n), "--lam",
            str(args.lam), "--levels", str(args.levels), "--forcing", args.forcing,
-           "--cpu-out", out, "--cheb-ratio", str(args.cheb_ratio)] + (
+           "--cpu-out", out, "--cheb-ratio", str(args.cheb_ratio), "--cpu-method", method] + (
                ["--cpu-safeguard"] if safeguard else [])
-    log(f"cpu baseline: FP64 oracle solve at n={args.n} with {threads} threads"
+    log(f"cpu baseline: FP64 oracle {method} solve at n={args.n} with {threads} threads"
         f"{' (final-solve safeguard)' if safeguard else ''}")
     p = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=args.cpu_timeout)
     if p.returncode != 0:
         raise RuntimeError("cpu leg failed: " + p.stderr[-2000:])
     info = json.loads(p.stdout.strip().splitlines()[-1])
     return info, out
+
+
+def chebyshev_outer_leg(H, ctx, args, params, psafe) -> dict:
+    """The reference's Chebyshev outer iteration (chebyshev_inexact,
+    krylov_newton.hpp:116-189: per step J z = F and J u = F''[z,z] with the
+    central-difference F'', x+ = x - z - u/2 when it lowers ||F||) on the same
+    256^3 workload: the plain FP64 method (reference forcing) and the mixed
+    one (safeguard), timed as the Newton legs, each against the CPU oracle's
+    Chebyshev solve with the same policy."""
+    import numpy as np
+    cfgs = leg_configs(H, args.levels, args.smooth_rows, args.cheb_ratio)
+    out = {"workload": f"bratu3d_{args.n}^3_lambda{args.lam}_chebyshev_inexact_fd_d2_"
+                       f"gmres30_gmg{args.levels}_cheb3_v22"}
+    sols = {}
+    for name, pp in (("fp64", params), ("mixed", psafe)):
+        c0 = cfgs[name]
+        cfg = H.solver_config(method=H.METHOD_CHEBYSHEV, precond=c0.precond,
+                              precision=c0.precision, flexible=c0.flexible, jv_mode=c0.jv_mode,
+                              d2_mode=H.DERIV_FD, mg=c0.mg)
+        s, x, r, t = time_leg(H, ctx, args.n, args.lam, cfg, pp, args.steps, args.warmup)
+        out[name] = {"s": round(float(np.mean(t)), 6), "outer_iters": int(r.iters),
+                     "gmres_iters": int(r.n_lin_iters), "n_F": int(r.n_F),
+                     "n_JVP": int(r.n_JVP), "n_D2": int(r.n_D2),
+                     "converged": int(r.converged), "final_rel_residual": r.res / r.res0,
+                     "policy": "reference forcing" if pp is params else "safeguard"}
+        sols[name] = x.numpy()
+        s.close()
+        x.free()
+        log(f"chebyshev outer {name}: {out[name]}")
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["s"] / out["mixed"]["s"], 3)
+    if not args.no_cpu:
+        for name, safe in (("fp64", False), ("mixed", True)):
+            info, path = run_cpu_baseline(args, safeguard=safe, method="cheb")
+            xc = np.load(path)
+            os.remove(path)
+            res = info["result"]
+            out[name]["cpu_fp64"] = {"s": round(info["seconds"], 3), "cores": info["threads"],
+                                     "outer_iters": res["iters"], "n_D2": res["n_D2"],
+                                     "gmres_iters": res["n_lin_iters"]}
+            out[name]["rel_l2_vs_cpu_fp64"] = float(np.linalg.norm(sols[name] - xc) /
+                                                    np.linalg.norm(xc))
+    return out
 
 
 CFG0_N, CFG0_LAM = 32, 6.0
@@ -639,6 +683,8 @@ def main() -> int:
                     help="run only this leg and print its JSON (for profiling); batch: the "
                          "sweep's batched solver alone")
     ap.add_argument("--no-config0", action="store_true", help="skip the 32^3 plain-method leg")
+    ap.add_argument("--no-chebyshev", action="store_true",
+                    help="skip the Chebyshev outer-iteration legs")
     ap.add_argument("--no-safeguard", action="store_true",
                     help="skip the legs timed with the other forcing policy")
     ap.add_argument("--cheb-ratio", type=float, default=CHEB_RATIO,
@@ -648,6 +694,7 @@ def main() -> int:
                          "safeguard); parity then holds only along the CPU's Krylov path")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-safeguard", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-method", default="newton", help=argparse.SUPPRESS)
     ap.add_argument("--config0-cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cfg0-reps", type=int, default=5, help=argparse.SUPPRESS)
     ap.add_argument("--chandra-cpu-leg", action="store_true", help=argparse.SUPPRESS)
@@ -865,6 +912,8 @@ def main() -> int:
                                "gmres_iters": info["result"]["n_lin_iters"]}
     for v in other.values():
         v.pop("x", None)
+    if not args.no_chebyshev:
+        out["chebyshev_outer"] = chebyshev_outer_leg(H, ctx, args, params, psafe)
     if not args.no_chandra:
         ch = chandra_leg(H, ctx, args)
         if not args.no_cpu:
