@@ -805,9 +805,12 @@ def main() -> int:
                 "frac_back_to_back": round(sm["GBs"] / HBM_PEAK_GBS, 4)}
     if kbase == "k_poly_fused<float,3,2>":
         # counted in the AMD assembly of this kernel's main loop
-        roofline["limiter"] = ("VALU issue: 61 VALU instructions per plane per thread for 6 "
-                               "node updates (~55% of the launch; packed FP32 is not used), "
-                               "halo columns, rows and planes recomputed (~1.8x the useful updates)")
+        roofline["limiter"] = ("VALU issue and the per-plane barrier: 61 VALU instructions "
+                               "per plane per thread for 6 node updates (~24 us of issue; packed "
+                               "FP32 is not used), halo columns, rows and planes recomputed "
+                               "(~1.7x the useful updates), one workgroup barrier per plane "
+                               "(~9 us); the plane loads ~6 us (measured on copies of the kernel "
+                               "without them)")
 
     out = {"metric": METRIC, "value": round(t_main, 6), "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_main * 1e3, 3),
