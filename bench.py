@@ -134,7 +134,10 @@ def fused_grid_threads(n: int, steps: int, rows: int, ichunk: int) -> int:
     """Threads of a k_poly_fused launch (64 x 16*rows regions less 2*steps
     halo per axis, plane chunks, blocks rounded up to a multiple of 8)."""
     out_w, out_h = 64 - 2 * steps, 16 * rows - 2 * steps
-    tiles = -(-n // out_w) * -(-n // out_h) * -(-n // ichunk)
+    # a region as wide (tall) as the domain covers that axis alone, no halo
+    tk = 1 if n <= 64 else -(-n // out_w)
+    tj = 1 if n <= 16 * rows else -(-n // out_h)
+    tiles = tk * tj * -(-n // ichunk)
     return -(-tiles // 8) * 8 * 1024
 
 

@@ -683,7 +683,7 @@ template <class T, int S, int R>
 inline int fused_fill_chunk_sr(int n, int cu_count) {
     using G = FusedGeom<S, R>;
     static const int per_cu = blocks_per_cu(k_poly_fused<T, S, R>, kFusedThreads);
-    const int64_t tiles = (int64_t)((n + G::kOutW - 1) / G::kOutW) * ((n + G::kOutH - 1) / G::kOutH);
+    const int64_t tiles = (int64_t)G::tiles_k(n) * G::tiles_j(n);
     return fill_chunk(n, tiles, (int64_t)per_cu * cu_count, 2 * S, 4, 128);
 }
 

@@ -10,8 +10,8 @@
 //
 // Work decomposition.  A workgroup of 1024 threads (16 waves) owns a region
 // of 64 (k, one lane each) x 16R (j, R rows per thread) columns and a chunk of
-// planes along i.  Few rows per thread keep the register rings small, so two
-// such workgroups (32 waves) fit a CU and keep enough loads in flight.  It marches over the planes: at plane q it loads x and b,
+// planes along i.  Few rows per thread keep the register rings small (and out
+// of scratch).  It marches over the planes: at plane q it loads x and b,
 // then step s advances the column to x_{s+1} at plane q-1-s.  The i-1 / i / i+1
 // values of every level live in registers, the j and k neighbours of the plane
 // being advanced in a 2-plane LDS ring per level (planes padded with a zero
@@ -20,7 +20,10 @@
 // the outputs are the columns at least STEPS cells inside the region (or at
 // the domain boundary, where the zero Dirichlet values are exact) and the
 // planes of the chunk; the halo planes and columns are recomputed by the
-// neighbouring workgroups.  Out-of-domain columns and planes hold zeros.
+// neighbouring workgroups.  Out-of-domain columns and planes hold zeros.  On a
+// grid no wider (taller) than the region, one region at the origin spans that
+// axis with no halo: both its edges are the boundary, and every node is an
+// output (FusedGeom::whole_k / whole_j).
 #pragma once
 
 #include <type_traits>
@@ -82,6 +85,13 @@ struct FusedGeom {
     static constexpr int kW = 64, kH = (kFusedThreads / 64) * R;
     static constexpr int kOutW = kW - 2 * STEPS, kOutH = kH - 2 * STEPS;
     static_assert(kOutW > 0 && kOutH > 0, "halo larger than the region");
+    // A region as wide (tall) as the domain needs no halo columns (rows):
+    // both of its edges are the zero Dirichlet boundary, so one region at the
+    // domain's origin covers that axis and every node in it is an output.
+    static constexpr bool whole_k(int n) { return n <= kW; }
+    static constexpr bool whole_j(int n) { return n <= kH; }
+    static constexpr int tiles_k(int n) { return whole_k(n) ? 1 : (n + kOutW - 1) / kOutW; }
+    static constexpr int tiles_j(int n) { return whole_j(n) ? 1 : (n + kOutH - 1) / kOutH; }
 };
 
 // Planes of the b register ring and of the unrolled march (see k_poly_fused):
@@ -191,7 +201,8 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
             flat[e] = T(0);
     }
     const int tk = lin % tiles_k, tj = (lin / tiles_k) % tiles_j, ch = lin / (tiles_k * tiles_j);
-    const int k0 = tk * G::kOutW - STEPS, j0 = tj * G::kOutH - STEPS;
+    const bool wk = G::whole_k(n), wj = G::whole_j(n);
+    const int k0 = wk ? 0 : tk * G::kOutW - STEPS, j0 = wj ? 0 : tj * G::kOutH - STEPS;
     const int ib = ch * ichunk, ie = min(n, ib + ichunk);
 
     const A h2inv = static_cast<A>(h2inv_d);
@@ -212,7 +223,7 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
     // the loads (kNoOffset outside the domain) and for the output store
     // (kNoOffset unless an output column of this region)
     const uint32_t plane_bytes = (uint32_t)nn * (uint32_t)sizeof(T);
-    const bool out_k = lane >= STEPS && lane < W - STEPS;
+    const bool out_k = wk || (lane >= STEPS && lane < W - STEPS);
     uint32_t off[R], off_out[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -221,7 +232,8 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
         col_in[r] = (unsigned)j < (unsigned)n && (unsigned)k < (unsigned)n;
         h_row[r] = col_in[r] ? h2inv : A(0);
         off[r] = col_in[r] ? (uint32_t)(j * n + k) * (uint32_t)sizeof(T) : kNoOffset;
-        off_out[r] = out_k && jrow[r] >= STEPS && jrow[r] < H - STEPS ? off[r] : kNoOffset;
+        off_out[r] = out_k && (wj || (jrow[r] >= STEPS && jrow[r] < H - STEPS)) ? off[r]
+                                                                                  : kNoOffset;
     }
 
     T xs[STEPS][R][3];
@@ -358,8 +370,8 @@ k_poly_fused(int n, int ichunk, int tiles_k, int tiles_j, int chunks, double h2i
 template <int STEPS, int R>
 inline dim3 fused_grid(int n, int ichunk, int& tiles_k, int& tiles_j, int& chunks) {
     using G = FusedGeom<STEPS, R>;
-    tiles_k = (n + G::kOutW - 1) / G::kOutW;
-    tiles_j = (n + G::kOutH - 1) / G::kOutH;
+    tiles_k = G::tiles_k(n);
+    tiles_j = G::tiles_j(n);
     chunks = (n + ichunk - 1) / ichunk;
     const int64_t total = (int64_t)tiles_k * tiles_j * chunks;
     return dim3((unsigned)((total + 7) / 8 * 8));
