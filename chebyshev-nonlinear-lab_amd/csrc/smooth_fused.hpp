@@ -225,6 +225,7 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
     const uint32_t plane_bytes = (uint32_t)nn * (uint32_t)sizeof(T);
     const bool out_k = wk || (lane >= STEPS && lane < W - STEPS);
     uint32_t off[R], off_out[R];
+    T out_mask[R];  // NORM: 1 for this thread's output columns, else 0
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         jrow[r] = ty + (kFusedThreads / 64) * r;
@@ -234,6 +235,7 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
         off[r] = col_in[r] ? (uint32_t)(j * n + k) * (uint32_t)sizeof(T) : kNoOffset;
         off_out[r] = out_k && (wj || (jrow[r] >= STEPS && jrow[r] < H - STEPS)) ? off[r]
                                                                                   : kNoOffset;
+        out_mask[r] = off_out[r] != kNoOffset ? T(1) : T(0);
     }
 
     T xs[STEPS][R][3];
@@ -334,8 +336,10 @@ __device__ __forceinline__ void poly_fused_body(GridPos g, int n, int ichunk, in
                         plane_rsrc(xo + (int64_t)(p_out ? p : 0) * nn, p_out ? plane_bytes : 0u);
                     buf_store<T>(xn, ro, off_out[r]);
                     if constexpr (NORM) {
-                        const double v = p_out && off_out[r] != kNoOffset ? (double)xn : 0.0;
-                        nsq += v * v;
+                        if (p_out) {  // uniform; x * 1 and x * 0 are exact
+                            const double v = static_cast<double>(xn * out_mask[r]);
+                            nsq = __builtin_fma(v, v, nsq);
+                        }
                     }
                 }
             }
