@@ -695,7 +695,7 @@ private:
             std::map<int64_t, int> sizes;
             for (auto& I : inst_) ++sizes[I.lv[l].n];
             std::ostringstream key;
-            key << dev << "|batch-smooth-v3|f32|deg=" << cfg_.mg.degree << "|pre=" << cfg_.mg.pre
+            key << dev << "|batch-smooth-v4|f32|deg=" << cfg_.mg.degree << "|pre=" << cfg_.mg.pre
                 << "|n=";
             for (auto& kv : sizes) key << kv.first << "x" << kv.second << ",";
             std::string val;

@@ -1307,7 +1307,7 @@ private:
             Level& L = levels_[l];
             if (L.fuse != NUMA_FUSE_AUTO) continue;
             std::ostringstream key;
-            key << dev << "|mg-smooth-v4|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
+            key << dev << "|mg-smooth-v5|" << (sizeof(T) == 4 ? "f32" : "f64") << "|n=" << L.n
                 << "|sm=" << cfg_.smoother << "|deg=" << cfg_.degree << "|pre=" << cfg_.pre;
             std::string val;
             const auto held = TuneCache::get().tuning(key.str());

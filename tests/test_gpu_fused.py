@@ -93,7 +93,7 @@ def test_autotune_records_choice(ctx):
     assert all(m in (H.FUSE_NONE, H.FUSE_SWEEP, H.FUSE_PHASE) for m, _, _ in fus), fus
     with open(path) as f:
         cache = json.load(f)
-    keys = [k for k in cache if "|mg-smooth-v4|f32|n=96|" in k]
+    keys = [k for k in cache if "|mg-smooth-v5|f32|n=96|" in k]
     assert keys, cache
     mode, ichunk, rows = (int(v) for v in cache[keys[0]].split(","))
     assert (mode, ichunk, rows) == fus[0]
