@@ -720,16 +720,27 @@ inline int fused_fill_chunk(int n, int steps, int rows, int cu_count) {
 // result D[l/16 + 4r][l%16], r = 0..3); three passes, the scaling, three
 // passes back, the block double-buffered in LDS.  `tab` holds S (16 x 16,
 // padded) and c (16), computed on the host.
+//
+// Every pass reads S as S[l%16][4c + l/16] (the k pass as its B operand, the
+// j and i passes as A): four values per lane, read from LDS once and held in
+// registers.  LDS rows (of the block and of S) are kDstRow = 17 doubles apart:
+// the k pass reads its A operand down a column (16 lanes, one row each),
+// which at 16 doubles (128 bytes) per row put all 16 lanes on one bank pair
+// (about 2 us per pass, 15 us per solve); at 17 they spread over all banks.
+// The values and the order of every product are those of the unpadded
+// layout.
 constexpr int kDstTab = 16 * 16 + 16;
+constexpr int kDstRow = 17;
 
 template <class T>
 __device__ inline void coarse_dst_body(int n, double h2inv, const double* __restrict__ tab,
                                        const T* __restrict__ b, T* __restrict__ xout) {
 #pragma clang fp contract(off)
-    __shared__ double u[2][4096];  // (i*16 + j)*16 + k
-    __shared__ double S[16][16];
+    __shared__ double u[2][256 * kDstRow];  // row i*16 + j, column k
+    __shared__ double S[16][kDstRow];
     __shared__ double cp[16];
     const int t = threadIdx.x, lane = t & 63, g = t >> 6;
+    const int r16 = lane & 15, q = lane >> 4;
     // all loads (table and right-hand side) issued before the first LDS write;
     // indices outside the level load element 0 and are replaced
     const double ts = tab[t < 256 ? t : 0];
@@ -746,60 +757,75 @@ __device__ inline void coarse_dst_body(int n, double h2inv, const double* __rest
     if (t < 256) S[t >> 4][t & 15] = ts;
     if (t < 16) cp[t] = tc;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) u[0][t + 1024 * m] = bok[m] ? static_cast<double>(bv[m]) : 0.0;
+    for (int m = 0; m < 4; ++m) {
+        const int id = t + 1024 * m;
+        u[0][(id >> 4) * kDstRow + (id & 15)] = bok[m] ? static_cast<double>(bv[m]) : 0.0;
+    }
     __syncthreads();
-    const int r16 = lane & 15, q = lane >> 4;
+    double sv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sv[c] = S[r16][4 * c + q];
     using d4 = __attribute__((ext_vector_type(4))) double;
     int cur = 0;
-    // Pass with A(row, kk) / B(kk, col) read through `ld_a` / `ld_b` and
-    // D(row, col) written through `st`; wave g takes product g.
+    // Pass with step c's A and B operands from `ld_a(in, c)` / `ld_b(in, c)`
+    // and D(row, col) written through `st`; wave g takes product g.
     auto pass = [&](auto ld_a, auto ld_b, auto st) {
         const double* in = u[cur];
         double* out = u[cur ^ 1];
         d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ld_a(in, r16, 4 * c + q),
-                                                       ld_b(in, 4 * c + q, r16), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ld_a(in, c), ld_b(in, c), acc, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) st(out, q + 4 * r, r16, acc[r]);
         __syncthreads();
         cur ^= 1;
     };
+    auto s_op = [&](const double*, int c) { return sv[c]; };
     // along k: D[j][p] = sum_k U[g][j][k] S[p][k]
     auto pass_k = [&]() {
-        pass([&](const double* in, int row, int kk) { return in[(g * 16 + row) * 16 + kk]; },
-             [&](const double*, int kk, int col) { return S[col][kk]; },
-             [&](double* out, int row, int col, double v) { out[(g * 16 + row) * 16 + col] = v; });
+        pass([&](const double* in, int c) { return in[(g * 16 + r16) * kDstRow + 4 * c + q]; }, s_op,
+             [&](double* out, int row, int col, double v) {
+                 out[(g * 16 + row) * kDstRow + col] = v;
+             });
     };
     // along j: D[p][k] = sum_j S[p][j] U[g][j][k]
     auto pass_j = [&]() {
-        pass([&](const double*, int row, int kk) { return S[row][kk]; },
-             [&](const double* in, int kk, int col) { return in[(g * 16 + kk) * 16 + col]; },
-             [&](double* out, int row, int col, double v) { out[(g * 16 + row) * 16 + col] = v; });
+        pass(s_op,
+             [&](const double* in, int c) { return in[(g * 16 + 4 * c + q) * kDstRow + r16]; },
+             [&](double* out, int row, int col, double v) {
+                 out[(g * 16 + row) * kDstRow + col] = v;
+             });
     };
     // along i: D[p][k] = sum_i S[p][i] U[i][g][k]
     auto pass_i = [&]() {
-        pass([&](const double*, int row, int kk) { return S[row][kk]; },
-             [&](const double* in, int kk, int col) { return in[(kk * 16 + g) * 16 + col]; },
-             [&](double* out, int row, int col, double v) { out[(row * 16 + g) * 16 + col] = v; });
+        pass(s_op,
+             [&](const double* in, int c) { return in[((4 * c + q) * 16 + g) * kDstRow + r16]; },
+             [&](double* out, int row, int col, double v) {
+                 out[(row * 16 + g) * kDstRow + col] = v;
+             });
     };
     pass_k();
     pass_j();
     pass_i();
     const double f = 8.0 / ((double)(n + 1) * (double)(n + 1) * (double)(n + 1));
-    for (int id = t; id < 4096; id += 1024) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int id = t + 1024 * m;
         const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
-        if (i < n && j < n && k < n)
-            u[cur][id] = f * (u[cur][id] / (h2inv * (cp[i] + cp[j] + cp[k])));
+        double& e = u[cur][(id >> 4) * kDstRow + k];
+        if (i < n && j < n && k < n) e = f * (e / (h2inv * (cp[i] + cp[j] + cp[k])));
     }
     __syncthreads();
     pass_k();
     pass_j();
     pass_i();
-    for (int id = t; id < 4096; id += 1024) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int id = t + 1024 * m;
         const int i = id >> 8, j = (id >> 4) & 15, k = id & 15;
-        if (i < n && j < n && k < n) xout[(i * n + j) * n + k] = static_cast<T>(u[cur][id]);
+        if (i < n && j < n && k < n)
+            xout[(i * n + j) * n + k] = static_cast<T>(u[cur][(id >> 4) * kDstRow + k]);
     }
 }
 template <class T>
