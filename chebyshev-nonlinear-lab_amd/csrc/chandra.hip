@@ -41,6 +41,7 @@ constexpr int kChBX = 16;  // blocks per instance of the batched streaming kerne
 // keeps the generating kernels instead.
 constexpr size_t kChKStoreMax = size_t(16) << 30;
 constexpr int kKdRT = 1, kKdCT = 4;  // k_gemm_f64_kd row, column tiles per wave
+constexpr int kKdNarrow = -1;        // packed FP64 block: column tile by width (gemm_f64)
 // k_gemm_bf16x3_k: waves, row tiles per wave; 4 column tiles per wave (k chunk
 // 64) where the padded batch is a multiple of 128, else 2 (k chunk 128)
 constexpr int kKbWR = 4, kKbRT = 1;
@@ -212,18 +213,49 @@ __global__ __launch_bounds__(256) void k_ch_select(int N, int Np, const double* 
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) dst[o + i] = src[o + i];
 }
 
-// y_b += sum_k coef[k*B + b] V_k,b   (k < count)
-__global__ __launch_bounds__(256) void k_ch_combine(int N, int Np, int B, int count,
+// y_b += sum_k coef[k*B + b] V_k,b   (k < count[b]: the basis vectors built
+// while instance b was iterating; later ones may hold another instance's
+// leftovers, see the packed contractions)
+__global__ __launch_bounds__(256) void k_ch_combine(int N, int Np, int B, const int* count,
                                                      const double* const* V, const double* coef,
                                                      double* y) {
 #pragma clang fp contract(off)
     const int b = blockIdx.y;
+    const int kb = count[b];
+    if (kb == 0) return;
     const size_t o = (size_t)b * Np;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
         double s = y[o + i];
-        for (int k = 0; k < count; ++k) s += coef[k * B + b] * V[k][o + i];
+        for (int k = 0; k < kb; ++k) s += coef[k * B + b] * V[k][o + i];
         y[o + i] = s;
     }
+}
+
+// Packed contractions: when only some instances use a contraction's result
+// (the rest have converged in the lockstep iteration), their columns are
+// gathered into a packed block of npad columns (a multiple of the
+// contraction's column tile), contracted, and the results scattered back.
+// Every column of a contraction is computed by the same sequence of
+// operations wherever it sits in the block, so a packed result is bit for
+// bit the full one.  dst column p = src column map[p] (p < nact), 0 for
+// nact <= p < npad.
+__global__ __launch_bounds__(256) void k_ch_gather(int Np, int nact, const int* map,
+                                                    const double* src, double* dst) {
+    const int p = blockIdx.y;
+    const bool in = p < nact;
+    const size_t so = in ? (size_t)map[p] * Np : 0, o = (size_t)p * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < Np; i += gridDim.x * 256)
+        dst[o + i] = in ? src[so + i] : 0.0;
+}
+// dst column map[p] = src column p (p < nact = gridDim.y), for each of
+// gridDim.z slices (strides sstride / dstride elements).
+template <class T>
+__global__ __launch_bounds__(256) void k_ch_scatter(int Np, const int* map, const T* src, T* dst,
+                                                     int64_t sstride, int64_t dstride) {
+    const int p = blockIdx.y;
+    const T* in = src + (int64_t)blockIdx.z * sstride + (int64_t)p * Np;
+    T* out = dst + (int64_t)blockIdx.z * dstride + (int64_t)map[p] * Np;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < Np; i += gridDim.x * 256) out[i] = in[i];
 }
 
 // user [B][N] <-> padded [Bp][Np]
@@ -340,7 +372,7 @@ public:
             const int64_t tiles = (Np_ / (kKbWR * 32 * kKbRT)) * (Bp_ / (32 * kb_ct_));
             ksplit_ = 1;
             while (ksplit_ < 4 && tiles * ksplit_ < 512) ksplit_ *= 2;
-            P_ = static_cast<float*>(alloc((size_t)ksplit_ * vec * sizeof(float)));
+            P_ = static_cast<float*>(zalloc((size_t)ksplit_ * vec * sizeof(float)));
         }
         // per-instance scalars: H ((m+1) x m x B), s ((m+1) x B), misc
         nsc_ = ((m_ + 1) * m_ + (m_ + 1) + 8) * Bp_;
@@ -351,11 +383,24 @@ public:
         vptr_ = static_cast<const double**>(alloc((size_t)(m_ + 1) * sizeof(double*)));
         coef_ = static_cast<double*>(alloc((size_t)(m_ + 1) * Bp_ * sizeof(double)));
         NUMA_HIP_CHECK(hipMemcpy(vptr_, V_.data(), (m_ + 1) * sizeof(double*), hipMemcpyHostToDevice));
+        kcnt_ = static_cast<int*>(zalloc((size_t)Bp_ * sizeof(int)));
         store_k();
+        const char* pk = std::getenv("NUMA_HIP_CHANDRA_PACK");
+        if (Kd_ && !(pk && pk[0] == '0' && pk[1] == 0)) {
+            pack_ = true;
+            gmap_ = static_cast<int*>(zalloc((size_t)Bp_ * sizeof(int)));
+            void* hm = nullptr;
+            NUMA_HIP_CHECK(hipHostMalloc(&hm, (size_t)Bp_ * sizeof(int), hipHostMallocDefault));
+            gmap_h_ = static_cast<int*>(hm);
+            Xg_ = static_cast<double*>(zalloc(vec * sizeof(double)));
+            Yg_ = static_cast<double*>(zalloc(vec * sizeof(double)));
+            if (Kh_) Pg_ = static_cast<float*>(zalloc((size_t)ksplit_ * vec * sizeof(float)));
+        }
     }
     ~ChandraBatch() {
         for (void* p : allocs_) (void)hipFree(p);
         if (hs_) (void)hipHostFree(hs_);
+        if (gmap_h_) (void)hipHostFree(gmap_h_);
     }
 
     int64_t size() const { return N_ * B_; }
@@ -519,7 +564,70 @@ private:
         NUMA_HIP_LAUNCH_CHECK();
     }
 
-    void gemm_f64(const double* Xin, double* Yout) {
+    // Columns of a packed contraction for the instances sel[b] != 0: uploads
+    // their map and returns the packed width (nact rounded up to `tile`), or 0
+    // when packing would not drop a column tile (then the full contraction
+    // runs).
+    int64_t pack_cols(const std::vector<char>* sel, int64_t tile, int64_t& nact) {
+        if (!pack_ || !sel) return 0;
+        std::vector<int> map;
+        for (int64_t b = 0; b < B_; ++b)
+            if ((*sel)[(size_t)b]) map.push_back((int)b);
+        nact = (int64_t)map.size();
+        if (tile == kKdNarrow) tile = nact <= 32 ? 16 : nact <= 64 ? 32 : 16 * kKdCT;
+        const int64_t npad = std::max<int64_t>(1, (nact + tile - 1) / tile) * tile;
+        if (npad >= Bp_) return 0;
+        if (nact > 0 && map != gmap_last_) {
+            c_.sync();  // the pinned staging may still feed the last upload
+            std::copy(map.begin(), map.end(), gmap_h_);
+            NUMA_HIP_CHECK(hipMemcpyAsync(gmap_, gmap_h_, map.size() * sizeof(int),
+                                          hipMemcpyHostToDevice, c_.stream));
+            gmap_last_ = map;
+        }
+        return npad;
+    }
+    void gather(const double* src, int64_t nact, int64_t npad) {
+        hipLaunchKernelGGL(k_ch_gather, dim3((unsigned)nbx(), (unsigned)npad), dim3(256), 0,
+                           c_.stream, (int)Np_, (int)nact, (const int*)gmap_, src, Xg_);
+        NUMA_HIP_LAUNCH_CHECK();
+    }
+
+    // Yout = K Xin (columns sel only, when given)
+    void gemm_f64(const double* Xin, double* Yout, const std::vector<char>* sel = nullptr) {
+        int64_t nact = 0;
+        // up to 128 columns: 4 chunks of loads in flight, with narrower wave
+        // tiles (16 / 32 / 64 columns) for narrower blocks, which stream all
+        // of K for few MFMAs; past 128 the full-width kernel.  Each element's
+        // chain of MFMA steps is the same in every form.
+        const int64_t npad = Kd_ ? pack_cols(sel, kKdNarrow, nact) : 0;
+        if (npad > 0) {
+            if (nact == 0) return;
+            gather(Xin, nact, npad);
+            const unsigned gx = (unsigned)(Np_ / (64 * kKdRT));
+            static_assert(kChPadN % (kGdK * 4) == 0, "prefetch ring of 4 chunks");
+            auto pf = [&](auto kern, int ct) {
+                hipLaunchKernelGGL(kern, dim3(gx, (unsigned)(npad / (16 * ct))), dim3(256), 0,
+                                   c_.stream, (int)Np_, (int)ldd(), (const double*)Kd_,
+                                   (const double*)Xg_, Yg_);
+            };
+            if (npad <= 32)
+                pf(k_gemm_f64_kd_pf<kKdRT, 1, 4>, 1);
+            else if (npad <= 64)
+                pf(k_gemm_f64_kd_pf<kKdRT, 2, 4>, 2);
+            else if (npad <= 128)
+                pf(k_gemm_f64_kd_pf<kKdRT, 4, 4>, 4);
+            else
+                hipLaunchKernelGGL((k_gemm_f64_kd<kKdRT, kKdCT>),
+                                   dim3(gx, (unsigned)(npad / (16 * kKdCT))), dim3(256), 0,
+                                   c_.stream, (int)Np_, (int)ldd(), (const double*)Kd_,
+                                   (const double*)Xg_, Yg_);
+            NUMA_HIP_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_ch_scatter<double>, dim3((unsigned)nbx(), (unsigned)nact, 1),
+                               dim3(256), 0, c_.stream, (int)Np_, (const int*)gmap_,
+                               (const double*)Yg_, Yout, (int64_t)0, (int64_t)0);
+            NUMA_HIP_LAUNCH_CHECK();
+            return;
+        }
         if (Kd_) {
             hipLaunchKernelGGL((k_gemm_f64_kd<kKdRT, kKdCT>),
                                dim3((unsigned)(Np_ / (64 * kKdRT)), (unsigned)(Bp_ / (16 * kKdCT))),
@@ -535,15 +643,25 @@ private:
     }
     // Split X, contract into the split-K slices P_, and (Yout non-null) sum
     // them into Yout; the J v path passes null and sums them in k_ch_jv.
-    void gemm_bf16(const double* Xin, double* Yout) {
-        const int64_t total = (int64_t)Bp_ * Np_;
-        hipLaunchKernelGGL(k_split_x, dim3(2048), dim3(256), 0, c_.stream, total, Xin, Xh_, Xl_);
+    // With sel, only the split-K slices of the selected columns are written
+    // (to P_; Yout must be null).
+    void gemm_bf16(const double* Xin, double* Yout, const std::vector<char>* sel = nullptr) {
+        int64_t nact = 0;
+        // a packed block uses 2 column tiles per wave (64 columns a workgroup)
+        const int64_t npad = Kh_ && !Yout ? pack_cols(sel, 64, nact) : 0;
+        const int ct = npad > 0 ? 2 : kb_ct_;
+        if (npad > 0 && nact == 0) return;
+        const int64_t cols = npad > 0 ? npad : Bp_;
+        if (npad > 0) gather(Xin, nact, npad);
+        const int64_t total = cols * Np_;
+        hipLaunchKernelGGL(k_split_x, dim3(2048), dim3(256), 0, c_.stream, total,
+                           npad > 0 ? (const double*)Xg_ : Xin, Xh_, Xl_);
         NUMA_HIP_LAUNCH_CHECK();
         if (Kh_) {
-            const dim3 grid((unsigned)(Np_ / (kKbWR * 32 * kKbRT)), (unsigned)(Bp_ / (32 * kb_ct_)),
+            const dim3 grid((unsigned)(Np_ / (kKbWR * 32 * kKbRT)), (unsigned)(cols / (32 * ct)),
                             (unsigned)ksplit_);
             const int64_t klen = Np_ / ksplit_;
-            auto kern = kb_ct_ == 4 ? (klen % 64 == 0 ? k_gemm_bf16x3_k<kKbWR, kKbRT, 4, 64>
+            auto kern = ct == 4 ? (klen % 64 == 0 ? k_gemm_bf16x3_k<kKbWR, kKbRT, 4, 64>
                                                       : k_gemm_bf16x3_k<kKbWR, kKbRT, 4, 32>)
                                     : (klen % 128 == 0  ? k_gemm_bf16x3_k<kKbWR, kKbRT, 2, 128>
                                        : klen % 64 == 0 ? k_gemm_bf16x3_k<kKbWR, kKbRT, 2, 64>
@@ -551,7 +669,17 @@ private:
             hipLaunchKernelGGL(kern, grid, dim3(kKbWR * 64), 0, c_.stream, (int)Np_, (int)ldb(),
                                ksplit_,
                                (const __bf16*)Kh_, (const __bf16*)Kl_, (const __bf16*)Xh_,
-                               (const __bf16*)Xl_, P_);
+                               (const __bf16*)Xl_, npad > 0 ? Pg_ : P_);
+            if (npad > 0) {
+                NUMA_HIP_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_ch_scatter<float>,
+                                   dim3((unsigned)nbx(), (unsigned)nact, (unsigned)ksplit_),
+                                   dim3(256), 0, c_.stream, (int)Np_, (const int*)gmap_,
+                                   (const float*)Pg_, P_, (int64_t)(npad * Np_),
+                                   (int64_t)(Bp_ * Np_));
+                NUMA_HIP_LAUNCH_CHECK();
+                return;
+            }
         } else {
             hipLaunchKernelGGL(k_gemm_bf16x3,
                                dim3((unsigned)(Np_ / kGbRows), (unsigned)(Bp_ / kGbCols),
@@ -566,9 +694,11 @@ private:
         NUMA_HIP_LAUNCH_CHECK();
     }
 
-    // F and Ax at X; ||F_b|| -> norms
-    void eval_F(const double* X, double* F, double* Ax, double* norms) {
-        gemm_f64(X, Y_);
+    // F and Ax at X; ||F_b|| -> norms (sel: only these instances' values are
+    // used; the others' are left meaningless)
+    void eval_F(const double* X, double* F, double* Ax, double* norms,
+                const std::vector<char>* sel = nullptr) {
+        gemm_f64(X, Y_, sel);
         hipLaunchKernelGGL(k_ch_F, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, X, (const double*)Y_, Ax, F, c_.red);
         NUMA_HIP_LAUNCH_CHECK();
@@ -576,20 +706,21 @@ private:
     }
 
     // out = F''(X_)[V, V], FP64 contraction (Ax_ holds A X_)
-    void d2(const double* V, double* out) {
-        gemm_f64(V, Y_);
+    void d2(const double* V, double* out, const std::vector<char>* sel = nullptr) {
+        gemm_f64(V, Y_, sel);
         hipLaunchKernelGGL(k_ch_d2, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, (const double*)Ax_, V, (const double*)Y_, out);
         NUMA_HIP_LAUNCH_CHECK();
     }
 
     // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
-    void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep) {
+    void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep,
+            const std::vector<char>* sel = nullptr) {
         const bool split = prec_ == NUMA_PRECISION_MIXED;
         if (split)
-            gemm_bf16(V, nullptr);
+            gemm_bf16(V, nullptr, sel);
         else
-            gemm_f64(V, Y_);
+            gemm_f64(V, Y_, sel);
         hipLaunchKernelGGL(k_ch_jv, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, (const double*)X_, (const double*)Ax_, V,
                            (const double*)Y_, split ? (const float*)P_ : nullptr, ksplit_,
@@ -632,6 +763,16 @@ private:
     int64_t ldb() const { return Np_; }
     double* Kd_ = nullptr;                      // stored K (null: generated per use)
     __bf16 *Kh_ = nullptr, *Kl_ = nullptr;      // its bf16 parts (split-BF16 handles)
+    // packed contractions (stored K only; NUMA_HIP_CHANDRA_PACK=0 at creation
+    // turns them off): column map, its pinned staging and last upload, the
+    // packed operand / result blocks, per-instance combine term counts
+    bool pack_ = false;
+    int* gmap_ = nullptr;
+    int* gmap_h_ = nullptr;
+    std::vector<int> gmap_last_;
+    double *Xg_ = nullptr, *Yg_ = nullptr;
+    float* Pg_ = nullptr;
+    int* kcnt_ = nullptr;
     int64_t nsc_ = 0;
     double* sd_ = nullptr;
     double* hs_ = nullptr;
@@ -692,7 +833,7 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
             bdot(V_[0], V_[0], BEpi{EPI_NORM_SCALE, nullptr, MISC(2), S(0)});
         } else {
             const BEpi ep{EPI_NORM_SCALE, nullptr, MISC(2), S(0)};
-            jv(sol, nullptr, rhs, V_[0], &ep);
+            jv(sol, nullptr, rhs, V_[0], &ep, &notdone);
             for (int64_t b = 0; b < B; ++b)
                 if (notdone[b]) ++out[b].matvecs;
         }
@@ -716,7 +857,7 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
         first = false;
         for (size_t j = 0; j < m && any(cyc); ++j) {
             // w = s_j J V_j into V_{j+1}; MGS
-            jv(V_[j], S(j), nullptr, V_[j + 1], nullptr);
+            jv(V_[j], S(j), nullptr, V_[j + 1], nullptr, &cyc);
             bdot(V_[j + 1], V_[0], BEpi{EPI_STORE, S(0), H(0, j), nullptr});
             for (size_t i = 0; i <= j; ++i) {
                 const bool last = i == j;
@@ -789,10 +930,15 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
             ++out[b].precs;  // M^{-1}(V y) with the identity
         }
         if (kmax > 0) {
+            std::vector<int> kc((size_t)B, 0);
+            for (int64_t b = 0; b < B; ++b)
+                if (notdone[b] && !out[b].breakdown) kc[b] = (int)k[b];
             NUMA_HIP_CHECK(hipMemcpyAsync(coef_, coef.data(), kmax * B * sizeof(double),
                                           hipMemcpyHostToDevice, c_.stream));
+            NUMA_HIP_CHECK(hipMemcpyAsync(kcnt_, kc.data(), B * sizeof(int),
+                                          hipMemcpyHostToDevice, c_.stream));
             hipLaunchKernelGGL(k_ch_combine, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
-                               (int)B, (int)kmax, (const double* const*)vptr_,
+                               (int)B, (const int*)kcnt_, (const double* const*)vptr_,
                                (const double*)coef_, sol);
             NUMA_HIP_LAUNCH_CHECK();
             c_.sync();
@@ -886,7 +1032,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
             // chebyshev_inexact (krylov_newton.hpp:147-169): J u = F''[z, z] with
             // eta2 = min(eta2_max, eta2_factor eta1), then the full step
             // x - z - u/2 if it lowers ||F||, else the Newton step with backtracking
-            d2(Z_, W_);
+            d2(Z_, W_, &pending);
             std::vector<double> eta2((size_t)B, 0.0);
             for (int64_t b = 0; b < B; ++b) {
                 if (!pending[b]) continue;
@@ -909,7 +1055,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
                                    (int)Np_, (const double*)X_, (const double*)Z_,
                                    (const double*)U_, Xt_);
                 NUMA_HIP_LAUNCH_CHECK();
-                eval_F(Xt_, Ft_, Axt_, MISC(0));
+                eval_F(Xt_, Ft_, Axt_, MISC(0), &full);
                 read(MISC(0), B, fn.data());
                 std::vector<double> take((size_t)Bp_, 0.0);
                 for (int64_t b = 0; b < B; ++b) {
@@ -933,7 +1079,7 @@ void ChandraBatch::solve(const numa_krylov_params& p, double* xuser, numa_krylov
             hipLaunchKernelGGL(k_ch_trial, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
                                (const double*)X_, (const double*)Z_, (const double*)MISC(3), Xt_);
             NUMA_HIP_LAUNCH_CHECK();
-            eval_F(Xt_, Ft_, Axt_, MISC(0));
+            eval_F(Xt_, Ft_, Axt_, MISC(0), &pending);
             read(MISC(0), B, fn.data());
             std::vector<double> take((size_t)Bp_, 0.0);
             for (int64_t b = 0; b < B; ++b) {

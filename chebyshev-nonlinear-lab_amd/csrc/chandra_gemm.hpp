@@ -280,6 +280,85 @@ __global__ __launch_bounds__(256) void k_gemm_f64_kd(int Np, int ldk, const doub
         }
 }
 
+// k_gemm_f64_kd with the operand loads PF chunks ahead of the MFMAs (a ring
+// of PF register sets of K and of X values, the X slot of chunk c + 1 staged
+// to LDS while chunk c is contracted).  For narrow blocks (16 instances a
+// wave), where a wave's MFMAs per chunk are few and one chunk in flight
+// leaves it waiting on memory.  Every output element gets the same MFMA
+// steps in the same order as in k_gemm_f64_kd.  Needs Np % (32 PF) == 0.
+template <int RT, int CT, int PF>
+__global__ __launch_bounds__(256) void k_gemm_f64_kd_pf(int Np, int ldk,
+                                                         const double* __restrict__ Kd,
+                                                         const double* __restrict__ X,
+                                                         double* __restrict__ Y) {
+    constexpr int XT = 2 * CT;
+    __shared__ double xs[2][16 * CT * kGdStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * (64 * RT) + wave * 16 * RT, b0 = blockIdx.y * 16 * CT;
+    const int q = lane >> 4;
+    const int nch = Np / kGdK;
+    const double* krow[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) krow[r] = Kd + (size_t)(i0 + r * 16 + (lane & 15)) * ldk + 8 * q;
+    const int sc = threadIdx.x >> 5, sk = threadIdx.x & 31;
+    const double* xsrc = X + (size_t)(b0 + sc) * Np + sk;
+    double xr[PF][XT], a[PF][RT][8];
+    // chunk c (the last chunk again past the end) into ring slot S
+    auto load = [&](int c, auto slot) {
+        constexpr int S = decltype(slot)::value;
+        const int j = (c < nch ? c : nch - 1) * kGdK;
+#pragma unroll
+        for (int t = 0; t < XT; ++t) xr[S][t] = xsrc[(size_t)(8 * t) * Np + j];
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int s = 0; s < 8; s += 2) {
+                const double2 v2 = *reinterpret_cast<const double2*>(krow[r] + j + s);
+                a[S][r][s] = v2.x;
+                a[S][r][s + 1] = v2.y;
+            }
+    };
+    auto stage = [&](double* buf, auto slot) {
+        constexpr int S = decltype(slot)::value;
+#pragma unroll
+        for (int t = 0; t < XT; ++t) buf[(sc + 8 * t) * kGdStride + sk] = xr[S][t];
+    };
+    doublex4 acc[RT][CT] = {};
+    // chunk c in slot S = c % PF, LDS buffer c & 1 (PF even)
+    auto step = [&](auto slot, int c) {
+        constexpr int S = decltype(slot)::value;
+        constexpr int N1 = (S + 1) % PF;
+        const double* xb = xs[S & 1] + (lane & 15) * kGdStride + 8 * q;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+#pragma unroll
+            for (int t = 0; t < CT; ++t) {
+                const double bv = xb[t * 16 * kGdStride + s];
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+                    acc[r][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[S][r][s], bv, acc[r][t], 0, 0, 0);
+            }
+        }
+        stage(xs[(S + 1) & 1], std::integral_constant<int, N1>{});
+        load(c + PF, slot);
+        __syncthreads();
+    };
+    static_for<PF>([&](auto sl) { load(decltype(sl)::value, sl); });
+    stage(xs[0], std::integral_constant<int, 0>{});
+    __syncthreads();
+    for (int c = 0; c < nch; c += PF)
+        static_for<PF>([&](auto sl) { step(sl, c + decltype(sl)::value); });
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+            const int col = b0 + t * 16 + (lane & 15);
+#pragma unroll
+            for (int rr = 0; rr < 4; ++rr)
+                Y[(size_t)col * Np + i0 + r * 16 + 4 * rr + q] = acc[r][t][rr];
+        }
+}
+
 // Split-BF16 from stored Kh, Kl.  WR waves per workgroup, wave tile 32 RT rows x
 // 32 CT instances (WR 32 RT rows per workgroup), k chunks of KC (32, 64 or 128,
 // dividing the split-K slice); blockIdx.z = split-K slice, as in k_gemm_bf16x3.

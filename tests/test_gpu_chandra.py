@@ -140,6 +140,40 @@ def test_stored_k_matches_generated(ctx, monkeypatch, precision, n, batch):
     assert all(c for _, _, c in got[0][4])
 
 
+@pytest.mark.parametrize("method", ["newton", "chebyshev"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_packed_contractions_match_full(ctx, monkeypatch, precision, method):
+    """Once some instances have converged, the lockstep solve contracts only
+    the columns whose results are used (gathered into a packed block of whole
+    column tiles, results scattered back).  Every instance's iterate and
+    counters are bit for bit those of the full-width contractions
+    (NUMA_HIP_CHANDRA_PACK=0).  256 instances (padded batch 256: FP64 tiles of
+    64 columns, split-BF16 tiles of 128) with c from 0.2 to 0.9999, whose
+    Newton counts differ, so the packed width drops below the batch in both
+    precisions; the Chebyshev method also packs F''[z, z] and its trial F."""
+    import numa_hip as H
+    prec = H.PRECISION_FP64 if precision == "fp64" else H.PRECISION_MIXED
+    meth = H.METHOD_NEWTON if method == "newton" else H.METHOD_CHEBYSHEV
+    n, batch = 512, 256
+    cs = np.linspace(0.2, 0.9999, batch)
+    p = H.krylov_params(rtol_nl=1e-10, gmres_restart=20)
+    got = []
+    for pack in ("1", "0"):
+        monkeypatch.setenv("NUMA_HIP_CHANDRA_PACK", pack)
+        print(f"chandrasekhar packed={pack} {precision} {method} n={n} batch={batch}", flush=True)
+        ch = H.ChandrasekharBatch(ctx, n, cs, rule=1, form=1, precision=prec, restart=20,
+                                  method=meth)
+        xs, res = ch.solve(p)
+        got.append((xs, [(r.iters, r.n_F, r.n_JVP, r.n_D2, r.n_lin_iters, r.converged, r.res)
+                         for r in res]))
+        ch.close()
+    assert np.array_equal(got[0][0], got[1][0])
+    assert got[0][1] == got[1][1]
+    its = [r[0] for r in got[0][1]]
+    assert all(r[5] for r in got[0][1])
+    assert min(its) < max(its)  # instances drop out at different steps
+
+
 @pytest.mark.parametrize("n,rule,form", [(8, 0, 0), (64, 1, 1), (200, 1, 0)])
 def test_eval_d2_fp64(ctx, oracle, n, rule, form):
     import numa_hip as H
