@@ -57,8 +57,11 @@ struct BEpi {   // per-instance epilogue of a batched reduction
     double* out2;
 };
 
-__global__ __launch_bounds__(256) void k_bfinish(int nbx, const double* partials, BEpi ep) {
-    const int b = blockIdx.x;
+// Kernels over a selection of instances take `imap` (null: every instance):
+// workgroup row y serves instance imap[y].
+__global__ __launch_bounds__(256) void k_bfinish(int nbx, const double* partials, BEpi ep,
+                                                  const int* imap) {
+    const int b = imap ? imap[blockIdx.x] : (int)blockIdx.x;
     double s = 0.0;
     for (int x = threadIdx.x; x < nbx; x += 256) s += partials[b * nbx + x];
     s = block_sum<256>(s);
@@ -74,8 +77,8 @@ __global__ __launch_bounds__(256) void k_bfinish(int nbx, const double* partials
 }
 
 __global__ __launch_bounds__(256) void k_bdot(int N, int Np, const double* a, const double* c,
-                                               Reduce red) {
-    const int b = blockIdx.y;
+                                               Reduce red, const int* imap) {
+    const int b = imap ? imap[blockIdx.y] : (int)blockIdx.y;
     const double* pa = a + (size_t)b * Np;
     const double* pc = c + (size_t)b * Np;
     double acc = 0.0;
@@ -86,9 +89,9 @@ __global__ __launch_bounds__(256) void k_bdot(int N, int Np, const double* a, co
 // w_b -= (h_b s_b) V_b ;  partial of <w_b, next_b> or <w_b, w_b> (next == null)
 __global__ __launch_bounds__(256) void k_bmgs(int N, int Np, double* w, const double* V,
                                                const double* h, const double* s,
-                                               const double* next, Reduce red) {
+                                               const double* next, Reduce red, const int* imap) {
 #pragma clang fp contract(off)
-    const int b = blockIdx.y;
+    const int b = imap ? imap[blockIdx.y] : (int)blockIdx.y;
     const double coef = h[b] * s[b];
     double* pw = w + (size_t)b * Np;
     const double* pv = V + (size_t)b * Np;
@@ -132,9 +135,9 @@ __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const do
                                                 const double* V, const double* Y,
                                                 const float* P, int ksplit, int64_t slice,
                                                 const double* scale, const double* rhs,
-                                                double* out, Reduce red) {
+                                                double* out, Reduce red, const int* imap) {
 #pragma clang fp contract(off)
-    const int b = blockIdx.y;
+    const int b = imap ? imap[blockIdx.y] : (int)blockIdx.y;
     const size_t o = (size_t)b * Np;
     const double sc = scale ? scale[b] : 1.0;
     double acc = 0.0;
@@ -384,14 +387,15 @@ public:
         coef_ = static_cast<double*>(alloc((size_t)(m_ + 1) * Bp_ * sizeof(double)));
         NUMA_HIP_CHECK(hipMemcpy(vptr_, V_.data(), (m_ + 1) * sizeof(double*), hipMemcpyHostToDevice));
         kcnt_ = static_cast<int*>(zalloc((size_t)Bp_ * sizeof(int)));
+        gmap_ = static_cast<int*>(zalloc((size_t)Bp_ * sizeof(int)));
+        void* hm = nullptr;
+        NUMA_HIP_CHECK(hipHostMalloc(&hm, (size_t)Bp_ * sizeof(int), hipHostMallocDefault));
+        gmap_h_ = static_cast<int*>(hm);
         store_k();
         const char* pk = std::getenv("NUMA_HIP_CHANDRA_PACK");
-        if (Kd_ && !(pk && pk[0] == '0' && pk[1] == 0)) {
+        lean_ = !(pk && pk[0] == '0' && pk[1] == 0);
+        if (Kd_ && lean_) {
             pack_ = true;
-            gmap_ = static_cast<int*>(zalloc((size_t)Bp_ * sizeof(int)));
-            void* hm = nullptr;
-            NUMA_HIP_CHECK(hipHostMalloc(&hm, (size_t)Bp_ * sizeof(int), hipHostMallocDefault));
-            gmap_h_ = static_cast<int*>(hm);
             Xg_ = static_cast<double*>(zalloc(vec * sizeof(double)));
             Yg_ = static_cast<double*>(zalloc(vec * sizeof(double)));
             if (Kh_) Pg_ = static_cast<float*>(zalloc((size_t)ksplit_ * vec * sizeof(float)));
@@ -559,8 +563,9 @@ private:
         c_.sync();
     }
     void bfinish(const BEpi& ep) {
-        hipLaunchKernelGGL(k_bfinish, dim3((unsigned)B_), dim3(256), 0, c_.stream, (int)nbx(),
-                           (const double*)c_.red.partials, ep);
+        if (nrows() == 0) return;
+        hipLaunchKernelGGL(k_bfinish, dim3(nrows()), dim3(256), 0, c_.stream, (int)nbx(),
+                           (const double*)c_.red.partials, ep, imap());
         NUMA_HIP_LAUNCH_CHECK();
     }
 
@@ -570,22 +575,48 @@ private:
     // runs).
     int64_t pack_cols(const std::vector<char>* sel, int64_t tile, int64_t& nact) {
         if (!pack_ || !sel) return 0;
-        std::vector<int> map;
-        for (int64_t b = 0; b < B_; ++b)
-            if ((*sel)[(size_t)b]) map.push_back((int)b);
+        const std::vector<int> map = instance_list(*sel);
         nact = (int64_t)map.size();
         if (tile == kKdNarrow) tile = nact <= 32 ? 16 : nact <= 64 ? 32 : 16 * kKdCT;
         const int64_t npad = std::max<int64_t>(1, (nact + tile - 1) / tile) * tile;
         if (npad >= Bp_) return 0;
-        if (nact > 0 && map != gmap_last_) {
-            c_.sync();  // the pinned staging may still feed the last upload
-            std::copy(map.begin(), map.end(), gmap_h_);
-            NUMA_HIP_CHECK(hipMemcpyAsync(gmap_, gmap_h_, map.size() * sizeof(int),
-                                          hipMemcpyHostToDevice, c_.stream));
-            gmap_last_ = map;
-        }
+        upload_map(map);
         return npad;
     }
+    std::vector<int> instance_list(const std::vector<char>& sel) const {
+        std::vector<int> map;
+        for (int64_t b = 0; b < B_; ++b)
+            if (sel[(size_t)b]) map.push_back((int)b);
+        return map;
+    }
+    // the instance list in gmap_ (uploaded when it changed)
+    void upload_map(const std::vector<int>& map) {
+        if (map.empty() || map == gmap_last_) return;
+        c_.sync();  // the pinned staging may still feed the last upload
+        std::copy(map.begin(), map.end(), gmap_h_);
+        NUMA_HIP_CHECK(hipMemcpyAsync(gmap_, gmap_h_, map.size() * sizeof(int),
+                                      hipMemcpyHostToDevice, c_.stream));
+        gmap_last_ = map;
+    }
+    // The instances the per-instance kernels (J v, dots, MGS, finishes) run
+    // on until the next call: sel[b] != 0, or every instance (null).  The
+    // others' outputs are left as they were.
+    void select_instances(const std::vector<char>* sel) {
+        if (!sel || !lean_) {
+            nsel_ = -1;
+            return;
+        }
+        const std::vector<int> map = instance_list(*sel);
+        if ((int64_t)map.size() == B_) {
+            nsel_ = -1;
+            return;
+        }
+        upload_map(map);
+        nsel_ = (int64_t)map.size();
+    }
+    const int* imap() const { return nsel_ < 0 ? nullptr : gmap_; }
+    unsigned nrows() const { return (unsigned)(nsel_ < 0 ? B_ : nsel_); }
+    dim3 sgrid() const { return dim3(nbx(), nrows()); }
     void gather(const double* src, int64_t nact, int64_t npad) {
         hipLaunchKernelGGL(k_ch_gather, dim3((unsigned)nbx(), (unsigned)npad), dim3(256), 0,
                            c_.stream, (int)Np_, (int)nact, (const int*)gmap_, src, Xg_);
@@ -716,21 +747,24 @@ private:
     // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
     void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep,
             const std::vector<char>* sel = nullptr) {
+        if (nrows() == 0) return;
         const bool split = prec_ == NUMA_PRECISION_MIXED;
         if (split)
             gemm_bf16(V, nullptr, sel);
         else
             gemm_f64(V, Y_, sel);
-        hipLaunchKernelGGL(k_ch_jv, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
+        hipLaunchKernelGGL(k_ch_jv, sgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, (const double*)X_, (const double*)Ax_, V,
                            (const double*)Y_, split ? (const float*)P_ : nullptr, ksplit_,
-                           (int64_t)Bp_ * Np_, scale, rhs, out, c_.red);
+                           (int64_t)Bp_ * Np_, scale, rhs, out, c_.red, imap());
         NUMA_HIP_LAUNCH_CHECK();
         if (ep) bfinish(*ep);
     }
 
     void bdot(const double* a, const double* b, const BEpi& ep) {
-        hipLaunchKernelGGL(k_bdot, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, a, b, c_.red);
+        if (nrows() == 0) return;
+        hipLaunchKernelGGL(k_bdot, sgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, a, b, c_.red,
+                           imap());
         NUMA_HIP_LAUNCH_CHECK();
         bfinish(ep);
     }
@@ -763,10 +797,13 @@ private:
     int64_t ldb() const { return Np_; }
     double* Kd_ = nullptr;                      // stored K (null: generated per use)
     __bf16 *Kh_ = nullptr, *Kl_ = nullptr;      // its bf16 parts (split-BF16 handles)
-    // packed contractions (stored K only; NUMA_HIP_CHANDRA_PACK=0 at creation
-    // turns them off): column map, its pinned staging and last upload, the
-    // packed operand / result blocks, per-instance combine term counts
+    // packed contractions (stored K only) and the instance selection of the
+    // per-instance kernels (NUMA_HIP_CHANDRA_PACK=0 at creation turns both
+    // off): instance list, its pinned staging and last upload, the packed
+    // operand / result blocks, per-instance combine term counts
+    bool lean_ = true;   // packed contractions and instance selection (NUMA_HIP_CHANDRA_PACK)
     bool pack_ = false;
+    int64_t nsel_ = -1;  // select_instances(): instances of the per-instance kernels (-1: all)
     int* gmap_ = nullptr;
     int* gmap_h_ = nullptr;
     std::vector<int> gmap_last_;
@@ -791,6 +828,11 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
     const int64_t B = B_;
     const size_t m = (size_t)restart;
     out.assign((size_t)B, numa_gmres_result{});
+    select_instances(nullptr);
+    struct AllOnExit {  // every instance again when this returns or throws
+        ChandraBatch& h;
+        ~AllOnExit() { h.select_instances(nullptr); }
+    } all_on_exit{*this};
     // x = 0 for everybody (the Newton caller passes z = 0)
     NUMA_HIP_CHECK(hipMemsetAsync(sol, 0, (size_t)Bp_ * Np_ * sizeof(double), c_.stream));
     std::vector<double> bnorm((size_t)B), beta((size_t)B);
@@ -833,7 +875,9 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
             bdot(V_[0], V_[0], BEpi{EPI_NORM_SCALE, nullptr, MISC(2), S(0)});
         } else {
             const BEpi ep{EPI_NORM_SCALE, nullptr, MISC(2), S(0)};
+            select_instances(&notdone);
             jv(sol, nullptr, rhs, V_[0], &ep, &notdone);
+            select_instances(nullptr);
             for (int64_t b = 0; b < B; ++b)
                 if (notdone[b]) ++out[b].matvecs;
         }
@@ -857,18 +901,20 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
         first = false;
         for (size_t j = 0; j < m && any(cyc); ++j) {
             // w = s_j J V_j into V_{j+1}; MGS
+            select_instances(&cyc);
             jv(V_[j], S(j), nullptr, V_[j + 1], nullptr, &cyc);
             bdot(V_[j + 1], V_[0], BEpi{EPI_STORE, S(0), H(0, j), nullptr});
             for (size_t i = 0; i <= j; ++i) {
                 const bool last = i == j;
-                hipLaunchKernelGGL(k_bmgs, bgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
+                hipLaunchKernelGGL(k_bmgs, sgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
                                    V_[j + 1], (const double*)V_[i], (const double*)H(i, j),
                                    (const double*)S(i), last ? nullptr : (const double*)V_[i + 1],
-                                   c_.red);
+                                   c_.red, imap());
                 NUMA_HIP_LAUNCH_CHECK();
                 bfinish(last ? BEpi{EPI_NORM_SCALE, nullptr, H(j + 1, j), S(j + 1)}
                              : BEpi{EPI_STORE, S(i + 1), H(i + 1, j), nullptr});
             }
+            select_instances(nullptr);
             read(H(0, j), (int64_t)(j + 2) * Bp_, col.data());
             for (int64_t b = 0; b < B; ++b) {
                 if (!cyc[b]) continue;

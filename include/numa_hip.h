@@ -426,8 +426,9 @@ int numa_hip_prolong3d(numa_hip_ctx* ctx, const double* coarse_dev, int64_t nc,
  * 16 GiB and the device can allocate it; NUMA_HIP_CHANDRA_STORE_K=0 in the environment at creation keeps the
  * kernels that generate it per use (the results are bit-identical).  With the stored matrix, the
  * lockstep solve contracts only the instances still iterating (packed blocks of columns, 64 MB more
- * device memory at N=8192, B=256 for MIXED, 32 MB for FP64); NUMA_HIP_CHANDRA_PACK=0 at creation
- * contracts every instance every time (same results, bit for bit). */
+ * device memory at N=8192, B=256 for MIXED, 32 MB for FP64) and runs its per-instance Arnoldi
+ * kernels only for them; NUMA_HIP_CHANDRA_PACK=0 at creation computes every instance every time
+ * (same results, bit for bit). */
 typedef struct numa_hip_chandra numa_hip_chandra;
 int numa_hip_chandra_create(numa_hip_ctx* ctx, int64_t n, int64_t batch, const double* c_host,
                             int64_t rule, int64_t form, int64_t precision,

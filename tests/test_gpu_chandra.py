@@ -145,8 +145,10 @@ def test_stored_k_matches_generated(ctx, monkeypatch, precision, n, batch):
 def test_packed_contractions_match_full(ctx, monkeypatch, precision, method):
     """Once some instances have converged, the lockstep solve contracts only
     the columns whose results are used (gathered into a packed block of whole
-    column tiles, results scattered back).  Every instance's iterate and
-    counters are bit for bit those of the full-width contractions
+    column tiles, results scattered back) and runs the Arnoldi step's J v,
+    dots, MGS passes and finishes only for the instances still in their
+    cycle.  Every instance's iterate and counters are bit for bit those of
+    the run that computes everything for every instance
     (NUMA_HIP_CHANDRA_PACK=0).  256 instances (padded batch 256: FP64 tiles of
     64 columns, split-BF16 tiles of 128) with c from 0.2 to 0.9999, whose
     Newton counts differ, so the packed width drops below the batch in both
