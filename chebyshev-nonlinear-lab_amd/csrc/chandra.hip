@@ -135,7 +135,8 @@ __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const do
                                                 const double* V, const double* Y,
                                                 const float* P, int ksplit, int64_t slice,
                                                 const double* scale, const double* rhs,
-                                                double* out, Reduce red, const int* imap) {
+                                                double* out, Reduce red, const int* imap,
+                                                const double* dotv) {
 #pragma clang fp contract(off)
     const int b = imap ? imap[blockIdx.y] : (int)blockIdx.y;
     const size_t o = (size_t)b * Np;
@@ -160,7 +161,10 @@ __global__ __launch_bounds__(256) void k_ch_jv(int N, int Np, int form, const do
         jv = jv * sc;
         if (rhs) jv = rhs[o + i] - jv;
         out[o + i] = jv;
-        acc += jv * jv;
+        if (dotv)  // <out, dotv>, term for term as k_bdot adds it (one fused multiply-add)
+            acc = __builtin_fma(jv, dotv[o + i], acc);
+        else
+            acc += jv * jv;
     }
     block_partial<256>(acc, red, b * gridDim.x + blockIdx.x);
 }
@@ -745,8 +749,9 @@ private:
     }
 
     // out = scale * J(X_) V  (or rhs - J V); ||out_b||^2 partials -> ep if given
+    // dotv: the partials are of <out_b, dotv_b> instead of ||out_b||^2
     void jv(const double* V, const double* scale, const double* rhs, double* out, const BEpi* ep,
-            const std::vector<char>* sel = nullptr) {
+            const std::vector<char>* sel = nullptr, const double* dotv = nullptr) {
         if (nrows() == 0) return;
         const bool split = prec_ == NUMA_PRECISION_MIXED;
         if (split)
@@ -756,7 +761,7 @@ private:
         hipLaunchKernelGGL(k_ch_jv, sgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_, form_,
                            (const double*)c_dev_, (const double*)X_, (const double*)Ax_, V,
                            (const double*)Y_, split ? (const float*)P_ : nullptr, ksplit_,
-                           (int64_t)Bp_ * Np_, scale, rhs, out, c_.red, imap());
+                           (int64_t)Bp_ * Np_, scale, rhs, out, c_.red, imap(), dotv);
         NUMA_HIP_LAUNCH_CHECK();
         if (ep) bfinish(*ep);
     }
@@ -902,8 +907,9 @@ void ChandraBatch::gmres_batch(const std::vector<char>& active, const std::vecto
         for (size_t j = 0; j < m && any(cyc); ++j) {
             // w = s_j J V_j into V_{j+1}; MGS
             select_instances(&cyc);
-            jv(V_[j], S(j), nullptr, V_[j + 1], nullptr, &cyc);
-            bdot(V_[j + 1], V_[0], BEpi{EPI_STORE, S(0), H(0, j), nullptr});
+            // w = s_j J V_j into V_{j+1} and the first MGS dot s_0 <w, V_0raw>
+            const BEpi h0{EPI_STORE, S(0), H(0, j), nullptr};
+            jv(V_[j], S(j), nullptr, V_[j + 1], &h0, &cyc, V_[0]);
             for (size_t i = 0; i <= j; ++i) {
                 const bool last = i == j;
                 hipLaunchKernelGGL(k_bmgs, sgrid(), dim3(256), 0, c_.stream, (int)N_, (int)Np_,
